@@ -1,0 +1,261 @@
+/*
+ * rtdl_mi355x.h — C-ABI of the MI355X-native RTDL backend (librtdl_mi355x.so).
+ *
+ * The plain per-call and prepared entry points replace the reference's
+ * HIPRT C-ABI one for one (same record layouts, same argument meaning, same
+ * status/error-buffer convention); each declaration cites the reference entry
+ * point it replaces.  The device-resident batch and serving entry points at
+ * the end have no counterpart in the reference: they extend the backend with
+ * the batched and served execution patterns.
+ *
+ * Conventions (reference: src/native/hiprt/rtdl_hiprt_core.cpp:3323-3335,
+ * src/native/hiprt/rtdl_hiprt_api.cpp:288-330):
+ *   - return 0 on success; non-zero on failure with a NUL-terminated message
+ *     truncated into `error_out` (size `error_size`);
+ *   - `*rows_out` / `*row_count_out` are zeroed on entry; row buffers are
+ *     allocated by the library and released with rtdl_mi355x_free_rows;
+ *   - a null input pointer with a non-zero count is an error;
+ *   - an empty probe set yields 0 rows; an empty build set yields all-zero
+ *     any-hit / hit-count rows and no neighbour, closest-hit or PIP rows;
+ *   - inputs are borrowed for the duration of the call only; prepared handles
+ *     are owned by the caller and released with the matching destroy call.
+ *   - all entry points are thread-safe; calls on one prepared handle are
+ *     serialised by the handle.
+ */
+#ifndef RTDL_MI355X_H
+#define RTDL_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- input records (reference: src/rtdsl/hiprt_runtime.py:108-260) ---- */
+
+typedef struct RtdlPoint {
+  uint32_t id;
+  double x;
+  double y;
+} RtdlPoint; /* 24 bytes */
+
+typedef struct RtdlPoint3D {
+  uint32_t id;
+  double x;
+  double y;
+  double z;
+} RtdlPoint3D; /* 32 bytes */
+
+typedef struct RtdlPolygonRef {
+  uint32_t id;
+  uint32_t vertex_offset; /* in vertices, into vertices_xy (2 doubles each) */
+  uint32_t vertex_count;
+} RtdlPolygonRef; /* 12 bytes */
+
+#pragma pack(push, 1)
+typedef struct RtdlTriangle3D {
+  uint32_t id;
+  double x0, y0, z0, x1, y1, z1, x2, y2, z2;
+} RtdlTriangle3D; /* 76 bytes, packed */
+
+typedef struct RtdlRay3D {
+  uint32_t id;
+  double ox, oy, oz, dx, dy, dz, tmax;
+} RtdlRay3D; /* 60 bytes, packed */
+#pragma pack(pop)
+
+/* ---- output rows (reference: src/rtdsl/hiprt_runtime.py:172-267,
+ *      src/rtdsl/embree_runtime.py:420-479) ---- */
+
+typedef struct RtdlFixedRadiusNeighborRow {
+  uint32_t query_id;
+  uint32_t neighbor_id;
+  double distance;
+} RtdlFixedRadiusNeighborRow; /* 16 bytes */
+
+typedef struct RtdlKnnNeighborRow {
+  uint32_t query_id;
+  uint32_t neighbor_id;
+  double distance;
+  uint32_t neighbor_rank;
+} RtdlKnnNeighborRow; /* 24 bytes */
+
+typedef struct RtdlRayAnyHitRow {
+  uint32_t ray_id;
+  uint32_t any_hit;
+} RtdlRayAnyHitRow;
+
+typedef struct RtdlRayHitCountRow {
+  uint32_t ray_id;
+  uint32_t hit_count;
+} RtdlRayHitCountRow;
+
+typedef struct RtdlRayClosestHitRow {
+  uint32_t ray_id;
+  uint32_t triangle_id;
+  double t;
+} RtdlRayClosestHitRow; /* 16 bytes */
+
+typedef struct RtdlPipRow {
+  uint32_t point_id;
+  uint32_t polygon_id;
+  uint32_t contains;
+} RtdlPipRow;
+
+/* ---- library / device ---- */
+
+/* replaces rtdl_hiprt_get_version (src/native/hiprt/rtdl_hiprt_api.cpp:1-9) */
+int rtdl_mi355x_get_version(int* major, int* minor, int* patch);
+
+/* replaces rtdl_hiprt_context_probe (src/native/hiprt/rtdl_hiprt_api.cpp:1059-1100):
+ * device_type = 1 for a GPU; api_version = HIP runtime version. */
+int rtdl_mi355x_context_probe(char* device_name, size_t device_name_size, int* device_type,
+                              int* api_version, char* error_out, size_t error_size);
+
+/* replaces rtdl_hiprt_free_rows (src/native/hiprt/rtdl_hiprt_api.cpp:11-13) */
+void rtdl_mi355x_free_rows(void* rows);
+
+/* Per-phase wall times of the last call on this thread, in seconds:
+ * [0] upload+pack, [1] build, [2] traverse, [3] emit, [4] copy-back, [5] total.
+ * (cf. src/rtdsl/optix_runtime.py:4880-4902).  Returns the count written. */
+int rtdl_mi355x_get_last_phase_timings(double* seconds_out, size_t capacity);
+
+/* ---- plain per-call entry points ---- */
+
+/* replaces rtdl_hiprt_run_fixed_radius_neighbors_2d (src/native/hiprt/rtdl_hiprt_api.cpp:488-515) */
+int rtdl_mi355x_run_fixed_radius_neighbors_2d(const RtdlPoint* queries, size_t query_count,
+                                              const RtdlPoint* search_points, size_t search_count,
+                                              double radius, uint32_t k_max,
+                                              RtdlFixedRadiusNeighborRow** rows_out,
+                                              size_t* row_count_out, char* error_out,
+                                              size_t error_size);
+
+/* replaces rtdl_hiprt_run_fixed_radius_neighbors_3d (src/native/hiprt/rtdl_hiprt_api.cpp:410-437) */
+int rtdl_mi355x_run_fixed_radius_neighbors_3d(const RtdlPoint3D* queries, size_t query_count,
+                                              const RtdlPoint3D* search_points, size_t search_count,
+                                              double radius, uint32_t k_max,
+                                              RtdlFixedRadiusNeighborRow** rows_out,
+                                              size_t* row_count_out, char* error_out,
+                                              size_t error_size);
+
+/* replaces the KNN path of run_hiprt (src/rtdsl/hiprt_runtime.py:1398-1475), with the
+ * row shape of rtdl_oracle_run_k_closest_hits (src/native/oracle/rtdl_oracle_abi.h:390-409) */
+int rtdl_mi355x_run_knn_rows_2d(const RtdlPoint* queries, size_t query_count,
+                                const RtdlPoint* search_points, size_t search_count, uint32_t k,
+                                RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                char* error_out, size_t error_size);
+int rtdl_mi355x_run_knn_rows_3d(const RtdlPoint3D* queries, size_t query_count,
+                                const RtdlPoint3D* search_points, size_t search_count, uint32_t k,
+                                RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                char* error_out, size_t error_size);
+
+/* replaces the bounded-KNN path of run_hiprt (src/rtdsl/hiprt_runtime.py:1478-1510) */
+int rtdl_mi355x_run_bounded_knn_rows_2d(const RtdlPoint* queries, size_t query_count,
+                                        const RtdlPoint* search_points, size_t search_count,
+                                        double radius, uint32_t k_max,
+                                        RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                        char* error_out, size_t error_size);
+int rtdl_mi355x_run_bounded_knn_rows_3d(const RtdlPoint3D* queries, size_t query_count,
+                                        const RtdlPoint3D* search_points, size_t search_count,
+                                        double radius, uint32_t k_max,
+                                        RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                        char* error_out, size_t error_size);
+
+/* replaces rtdl_hiprt_run_ray_anyhit_3d (src/native/hiprt/rtdl_hiprt_api.cpp:288-408) */
+int rtdl_mi355x_run_ray_anyhit_3d(const RtdlRay3D* rays, size_t ray_count,
+                                  const RtdlTriangle3D* triangles, size_t triangle_count,
+                                  RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
+                                  char* error_out, size_t error_size);
+
+/* replaces rtdl_hiprt_run_ray_hitcount_3d (src/native/hiprt/rtdl_hiprt_api.cpp:183-286) */
+int rtdl_mi355x_run_ray_hitcount_3d(const RtdlRay3D* rays, size_t ray_count,
+                                    const RtdlTriangle3D* triangles, size_t triangle_count,
+                                    RtdlRayHitCountRow** rows_out, size_t* row_count_out,
+                                    char* error_out, size_t error_size);
+
+/* replaces the closest-hit runner of the Embree backend
+ * (src/native/embree/rtdl_embree_api.cpp:1195, rtdl_embree_run_ray_closest_hit_3d) */
+int rtdl_mi355x_run_ray_closest_hit_3d(const RtdlRay3D* rays, size_t ray_count,
+                                       const RtdlTriangle3D* triangles, size_t triangle_count,
+                                       RtdlRayClosestHitRow** rows_out, size_t* row_count_out,
+                                       char* error_out, size_t error_size);
+
+/* replaces rtdl_hiprt_run_point_primitive_anyhit_packet (src/native/hiprt/rtdl_hiprt_api.cpp:574-593);
+ * positive_only follows the Embree signature (src/rtdsl/embree_runtime.py:3933):
+ * 0 = full_matrix rows, 1 = positive_hits rows. */
+int rtdl_mi355x_run_point_in_polygon(const RtdlPoint* points, size_t point_count,
+                                     const RtdlPolygonRef* polygons, size_t polygon_count,
+                                     const double* vertices_xy, size_t vertex_xy_count,
+                                     uint32_t positive_only, RtdlPipRow** rows_out,
+                                     size_t* row_count_out, char* error_out, size_t error_size);
+
+/* ---- prepared (scene-resident) entry points ---- */
+
+/* replaces rtdl_hiprt_prepare_fixed_radius_neighbors_3d (src/native/hiprt/rtdl_hiprt_api.cpp:439-454).
+ * The prepared point scene serves fixed-radius, KNN and bounded-KNN probes. */
+int rtdl_mi355x_prepare_points_2d(const RtdlPoint* search_points, size_t search_count,
+                                  void** prepared_out, char* error_out, size_t error_size);
+int rtdl_mi355x_prepare_points_3d(const RtdlPoint3D* search_points, size_t search_count,
+                                  void** prepared_out, char* error_out, size_t error_size);
+
+/* replaces rtdl_hiprt_run_prepared_fixed_radius_neighbors_3d (src/native/hiprt/rtdl_hiprt_api.cpp:460-486);
+ * `queries` points to RtdlPoint or RtdlPoint3D records matching the prepared dimension. */
+int rtdl_mi355x_run_prepared_fixed_radius_neighbors(void* prepared, const void* queries,
+                                                    size_t query_count, double radius,
+                                                    uint32_t k_max,
+                                                    RtdlFixedRadiusNeighborRow** rows_out,
+                                                    size_t* row_count_out, char* error_out,
+                                                    size_t error_size);
+int rtdl_mi355x_run_prepared_knn_rows(void* prepared, const void* queries, size_t query_count,
+                                      uint32_t k, RtdlKnnNeighborRow** rows_out,
+                                      size_t* row_count_out, char* error_out, size_t error_size);
+int rtdl_mi355x_run_prepared_bounded_knn_rows(void* prepared, const void* queries,
+                                              size_t query_count, double radius, uint32_t k_max,
+                                              RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                              char* error_out, size_t error_size);
+
+/* replaces rtdl_hiprt_prepare_ray_hitcount_3d / _run_prepared_ray_hitcount_3d
+ * (src/native/hiprt/rtdl_hiprt_api.cpp:15-79); the prepared triangle scene serves
+ * any-hit, hit-count and closest-hit probes. */
+int rtdl_mi355x_prepare_triangles_3d(const RtdlTriangle3D* triangles, size_t triangle_count,
+                                     void** prepared_out, char* error_out, size_t error_size);
+int rtdl_mi355x_run_prepared_ray_anyhit_3d(void* prepared, const RtdlRay3D* rays,
+                                           size_t ray_count, RtdlRayAnyHitRow** rows_out,
+                                           size_t* row_count_out, char* error_out,
+                                           size_t error_size);
+int rtdl_mi355x_run_prepared_ray_hitcount_3d(void* prepared, const RtdlRay3D* rays,
+                                             size_t ray_count, RtdlRayHitCountRow** rows_out,
+                                             size_t* row_count_out, char* error_out,
+                                             size_t error_size);
+int rtdl_mi355x_run_prepared_ray_closest_hit_3d(void* prepared, const RtdlRay3D* rays,
+                                                size_t ray_count, RtdlRayClosestHitRow** rows_out,
+                                                size_t* row_count_out, char* error_out,
+                                                size_t error_size);
+
+/* replaces the prepared PIP path of the Embree backend
+ * (src/rtdsl/embree_runtime.py:646-768 prepared executions over polygon scenes) */
+int rtdl_mi355x_prepare_polygons(const RtdlPolygonRef* polygons, size_t polygon_count,
+                                 const double* vertices_xy, size_t vertex_xy_count,
+                                 void** prepared_out, char* error_out, size_t error_size);
+int rtdl_mi355x_run_prepared_point_in_polygon(void* prepared, const RtdlPoint* points,
+                                              size_t point_count, uint32_t positive_only,
+                                              RtdlPipRow** rows_out, size_t* row_count_out,
+                                              char* error_out, size_t error_size);
+
+/* replaces rtdl_hiprt_destroy_prepared_* (src/native/hiprt/rtdl_hiprt_api.cpp:48-50,456-458) */
+void rtdl_mi355x_destroy_prepared(void* prepared);
+
+/* ---- diagnostics (no counterpart in the reference) ---- */
+
+/* Evaluates sqrt(a[i]) and a[i] / b[i] in FP64 on the device with the same
+ * code generation as the refine kernels, so tests can confirm both are
+ * correctly rounded (the refine step relies on it for bit-exact rows). */
+int rtdl_mi355x_selftest_fp64(const double* a, const double* b, size_t n, double* sqrt_out,
+                              double* div_out, char* error_out, size_t error_size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTDL_MI355X_H */

@@ -1,0 +1,388 @@
+// extern "C" entry points of librtdl_mi355x.so (declared in
+// include/rtdl_mi355x.h).  Each wraps a runtime pipeline in the status /
+// error-buffer convention of the reference's native backends
+// (reference: src/native/hiprt/rtdl_hiprt_core.cpp:3323-3335).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "../../include/rtdl_mi355x.h"
+#include "common.hpp"
+#include "io.hpp"
+#include "runtime.hpp"
+
+using namespace rtdl;
+
+namespace {
+
+constexpr int kVersionMajor = 0;
+constexpr int kVersionMinor = 1;
+constexpr int kVersionPatch = 0;
+
+template <class Row>
+void zero_outputs(Row** rows_out, size_t* n_out) {
+  require(rows_out != nullptr && n_out != nullptr, "output pointers must not be null");
+  *rows_out = nullptr;
+  *n_out = 0;
+}
+
+// Per-call runs build a throw-away scene on the calling thread's stream.
+template <class Build, class Run>
+void per_call(Build&& build, Run&& run) {
+  PhaseClock& clock = thread_clock();
+  clock.begin();
+  Prepared scene(false);
+  build(scene);
+  RTDL_HIP(hipStreamSynchronize(scene.stream));
+  clock.lap(1);
+  run(scene);
+  clock.end();
+}
+
+template <class Run>
+void on_prepared(void* handle, Run&& run) {
+  Prepared* p = prepared_from(handle);
+  std::lock_guard<std::mutex> g(p->lock);
+  PhaseClock& clock = thread_clock();
+  clock.begin();
+  run(*p);
+  clock.end();
+}
+
+template <class Build>
+void make_prepared(void** prepared_out, Build&& build) {
+  require(prepared_out != nullptr, "prepared_out must not be null");
+  *prepared_out = nullptr;
+  PhaseClock& clock = thread_clock();
+  clock.begin();
+  auto p = std::make_unique<Prepared>(true);
+  build(*p);
+  RTDL_HIP(hipStreamSynchronize(p->stream));
+  clock.lap(1);
+  clock.end();
+  *prepared_out = p.release();
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtdl_mi355x_get_version(int* major, int* minor, int* patch) {
+  if (!major || !minor || !patch) return 1;
+  *major = kVersionMajor;
+  *minor = kVersionMinor;
+  *patch = kVersionPatch;
+  return 0;
+}
+
+int rtdl_mi355x_context_probe(char* device_name, size_t device_name_size, int* device_type,
+                              int* api_version, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(device_name != nullptr && device_name_size > 0 && device_type != nullptr &&
+                api_version != nullptr,
+            "null output passed to rtdl_mi355x_context_probe");
+    device_name[0] = '\0';
+    *device_type = -1;
+    int runtime = 0;
+    RTDL_HIP(hipRuntimeGetVersion(&runtime));
+    *api_version = runtime;
+    int count = 0;
+    RTDL_HIP(hipGetDeviceCount(&count));
+    require(count > 0, "no HIP device is visible");
+    hipDeviceProp_t props;
+    RTDL_HIP(hipGetDeviceProperties(&props, 0));
+    std::snprintf(device_name, device_name_size, "%s (%s)", props.name, props.gcnArchName);
+    *device_type = 1;
+  });
+}
+
+void rtdl_mi355x_free_rows(void* rows) { std::free(rows); }
+
+int rtdl_mi355x_get_last_phase_timings(double* seconds_out, size_t capacity) {
+  if (!seconds_out) return 0;
+  const auto& s = thread_clock().seconds;
+  const size_t n = capacity < s.size() ? capacity : s.size();
+  for (size_t i = 0; i < n; ++i) seconds_out[i] = s[i];
+  return (int)n;
+}
+
+// ---------------------------------------------------------- neighbours --
+
+static int neighbors_per_call(const void* queries, size_t nq, const void* search, size_t ns,
+                              int dim, NeighborMode mode, double radius, uint32_t k,
+                              void** rows_out, size_t* n_out, char* error_out,
+                              size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, n_out);
+    require((nq == 0 || queries != nullptr) && (ns == 0 || search != nullptr),
+            "input pointers must not be null when counts are nonzero");
+    require(k >= 1, mode == NeighborMode::Knn ? "knn_rows k must be positive"
+                                              : "k_max must be positive");
+    require(!(radius < 0.0), "radius must be non-negative");
+    per_call([&](Prepared& p) { prepare_points(p, search, ns, dim); },
+             [&](Prepared& p) { run_neighbors(p, queries, nq, mode, radius, k, rows_out, n_out); });
+  });
+}
+
+int rtdl_mi355x_run_fixed_radius_neighbors_2d(const RtdlPoint* queries, size_t query_count,
+                                              const RtdlPoint* search_points, size_t search_count,
+                                              double radius, uint32_t k_max,
+                                              RtdlFixedRadiusNeighborRow** rows_out,
+                                              size_t* row_count_out, char* error_out,
+                                              size_t error_size) {
+  return neighbors_per_call(queries, query_count, search_points, search_count, 2,
+                            NeighborMode::FixedRadius, radius, k_max, (void**)rows_out,
+                            row_count_out, error_out, error_size);
+}
+
+int rtdl_mi355x_run_fixed_radius_neighbors_3d(const RtdlPoint3D* queries, size_t query_count,
+                                              const RtdlPoint3D* search_points, size_t search_count,
+                                              double radius, uint32_t k_max,
+                                              RtdlFixedRadiusNeighborRow** rows_out,
+                                              size_t* row_count_out, char* error_out,
+                                              size_t error_size) {
+  return neighbors_per_call(queries, query_count, search_points, search_count, 3,
+                            NeighborMode::FixedRadius, radius, k_max, (void**)rows_out,
+                            row_count_out, error_out, error_size);
+}
+
+int rtdl_mi355x_run_knn_rows_2d(const RtdlPoint* queries, size_t query_count,
+                                const RtdlPoint* search_points, size_t search_count, uint32_t k,
+                                RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                char* error_out, size_t error_size) {
+  return neighbors_per_call(queries, query_count, search_points, search_count, 2,
+                            NeighborMode::Knn, 0.0, k, (void**)rows_out, row_count_out, error_out,
+                            error_size);
+}
+
+int rtdl_mi355x_run_knn_rows_3d(const RtdlPoint3D* queries, size_t query_count,
+                                const RtdlPoint3D* search_points, size_t search_count, uint32_t k,
+                                RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                char* error_out, size_t error_size) {
+  return neighbors_per_call(queries, query_count, search_points, search_count, 3,
+                            NeighborMode::Knn, 0.0, k, (void**)rows_out, row_count_out, error_out,
+                            error_size);
+}
+
+int rtdl_mi355x_run_bounded_knn_rows_2d(const RtdlPoint* queries, size_t query_count,
+                                        const RtdlPoint* search_points, size_t search_count,
+                                        double radius, uint32_t k_max,
+                                        RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                        char* error_out, size_t error_size) {
+  return neighbors_per_call(queries, query_count, search_points, search_count, 2,
+                            NeighborMode::BoundedKnn, radius, k_max, (void**)rows_out,
+                            row_count_out, error_out, error_size);
+}
+
+int rtdl_mi355x_run_bounded_knn_rows_3d(const RtdlPoint3D* queries, size_t query_count,
+                                        const RtdlPoint3D* search_points, size_t search_count,
+                                        double radius, uint32_t k_max,
+                                        RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                        char* error_out, size_t error_size) {
+  return neighbors_per_call(queries, query_count, search_points, search_count, 3,
+                            NeighborMode::BoundedKnn, radius, k_max, (void**)rows_out,
+                            row_count_out, error_out, error_size);
+}
+
+// ------------------------------------------------------------------ rays --
+
+static int rays_per_call(const RtdlRay3D* rays, size_t nr, const RtdlTriangle3D* tris, size_t nt,
+                         int mode, void** rows_out, size_t* n_out, char* error_out,
+                         size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, n_out);
+    require((nr == 0 || rays != nullptr) && (nt == 0 || tris != nullptr),
+            "input pointers must not be null when counts are nonzero");
+    per_call([&](Prepared& p) { prepare_triangles(p, tris, nt); },
+             [&](Prepared& p) { run_rays(p, rays, nr, mode, rows_out, n_out); });
+  });
+}
+
+int rtdl_mi355x_run_ray_anyhit_3d(const RtdlRay3D* rays, size_t ray_count,
+                                  const RtdlTriangle3D* triangles, size_t triangle_count,
+                                  RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
+                                  char* error_out, size_t error_size) {
+  return rays_per_call(rays, ray_count, triangles, triangle_count, 0, (void**)rows_out,
+                       row_count_out, error_out, error_size);
+}
+
+int rtdl_mi355x_run_ray_hitcount_3d(const RtdlRay3D* rays, size_t ray_count,
+                                    const RtdlTriangle3D* triangles, size_t triangle_count,
+                                    RtdlRayHitCountRow** rows_out, size_t* row_count_out,
+                                    char* error_out, size_t error_size) {
+  return rays_per_call(rays, ray_count, triangles, triangle_count, 1, (void**)rows_out,
+                       row_count_out, error_out, error_size);
+}
+
+int rtdl_mi355x_run_ray_closest_hit_3d(const RtdlRay3D* rays, size_t ray_count,
+                                       const RtdlTriangle3D* triangles, size_t triangle_count,
+                                       RtdlRayClosestHitRow** rows_out, size_t* row_count_out,
+                                       char* error_out, size_t error_size) {
+  return rays_per_call(rays, ray_count, triangles, triangle_count, 2, (void**)rows_out,
+                       row_count_out, error_out, error_size);
+}
+
+// ------------------------------------------------------------------- PIP --
+
+int rtdl_mi355x_run_point_in_polygon(const RtdlPoint* points, size_t point_count,
+                                     const RtdlPolygonRef* polygons, size_t polygon_count,
+                                     const double* vertices_xy, size_t vertex_xy_count,
+                                     uint32_t positive_only, RtdlPipRow** rows_out,
+                                     size_t* row_count_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, row_count_out);
+    require((point_count == 0 || points != nullptr) && (polygon_count == 0 || polygons != nullptr),
+            "input pointers must not be null when counts are nonzero");
+    per_call(
+        [&](Prepared& p) { prepare_polygons(p, polygons, polygon_count, vertices_xy, vertex_xy_count); },
+        [&](Prepared& p) {
+          run_pip(p, points, point_count, positive_only != 0, (void**)rows_out, row_count_out);
+        });
+  });
+}
+
+// -------------------------------------------------------------- prepared --
+
+int rtdl_mi355x_prepare_points_2d(const RtdlPoint* search_points, size_t search_count,
+                                  void** prepared_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    make_prepared(prepared_out, [&](Prepared& p) { prepare_points(p, search_points, search_count, 2); });
+  });
+}
+
+int rtdl_mi355x_prepare_points_3d(const RtdlPoint3D* search_points, size_t search_count,
+                                  void** prepared_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    make_prepared(prepared_out, [&](Prepared& p) { prepare_points(p, search_points, search_count, 3); });
+  });
+}
+
+int rtdl_mi355x_run_prepared_fixed_radius_neighbors(void* prepared, const void* queries,
+                                                    size_t query_count, double radius,
+                                                    uint32_t k_max,
+                                                    RtdlFixedRadiusNeighborRow** rows_out,
+                                                    size_t* row_count_out, char* error_out,
+                                                    size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, row_count_out);
+    require(k_max >= 1, "k_max must be positive");
+    on_prepared(prepared, [&](Prepared& p) {
+      run_neighbors(p, queries, query_count, NeighborMode::FixedRadius, radius, k_max,
+                    (void**)rows_out, row_count_out);
+    });
+  });
+}
+
+int rtdl_mi355x_run_prepared_knn_rows(void* prepared, const void* queries, size_t query_count,
+                                      uint32_t k, RtdlKnnNeighborRow** rows_out,
+                                      size_t* row_count_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, row_count_out);
+    require(k >= 1, "knn_rows k must be positive");
+    on_prepared(prepared, [&](Prepared& p) {
+      run_neighbors(p, queries, query_count, NeighborMode::Knn, 0.0, k, (void**)rows_out,
+                    row_count_out);
+    });
+  });
+}
+
+int rtdl_mi355x_run_prepared_bounded_knn_rows(void* prepared, const void* queries,
+                                              size_t query_count, double radius, uint32_t k_max,
+                                              RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
+                                              char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, row_count_out);
+    require(k_max >= 1, "k_max must be positive");
+    on_prepared(prepared, [&](Prepared& p) {
+      run_neighbors(p, queries, query_count, NeighborMode::BoundedKnn, radius, k_max,
+                    (void**)rows_out, row_count_out);
+    });
+  });
+}
+
+int rtdl_mi355x_prepare_triangles_3d(const RtdlTriangle3D* triangles, size_t triangle_count,
+                                     void** prepared_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    make_prepared(prepared_out, [&](Prepared& p) { prepare_triangles(p, triangles, triangle_count); });
+  });
+}
+
+static int rays_prepared(void* prepared, const RtdlRay3D* rays, size_t nr, int mode,
+                         void** rows_out, size_t* n_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, n_out);
+    on_prepared(prepared, [&](Prepared& p) { run_rays(p, rays, nr, mode, rows_out, n_out); });
+  });
+}
+
+int rtdl_mi355x_run_prepared_ray_anyhit_3d(void* prepared, const RtdlRay3D* rays,
+                                           size_t ray_count, RtdlRayAnyHitRow** rows_out,
+                                           size_t* row_count_out, char* error_out,
+                                           size_t error_size) {
+  return rays_prepared(prepared, rays, ray_count, 0, (void**)rows_out, row_count_out, error_out,
+                       error_size);
+}
+
+int rtdl_mi355x_run_prepared_ray_hitcount_3d(void* prepared, const RtdlRay3D* rays,
+                                             size_t ray_count, RtdlRayHitCountRow** rows_out,
+                                             size_t* row_count_out, char* error_out,
+                                             size_t error_size) {
+  return rays_prepared(prepared, rays, ray_count, 1, (void**)rows_out, row_count_out, error_out,
+                       error_size);
+}
+
+int rtdl_mi355x_run_prepared_ray_closest_hit_3d(void* prepared, const RtdlRay3D* rays,
+                                                size_t ray_count, RtdlRayClosestHitRow** rows_out,
+                                                size_t* row_count_out, char* error_out,
+                                                size_t error_size) {
+  return rays_prepared(prepared, rays, ray_count, 2, (void**)rows_out, row_count_out, error_out,
+                       error_size);
+}
+
+int rtdl_mi355x_prepare_polygons(const RtdlPolygonRef* polygons, size_t polygon_count,
+                                 const double* vertices_xy, size_t vertex_xy_count,
+                                 void** prepared_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    make_prepared(prepared_out, [&](Prepared& p) {
+      prepare_polygons(p, polygons, polygon_count, vertices_xy, vertex_xy_count);
+    });
+  });
+}
+
+int rtdl_mi355x_run_prepared_point_in_polygon(void* prepared, const RtdlPoint* points,
+                                              size_t point_count, uint32_t positive_only,
+                                              RtdlPipRow** rows_out, size_t* row_count_out,
+                                              char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, row_count_out);
+    on_prepared(prepared, [&](Prepared& p) {
+      run_pip(p, points, point_count, positive_only != 0, (void**)rows_out, row_count_out);
+    });
+  });
+}
+
+void rtdl_mi355x_destroy_prepared(void* prepared) {
+  if (!prepared) return;
+  auto* p = static_cast<Prepared*>(prepared);
+  if (p->magic != 0x52544D58u) return;
+  delete p;
+}
+
+int rtdl_mi355x_selftest_fp64(const double* a, const double* b, size_t n, double* sqrt_out,
+                              double* div_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(n == 0 || (a && b && sqrt_out && div_out), "null pointer with nonzero count");
+    if (!n) return;
+    hipStream_t s = thread_stream();
+    DeviceArray<double> da(n), db(n), ds(n), dd(n);
+    RTDL_HIP(hipMemcpyAsync(da.get(), a, n * sizeof(double), hipMemcpyHostToDevice, s));
+    RTDL_HIP(hipMemcpyAsync(db.get(), b, n * sizeof(double), hipMemcpyHostToDevice, s));
+    fp64_selftest(da.get(), db.get(), n, ds.get(), dd.get(), s);
+    RTDL_HIP(hipMemcpyAsync(sqrt_out, ds.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+    RTDL_HIP(hipMemcpyAsync(div_out, dd.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+    RTDL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

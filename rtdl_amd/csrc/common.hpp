@@ -1,0 +1,160 @@
+// Shared host/device infrastructure of librtdl_mi355x: error handling,
+// device buffers, the BVH node format and the conservative FP32 bounds.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+namespace rtdl {
+
+// ---------------------------------------------------------------- errors --
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define RTDL_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t rtdl_err_ = (call);                                                  \
+    if (rtdl_err_ != hipSuccess) {                                                  \
+      throw ::rtdl::Error(std::string(#call) + " failed: " + hipGetErrorString(rtdl_err_) + \
+                          " (" __FILE__ ":" + std::to_string(__LINE__) + ")");      \
+    }                                                                               \
+  } while (0)
+
+#define RTDL_LAUNCH_CHECK() RTDL_HIP(hipGetLastError())
+
+inline void require(bool ok, const std::string& message) {
+  if (!ok) throw Error(message);
+}
+
+// ------------------------------------------------------- device buffers --
+
+// Device allocations go through a process-wide caching pool so that the
+// per-call path does not pay hipMalloc/hipFree for every temporary.
+void* pool_alloc(size_t bytes);
+void pool_free(void* ptr);
+void pool_trim();
+
+template <class T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(size_t n) { resize(n); }
+  ~DeviceArray() { reset(); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : ptr_(o.ptr_), n_(o.n_) {
+    o.ptr_ = nullptr;
+    o.n_ = 0;
+  }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) {
+      reset();
+      ptr_ = o.ptr_;
+      n_ = o.n_;
+      o.ptr_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  void resize(size_t n) {
+    reset();
+    if (n) ptr_ = static_cast<T*>(pool_alloc(n * sizeof(T)));
+    n_ = n;
+  }
+  void reset() {
+    if (ptr_) pool_free(ptr_);
+    ptr_ = nullptr;
+    n_ = 0;
+  }
+  T* get() const { return ptr_; }
+  size_t size() const { return n_; }
+  size_t bytes() const { return n_ * sizeof(T); }
+
+ private:
+  T* ptr_ = nullptr;
+  size_t n_ = 0;
+};
+
+// ------------------------------------------------------- BVH node format --
+
+// One BVH node is 32 bytes: an FP32 box and two links.  Nodes are visited in
+// depth-first preorder without a stack: on a box hit an internal node
+// continues at its left child (`link`), a leaf tests its primitives; on a
+// miss, or after a leaf, traversal continues at `escape` (the next node in
+// preorder once this subtree is skipped).
+//   internal node: link = node index of the left child
+//   leaf node:     link = kLeafFlag | (count - 1) << 28 | first primitive
+constexpr uint32_t kLeafFlag = 0x80000000u;
+constexpr uint32_t kEndNode = 0xFFFFFFFFu;
+constexpr uint32_t kMaxLeafSize = 8;
+constexpr uint32_t kMaxLeafStart = (1u << 28) - 1;
+
+struct alignas(16) BvhNode {
+  float lo[3];
+  uint32_t link;
+  float hi[3];
+  uint32_t escape;
+};
+static_assert(sizeof(BvhNode) == 32, "BvhNode must be 32 bytes");
+
+__host__ __device__ inline bool node_is_leaf(uint32_t link) { return (link & kLeafFlag) != 0; }
+__host__ __device__ inline uint32_t leaf_first(uint32_t link) { return link & kMaxLeafStart; }
+__host__ __device__ inline uint32_t leaf_count(uint32_t link) { return ((link >> 28) & 7u) + 1u; }
+
+// ---------------------------------------------- conservative FP32 bounds --
+//
+// Traversal and primitive prefilters run in FP32; the final predicate is
+// always decided in FP64.  For a query q (FP64) and a search distance rho,
+// with M >= every |coordinate| of the scene and the query, the FP32 squared
+// distance of any point or box within true distance rho of q is at most
+//   thr2(rho) = ((rho + 2^-21 * M) * (1 + 2^-18))^2,
+// since each FP32 coordinate difference is within 4 * 2^-24 * M of the exact
+// one (two roundings of the inputs and one of the difference), the Euclidean
+// error is at most sqrt(3) * 4 * 2^-24 * M < 2^-21 * M, and the three
+// squarings and two additions add at most 4 * 2^-24 relative error.  The
+// (1 + 2^-18) factor also covers the FP64 evaluation of the bound itself.
+__device__ inline float frn_threshold2(double rho, float m) {
+  if (!(rho < 3.0e38)) return __builtin_inff();
+  const double slack = (double)m * 4.76837158203125e-07;  // 2^-21
+  const double t = (rho + slack) * 1.000003814697265625;  // 1 + 2^-18
+  return __double2float_ru(t * t);
+}
+
+// M for one query: the scene bound and |q|, each rounded up.
+__device__ inline float coord_bound(float scene_m, double x, double y, double z) {
+  float m = scene_m;
+  m = fmaxf(m, __double2float_ru(fabs(x)));
+  m = fmaxf(m, __double2float_ru(fabs(y)));
+  m = fmaxf(m, __double2float_ru(fabs(z)));
+  return m;
+}
+
+__device__ inline float box_dist2(const float q[3], const float lo[3], const float hi[3]) {
+  float d2 = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float g = fmaxf(fmaxf(lo[a] - q[a], q[a] - hi[a]), 0.0f);
+    d2 = __builtin_fmaf(g, g, d2);
+  }
+  return d2;
+}
+
+// Strict (distance, id) order of the reference's candidate sort.
+__host__ __device__ inline bool key_less(double d0, uint32_t i0, double d1, uint32_t i1) {
+  return d0 < d1 || (d0 == d1 && i0 < i1);
+}
+
+// ----------------------------------------------------------- misc helpers --
+
+inline unsigned blocks_for(size_t n, unsigned block) {
+  return static_cast<unsigned>((n + block - 1) / block);
+}
+
+}  // namespace rtdl

@@ -1,0 +1,574 @@
+// LBVH construction for points, triangles and polygons.
+//
+//   1. bounds:   per-primitive FP32 boxes rounded outward from the FP64 input,
+//                per-instance centroid bounds, the coordinate bound M and
+//                whether every coordinate is exact in FP32;
+//   2. keys:     (instance << 32) | Morton code of the centroid (30-bit 3-D,
+//                32-bit 2-D), sorted with the LDS radix sort (stable, so equal
+//                codes keep input order);
+//   3. gather:   primitives copied into Morton order;
+//   4. leaves:   runs of up to `leaf_size` consecutive primitives of one
+//                instance; Karras' binary radix tree over the leaf keys
+//                (ties broken by leaf index);
+//   5. ropes:    the escape link of a node whose leaf range ends at leaf b is
+//                the highest node whose range starts at leaf b + 1 (found by
+//                an atomicMax over range lengths), so traversal needs no
+//                stack;
+//   6. refit:    bottom-up box unions, one climbing thread per leaf, the
+//                second thread to reach a node continues.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+#include "primitives.hpp"
+#include "scene.hpp"
+
+namespace rtdl {
+namespace {
+
+__device__ inline int ordered_from_float(float f) {
+  int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__host__ inline float float_from_ordered(int i) {
+  int b = i >= 0 ? i : i ^ 0x7FFFFFFF;
+  float f;
+  std::memcpy(&f, &b, sizeof(f));
+  return f;
+}
+
+struct BoundsTask {
+  uint32_t instance;
+  uint32_t begin;
+  uint32_t end;
+};
+
+__device__ inline void prim_box(PrimKind kind, size_t p, const double* coords, const uint4* refs,
+                                const double* verts, double lo[3], double hi[3], double c[3],
+                                bool* exact, double* maxabs) {
+  *exact = true;
+  *maxabs = 0.0;
+  auto see = [&](double v) {
+    *exact = *exact && ((double)(float)v == v);
+    *maxabs = fmax(*maxabs, fabs(v));
+  };
+  if (kind == PrimKind::Points) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v = coords[3 * p + a];
+      lo[a] = hi[a] = c[a] = v;
+      see(v);
+    }
+  } else if (kind == PrimKind::Triangles) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v0 = coords[9 * p + a], v1 = coords[9 * p + 3 + a], v2 = coords[9 * p + 6 + a];
+      lo[a] = fmin(fmin(v0, v1), v2);
+      hi[a] = fmax(fmax(v0, v1), v2);
+      c[a] = (v0 + v1 + v2) / 3.0;
+      see(v0);
+      see(v1);
+      see(v2);
+    }
+  } else {
+    const uint4 r = refs[p];
+    lo[0] = lo[1] = __builtin_inf();
+    hi[0] = hi[1] = -__builtin_inf();
+    for (uint32_t k = 0; k < r.y; ++k) {
+      const double x = verts[2 * ((size_t)r.x + k)], y = verts[2 * ((size_t)r.x + k) + 1];
+      lo[0] = fmin(lo[0], x);
+      hi[0] = fmax(hi[0], x);
+      lo[1] = fmin(lo[1], y);
+      hi[1] = fmax(hi[1], y);
+      see(x);
+      see(y);
+    }
+    lo[2] = hi[2] = 0.0;
+    c[0] = 0.5 * (lo[0] + hi[0]);
+    c[1] = 0.5 * (lo[1] + hi[1]);
+    c[2] = 0.0;
+  }
+}
+
+__global__ void bounds_kernel(PrimKind kind, const BoundsTask* tasks, const double* coords,
+                              const uint4* refs, const double* verts, float4* box_lo,
+                              float4* box_hi, int* inst_bounds, uint32_t* maxabs_bits,
+                              uint32_t* exact_flag) {
+  const BoundsTask task = tasks[blockIdx.x];
+  __shared__ int red[6];
+  __shared__ float red_m;
+  __shared__ uint32_t red_exact;
+  if (threadIdx.x < 3) {
+    red[threadIdx.x] = 0x7FFFFFFF;
+    red[3 + threadIdx.x] = (int)0x80000000;
+  }
+  if (threadIdx.x == 0) {
+    red_m = 0.0f;
+    red_exact = 1u;
+  }
+  __syncthreads();
+  float cmin[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+  float cmax[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+  float local_m = 0.0f;
+  bool local_exact = true;
+  for (uint32_t p = task.begin + threadIdx.x; p < task.end; p += blockDim.x) {
+    double lo[3], hi[3], c[3], maxabs;
+    bool exact;
+    prim_box(kind, p, coords, refs, verts, lo, hi, c, &exact, &maxabs);
+    box_lo[p] = make_float4(__double2float_rd(lo[0]), __double2float_rd(lo[1]),
+                            __double2float_rd(lo[2]), 0.0f);
+    box_hi[p] = make_float4(__double2float_ru(hi[0]), __double2float_ru(hi[1]),
+                            __double2float_ru(hi[2]), 0.0f);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float cf = (float)c[a];
+      cmin[a] = fminf(cmin[a], cf);
+      cmax[a] = fmaxf(cmax[a], cf);
+    }
+    local_m = fmaxf(local_m, __double2float_ru(maxabs));
+    local_exact = local_exact && exact;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (cmin[a] <= cmax[a]) {
+      atomicMin(&red[a], ordered_from_float(cmin[a]));
+      atomicMax(&red[3 + a], ordered_from_float(cmax[a]));
+    }
+  }
+  atomicMax((unsigned int*)&red_m, __float_as_uint(local_m));
+  if (!local_exact) red_exact = 0u;
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    atomicMin(&inst_bounds[6 * task.instance + threadIdx.x], red[threadIdx.x]);
+    atomicMax(&inst_bounds[6 * task.instance + 3 + threadIdx.x], red[3 + threadIdx.x]);
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(maxabs_bits, __float_as_uint(red_m));
+    if (!red_exact) atomicAnd(exact_flag, 0u);
+  }
+}
+
+__device__ inline uint32_t spread10(uint32_t v) {  // 10 bits -> every third bit
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ inline uint32_t spread16(uint32_t v) {  // 16 bits -> every second bit
+  v &= 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+__device__ inline uint32_t quantize(float v, float lo, float hi, uint32_t cells) {
+  const float ext = hi - lo;
+  if (!(ext > 0.0f)) return 0u;
+  float f = (v - lo) / ext * (float)cells;
+  f = fminf(fmaxf(f, 0.0f), (float)(cells - 1));
+  return (uint32_t)f;
+}
+
+__global__ void morton_kernel(size_t n, int dim, const float4* box_lo, const float4* box_hi,
+                              const uint64_t* inst_offsets, uint32_t n_inst,
+                              const float* inst_bounds, uint64_t* keys, uint32_t* vals) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  // instance of p: last i with inst_offsets[i] <= p
+  uint32_t lo = 0, hi = n_inst;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (inst_offsets[mid] <= p) lo = mid; else hi = mid;
+  }
+  const float* b = inst_bounds + 6 * lo;
+  const float4 l = box_lo[p], h = box_hi[p];
+  const float cx = 0.5f * (l.x + h.x), cy = 0.5f * (l.y + h.y), cz = 0.5f * (l.z + h.z);
+  uint32_t code;
+  if (dim == 3) {
+    code = (spread10(quantize(cx, b[0], b[3], 1024)) << 2) |
+           (spread10(quantize(cy, b[1], b[4], 1024)) << 1) | spread10(quantize(cz, b[2], b[5], 1024));
+  } else {
+    code = (spread16(quantize(cx, b[0], b[3], 65536)) << 1) | spread16(quantize(cy, b[1], b[4], 65536));
+  }
+  keys[p] = ((uint64_t)lo << 32) | code;
+  vals[p] = (uint32_t)p;
+}
+
+__global__ void gather_kernel(PrimKind kind, size_t n, const uint32_t* order, const double* coords,
+                              const uint32_t* ids, const uint4* refs, const float4* box_lo,
+                              const float4* box_hi, float4* prims, double* exact,
+                              uint32_t* slot_to_input, uint4* poly, double* poly_box,
+                              float4* sorted_lo, float4* sorted_hi) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t p = order[j];
+  slot_to_input[j] = p;
+  sorted_lo[j] = box_lo[p];
+  sorted_hi[j] = box_hi[p];
+  if (kind == PrimKind::Points) {
+    const double x = coords[3 * (size_t)p], y = coords[3 * (size_t)p + 1], z = coords[3 * (size_t)p + 2];
+    prims[j] = make_float4((float)x, (float)y, (float)z, __uint_as_float(ids[p]));
+    if (exact) {
+      exact[3 * j] = x;
+      exact[3 * j + 1] = y;
+      exact[3 * j + 2] = z;
+    }
+  } else if (kind == PrimKind::Triangles) {
+    const double* c = coords + 9 * (size_t)p;
+    prims[3 * j] = make_float4((float)c[0], (float)c[1], (float)c[2], __uint_as_float(ids[p]));
+    prims[3 * j + 1] = make_float4((float)c[3], (float)c[4], (float)c[5], 0.0f);
+    prims[3 * j + 2] = make_float4((float)c[6], (float)c[7], (float)c[8], 0.0f);
+    if (exact) {
+#pragma unroll
+      for (int a = 0; a < 9; ++a) exact[9 * j + a] = c[a];
+    }
+  } else {
+    const uint4 r = refs[p];
+    poly[j] = r;
+    // exact FP64 box of the polygon (reference pip_cpu bounds, min/max of vertices)
+    (void)poly_box;
+  }
+}
+
+__global__ void poly_box_kernel(size_t n, const uint4* poly, const double* verts, double* poly_box) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint4 r = poly[j];
+  double x0 = verts[2 * (size_t)r.x], y0 = verts[2 * (size_t)r.x + 1], x1 = x0, y1 = y0;
+  for (uint32_t k = 1; k < r.y; ++k) {
+    const double x = verts[2 * ((size_t)r.x + k)], y = verts[2 * ((size_t)r.x + k) + 1];
+    x0 = fmin(x0, x);
+    x1 = fmax(x1, x);
+    y0 = fmin(y0, y);
+    y1 = fmax(y1, y);
+  }
+  poly_box[4 * j] = x0;
+  poly_box[4 * j + 1] = y0;
+  poly_box[4 * j + 2] = x1;
+  poly_box[4 * j + 3] = y1;
+}
+
+// leaf l of instance i covers slots [first, first + count)
+__global__ void leaves_kernel(size_t m, const uint64_t* leaf_offsets, const uint64_t* inst_offsets,
+                              uint32_t n_inst, uint32_t leaf_size, const uint64_t* sorted_keys,
+                              uint32_t* leaf_first, uint32_t* leaf_count, uint64_t* leaf_keys) {
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= m) return;
+  uint32_t lo = 0, hi = n_inst;
+  while (hi - lo > 1) {  // last instance with leaf_offsets[i] <= l
+    const uint32_t mid = (lo + hi) / 2;
+    if (leaf_offsets[mid] <= l) lo = mid; else hi = mid;
+  }
+  // skip empty instances that share the same leaf offset
+  while (lo + 1 < n_inst && leaf_offsets[lo + 1] <= l) ++lo;
+  const uint64_t first = inst_offsets[lo] + (l - leaf_offsets[lo]) * (uint64_t)leaf_size;
+  const uint64_t end = inst_offsets[lo + 1];
+  const uint64_t count = end - first < leaf_size ? end - first : leaf_size;
+  leaf_first[l] = (uint32_t)first;
+  leaf_count[l] = (uint32_t)count;
+  leaf_keys[l] = sorted_keys[first];
+}
+
+__device__ inline int karras_delta(const uint64_t* keys, int m, int i, int j) {
+  if (j < 0 || j >= m) return -1;
+  const uint64_t a = keys[i], b = keys[j];
+  if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+  return __clzll(a ^ b);
+}
+
+__global__ void karras_kernel(int m, const uint64_t* keys, uint32_t* left, uint32_t* right,
+                              uint32_t* parent, uint32_t* range_lo, uint32_t* range_hi,
+                              unsigned long long* highest) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m - 1) return;
+  const int d = (karras_delta(keys, m, i, i + 1) - karras_delta(keys, m, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = karras_delta(keys, m, i, i - d);
+  int lmax = 2;
+  while (karras_delta(keys, m, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (karras_delta(keys, m, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = karras_delta(keys, m, i, j);
+  int s = 0;
+  int t = l;
+  do {
+    t = (t + 1) / 2;
+    if (karras_delta(keys, m, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int gamma = i + s * d + (d < 0 ? d : 0);
+  const int lo = i < j ? i : j;
+  const int hi = i < j ? j : i;
+  const uint32_t leaf_base = (uint32_t)(m - 1);
+  const uint32_t lc = (lo == gamma) ? leaf_base + (uint32_t)gamma : (uint32_t)gamma;
+  const uint32_t rc = (hi == gamma + 1) ? leaf_base + (uint32_t)(gamma + 1) : (uint32_t)(gamma + 1);
+  left[i] = lc;
+  right[i] = rc;
+  parent[lc] = (uint32_t)i;
+  parent[rc] = (uint32_t)i;
+  range_lo[i] = (uint32_t)lo;
+  range_hi[i] = (uint32_t)hi;
+  atomicMax(&highest[lo], ((unsigned long long)(hi - lo) << 32) | (unsigned long long)i);
+}
+
+__device__ inline uint32_t escape_after(uint32_t last_leaf, uint32_t m,
+                                        const unsigned long long* highest) {
+  if (last_leaf + 1 >= m) return kEndNode;
+  const unsigned long long h = highest[last_leaf + 1];
+  if (h) return (uint32_t)(h & 0xFFFFFFFFull);
+  return (m - 1) + last_leaf + 1;
+}
+
+__global__ void links_kernel(uint32_t m, const uint32_t* left, const uint32_t* range_hi,
+                             const unsigned long long* highest, BvhNode* nodes) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m - 1) return;
+  nodes[k].link = left[k];
+  nodes[k].escape = escape_after(range_hi[k], m, highest);
+}
+
+__global__ void refit_kernel(uint32_t m, const uint32_t* leaf_first, const uint32_t* leaf_count,
+                             const float4* sorted_lo, const float4* sorted_hi,
+                             const unsigned long long* highest, const uint32_t* parent,
+                             const uint32_t* left, const uint32_t* right, uint32_t* visits,
+                             BvhNode* nodes) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= m) return;
+  const uint32_t first = leaf_first[l], count = leaf_count[l];
+  float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+  float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+  for (uint32_t k = 0; k < count; ++k) {
+    const float4 a = sorted_lo[first + k], b = sorted_hi[first + k];
+    lo[0] = fminf(lo[0], a.x);
+    lo[1] = fminf(lo[1], a.y);
+    lo[2] = fminf(lo[2], a.z);
+    hi[0] = fmaxf(hi[0], b.x);
+    hi[1] = fmaxf(hi[1], b.y);
+    hi[2] = fmaxf(hi[2], b.z);
+  }
+  uint32_t node = (m - 1) + l;
+  BvhNode out;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    out.lo[a] = lo[a];
+    out.hi[a] = hi[a];
+  }
+  out.link = kLeafFlag | ((count - 1) << 28) | first;
+  out.escape = escape_after(l, m, highest);
+  nodes[node] = out;
+  while (node != 0) {
+    const uint32_t p = parent[node];
+    __threadfence();
+    if (atomicAdd(&visits[p], 1u) == 0) return;  // sibling not finished yet
+    __threadfence();
+    const BvhNode a = nodes[left[p]];
+    const BvhNode b = nodes[right[p]];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      nodes[p].lo[ax] = fminf(a.lo[ax], b.lo[ax]);
+      nodes[p].hi[ax] = fmaxf(a.hi[ax], b.hi[ax]);
+    }
+    node = p;
+  }
+}
+
+// Root of each instance: the node whose leaf range is exactly the instance's.
+__global__ void roots_kernel(uint32_t n_inst, uint32_t m, const uint64_t* leaf_offsets,
+                             const uint32_t* parent, const uint32_t* range_lo,
+                             const uint32_t* range_hi, const BvhNode* nodes, uint32_t* roots,
+                             uint32_t* ends) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inst) return;
+  const uint64_t a = leaf_offsets[i], b = leaf_offsets[i + 1];
+  if (a == b) {
+    roots[i] = kEndNode;
+    ends[i] = kEndNode;
+    return;
+  }
+  uint32_t node = (m - 1) + (uint32_t)a;
+  if (b - a > 1) {
+    node = parent[node];
+    while (node != kEndNode && !(range_lo[node] == a && range_hi[node] == b - 1)) node = parent[node];
+  }
+  roots[i] = node;
+  ends[i] = node == kEndNode ? kEndNode : nodes[node].escape;
+}
+
+}  // namespace
+
+void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* coords,
+                 const uint32_t* ids, const uint4* poly_refs, const double* poly_verts,
+                 size_t n_verts, const std::vector<uint64_t>& instance_offsets,
+                 uint32_t leaf_size, hipStream_t stream) {
+  require(instance_offsets.size() >= 2 && instance_offsets.front() == 0 &&
+              instance_offsets.back() == n,
+          "instance offsets must start at 0 and end at the primitive count");
+  require(leaf_size >= 1 && leaf_size <= kMaxLeafSize, "leaf size must be in [1, 8]");
+  require(n <= kMaxLeafStart, "scene exceeds 2^28 primitives");
+  const uint32_t n_inst = (uint32_t)(instance_offsets.size() - 1);
+  scene.kind = kind;
+  scene.dim = dim;
+  scene.leaf_size = leaf_size;
+  scene.n_prims = n;
+  scene.n_instances = n_inst;
+  scene.host_offsets = instance_offsets;
+
+  // leaves per instance
+  std::vector<uint64_t> leaf_offsets(n_inst + 1, 0);
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    const uint64_t ni = instance_offsets[i + 1] - instance_offsets[i];
+    require(instance_offsets[i + 1] >= instance_offsets[i], "instance offsets must be non-decreasing");
+    leaf_offsets[i + 1] = leaf_offsets[i] + (ni + leaf_size - 1) / leaf_size;
+  }
+  const size_t m = leaf_offsets[n_inst];
+  scene.n_leaves = m;
+  if (kind == PrimKind::Polygons && n_verts) {
+    scene.poly_verts.resize(2 * n_verts);
+    RTDL_HIP(hipMemcpyAsync(scene.poly_verts.get(), poly_verts, 2 * n_verts * sizeof(double),
+                            hipMemcpyDeviceToDevice, stream));
+  }
+  scene.host_roots.assign(n_inst, kEndNode);
+  scene.host_ends.assign(n_inst, kEndNode);
+  scene.roots.resize(n_inst ? n_inst : 1);
+  scene.ends.resize(n_inst ? n_inst : 1);
+  if (n == 0) {
+    scene.m = 0.0f;
+    scene.fp32_exact = true;
+    RTDL_HIP(hipMemcpyAsync(scene.roots.get(), scene.host_roots.data(), n_inst * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, stream));
+    RTDL_HIP(hipMemcpyAsync(scene.ends.get(), scene.host_ends.data(), n_inst * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, stream));
+    RTDL_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+
+  // 1. bounds
+  std::vector<BoundsTask> tasks;
+  constexpr uint32_t kChunk = 16384;
+  for (uint32_t i = 0; i < n_inst; ++i)
+    for (uint64_t b = instance_offsets[i]; b < instance_offsets[i + 1]; b += kChunk)
+      tasks.push_back({i, (uint32_t)b, (uint32_t)std::min<uint64_t>(b + kChunk, instance_offsets[i + 1])});
+  DeviceArray<BoundsTask> d_tasks(tasks.size());
+  DeviceArray<float4> box_lo(n), box_hi(n);
+  DeviceArray<int> inst_bounds(6 * (size_t)n_inst);
+  DeviceArray<uint32_t> flags(2);  // [0] max-abs bits, [1] exact flag
+  std::vector<int> init_bounds(6 * (size_t)n_inst);
+  for (uint32_t i = 0; i < n_inst; ++i)
+    for (int a = 0; a < 3; ++a) {
+      init_bounds[6 * i + a] = 0x7FFFFFFF;
+      init_bounds[6 * i + 3 + a] = (int)0x80000000;
+    }
+  const uint32_t init_flags[2] = {0u, 1u};
+  RTDL_HIP(hipMemcpyAsync(d_tasks.get(), tasks.data(), tasks.size() * sizeof(BoundsTask),
+                          hipMemcpyHostToDevice, stream));
+  RTDL_HIP(hipMemcpyAsync(inst_bounds.get(), init_bounds.data(), init_bounds.size() * sizeof(int),
+                          hipMemcpyHostToDevice, stream));
+  RTDL_HIP(hipMemcpyAsync(flags.get(), init_flags, sizeof(init_flags), hipMemcpyHostToDevice, stream));
+  bounds_kernel<<<(unsigned)tasks.size(), 256, 0, stream>>>(kind, d_tasks.get(), coords, poly_refs,
+                                                             poly_verts, box_lo.get(), box_hi.get(),
+                                                             inst_bounds.get(), flags.get(),
+                                                             flags.get() + 1);
+  RTDL_LAUNCH_CHECK();
+  std::vector<int> enc_bounds(6 * (size_t)n_inst);
+  uint32_t host_flags[2];
+  RTDL_HIP(hipMemcpyAsync(enc_bounds.data(), inst_bounds.get(), enc_bounds.size() * sizeof(int),
+                          hipMemcpyDeviceToHost, stream));
+  RTDL_HIP(hipMemcpyAsync(host_flags, flags.get(), sizeof(host_flags), hipMemcpyDeviceToHost, stream));
+  RTDL_HIP(hipStreamSynchronize(stream));
+  std::memcpy(&scene.m, &host_flags[0], sizeof(float));
+  scene.fp32_exact = host_flags[1] != 0;
+  std::vector<float> fbounds(enc_bounds.size());
+  for (size_t i = 0; i < enc_bounds.size(); ++i) fbounds[i] = float_from_ordered(enc_bounds[i]);
+
+  // 2. Morton keys and sort
+  DeviceArray<float> d_fbounds(fbounds.size());
+  DeviceArray<uint64_t> d_inst_offsets(n_inst + 1), d_leaf_offsets(n_inst + 1);
+  RTDL_HIP(hipMemcpyAsync(d_fbounds.get(), fbounds.data(), fbounds.size() * sizeof(float),
+                          hipMemcpyHostToDevice, stream));
+  RTDL_HIP(hipMemcpyAsync(d_inst_offsets.get(), instance_offsets.data(),
+                          (n_inst + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+  RTDL_HIP(hipMemcpyAsync(d_leaf_offsets.get(), leaf_offsets.data(), (n_inst + 1) * sizeof(uint64_t),
+                          hipMemcpyHostToDevice, stream));
+  DeviceArray<uint64_t> keys(n);
+  DeviceArray<uint32_t> order(n);
+  morton_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(n, dim, box_lo.get(), box_hi.get(),
+                                                         d_inst_offsets.get(), n_inst,
+                                                         d_fbounds.get(), keys.get(), order.get());
+  RTDL_LAUNCH_CHECK();
+  int inst_bits = 0;
+  while ((1u << inst_bits) < n_inst) ++inst_bits;
+  radix_sort_pairs(keys.get(), order.get(), n, 32 + inst_bits, stream);
+
+  // 3. gather into Morton order
+  if (kind == PrimKind::Points) scene.prims.resize(n);
+  if (kind == PrimKind::Triangles) scene.prims.resize(3 * n);
+  if (kind == PrimKind::Polygons) {
+    scene.prims.reset();
+    scene.poly.resize(n);
+    scene.poly_box.resize(4 * n);
+  }
+  scene.exact.reset();
+  if (!scene.fp32_exact && kind != PrimKind::Polygons)
+    scene.exact.resize((kind == PrimKind::Points ? 3 : 9) * n);
+  scene.slot_to_input.resize(n);
+  DeviceArray<float4> sorted_lo(n), sorted_hi(n);
+  gather_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(
+      kind, n, order.get(), coords, ids, poly_refs, box_lo.get(), box_hi.get(), scene.prims.get(),
+      scene.exact.get(), scene.slot_to_input.get(), scene.poly.get(), scene.poly_box.get(),
+      sorted_lo.get(), sorted_hi.get());
+  RTDL_LAUNCH_CHECK();
+  if (kind == PrimKind::Polygons) {
+    poly_box_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(n, scene.poly.get(), scene.poly_verts.get(),
+                                                             scene.poly_box.get());
+    RTDL_LAUNCH_CHECK();
+  }
+
+  // 4. leaves + Karras hierarchy
+  DeviceArray<uint32_t> leaf_first(m), leaf_count(m);
+  DeviceArray<uint64_t> leaf_keys(m);
+  leaves_kernel<<<blocks_for(m, 256), 256, 0, stream>>>(m, d_leaf_offsets.get(), d_inst_offsets.get(),
+                                                         n_inst, leaf_size, keys.get(),
+                                                         leaf_first.get(), leaf_count.get(),
+                                                         leaf_keys.get());
+  RTDL_LAUNCH_CHECK();
+  const size_t n_nodes = 2 * m - 1;
+  scene.nodes.resize(n_nodes);
+  const size_t n_internal = m - 1;
+  DeviceArray<uint32_t> left(n_internal ? n_internal : 1), right(n_internal ? n_internal : 1);
+  DeviceArray<uint32_t> parent(n_nodes), range_lo(n_internal ? n_internal : 1),
+      range_hi(n_internal ? n_internal : 1), visits(n_internal ? n_internal : 1);
+  DeviceArray<unsigned long long> highest(m);
+  RTDL_HIP(hipMemsetAsync(highest.get(), 0, m * sizeof(unsigned long long), stream));
+  RTDL_HIP(hipMemsetAsync(visits.get(), 0, visits.bytes(), stream));
+  RTDL_HIP(hipMemsetAsync(parent.get(), 0xFF, parent.bytes(), stream));
+  if (n_internal) {
+    karras_kernel<<<blocks_for(n_internal, 256), 256, 0, stream>>>(
+        (int)m, leaf_keys.get(), left.get(), right.get(), parent.get(), range_lo.get(),
+        range_hi.get(), highest.get());
+    RTDL_LAUNCH_CHECK();
+    links_kernel<<<blocks_for(n_internal, 256), 256, 0, stream>>>(
+        (uint32_t)m, left.get(), range_hi.get(), highest.get(), scene.nodes.get());
+    RTDL_LAUNCH_CHECK();
+  }
+  // 5./6. leaf records, ropes and bottom-up refit
+  refit_kernel<<<blocks_for(m, 256), 256, 0, stream>>>(
+      (uint32_t)m, leaf_first.get(), leaf_count.get(), sorted_lo.get(), sorted_hi.get(),
+      highest.get(), parent.get(), left.get(), right.get(), visits.get(), scene.nodes.get());
+  RTDL_LAUNCH_CHECK();
+  roots_kernel<<<blocks_for(n_inst, 64), 64, 0, stream>>>(
+      n_inst, (uint32_t)m, d_leaf_offsets.get(), parent.get(), range_lo.get(), range_hi.get(),
+      scene.nodes.get(), scene.roots.get(), scene.ends.get());
+  RTDL_LAUNCH_CHECK();
+  RTDL_HIP(hipMemcpyAsync(scene.host_roots.data(), scene.roots.get(), n_inst * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, stream));
+  RTDL_HIP(hipMemcpyAsync(scene.host_ends.data(), scene.ends.get(), n_inst * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, stream));
+  RTDL_HIP(hipStreamSynchronize(stream));
+}
+
+}  // namespace rtdl

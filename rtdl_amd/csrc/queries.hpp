@@ -1,0 +1,62 @@
+// Launch wrappers of the traversal, refine and emit kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "scene.hpp"
+
+namespace rtdl {
+
+// Largest per-query selection kept in registers; larger fixed-radius k_max
+// values use the count-then-collect path.
+constexpr uint32_t kMaxListK = 64;
+
+struct TraversalStats {
+  unsigned long long nodes;  // node records fetched
+  unsigned long long prims;  // primitive records tested
+};
+
+// ---- point queries --------------------------------------------------------
+// q: FP64 query coordinates (3 per query, z = 0 in 2-D), order: traversal
+// order (slot -> query index) or null.  For every query the k smallest
+// (distance, id) candidates with squared FP64 distance <= radius * radius are
+// written, ascending, to list_d/list_id[query * k + r], with count[query].
+// With `count_all` the radius is never shrunk and total[query] receives the
+// number of candidates inside the radius (k must then be kMaxListK).
+void launch_collect(const SceneView& s, const double* q, const uint32_t* order, uint32_t nq,
+                    double radius, uint32_t k, bool count_all, double* list_d, uint32_t* list_id,
+                    uint32_t* count, uint32_t* total, TraversalStats* stats, int block,
+                    hipStream_t stream);
+
+// Writes every candidate inside the radius for the queries listed in
+// `queries` (n entries) into [offset[i], offset[i + 1]) of out_d / out_id,
+// unsorted.
+void launch_collect_all(const SceneView& s, const double* q, const uint32_t* queries, uint32_t n,
+                        double radius, const uint64_t* offset, double* out_d, uint32_t* out_id,
+                        hipStream_t stream);
+
+// ---- rays ---------------------------------------------------------------
+// rays: 7 FP64 per ray (ox, oy, oz, dx, dy, dz, tmax).  mode 0: any-hit flag,
+// 1: hit count, 2: closest hit (triangle id in out_u, t in out_t, NaN t when
+// the ray hits nothing).
+void launch_rays(const SceneView& s, const double* rays, const uint32_t* order, uint32_t nr,
+                 int mode, uint32_t* out_u, double* out_t, TraversalStats* stats, int block,
+                 hipStream_t stream);
+
+// ---- point in polygon -----------------------------------------------------
+// pts: 2 FP64 per point.  Pass 1 (out_poly == null) writes count[point];
+// pass 2 writes the input indices of containing polygons, ascending, into
+// [offset[point], offset[point + 1]).
+void launch_pip(const SceneView& s, const double* pts, const uint32_t* order, uint32_t np,
+                bool positive_only, double eps_sq, uint32_t* count, const uint64_t* offset,
+                uint32_t* out_poly, TraversalStats* stats, int block, hipStream_t stream);
+
+// ---- helpers ----------------------------------------------------------------
+// Morton order of query positions inside the box [lo, hi] (3 FP64 per
+// query); returns the permutation in `order`.
+void query_morton_order(const double* q, int stride, uint32_t n, const float lo[3],
+                        const float hi[3], uint32_t* order, hipStream_t stream);
+
+}  // namespace rtdl

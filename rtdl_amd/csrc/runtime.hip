@@ -1,0 +1,527 @@
+// Host runtime: caching device allocator, per-thread streams, phase clock,
+// prepared scenes and the upload -> traverse -> emit pipelines behind the
+// C-ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "common.hpp"
+#include "io.hpp"
+#include "primitives.hpp"
+#include "runtime.hpp"
+
+namespace rtdl {
+
+// ------------------------------------------------------------- device pool --
+//
+// Freed blocks are cached by rounded size and handed back to later requests.
+// Every pipeline synchronises its stream before its temporaries go out of
+// scope, so a cached block is never reused while a kernel still reads it.
+
+namespace {
+std::mutex g_pool_mutex;
+std::multimap<size_t, void*> g_pool_free;
+std::map<void*, size_t> g_pool_size;
+size_t g_pool_cached = 0;
+constexpr size_t kPoolCacheLimit = size_t(32) << 30;
+
+size_t pool_round(size_t bytes) {
+  if (bytes <= 256) return 256;
+  if (bytes <= (size_t(1) << 20)) {
+    size_t r = 256;
+    while (r < bytes) r <<= 1;
+    return r;
+  }
+  const size_t gran = size_t(2) << 20;
+  return (bytes + gran - 1) / gran * gran;
+}
+}  // namespace
+
+void* pool_alloc(size_t bytes) {
+  const size_t size = pool_round(bytes);
+  {
+    std::lock_guard<std::mutex> g(g_pool_mutex);
+    auto it = g_pool_free.find(size);
+    if (it != g_pool_free.end()) {
+      void* p = it->second;
+      g_pool_free.erase(it);
+      g_pool_cached -= size;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t err = hipMalloc(&p, size);
+  if (err != hipSuccess) {
+    pool_trim();
+    (void)hipGetLastError();
+    err = hipMalloc(&p, size);
+  }
+  if (err != hipSuccess) {
+    throw Error("device allocation of " + std::to_string(size) + " bytes failed: " +
+                hipGetErrorString(err));
+  }
+  std::lock_guard<std::mutex> g(g_pool_mutex);
+  g_pool_size[p] = size;
+  return p;
+}
+
+void pool_free(void* ptr) {
+  if (!ptr) return;
+  std::lock_guard<std::mutex> g(g_pool_mutex);
+  auto it = g_pool_size.find(ptr);
+  if (it == g_pool_size.end()) return;
+  const size_t size = it->second;
+  if (g_pool_cached + size > kPoolCacheLimit) {
+    g_pool_size.erase(it);
+    (void)hipFree(ptr);
+    return;
+  }
+  g_pool_free.emplace(size, ptr);
+  g_pool_cached += size;
+}
+
+void pool_trim() {
+  std::lock_guard<std::mutex> g(g_pool_mutex);
+  for (auto& kv : g_pool_free) {
+    g_pool_size.erase(kv.second);
+    (void)hipFree(kv.second);
+  }
+  g_pool_free.clear();
+  g_pool_cached = 0;
+}
+
+// ------------------------------------------------------- streams and clock --
+
+namespace {
+struct ThreadStream {
+  hipStream_t s = nullptr;
+  ~ThreadStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+thread_local ThreadStream t_stream;
+thread_local PhaseClock t_clock;
+}  // namespace
+
+hipStream_t thread_stream() {
+  if (!t_stream.s) RTDL_HIP(hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking));
+  return t_stream.s;
+}
+
+PhaseClock& thread_clock() { return t_clock; }
+
+void PhaseClock::begin() {
+  seconds.fill(0.0);
+  start = mark = std::chrono::steady_clock::now();
+}
+void PhaseClock::lap(int phase) {
+  const auto now = std::chrono::steady_clock::now();
+  seconds[phase] += std::chrono::duration<double>(now - mark).count();
+  mark = now;
+}
+void PhaseClock::end() {
+  seconds[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+}
+
+// --------------------------------------------------------------- prepared --
+
+Prepared::Prepared(bool own_stream) : owns_stream(own_stream) {
+  if (own_stream)
+    RTDL_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  else
+    stream = thread_stream();
+}
+Prepared::~Prepared() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    if (owns_stream) (void)hipStreamDestroy(stream);
+  }
+  magic = 0;
+}
+
+Prepared* prepared_from(void* handle) {
+  require(handle != nullptr, "prepared handle must not be null");
+  auto* p = static_cast<Prepared*>(handle);
+  require(p->magic == 0x52544D58u, "invalid prepared handle");
+  return p;
+}
+
+namespace {
+
+constexpr uint32_t kPointLeaf = 4;
+constexpr uint32_t kTriangleLeaf = 2;
+constexpr uint32_t kPolygonLeaf = 1;
+constexpr uint32_t kSortQueriesAbove = 2048;
+
+template <class T>
+DeviceArray<T> upload(const T* host, size_t n, hipStream_t stream) {
+  DeviceArray<T> d(n);
+  if (n) RTDL_HIP(hipMemcpyAsync(d.get(), host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+  return d;
+}
+
+DeviceArray<uint8_t> upload_bytes(const void* host, size_t bytes, hipStream_t stream) {
+  return upload(static_cast<const uint8_t*>(host), bytes, stream);
+}
+
+void read_root_box(Prepared& p) {
+  const SceneView v = p.scene.view(0);
+  if (v.root == kEndNode) return;
+  BvhNode root;
+  RTDL_HIP(hipMemcpyAsync(&root, p.scene.nodes.get() + v.root, sizeof(root), hipMemcpyDeviceToHost,
+                          p.stream));
+  RTDL_HIP(hipStreamSynchronize(p.stream));
+  for (int a = 0; a < 3; ++a) {
+    p.root_lo[a] = root.lo[a];
+    p.root_hi[a] = root.hi[a];
+  }
+}
+
+void* host_rows(size_t count, size_t row_bytes) {
+  void* rows = std::malloc(count ? count * row_bytes : 1);
+  require(rows != nullptr, "host row allocation failed");
+  return rows;
+}
+
+void copy_rows_back(const void* device_rows, size_t count, size_t row_bytes, void** rows_out,
+                    size_t* n_out, hipStream_t stream) {
+  void* rows = host_rows(count, row_bytes);
+  if (count) {
+    hipError_t err =
+        hipMemcpyAsync(rows, device_rows, count * row_bytes, hipMemcpyDeviceToHost, stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(stream);
+    if (err != hipSuccess) {
+      std::free(rows);
+      throw Error(std::string("row copy-back failed: ") + hipGetErrorString(err));
+    }
+  }
+  *rows_out = rows;
+  *n_out = count;
+}
+
+uint64_t read_u64(const uint64_t* device, hipStream_t stream) {
+  uint64_t v = 0;
+  RTDL_HIP(hipMemcpyAsync(&v, device, sizeof(v), hipMemcpyDeviceToHost, stream));
+  RTDL_HIP(hipStreamSynchronize(stream));
+  return v;
+}
+
+// Emit order of neighbour rows: the reference sorts rows stably by query id
+// (reference.py:375), so queries are emitted in (id, input index) order.
+DeviceArray<uint32_t> emit_order(const uint32_t* qids, uint32_t nq, hipStream_t stream) {
+  DeviceArray<uint32_t> order;
+  if (nq <= 1 || is_nondecreasing(qids, nq, stream)) return order;
+  order.resize(nq);
+  DeviceArray<uint64_t> keys(nq);
+  keys_from_u32(qids, nullptr, nq, keys.get(), stream);
+  iota(order.get(), nq, stream);
+  radix_sort_pairs(keys.get(), order.get(), nq, 32, stream);
+  return order;
+}
+
+DeviceArray<uint32_t> traversal_order(const double* q, int stride, uint32_t n, const Prepared& p,
+                                      hipStream_t stream) {
+  DeviceArray<uint32_t> order;
+  if (n < kSortQueriesAbove) return order;
+  order.resize(n);
+  query_morton_order(q, stride, n, p.root_lo, p.root_hi, order.get(), stream);
+  return order;
+}
+
+}  // namespace
+
+void prepare_points(Prepared& p, const void* raw, size_t n, int dim) {
+  require(n == 0 || raw != nullptr, "search point pointer must not be null when count is nonzero");
+  require(n <= kMaxLeafStart, "too many search points");
+  hipStream_t s = p.stream;
+  p.type = dim == 3 ? SceneType::Points3D : SceneType::Points2D;
+  auto bytes = upload_bytes(raw, n * (dim == 3 ? 32 : 24), s);
+  DeviceArray<double> coords(3 * n);
+  DeviceArray<uint32_t> ids(n);
+  unpack_points(bytes.get(), n, dim, coords.get(), ids.get(), s);
+  build_scene(p.scene, PrimKind::Points, dim, n, coords.get(), ids.get(), nullptr, nullptr, 0,
+              {0, (uint64_t)n}, kPointLeaf, s);
+  read_root_box(p);
+}
+
+void prepare_triangles(Prepared& p, const void* raw, size_t n) {
+  require(n == 0 || raw != nullptr, "triangle pointer must not be null when count is nonzero");
+  require(n <= kMaxLeafStart, "too many triangles");
+  hipStream_t s = p.stream;
+  p.type = SceneType::Triangles3D;
+  auto bytes = upload_bytes(raw, n * 76, s);
+  DeviceArray<double> coords(9 * n);
+  DeviceArray<uint32_t> ids(n);
+  unpack_triangles(bytes.get(), n, coords.get(), ids.get(), s);
+  build_scene(p.scene, PrimKind::Triangles, 3, n, coords.get(), ids.get(), nullptr, nullptr, 0,
+              {0, (uint64_t)n}, kTriangleLeaf, s);
+  read_root_box(p);
+}
+
+void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* verts_xy,
+                      size_t n_doubles) {
+  require(n == 0 || refs != nullptr, "polygon pointer must not be null when count is nonzero");
+  require(n_doubles == 0 || verts_xy != nullptr,
+          "vertex pointer must not be null when vertex count is nonzero");
+  require(n_doubles % 2 == 0, "vertex_xy_count must be even (x, y pairs)");
+  require(n <= kMaxLeafStart, "too many polygons");
+  const size_t n_verts = n_doubles / 2;
+  // validate the references on the host before anything reaches the device
+  const uint8_t* r = static_cast<const uint8_t*>(refs);
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t off, cnt;
+    std::memcpy(&off, r + 12 * i + 4, 4);
+    std::memcpy(&cnt, r + 12 * i + 8, 4);
+    require(cnt >= 3, "polygon " + std::to_string(i) + " needs at least 3 vertices");
+    require((uint64_t)off + cnt <= n_verts,
+            "polygon " + std::to_string(i) + " vertex range exceeds vertex_xy_count");
+  }
+  hipStream_t s = p.stream;
+  p.type = SceneType::Polygons;
+  p.n_polygons = n;
+  auto bytes = upload_bytes(refs, n * 12, s);
+  p.refs_input.resize(n ? n : 1);
+  unpack_polygons(bytes.get(), n, p.refs_input.get(), s);
+  auto verts = upload(verts_xy, n_doubles, s);
+  build_scene(p.scene, PrimKind::Polygons, 2, n, nullptr, nullptr, p.refs_input.get(), verts.get(),
+              n_verts, {0, (uint64_t)n}, kPolygonLeaf, s);
+  read_root_box(p);
+}
+
+// -------------------------------------------------------------- neighbours --
+
+void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborMode mode,
+                   double radius, uint32_t k, void** rows_out, size_t* n_out) {
+  require(p.type == SceneType::Points2D || p.type == SceneType::Points3D,
+          "prepared handle is not a point scene");
+  require(nq_in == 0 || raw_queries != nullptr,
+          "query pointer must not be null when query count is nonzero");
+  require(nq_in < 0xFFFFFFFFull, "too many queries");
+  require(k >= 1, "k must be positive");
+  require(!(radius < 0.0), "radius must be non-negative");
+  const bool rank_rows = mode != NeighborMode::FixedRadius;
+  const size_t row_bytes = rank_rows ? 24 : 16;
+  PhaseClock& clock = thread_clock();
+  const uint32_t nq = (uint32_t)nq_in;
+  hipStream_t s = p.stream;
+  if (nq == 0 || p.scene.n_prims == 0) {
+    *rows_out = host_rows(0, row_bytes);
+    *n_out = 0;
+    return;
+  }
+  const int dim = p.type == SceneType::Points3D ? 3 : 2;
+  auto bytes = upload_bytes(raw_queries, (size_t)nq * (dim == 3 ? 32 : 24), s);
+  DeviceArray<double> q(3 * (size_t)nq);
+  DeviceArray<uint32_t> qid(nq);
+  unpack_points(bytes.get(), nq, dim, q.get(), qid.get(), s);
+  auto order = traversal_order(q.get(), 3, nq, p, s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(0);
+
+  const double search_radius = mode == NeighborMode::Knn ? __builtin_inf() : radius;
+  const SceneView view = p.scene.view(0);
+  DeviceArray<uint32_t> count(nq);
+  DeviceArray<double> cand_d;
+  DeviceArray<uint32_t> cand_id;
+  DeviceArray<uint64_t> src_start;  // only for the count-then-collect path
+  uint32_t stride = k;
+  const bool list_path = k <= kMaxListK;
+  if (list_path) {
+    cand_d.resize((size_t)nq * k);
+    cand_id.resize((size_t)nq * k);
+    launch_collect(view, q.get(), order.get(), nq, search_radius, k, false, cand_d.get(),
+                   cand_id.get(), count.get(), nullptr, nullptr, 256, s);
+  } else {
+    // Keep the 64 nearest per query and count everything inside the radius;
+    // queries with more than 64 candidates are collected in full and sorted.
+    stride = kMaxListK;
+    DeviceArray<uint32_t> total(nq);
+    DeviceArray<double> list_d((size_t)nq * kMaxListK);
+    DeviceArray<uint32_t> list_id((size_t)nq * kMaxListK);
+    launch_collect(view, q.get(), order.get(), nq, search_radius, kMaxListK, true, list_d.get(),
+                   list_id.get(), count.get(), total.get(), nullptr, 256, s);
+    std::vector<uint32_t> host_total(nq);
+    RTDL_HIP(hipMemcpyAsync(host_total.data(), total.get(), nq * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s));
+    RTDL_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> heavy;
+    std::vector<uint64_t> heavy_off(1, 0);
+    for (uint32_t i = 0; i < nq; ++i) {
+      if (host_total[i] > kMaxListK) {
+        heavy.push_back(i);
+        heavy_off.push_back(heavy_off.back() + host_total[i]);
+      }
+    }
+    const uint64_t heavy_n = heavy_off.back();
+    require(heavy_n < 0xFFFFFFFFull, "too many fixed-radius candidates for one call");
+    const size_t light_n = (size_t)nq * kMaxListK;
+    cand_d.resize(light_n + heavy_n);
+    cand_id.resize(light_n + heavy_n);
+    RTDL_HIP(hipMemcpyAsync(cand_d.get(), list_d.get(), light_n * sizeof(double),
+                            hipMemcpyDeviceToDevice, s));
+    RTDL_HIP(hipMemcpyAsync(cand_id.get(), list_id.get(), light_n * sizeof(uint32_t),
+                            hipMemcpyDeviceToDevice, s));
+    std::vector<uint64_t> host_start(nq);
+    std::vector<uint32_t> host_count(nq);
+    for (uint32_t i = 0; i < nq; ++i) {
+      host_start[i] = (uint64_t)i * kMaxListK;
+      host_count[i] = std::min(host_total[i], k);
+    }
+    if (!heavy.empty()) {
+      auto d_heavy = upload(heavy.data(), heavy.size(), s);
+      auto d_off = upload(heavy_off.data(), heavy_off.size(), s);
+      DeviceArray<double> hd(heavy_n), hd_sorted(heavy_n);
+      DeviceArray<uint32_t> hid(heavy_n), hid_sorted(heavy_n);
+      launch_collect_all(view, q.get(), d_heavy.get(), (uint32_t)heavy.size(), search_radius,
+                         d_off.get(), hd.get(), hid.get(), s);
+      // stable LSD sort of each segment by (distance, id): sort by id, then
+      // by distance, then by segment
+      DeviceArray<uint64_t> keys(heavy_n);
+      DeviceArray<uint32_t> perm(heavy_n), seg(heavy_n);
+      iota(perm.get(), heavy_n, s);
+      keys_from_u32(hid.get(), nullptr, heavy_n, keys.get(), s);
+      radix_sort_pairs(keys.get(), perm.get(), heavy_n, 32, s);
+      keys_from_f64(hd.get(), perm.get(), heavy_n, keys.get(), s);
+      radix_sort_pairs(keys.get(), perm.get(), heavy_n, 64, s);
+      segment_ids(d_off.get(), (uint32_t)heavy.size(), heavy_n, seg.get(), s);
+      keys_from_u32(seg.get(), perm.get(), heavy_n, keys.get(), s);
+      radix_sort_pairs(keys.get(), perm.get(), heavy_n, 32, s);
+      permute_f64(hd.get(), perm.get(), heavy_n, cand_d.get() + light_n, s);
+      permute_u32(hid.get(), perm.get(), heavy_n, cand_id.get() + light_n, s);
+      for (size_t h = 0; h < heavy.size(); ++h) host_start[heavy[h]] = light_n + heavy_off[h];
+      RTDL_HIP(hipStreamSynchronize(s));
+    }
+    src_start = upload(host_start.data(), nq, s);
+    RTDL_HIP(hipMemcpyAsync(count.get(), host_count.data(), nq * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, s));
+  }
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(2);
+
+  auto eorder = emit_order(qid.get(), nq, s);
+  DeviceArray<uint32_t> counts_pos(nq);
+  DeviceArray<uint64_t> offsets((size_t)nq + 1);
+  gather_counts(count.get(), eorder.get(), nq, 0, counts_pos.get(), s);
+  exclusive_scan(counts_pos.get(), offsets.get(), nq, s);
+  const uint64_t total_rows = read_u64(offsets.get() + nq, s);
+  DeviceArray<uint8_t> rows(total_rows * row_bytes);
+  emit_neighbor_rows(total_rows, offsets.get(), nq, eorder.get(), src_start.get(), stride,
+                     cand_d.get(), cand_id.get(), qid.get(), rank_rows, rows.get(), s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(3);
+  copy_rows_back(rows.get(), total_rows, row_bytes, rows_out, n_out, s);
+  clock.lap(4);
+}
+
+// -------------------------------------------------------------------- rays --
+
+void run_rays(Prepared& p, const void* raw_rays, size_t nr_in, int mode, void** rows_out,
+              size_t* n_out) {
+  require(p.type == SceneType::Triangles3D, "prepared handle is not a triangle scene");
+  require(nr_in == 0 || raw_rays != nullptr, "ray pointer must not be null when ray count is nonzero");
+  require(nr_in < 0xFFFFFFFFull, "too many rays");
+  const size_t row_bytes = mode == 2 ? 16 : 8;
+  PhaseClock& clock = thread_clock();
+  const uint32_t nr = (uint32_t)nr_in;
+  hipStream_t s = p.stream;
+  if (nr == 0) {
+    *rows_out = host_rows(0, row_bytes);
+    *n_out = 0;
+    return;
+  }
+  auto bytes = upload_bytes(raw_rays, (size_t)nr * 60, s);
+  DeviceArray<double> rays(7 * (size_t)nr);
+  DeviceArray<uint32_t> rid(nr);
+  unpack_rays(bytes.get(), nr, rays.get(), rid.get(), s);
+  auto order = traversal_order(rays.get(), 7, nr, p, s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(0);
+  DeviceArray<uint32_t> out_u(nr);
+  DeviceArray<double> out_t(mode == 2 ? nr : 0);
+  launch_rays(p.scene.view(0), rays.get(), order.get(), nr, mode, out_u.get(), out_t.get(), nullptr,
+              256, s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(2);
+  if (mode != 2) {
+    DeviceArray<uint8_t> rows((size_t)nr * 8);
+    emit_pair_rows(nr, rid.get(), out_u.get(), rows.get(), s);
+    RTDL_HIP(hipStreamSynchronize(s));
+    clock.lap(3);
+    copy_rows_back(rows.get(), nr, 8, rows_out, n_out, s);
+  } else {
+    DeviceArray<uint32_t> flags(nr);
+    DeviceArray<uint64_t> offsets((size_t)nr + 1);
+    hit_flags(nr, out_t.get(), flags.get(), s);
+    exclusive_scan(flags.get(), offsets.get(), nr, s);
+    const uint64_t total = read_u64(offsets.get() + nr, s);
+    DeviceArray<uint8_t> rows(total * 16);
+    emit_closest_rows(nr, offsets.get(), rid.get(), out_u.get(), out_t.get(), rows.get(), s);
+    RTDL_HIP(hipStreamSynchronize(s));
+    clock.lap(3);
+    copy_rows_back(rows.get(), total, 16, rows_out, n_out, s);
+  }
+  clock.lap(4);
+}
+
+// --------------------------------------------------------------------- PIP --
+
+void run_pip(Prepared& p, const void* raw_points, size_t np_in, bool positive_only,
+             void** rows_out, size_t* n_out) {
+  require(p.type == SceneType::Polygons, "prepared handle is not a polygon scene");
+  require(np_in == 0 || raw_points != nullptr,
+          "point pointer must not be null when point count is nonzero");
+  require(np_in < 0xFFFFFFFFull, "too many points");
+  PhaseClock& clock = thread_clock();
+  const uint32_t np = (uint32_t)np_in;
+  const uint32_t ng = (uint32_t)p.n_polygons;
+  hipStream_t s = p.stream;
+  if (np == 0 || ng == 0) {
+    *rows_out = host_rows(0, 12);
+    *n_out = 0;
+    return;
+  }
+  require(positive_only || (uint64_t)np * ng < (uint64_t(1) << 36),
+          "full_matrix PIP output exceeds 2^36 rows");
+  auto bytes = upload_bytes(raw_points, (size_t)np * 24, s);
+  DeviceArray<double> xy(2 * (size_t)np);
+  DeviceArray<uint32_t> pid(np);
+  unpack_points_xy(bytes.get(), np, xy.get(), pid.get(), s);
+  auto order = traversal_order(xy.get(), 2, np, p, s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(0);
+  const double eps_sq = std::pow(1.0e-12, 2.0);  // reference.py:757 `eps ** 2`
+  DeviceArray<uint32_t> count(np);
+  DeviceArray<uint64_t> offsets((size_t)np + 1);
+  const SceneView view = p.scene.view(0);
+  launch_pip(view, xy.get(), order.get(), np, positive_only, eps_sq, count.get(), nullptr, nullptr,
+             nullptr, 256, s);
+  exclusive_scan(count.get(), offsets.get(), np, s);
+  const uint64_t total = read_u64(offsets.get() + np, s);
+  DeviceArray<uint32_t> poly_idx(total ? total : 1);
+  launch_pip(view, xy.get(), order.get(), np, positive_only, eps_sq, count.get(), offsets.get(),
+             poly_idx.get(), nullptr, 256, s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(2);
+  if (positive_only) {
+    DeviceArray<uint8_t> rows(total * 12);
+    emit_pip_positive(total, offsets.get(), np, pid.get(), poly_idx.get(), p.refs_input.get(),
+                      rows.get(), s);
+    RTDL_HIP(hipStreamSynchronize(s));
+    clock.lap(3);
+    copy_rows_back(rows.get(), total, 12, rows_out, n_out, s);
+  } else {
+    const uint64_t cells = (uint64_t)np * ng;
+    DeviceArray<uint8_t> rows(cells * 12);
+    emit_pip_full(np, ng, offsets.get(), pid.get(), poly_idx.get(), p.refs_input.get(), rows.get(),
+                  s);
+    RTDL_HIP(hipStreamSynchronize(s));
+    clock.lap(3);
+    copy_rows_back(rows.get(), cells, 12, rows_out, n_out, s);
+  }
+  clock.lap(4);
+}
+
+}  // namespace rtdl

@@ -1,0 +1,98 @@
+// Device-resident scenes (LBVH over points, triangles or polygons) and the
+// POD view the traversal kernels receive.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+
+namespace rtdl {
+
+enum class PrimKind : int { Points = 0, Triangles = 1, Polygons = 2 };
+
+// What a traversal kernel needs to know about a scene.  Primitives are
+// stored in Morton order ("slots").  For points `prims` holds one float4
+// (x, y, z, id bits) per slot; for triangles three float4 per slot
+// ((v0, id), (v1, 0), (v2, 0)).  `exact` holds the FP64 coordinates per
+// slot (3 per point, 9 per triangle) and is null when every coordinate is
+// exactly representable in FP32, in which case the FP32 values are exact.
+// Polygons keep FP64 vertices; `poly` holds (vertex offset, count, id, 0)
+// per slot and `poly_box` the FP64 bounding box (x0, y0, x1, y1) per slot.
+struct SceneView {
+  const BvhNode* nodes;
+  const float4* prims;
+  const double* exact;
+  const uint32_t* slot_to_input;
+  const uint4* poly;
+  const double* poly_box;
+  const double* poly_verts;
+  uint32_t root;
+  uint32_t end;  // traversal stops when it reaches this node (escape of root)
+  uint32_t n_prims;
+  uint32_t n_nodes;
+  float m;       // >= every |coordinate| in the scene, rounded up
+};
+
+// A set of instances built into one LBVH: instance i owns primitives
+// [offsets[i], offsets[i+1]) of the input (contiguous) and gets its own
+// subtree; `roots[i]` / `ends[i]` delimit its traversal.
+class Scene {
+ public:
+  PrimKind kind = PrimKind::Points;
+  int dim = 3;
+  uint32_t leaf_size = 4;
+  size_t n_prims = 0;
+  size_t n_leaves = 0;
+  size_t n_instances = 0;
+  bool fp32_exact = true;
+  float m = 0.0f;
+  DeviceArray<BvhNode> nodes;
+  DeviceArray<float4> prims;
+  DeviceArray<double> exact;
+  DeviceArray<uint32_t> slot_to_input;
+  DeviceArray<uint4> poly;
+  DeviceArray<double> poly_box;
+  DeviceArray<double> poly_verts;
+  DeviceArray<uint32_t> roots;  // per instance
+  DeviceArray<uint32_t> ends;   // per instance
+  std::vector<uint32_t> host_roots;
+  std::vector<uint32_t> host_ends;
+  std::vector<uint64_t> host_offsets;  // per instance, n_instances + 1
+
+  SceneView view(size_t instance = 0) const {
+    SceneView v{};
+    v.nodes = nodes.get();
+    v.prims = prims.get();
+    v.exact = fp32_exact ? nullptr : exact.get();
+    v.slot_to_input = slot_to_input.get();
+    v.poly = poly.get();
+    v.poly_box = poly_box.get();
+    v.poly_verts = poly_verts.get();
+    v.root = host_roots.empty() ? kEndNode : host_roots[instance];
+    v.end = host_ends.empty() ? kEndNode : host_ends[instance];
+    v.n_prims = (uint32_t)n_prims;
+    v.n_nodes = (uint32_t)nodes.size();
+    v.m = m;
+    return v;
+  }
+  size_t device_bytes() const {
+    return nodes.bytes() + prims.bytes() + exact.bytes() + slot_to_input.bytes() + poly.bytes() +
+           poly_box.bytes() + poly_verts.bytes() + roots.bytes() + ends.bytes();
+  }
+};
+
+// Canonical FP64 input already on the device:
+//   points:    coords[3 * n] (x, y, z; z = 0 for 2-D), ids[n]
+//   triangles: coords[9 * n], ids[n]
+//   polygons:  refs[n] = (vertex offset, vertex count, id, 0), verts[2 * nv]
+// `instance_offsets` (host, n_instances + 1 entries) splits the input into
+// independently traversable instances; pass {0, n} for a single scene.
+void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* coords,
+                 const uint32_t* ids, const uint4* poly_refs, const double* poly_verts,
+                 size_t n_verts, const std::vector<uint64_t>& instance_offsets,
+                 uint32_t leaf_size, hipStream_t stream);
+
+}  // namespace rtdl
