@@ -1,0 +1,241 @@
+"""Pins the oracle before it is trusted as the checker.
+
+* The reference's own authored cases and their hard-coded expectations
+  (tests/goal198, goal204, goal548, goal552, goal632 in the reference).
+* Closed forms a plausible mistake would break: inclusive radius boundary,
+  truncation to k_max, (distance, id) tie order, grouping by query id, no
+  KNN padding, KNN rows as a superset of FRN rows at the k-th distance,
+  Moller-Trumbore special cases (det threshold, u/v/t bounds, tmax), PIP on
+  vertices and edges and the even-odd rule on a concave polygon.
+* The two independent restatements (pure Python and C) agree bit for bit on
+  random inputs, with and without the C grid filter.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import cpu_python_reference as ref
+from oracle import native
+from rtdl_amd.records import Point, Point3D, Polygon, Ray2D, Ray3D, Triangle, Triangle3D
+from tests import cases
+
+# ------------------------------------------------------------ authored cases --
+
+
+def _authored_points():
+    return (
+        (Point(100, 0.0, 0.0), Point(101, 3.0, 0.0)),
+        (Point(1, 0.0, 0.0), Point(2, 0.3, 0.0), Point(3, -0.3, 0.0), Point(4, 3.2, 0.0),
+         Point(5, 4.0, 0.0)),
+    )
+
+
+def test_frn_authored_rows_goal198():
+    q, s = _authored_points()
+    rows = ref.fixed_radius_neighbors_cpu(q, s, radius=0.5, k_max=3)
+    assert len(rows) == 4
+    assert [r["query_id"] for r in rows] == [100, 100, 100, 101]
+    assert [r["neighbor_id"] for r in rows] == [1, 2, 3, 4]
+    assert rows[0]["distance"] == 0.0
+    assert math.isclose(rows[1]["distance"], 0.3, rel_tol=1e-9)
+    assert math.isclose(rows[2]["distance"], 0.3, rel_tol=1e-9)
+    assert math.isclose(rows[3]["distance"], 0.2, rel_tol=1e-9)
+
+
+def test_frn_grouping_and_zero_radius_goal198():
+    rows = ref.fixed_radius_neighbors_cpu(
+        (Point(20, 3.0, 0.0), Point(10, 0.0, 0.0)), (Point(1, 0.0, 0.0), Point(2, 3.0, 0.0)),
+        radius=0.1, k_max=2)
+    assert [r["query_id"] for r in rows] == [10, 20]
+    rows = ref.fixed_radius_neighbors_cpu(
+        (Point(10, 1.0, 1.0),), (Point(1, 1.0, 1.0), Point(2, 1.0, 1.0001)), radius=0.0, k_max=4)
+    assert rows == ({"query_id": 10, "neighbor_id": 1, "distance": 0.0},)
+
+
+def test_knn_authored_rows_goal204():
+    q, s = _authored_points()
+    rows = ref.knn_rows_cpu(q, s, k=3)
+    assert len(rows) == 6
+    assert [(r["neighbor_id"], r["neighbor_rank"]) for r in rows[:3]] == [(1, 1), (2, 2), (3, 3)]
+    assert rows[3]["query_id"] == 101 and rows[3]["neighbor_rank"] == 1
+    rows = ref.knn_rows_cpu((Point(20, 3.0, 0.0), Point(10, 0.0, 0.0)),
+                            (Point(1, 0.0, 0.0), Point(2, 3.0, 0.0)), k=2)
+    assert [r["query_id"] for r in rows] == [10, 10, 20, 20]
+    rows = ref.knn_rows_cpu((Point(10, 1.0, 1.0),), (Point(1, 1.0, 1.0),), k=4)
+    assert rows == ({"query_id": 10, "neighbor_id": 1, "distance": 0.0, "neighbor_rank": 1},)
+
+
+def test_frn_3d_authored_goal548():
+    q = (Point3D(10, 0.0, 0.0, 0.0), Point3D(20, 5.0, 0.0, 0.0))
+    s = (Point3D(1, 0.0, 0.0, 0.0), Point3D(2, 0.0, 0.0, 0.6), Point3D(3, 0.0, 0.8, 0.0),
+         Point3D(4, 5.0, 0.0, 0.9))
+    rows = ref.fixed_radius_neighbors_cpu(q, s, radius=1.0, k_max=3)
+    assert [(r["query_id"], r["neighbor_id"]) for r in rows] == [(10, 1), (10, 2), (10, 3), (20, 4)]
+    assert [r["distance"] for r in rows] == [0.0, 0.6, 0.8, 0.9]
+
+
+def test_any_hit_authored_goal632():
+    rays = (Ray2D(1, 0.0, 0.0, 1.0, 0.0, 10.0), Ray2D(2, 0.0, 0.0, 0.0, 1.0, 2.0))
+    tris = (Triangle(10, 2.0, -1.0, 3.0, 1.0, 4.0, -1.0), Triangle(11, 6.0, -1.0, 7.0, 1.0, 8.0, -1.0),
+            Triangle(12, -1.0, 3.0, 1.0, 3.0, 0.0, 4.0))
+    assert ref.ray_triangle_any_hit_cpu(rays, tris) == ({"ray_id": 1, "any_hit": 1}, {"ray_id": 2, "any_hit": 0})
+    assert [r["hit_count"] for r in ref.ray_triangle_hit_count_cpu(rays, tris)] == [2, 0]
+    rays3 = (Ray3D(1, 0.0, 0.0, 0.0, 10.0, 0.0, 0.0, 1.0), Ray3D(2, 0.0, 2.0, 0.0, 10.0, 0.0, 0.0, 1.0))
+    tri3 = (Triangle3D(20, 5.0, -1.0, -1.0, 5.0, 1.0, -1.0, 5.0, 0.0, 1.0),)
+    assert ref.ray_triangle_any_hit_cpu(rays3, tri3) == ({"ray_id": 1, "any_hit": 1}, {"ray_id": 2, "any_hit": 0})
+
+
+def _square(pid, x0, y0, size):
+    return Polygon(pid, ((x0, y0), (x0 + size, y0), (x0 + size, y0 + size), (x0, y0 + size)))
+
+
+def test_pip_authored_squares_goal552():
+    points = (Point(1, 0.0, 0.0), Point(2, 0.5, 0.5), Point(3, 3.0, 3.0))
+    polys = (_square(10, -0.25, -0.25, 1.0), _square(20, 2.0, 2.0, 1.0))
+    full = ref.pip_cpu(points, polys)
+    assert [(r["point_id"], r["polygon_id"], r["contains"]) for r in full] == [
+        (1, 10, 1), (1, 20, 0), (2, 10, 1), (2, 20, 0), (3, 10, 0), (3, 20, 1)]
+    pos = ref.pip_cpu(points, polys, result_mode="positive_hits")
+    assert [(r["point_id"], r["polygon_id"]) for r in pos] == [(1, 10), (2, 10), (3, 20)]
+
+
+# ------------------------------------------------------------ closed forms --
+
+
+def test_frn_radius_is_inclusive_and_truncates_by_distance_then_id():
+    q = (Point(0, 0.0, 0.0),)
+    s = (Point(9, 0.5, 0.0), Point(3, 0.0, 0.5), Point(7, 0.0, -0.5), Point(1, 0.25, 0.0), Point(2, 0.6, 0.0))
+    rows = ref.fixed_radius_neighbors_cpu(q, s, radius=0.5, k_max=3)
+    assert [r["neighbor_id"] for r in rows] == [1, 3, 7]  # 0.25, then the 0.5 tie by id
+    rows = ref.fixed_radius_neighbors_cpu(q, s, radius=0.5, k_max=10)
+    assert [r["neighbor_id"] for r in rows] == [1, 3, 7, 9]  # 0.5 is inside, 0.6 is not
+
+
+def test_knn_contains_frn_rows_at_kth_distance():
+    rng = np.random.default_rng(3)
+    pts = [Point3D(i, *map(float, rng.random(3))) for i in range(300)]
+    q = pts[:20]
+    knn = ref.knn_rows_cpu(q, pts, k=7)
+    for query in q:
+        mine = [r for r in knn if r["query_id"] == query.id]
+        kth = mine[-1]["distance"]
+        frn = ref.fixed_radius_neighbors_cpu((query,), pts, radius=kth, k_max=7)
+        assert {r["neighbor_id"] for r in frn} <= {r["neighbor_id"] for r in mine}
+        assert [r["neighbor_rank"] for r in mine] == list(range(1, 8))
+
+
+def test_moller_trumbore_special_cases():
+    tri = Triangle3D(1, 0.0, 0.0, 1.0, 1.0, 0.0, 1.0, 0.0, 1.0, 1.0)
+    hit = ref._hit_t_3d
+    assert hit(Ray3D(0, 0.2, 0.2, 0.0, 0.0, 0.0, 1.0, 5.0), tri) == 1.0
+    assert hit(Ray3D(0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 5.0), tri) == 1.0  # vertex, u = v = 0
+    assert hit(Ray3D(0, 0.5, 0.5, 0.0, 0.0, 0.0, 1.0, 5.0), tri) == 1.0  # hypotenuse, u + v = 1
+    assert hit(Ray3D(0, 0.6, 0.6, 0.0, 0.0, 0.0, 1.0, 5.0), tri) is None
+    assert hit(Ray3D(0, 0.2, 0.2, 0.0, 0.0, 0.0, 1.0, 1.0), tri) == 1.0  # t == tmax inclusive
+    assert hit(Ray3D(0, 0.2, 0.2, 0.0, 0.0, 0.0, 1.0, 0.5), tri) is None
+    assert hit(Ray3D(0, 0.2, 0.2, 2.0, 0.0, 0.0, 1.0, 5.0), tri) is None  # behind (t < 0)
+    assert hit(Ray3D(0, 0.2, 0.2, 0.0, 1.0, 0.0, 0.0, 5.0), tri) is None  # parallel, det = 0
+    # det just above / below the 1e-8 threshold: det = d_z * (e1 x e2)_z
+    small = Triangle3D(1, 0.0, 0.0, 1.0, 1e-4, 0.0, 1.0, 0.0, 1e-4, 1.0)
+    assert hit(Ray3D(0, 1e-5, 1e-5, 0.0, 0.0, 0.0, 1.0, 5.0), small) is None  # det = 1e-8 -> miss
+    assert hit(Ray3D(0, 1e-5, 1e-5, 0.0, 0.0, 0.0, 2.0, 5.0), small) == 0.5
+
+
+def test_pip_on_edges_vertices_and_concave_even_odd():
+    concave = ((0.0, 0.0), (4.0, 0.0), (4.0, 4.0), (2.0, 1.0), (0.0, 4.0))
+    test = ref.point_in_polygon_test
+    assert test(1.0, 0.5, concave)
+    assert not test(2.0, 2.0, concave)  # inside the notch
+    assert test(2.0, 1.0, concave)  # on the notch vertex
+    assert test(3.0, 2.5, concave)
+    assert test(4.0, 2.0, concave)  # on an edge
+    assert test(3.0, 1.5, concave)  # on the slanted edge (4,4)-(2,1)
+    assert not test(-1e-9, 2.0, concave)
+    assert test(-1e-13, 2.0, concave)  # within the 1e-12 edge tolerance
+
+
+# --------------------------------------------- two restatements cross-check --
+
+
+def _as_records(cols, dim):
+    if dim == 3:
+        return tuple(Point3D(int(i), float(x), float(y), float(z))
+                     for i, x, y, z in zip(cols["id"], cols["x"], cols["y"], cols["z"]))
+    return tuple(Point(int(i), float(x), float(y)) for i, x, y in zip(cols["id"], cols["x"], cols["y"]))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("brute", [False, True])
+def test_c_oracle_matches_python_oracle_neighbors(dim, brute):
+    rng = np.random.default_rng(40 + dim)
+    q = cases.points(rng, 150, dim, fp32=False, ids=rng.permutation(150))
+    s = cases.points(rng, 250, dim, kind="mixture", fp32=False)
+    s["x"][:5] = q["x"][:5]
+    s["y"][:5] = q["y"][:5]
+    if dim == 3:
+        s["z"][:5] = q["z"][:5]
+    qr, sr = _as_records(q, dim), _as_records(s, dim)
+    for radius, k in ((0.05, 3), (0.2, 1000), (0.0, 2)):
+        want = ref.fixed_radius_neighbors_cpu(qr, sr, radius=radius, k_max=k)
+        got = native.neighbors("frn", q, s, radius=radius, k=k, brute=brute)
+        assert [(r["query_id"], r["neighbor_id"], r["distance"]) for r in want] == \
+            list(zip(got["qid"].tolist(), got["nid"].tolist(), got["dist"].tolist()))
+        want = ref.bounded_knn_rows_cpu(qr, sr, radius=radius, k_max=k)
+        got = native.neighbors("bounded_knn", q, s, radius=radius, k=k, brute=brute)
+        assert [r["neighbor_rank"] for r in want] == got["rank"].tolist()
+    for k in (1, 9, 300):
+        want = ref.knn_rows_cpu(qr, sr, k=k)
+        got = native.neighbors("knn", q, s, k=k, brute=brute)
+        assert [(r["query_id"], r["neighbor_id"], r["distance"], r["neighbor_rank"]) for r in want] == \
+            list(zip(got["qid"].tolist(), got["nid"].tolist(), got["dist"].tolist(), got["rank"].tolist()))
+
+
+@pytest.mark.parametrize("brute", [False, True])
+def test_c_oracle_matches_python_oracle_rays(brute):
+    rng = np.random.default_rng(50)
+    t = cases.triangles(rng, 300, sigma=0.05, fp32=False)
+    r = cases.rays(rng, 200, tmax=0.5, fp32=False)
+    tm, rm = cases.tri_matrix(t), cases.ray_matrix(r)
+    tris = tuple(Triangle3D(int(t["id"][i]), *map(float, tm[i])) for i in range(len(tm)))
+    rays = tuple(Ray3D(int(r["id"][i]), *map(float, rm[i])) for i in range(len(rm)))
+    assert [x["any_hit"] for x in ref.ray_triangle_any_hit_cpu(rays, tris)] == \
+        native.rays_3d("any", rm, tm, t["id"], brute=brute).tolist()
+    assert [x["hit_count"] for x in ref.ray_triangle_hit_count_cpu(rays, tris)] == \
+        native.rays_3d("count", rm, tm, t["id"], brute=brute).tolist()
+    tid, tt, has = native.rays_3d("closest", rm, tm, t["id"], brute=brute)
+    want = ref.ray_triangle_closest_hit_cpu(rays, tris)
+    assert [(x["ray_id"], x["triangle_id"], x["t"]) for x in want] == \
+        [(int(i), int(tid[i]), float(tt[i])) for i in np.flatnonzero(has)]
+
+
+@pytest.mark.parametrize("positive_only", [True, False])
+@pytest.mark.parametrize("brute", [False, True])
+def test_c_oracle_matches_python_oracle_pip(positive_only, brute):
+    rng = np.random.default_rng(60)
+    polys = cases.star_polygons(rng, 40, rmin=0.05, rmax=0.2, fp32=False)
+    refs, v = polys["refs"], polys["vertices_xy"]
+    pts = cases.points(rng, 400, 2, fp32=False)
+    vv = v.reshape(-1, 2)
+    pts["x"][:20], pts["y"][:20] = vv[:20, 0], vv[:20, 1]
+    pts["x"][20:40] = (vv[20:40, 0] + vv[21:41, 0]) / 2
+    pts["y"][20:40] = (vv[20:40, 1] + vv[21:41, 1]) / 2
+    recs = tuple(Polygon(int(refs["id"][g]), tuple(map(tuple, vv[refs["vertex_offset"][g]:
+                 refs["vertex_offset"][g] + refs["vertex_count"][g]].tolist()))) for g in range(len(refs)))
+    probes = _as_records(pts, 2)
+    mode = "positive_hits" if positive_only else "full_matrix"
+    want = [(r["point_id"], r["polygon_id"]) for r in ref.pip_cpu(probes, recs, result_mode=mode) if r["contains"]]
+    pairs = native.pip_pairs(pts["x"], pts["y"], refs["vertex_offset"], refs["vertex_count"], v,
+                             positive_only=positive_only, brute=brute)
+    assert want == [(int(pts["id"][i]), int(refs["id"][g])) for i, g in pairs]
+
+
+def test_libm_pow_is_not_the_correctly_rounded_square():
+    """Documents why the on-edge test can differ at its tolerance boundary:
+    the reference's `x ** 2` is libm pow, which is not always x * x."""
+    rng = np.random.default_rng(0)
+    xs = rng.random(200000).tolist()
+    diffs = sum(1 for x in xs if math.pow(x, 2.0) != x * x)
+    assert diffs < len(xs) * 0.01
