@@ -1,0 +1,550 @@
+#!/usr/bin/env python3
+"""Benchmark: served request stream vs the plain per-call backend on 1 MI355X.
+
+Workload (the served request mix): Poisson arrivals at 20,000 requests/s;
+each request is a fixed-radius (r = 0.01, k_max = 64), KNN (k = 8), 3-D ray
+any-hit (tmax = 0.5) or point-in-polygon (positive hits) query set, 25 % each,
+with batch size floor(exp(U[0, ln 65536])), against 16 resident scenes whose
+sizes are log-spaced in [1e5, 1e7] primitives (4 per request type,
+alternating uniform / 64-component Gaussian mixture, sigma 0.02, FP32).
+PIP scenes are star polygons (8-32 vertices) whose radii U[0.005, 0.02] are
+scaled by sqrt(1000 / N) so that a point lies in about as many polygons as
+with 1,000 polygons.
+
+One step = one second of the request stream.  `value` is the number of
+queries completed by the served backend divided by the wall time from the
+start of the timed replay until its last request completed (inputs resident
+in HBM before the clock starts).  Beside it: p50/p99 latency (arrival to rows
+ready in device memory), the plain per-call C-ABI backend on a prefix of the
+same stream (host records in, host rows out, scene built per call), a
+per-request resident-scene run (one synchronous batch per request), the HBM
+roofline of the dominant traversal kernel and the multi-threaded C oracle
+on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "queries/s + p99 latency, batched/served vs plain per-call backend, 1 MI355X"
+SEED = 20261203 + 4
+RADIUS = 0.01
+K_MAX = 64
+K_KNN = 8
+TMAX = 0.5
+HBM_PEAK = 8.0e12
+KINDS = ("frn", "knn", "ray", "pip")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# --------------------------------------------------------------- scenes --
+
+
+def make_scenes(torch, B, rng, max_prims: float, n_scenes: int = 16):
+    """Builds the resident scenes on the device; returns a list of dicts."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED)
+    centres3 = torch.rand(64, 3, device="cuda", generator=gen)
+    centres2 = torch.rand(64, 2, device="cuda", generator=gen)
+    scenes = []
+    for i in range(n_scenes):
+        n = int(round(1e5 * (max_prims / 1e5) ** (i / (n_scenes - 1))))
+        kind = KINDS[i % 4]
+        dist = "uniform" if (i // 4) % 2 == 0 else "mixture"
+
+        def pts(count, dim):
+            if dist == "uniform":
+                return torch.rand(count, dim, device="cuda", generator=gen)
+            c = centres3 if dim == 3 else centres2
+            pick = torch.randint(0, 64, (count,), device="cuda", generator=gen)
+            return c[pick] + 0.02 * torch.randn(count, dim, device="cuda", generator=gen)
+
+        t0 = time.perf_counter()
+        if kind in ("frn", "knn"):
+            data = pts(n, 3).float().contiguous()
+            handle = B.prepare_point_instances(data, [0, n])
+        elif kind == "ray":
+            c = pts(n, 3)
+            data = (c[:, None, :] + 0.005 * torch.randn(n, 3, 3, device="cuda", generator=gen))
+            data = data.reshape(n, 9).float().contiguous()
+            handle = B.prepare_triangle_instances(data, [0, n])
+        else:
+            counts = torch.randint(8, 33, (n,), device="cuda", generator=gen, dtype=torch.int64)
+            offsets = torch.cumsum(counts, 0) - counts
+            total = int(counts.sum())
+            seg = torch.repeat_interleave(torch.arange(n, device="cuda"), counts)
+            key = seg.double() + torch.rand(total, device="cuda", generator=gen, dtype=torch.float64)
+            key, _ = torch.sort(key)
+            ang = (key - seg.double()) * 2.0 * math.pi
+            scale = math.sqrt(1000.0 / n)
+            rad = (0.005 + 0.015 * torch.rand(total, device="cuda", generator=gen, dtype=torch.float64)) * scale
+            cen = pts(n, 2).double()[seg]
+            verts = torch.stack([cen[:, 0] + rad * torch.cos(ang), cen[:, 1] + rad * torch.sin(ang)], 1)
+            verts = verts.float().contiguous()
+            voff = offsets.to(torch.int32).contiguous()
+            vcnt = counts.to(torch.int32).contiguous()
+            handle = B.prepare_polygon_instances(voff, vcnt, verts, [0, n])
+            data = (voff, vcnt, verts)
+        torch.cuda.synchronize()
+        scenes.append({"index": i, "kind": kind, "dist": dist, "n": n, "handle": handle, "data": data,
+                       "build_s": time.perf_counter() - t0})
+        log(f"scene {i:2d}: {kind:3s} {dist:7s} n={n:>9d} built in {scenes[-1]['build_s']:.2f}s")
+    return scenes, centres3, centres2
+
+
+# ------------------------------------------------------- request stream --
+
+
+def make_stream(rng, seconds: float, rate: float, scenes, t_offset: float = 0.0):
+    n = rng.poisson(rate * seconds)
+    arrivals = np.sort(rng.uniform(0.0, seconds, n)) + t_offset
+    kinds = rng.integers(0, 4, n)
+    sizes = np.floor(np.exp(rng.uniform(0.0, math.log(65536.0), n))).astype(np.int64)
+    sizes = np.clip(sizes, 1, 65536)
+    by_kind = {k: [s for s in scenes if s["kind"] == KINDS[k]] for k in range(4)}
+    scene_idx = np.array([by_kind[k][rng.integers(0, len(by_kind[k]))]["index"] for k in kinds])
+    return {"arrival": arrivals, "kind": kinds, "size": sizes, "scene": scene_idx}
+
+
+def make_arenas(torch, stream, scenes, centres3, centres2, seed: int):
+    """Query arenas on the device; returns (arenas, per-request offsets)."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    n = len(stream["kind"])
+    offsets = np.zeros(n, dtype=np.int64)
+    # neighbour requests share the point arena, drawn per request from its
+    # scene's distribution; rays and PIP points are uniform in the unit box
+    dists = np.array([scenes[s]["dist"] == "mixture" for s in stream["scene"]])
+    arenas = {}
+    totals = {}
+    for group, kinds in (("point", (0, 1)), ("ray", (2,)), ("pip", (3,))):
+        sel = np.isin(stream["kind"], kinds)
+        idx = np.flatnonzero(sel)
+        sizes = stream["size"][idx]
+        offsets[idx] = np.concatenate([[0], np.cumsum(sizes)[:-1]]) if len(idx) else []
+        total = int(sizes.sum())
+        totals[group] = total
+        if group == "point":
+            q = torch.rand(total, 3, device="cuda", generator=gen)
+            mix = np.repeat(dists[idx], sizes)
+            m = torch.from_numpy(mix).cuda()
+            pick = torch.randint(0, 64, (total,), device="cuda", generator=gen)
+            g = centres3[pick] + 0.02 * torch.randn(total, 3, device="cuda", generator=gen)
+            q = torch.where(m[:, None], g, q)
+        elif group == "ray":
+            o = torch.rand(total, 3, device="cuda", generator=gen)
+            d = torch.randn(total, 3, device="cuda", generator=gen)
+            d = d / d.norm(dim=1, keepdim=True)
+            q = torch.cat([o, d, torch.full((total, 1), TMAX, device="cuda")], 1)
+        else:
+            q = torch.rand(total, 2, device="cuda", generator=gen)
+            mix = np.repeat(dists[idx], sizes)
+            m = torch.from_numpy(mix).cuda()
+            pick = torch.randint(0, 64, (total,), device="cuda", generator=gen)
+            g = centres2[pick] + 0.02 * torch.randn(total, 2, device="cuda", generator=gen)
+            q = torch.where(m[:, None], g, q)
+        arenas[group] = q.float().contiguous()
+    torch.cuda.synchronize()
+    return arenas, offsets, totals
+
+
+JOB_OF_KIND = None
+
+
+def requests_array(B, stream, offsets, scene_slot, capture_every: int = 0):
+    kind_map = {0: B.JOB_FIXED_RADIUS, 1: B.JOB_KNN, 2: B.JOB_RAY_ANYHIT, 3: B.JOB_PIP_POSITIVE}
+    n = len(stream["kind"])
+    reqs = np.zeros(n, dtype=B.SERVE_REQUEST_DTYPE)
+    reqs["kind"] = [kind_map[int(k)] for k in stream["kind"]]
+    reqs["scene"] = [scene_slot[int(s)] for s in stream["scene"]]
+    reqs["query_offset"] = offsets
+    reqs["query_count"] = stream["size"]
+    reqs["arrival_s"] = stream["arrival"]
+    if capture_every:
+        reqs["capture"][::capture_every] = 1
+    return reqs
+
+
+# ------------------------------------------------------------- roofline --
+
+
+def roofline_leg(torch, B, scenes, stream, offsets, arenas, limit_queries: int):
+    """One batch per predicate family over the first requests of the stream.
+    Kernel seconds from events on the launch stream (timed run), node and
+    primitive visits from a separate counting run (untimed)."""
+    kind_map = {0: B.JOB_FIXED_RADIUS, 1: B.JOB_KNN, 2: B.JOB_RAY_ANYHIT, 3: B.JOB_PIP_POSITIVE}
+    group_of = {0: "point", 1: "point", 2: "ray", 3: "pip"}
+    stride = {0: 3, 1: 3, 2: 7, 3: 2}
+    probe_bytes = {0: 12, 1: 12, 2: 28, 3: 8}
+    prim_bytes = {0: 12, 1: 12, 2: 36, 3: None}
+    row_bytes = {0: 16, 1: 24, 2: 1, 3: 8}
+    out = {}
+    for k in range(4):
+        idx = np.flatnonzero(stream["kind"] == k)
+        jobs, q_total = [], 0
+        for i in idx:
+            if q_total >= limit_queries:
+                break
+            n = int(stream["size"][i])
+            o = int(offsets[i])
+            s = scenes[int(stream["scene"][i])]
+            jobs.append(B.BatchJob(s["handle"], kind_map[k], arenas[group_of[k]][o:o + n],
+                                   radius=RADIUS, k=K_KNN if k == 1 else K_MAX))
+            q_total += n
+        if not jobs:
+            continue
+        B.run_batch_profiled(jobs)  # warm
+        res, fam, _ = B.run_batch_profiled(jobs)
+        _, _, visits = B.run_batch_profiled(jobs, count_visits=True)
+        fname = {0: "neighbors", 1: "neighbors", 2: "ray_any_hit", 3: "pip"}[k]
+        t = fam[fname]
+        rows = sum(r["rows"] for r in res)
+        nodes, prims = visits
+        if k == 3:
+            # polygon refs (16 B) plus 8 B per vertex; the mean star polygon has 20 vertices
+            pb = 16 + 8 * 20
+        else:
+            pb = prim_bytes[k]
+        w = q_total * probe_bytes[k] + 32 * nodes + pb * prims + row_bytes[k] * rows
+        out[KINDS[k]] = {
+            "queries": q_total, "jobs": len(jobs), "kernel_s": t, "node_visits": nodes,
+            "prim_tests": prims, "rows": rows, "algorithmic_bytes": w,
+            "achieved_GBps": w / t / 1e9 if t > 0 else 0.0,
+            "queries_per_s": q_total / t if t > 0 else 0.0,
+        }
+        log(f"kernel {KINDS[k]}: {q_total} q in {t*1e3:.2f} ms ({q_total/t/1e6:.1f} Mq/s), "
+            f"{nodes/q_total:.1f} nodes/q, {prims/q_total:.1f} prims/q, W/t = {w/t/1e9:.0f} GB/s")
+    return out
+
+
+# ------------------------------------------------------------ baselines --
+
+
+def host_scene_records(torch, rtm, s):
+    if "records" in s:
+        return s["records"]
+    if s["kind"] in ("frn", "knn"):
+        d = s["data"].double().cpu().numpy()
+        rec = np.zeros(len(d), dtype=rtm.POINT3D_DTYPE)
+        rec["id"] = np.arange(len(d))
+        rec["x"], rec["y"], rec["z"] = d[:, 0], d[:, 1], d[:, 2]
+    elif s["kind"] == "ray":
+        d = s["data"].double().cpu().numpy()
+        rec = np.zeros(len(d), dtype=rtm.TRIANGLE3D_DTYPE)
+        rec["id"] = np.arange(len(d))
+        for j, f in enumerate(["x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2"]):
+            rec[f] = d[:, j]
+    else:
+        voff, vcnt, verts = s["data"]
+        refs = np.zeros(len(voff), dtype=rtm.POLYGON_REF_DTYPE)
+        refs["id"] = np.arange(len(voff))
+        refs["vertex_offset"] = voff.cpu().numpy()
+        refs["vertex_count"] = vcnt.cpu().numpy()
+        rec = (refs, verts.double().cpu().numpy().reshape(-1))
+    s["records"] = rec
+    return rec
+
+
+def host_queries(rtm, arenas, kind, o, n):
+    if kind in (0, 1):
+        q = arenas["point"][o:o + n].double().cpu().numpy()
+        rec = np.zeros(n, dtype=rtm.POINT3D_DTYPE)
+        rec["id"] = np.arange(n)
+        rec["x"], rec["y"], rec["z"] = q[:, 0], q[:, 1], q[:, 2]
+    elif kind == 2:
+        q = arenas["ray"][o:o + n].double().cpu().numpy()
+        rec = np.zeros(n, dtype=rtm.RAY3D_DTYPE)
+        rec["id"] = np.arange(n)
+        for j, f in enumerate(["ox", "oy", "oz", "dx", "dy", "dz", "tmax"]):
+            rec[f] = q[:, j]
+    else:
+        q = arenas["pip"][o:o + n].double().cpu().numpy()
+        rec = np.zeros(n, dtype=rtm.POINT2D_DTYPE)
+        rec["id"] = np.arange(n)
+        rec["x"], rec["y"] = q[:, 0], q[:, 1]
+    return rec
+
+
+def plain_per_call_leg(torch, rtm, scenes, stream, offsets, arenas, budget_s: float):
+    """Plain per-call C-ABI: host records in, scene built per call, host rows out."""
+    p = rtm._ptr
+    done_q, spent, calls = 0, 0.0, 0
+    for i in range(len(stream["kind"])):
+        if spent >= budget_s:
+            break
+        k = int(stream["kind"][i])
+        n = int(stream["size"][i])
+        s = scenes[int(stream["scene"][i])]
+        scene_rec = host_scene_records(torch, rtm, s)
+        q = host_queries(rtm, arenas, k, int(offsets[i]), n)
+        t0 = time.perf_counter()
+        if k == 0:
+            rtm._call_rows("rtdl_mi355x_run_fixed_radius_neighbors_3d", rtm.FRN_ROW_DTYPE, p(q), n,
+                           p(scene_rec), len(scene_rec), RADIUS, K_MAX)
+        elif k == 1:
+            rtm._call_rows("rtdl_mi355x_run_knn_rows_3d", rtm.KNN_ROW_DTYPE, p(q), n, p(scene_rec),
+                           len(scene_rec), K_KNN)
+        elif k == 2:
+            rtm._call_rows("rtdl_mi355x_run_ray_anyhit_3d", rtm.ANYHIT_ROW_DTYPE, p(q), n, p(scene_rec),
+                           len(scene_rec))
+        else:
+            refs, verts = scene_rec
+            rtm._call_rows("rtdl_mi355x_run_point_in_polygon", rtm.PIP_ROW_DTYPE, p(q), n, p(refs),
+                           len(refs), p(verts), len(verts), 1)
+        spent += time.perf_counter() - t0
+        done_q += n
+        calls += 1
+    return {"queries": done_q, "seconds": spent, "calls": calls,
+            "queries_per_s": done_q / spent if spent else 0.0}
+
+
+def per_request_resident_leg(torch, B, scenes, stream, offsets, arenas, budget_s: float):
+    """Resident scenes, one synchronous single-job batch per request."""
+    kind_map = {0: B.JOB_FIXED_RADIUS, 1: B.JOB_KNN, 2: B.JOB_RAY_ANYHIT, 3: B.JOB_PIP_POSITIVE}
+    group_of = {0: "point", 1: "point", 2: "ray", 3: "pip"}
+    done_q, calls = 0, 0
+    t0 = time.perf_counter()
+    for i in range(len(stream["kind"])):
+        if time.perf_counter() - t0 >= budget_s:
+            break
+        k = int(stream["kind"][i])
+        n = int(stream["size"][i])
+        o = int(offsets[i])
+        s = scenes[int(stream["scene"][i])]
+        job = B.BatchJob(s["handle"], kind_map[k], arenas[group_of[k]][o:o + n], radius=RADIUS,
+                         k=K_KNN if k == 1 else K_MAX)
+        B.run_batch([job], morton_sort=True)
+        done_q += n
+        calls += 1
+    spent = time.perf_counter() - t0
+    return {"queries": done_q, "seconds": spent, "calls": calls,
+            "queries_per_s": done_q / spent if spent else 0.0}
+
+
+def cpu_baseline_leg(torch, rtm, scenes, stream, offsets, arenas, budget_s: float):
+    """The C oracle (FP64, multi-threaded over all host cores) on a bounded
+    sample of the same requests, grouped per scene (one index per scene)."""
+    from oracle import native as oracle
+
+    threads = oracle.default_threads()
+    done_q, spent = 0, 0.0
+    # take requests in stream order; each scene's pending requests run as one
+    # call (one index build per scene) once they reach `flush_at` queries
+    flush_at = 16384
+    pending: dict[int, list[int]] = {}
+    batches = []
+    for i in range(len(stream["kind"])):
+        sidx = int(stream["scene"][i])
+        pending.setdefault(sidx, []).append(i)
+        if sum(int(stream["size"][j]) for j in pending[sidx]) >= flush_at:
+            batches.append((sidx, pending.pop(sidx)))
+    batches.extend(pending.items())
+    for sidx, reqs in batches:
+        if spent >= budget_s:
+            break
+        if True:
+            s = scenes[sidx]
+            k = int(stream["kind"][reqs[0]])
+            qs = [host_queries(rtm, arenas, k, int(offsets[i]), int(stream["size"][i])) for i in reqs]
+            q = np.concatenate(qs)
+            rec = host_scene_records(torch, rtm, s)
+            t0 = time.perf_counter()
+            if k in (0, 1):
+                qd = {"id": np.arange(len(q), dtype=np.uint32), "x": q["x"], "y": q["y"], "z": q["z"]}
+                sd = {"id": rec["id"], "x": rec["x"], "y": rec["y"], "z": rec["z"]}
+                if k == 0:
+                    oracle.neighbors("frn", qd, sd, radius=RADIUS, k=K_MAX, threads=threads)
+                else:
+                    oracle.neighbors("knn", qd, sd, k=K_KNN, threads=threads)
+            elif k == 2:
+                rm = np.stack([q[f] for f in ["ox", "oy", "oz", "dx", "dy", "dz", "tmax"]], 1)
+                tm = np.stack([rec[f] for f in ["x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2"]], 1)
+                oracle.rays_3d("any", rm, tm, rec["id"], threads=threads)
+            else:
+                refs, verts = rec
+                oracle.pip_pairs(q["x"], q["y"], refs["vertex_offset"], refs["vertex_count"], verts,
+                                 positive_only=True, threads=threads)
+            spent += time.perf_counter() - t0
+            done_q += len(q)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": done_q / spent if spent else 0.0, "unit": "queries/s", "cores": threads,
+            "kind": "port", "cpu_model": model,
+            "sample": f"{done_q} queries of the timed stream, grouped per scene (grid index built per "
+                      f"group, included), C FP64 oracle, {spent:.1f} s"}
+
+
+# ----------------------------------------------------------------- main --
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="seconds of request stream timed")
+    ap.add_argument("--warmup", type=int, default=2, help="seconds of request stream replayed untimed")
+    ap.add_argument("--rate", type=float, default=20000.0)
+    ap.add_argument("--max-prims", type=float, default=1e7)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-batch-queries", type=int, default=4 << 20)
+    ap.add_argument("--percall-seconds", type=float, default=8.0)
+    ap.add_argument("--resident-seconds", type=float, default=4.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--roofline-queries", type=int, default=2 << 20)
+    ap.add_argument("--skip-baselines", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every query set is independent, multi-GPU sharding is a "
+              "design note only; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from rtdl_amd import batch as B
+    from rtdl_amd import mi355x_runtime as rtm
+
+    info = rtm.mi355x_context_probe()
+    log(f"device {info}")
+    rng = np.random.default_rng(SEED)
+    scenes, c3, c2 = make_scenes(torch, B, rng, args.max_prims)
+    scene_slot = {s["index"]: s["index"] for s in scenes}
+    handles = [s["handle"] for s in scenes]
+
+    warm = make_stream(rng, args.warmup, args.rate, scenes)
+    timed = make_stream(rng, args.steps, args.rate, scenes)
+    log(f"stream: warmup {len(warm['kind'])} requests, timed {len(timed['kind'])} requests, "
+        f"{int(timed['size'].sum())} queries")
+    serve_kw = dict(radius=RADIUS, k_max=K_MAX, k=K_KNN, slots=args.slots,
+                    memory_budget_bytes=int(args.budget_gb * (1 << 30)),
+                    max_batch_queries=args.max_batch_queries)
+    per_slot = min(serve_kw["memory_budget_bytes"] // args.slots,
+                   (args.max_batch_queries + 65536) * K_MAX * 24 + (64 << 20))
+    log(f"admission budget {args.budget_gb:.0f} GiB over {args.slots} slots; "
+        f"{per_slot * args.slots / (1 << 30):.1f} GiB of row buffers resident")
+
+    if len(warm["kind"]):
+        wa, wo, _ = make_arenas(torch, warm, scenes, c3, c2, SEED + 1)
+        wreq = requests_array(B, warm, wo, scene_slot)
+        B.serve_replay(handles, wreq, point_queries=wa["point"], ray_queries=wa["ray"],
+                       pip_queries=wa["pip"], **serve_kw)
+        del wa
+        torch.cuda.synchronize()
+        log("warmup replay done")
+
+    arenas, offsets, totals = make_arenas(torch, timed, scenes, c3, c2, SEED + 2)
+    reqs = requests_array(B, timed, offsets, scene_slot)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    log(f"{free / (1 << 30):.1f} GiB free before the timed replay")
+    t0 = time.perf_counter()
+    results, stats, _ = B.serve_replay(handles, reqs, point_queries=arenas["point"],
+                                       ray_queries=arenas["ray"], pip_queries=arenas["pip"], **serve_kw)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ok = results["status"] == 0
+    lat = (results["done_s"] - results["arrival_s"])[ok] * 1e3
+    served_q = int(timed["size"][ok].sum())
+    value = served_q / stats["wall_s"]
+    log(f"served: {served_q} queries in {stats['wall_s']:.3f}s ({value/1e6:.2f} Mq/s), "
+        f"p50 {np.percentile(lat, 50):.2f} ms, p99 {np.percentile(lat, 99):.2f} ms, "
+        f"{stats['batches']} batches, rejected {int((~ok).sum())}")
+    per_kind = {}
+    for k in range(4):
+        sel = (timed["kind"] == k) & ok
+        if sel.any():
+            lk = (results["done_s"] - results["arrival_s"])[sel] * 1e3
+            per_kind[KINDS[k]] = {"requests": int(sel.sum()), "queries": int(timed["size"][sel].sum()),
+                                  "p50_ms": float(np.percentile(lk, 50)), "p99_ms": float(np.percentile(lk, 99))}
+
+    roof = roofline_leg(torch, B, scenes, timed, offsets, arenas, args.roofline_queries)
+    dominant = max(roof, key=lambda k: roof[k]["kernel_s"] / max(1, roof[k]["queries"])
+                   * max(1, int(timed["size"][timed["kind"] == KINDS.index(k)].sum())))
+    d = roof[dominant]
+
+    baselines = {}
+    cpu = None
+    if not args.skip_baselines:
+        baselines["plain_per_call"] = plain_per_call_leg(torch, rtm, scenes, timed, offsets, arenas,
+                                                         args.percall_seconds)
+        log(f"plain per-call: {baselines['plain_per_call']}")
+        baselines["per_request_resident"] = per_request_resident_leg(torch, B, scenes, timed, offsets,
+                                                                      arenas, args.resident_seconds)
+        log(f"per-request resident: {baselines['per_request_resident']}")
+        cpu = cpu_baseline_leg(torch, rtm, scenes, timed, offsets, arenas, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+
+    plain = baselines.get("plain_per_call", {}).get("queries_per_s")
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": stats["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "served request stream: Poisson 20k req/s, 25% each fixed-radius r=0.01 k_max=64 / "
+                        "KNN k=8 / 3-D ray any-hit tmax=0.5 / PIP positive hits, batch log-uniform [1, 65536], "
+                        "16 resident scenes of 1e5-1e7 FP32 primitives (uniform / 64-Gaussian mixture)",
+            "stream_seconds": args.steps,
+            "requests": int(len(timed["kind"])),
+            "queries": int(timed["size"].sum()),
+            "slots": args.slots,
+            "memory_budget_gb": args.budget_gb,
+            "row_region_gb_per_slot": stats["region_bytes"] / (1 << 30),
+            "max_prims": args.max_prims,
+        },
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "rows_per_s": stats["rows"] / stats["wall_s"],
+        "served": {"wall_s": stats["wall_s"], "batches": stats["batches"], "rows": stats["rows"],
+                   "rejected": int((~ok).sum()), "peak_inflight_bytes": stats["peak_inflight_bytes"],
+                   "per_kind": per_kind},
+        "baselines": baselines,
+        "speedup_vs_plain_per_call": (value / plain) if plain else None,
+        "speedup_vs_per_request_resident": (value / baselines["per_request_resident"]["queries_per_s"])
+        if baselines.get("per_request_resident", {}).get("queries_per_s") else None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"batch_{'points' if dominant in ('frn', 'knn') else dominant}_kernel ({dominant})",
+            "achieved": d["achieved_GBps"],
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": d["achieved_GBps"] * 1e9 / HBM_PEAK,
+            "traffic": None,
+            "per_kind": roof,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

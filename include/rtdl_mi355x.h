@@ -246,6 +246,139 @@ int rtdl_mi355x_run_prepared_point_in_polygon(void* prepared, const RtdlPoint* p
 /* replaces rtdl_hiprt_destroy_prepared_* (src/native/hiprt/rtdl_hiprt_api.cpp:48-50,456-458) */
 void rtdl_mi355x_destroy_prepared(void* prepared);
 
+/* ---- batched and served execution (no counterpart in the reference) ----
+ *
+ * These entry points take device-resident inputs (FP32 query arrays whose
+ * values are exact in FP64) and write device-resident row buffers.  Query
+ * ids in the rows are indices into the job's query array.  Neighbour rows
+ * use the RtdlFixedRadiusNeighborRow / RtdlKnnNeighborRow layouts, ray rows
+ * RtdlRayAnyHitRow / RtdlRayHitCountRow / RtdlRayClosestHitRow, PIP rows
+ * RtdlPipRow (positive hits only, at most 16 per point). */
+
+enum {
+  RTDL_JOB_FIXED_RADIUS = 0,
+  RTDL_JOB_KNN = 1,
+  RTDL_JOB_BOUNDED_KNN = 2,
+  RTDL_JOB_RAY_ANYHIT = 3,
+  RTDL_JOB_RAY_HITCOUNT = 4,
+  RTDL_JOB_RAY_CLOSEST = 5,
+  RTDL_JOB_PIP_POSITIVE = 6
+};
+
+/* status bits of a job / request */
+#define RTDL_STATUS_ROW_OVERFLOW 1u
+#define RTDL_STATUS_PIP_OVERFLOW 2u
+#define RTDL_STATUS_REJECTED 4u
+
+typedef struct RtdlBatchJob {
+  void* scene;           /* prepared handle */
+  uint32_t instance;     /* instance of a multi-instance scene, 0 otherwise */
+  uint32_t kind;         /* RTDL_JOB_* */
+  const float* queries;  /* device pointer, query_stride floats per query */
+  uint32_t query_count;
+  uint32_t query_stride; /* 2 (2-D points), 3 (3-D points, z = 0 for 2-D) or 7 (rays) */
+  double radius;         /* fixed radius / bounded KNN */
+  uint32_t k;            /* k (KNN) or k_max (fixed radius, bounded KNN), 1..64 */
+  uint32_t pad;
+  void* rows;            /* device row buffer */
+  uint64_t row_capacity; /* rows the buffer holds */
+} RtdlBatchJob;
+
+typedef struct RtdlBatchJobResult {
+  uint64_t rows;      /* rows written */
+  uint32_t status;    /* RTDL_STATUS_* bits */
+  uint32_t ambiguous; /* selections recomputed with the exact FP64 list */
+} RtdlBatchJobResult;
+
+/* Runs all jobs as one batch: one persistent launch per predicate family
+ * over a device work queue, optional (job, Morton) query ordering, rows
+ * compacted per job by wavefront prefix sums.  Blocks until the batch is
+ * done; seconds_out (optional) receives the device time of the batch. */
+int rtdl_mi355x_run_batch(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                          RtdlBatchJobResult* results, double* seconds_out, char* error_out,
+                          size_t error_size);
+
+/* rtdl_mi355x_run_batch with measurement: family_seconds (5 entries:
+ * neighbour, ray any-hit, ray hit-count, ray closest-hit, PIP) receives each
+ * family's kernel time from events on the launch stream; visits (2 entries,
+ * optional, counting kernels) receives the node records fetched and the
+ * primitive records tested by all traversals of the batch. */
+int rtdl_mi355x_run_batch_profiled(const RtdlBatchJob* jobs, size_t job_count,
+                                   uint32_t morton_sort, RtdlBatchJobResult* results,
+                                   double* family_seconds, uint64_t* visits, char* error_out,
+                                   size_t error_size);
+
+/* Multi-instance scenes for batched jobs: instance i owns the primitives
+ * [instance_offsets[i], instance_offsets[i + 1]) of the device input and
+ * gets its own LBVH subtree; neighbour / polygon ids are instance-local
+ * indices.  coords: 3 FP32 per point (z = 0 for 2-D instances).
+ * Polygons: refs (vertex offset, vertex count) per polygon with offsets into
+ * the device vertex array (2 FP32 per vertex). */
+int rtdl_mi355x_prepare_point_instances(const float* coords, size_t point_count,
+                                        const uint64_t* instance_offsets, size_t instance_count,
+                                        void** prepared_out, char* error_out, size_t error_size);
+/* coords: 9 FP32 per triangle (three vertices); ids are instance-local. */
+int rtdl_mi355x_prepare_triangle_instances(const float* coords, size_t triangle_count,
+                                           const uint64_t* instance_offsets, size_t instance_count,
+                                           void** prepared_out, char* error_out, size_t error_size);
+int rtdl_mi355x_prepare_polygon_instances(const uint32_t* vertex_offsets,
+                                          const uint32_t* vertex_counts, size_t polygon_count,
+                                          const float* vertices_xy, size_t vertex_count,
+                                          const uint64_t* instance_offsets, size_t instance_count,
+                                          void** prepared_out, char* error_out, size_t error_size);
+
+typedef struct RtdlServeRequest {
+  uint32_t kind;         /* RTDL_JOB_* */
+  uint32_t scene;        /* index into the scene handle array */
+  uint64_t query_offset; /* first query of the request in the kind's arena */
+  uint32_t query_count;
+  uint32_t capture;      /* 1: return this request's rows to the host */
+  double arrival_s;      /* arrival time from the start of the replay; non-decreasing */
+} RtdlServeRequest;
+
+typedef struct RtdlServeConfig {
+  double radius;                /* fixed-radius requests */
+  uint32_t k_max;               /* fixed-radius requests, 1..64 */
+  uint32_t k;                   /* KNN requests, 1..64 */
+  uint32_t slots;               /* batches in flight, each on its own stream */
+  uint32_t morton_sort;         /* order each batch's queries by (request, Morton code) */
+  uint64_t memory_budget_bytes; /* device bytes for in-flight row buffers, split over slots */
+  uint64_t max_batch_queries;
+  const float* point_queries;   /* device arena, 3 FP32 per query (neighbour requests) */
+  const float* ray_queries;     /* device arena, 7 FP32 per ray */
+  const float* pip_queries;     /* device arena, 2 FP32 per point */
+  uint32_t realtime;            /* 1: honour arrival times; 0: everything arrives at 0 */
+  uint32_t pad;
+} RtdlServeConfig;
+
+typedef struct RtdlServeResult {
+  double arrival_s;
+  double dispatch_s;
+  double done_s;
+  uint64_t rows;
+  uint32_t status;     /* RTDL_STATUS_* bits */
+  uint32_t batch;
+  void* captured_rows; /* with capture: host rows, release with rtdl_mi355x_free_rows */
+} RtdlServeResult;
+
+typedef struct RtdlServeStats {
+  double wall_s;
+  uint64_t batches;
+  uint64_t queries;
+  uint64_t rows;
+  uint64_t peak_inflight_bytes;
+  uint64_t region_bytes;
+} RtdlServeStats;
+
+/* Replays a request stream against resident scenes with continuous
+ * batching: up to `slots` batches in flight, each admitted from the
+ * requests that have arrived, in order, while their row buffers fit the
+ * slot's share of the memory budget.  Latency = done_s - arrival_s. */
+int rtdl_mi355x_serve_replay(void* const* scenes, size_t scene_count,
+                             const RtdlServeRequest* requests, size_t request_count,
+                             const RtdlServeConfig* config, RtdlServeResult* results,
+                             RtdlServeStats* stats, char* error_out, size_t error_size);
+
 /* ---- diagnostics (no counterpart in the reference) ---- */
 
 /* Evaluates sqrt(a[i]) and a[i] / b[i] in FP64 on the device with the same

@@ -656,8 +656,15 @@ int oracle_rays_3d(int mode, size_t nr, const double* rays, size_t nt, const dou
         if (bmin[t * 3 + a] < lo[a]) lo[a] = bmin[t * 3 + a];
         if (bmax[t * 3 + a] > hi[a]) hi[a] = bmax[t * 3 + a];
       }
-    for (int a = 0; a < 3; ++a) vol *= (hi[a] - lo[a]) > 0 ? (hi[a] - lo[a]) : 1.0;
-    double want = pow(vol * 2.0 / (double)nt, 1.0 / 3.0);
+    double span = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      vol *= (hi[a] - lo[a]) > 0 ? (hi[a] - lo[a]) : 1.0;
+      if (hi[a] - lo[a] > span) span = hi[a] - lo[a];
+    }
+    /* about 16 triangles per cell, and at most ~48 cells per axis: a long
+     * segment enumerates every cell of its bounding box */
+    double want = pow(vol * 16.0 / (double)nt, 1.0 / 3.0);
+    if (want < span / 48.0) want = span / 48.0;
     grid_build(&c.grid, 3, nt, bmin, bmax, want);
     free(bmin);
     free(bmax);

@@ -9,8 +9,10 @@
 
 #include "../../include/rtdl_mi355x.h"
 #include "common.hpp"
+#include "engine.hpp"
 #include "io.hpp"
 #include "runtime.hpp"
+#include "serve.hpp"
 
 using namespace rtdl;
 
@@ -367,6 +369,145 @@ void rtdl_mi355x_destroy_prepared(void* prepared) {
   auto* p = static_cast<Prepared*>(prepared);
   if (p->magic != 0x52544D58u) return;
   delete p;
+}
+
+// ------------------------------------------------------ batched / served --
+
+int rtdl_mi355x_prepare_point_instances(const float* coords, size_t point_count,
+                                        const uint64_t* instance_offsets, size_t instance_count,
+                                        void** prepared_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(instance_offsets != nullptr && instance_count >= 1, "instance offsets are required");
+    std::vector<uint64_t> offs(instance_offsets, instance_offsets + instance_count + 1);
+    make_prepared(prepared_out, [&](Prepared& p) { prepare_point_instances(p, coords, point_count, offs); });
+  });
+}
+
+int rtdl_mi355x_prepare_triangle_instances(const float* coords, size_t triangle_count,
+                                           const uint64_t* instance_offsets, size_t instance_count,
+                                           void** prepared_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(instance_offsets != nullptr && instance_count >= 1, "instance offsets are required");
+    std::vector<uint64_t> offs(instance_offsets, instance_offsets + instance_count + 1);
+    make_prepared(prepared_out, [&](Prepared& p) { prepare_triangle_instances(p, coords, triangle_count, offs); });
+  });
+}
+
+int rtdl_mi355x_prepare_polygon_instances(const uint32_t* vertex_offsets,
+                                          const uint32_t* vertex_counts, size_t polygon_count,
+                                          const float* vertices_xy, size_t vertex_count,
+                                          const uint64_t* instance_offsets, size_t instance_count,
+                                          void** prepared_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(instance_offsets != nullptr && instance_count >= 1, "instance offsets are required");
+    std::vector<uint64_t> offs(instance_offsets, instance_offsets + instance_count + 1);
+    make_prepared(prepared_out, [&](Prepared& p) {
+      prepare_polygon_instances(p, vertex_offsets, vertex_counts, polygon_count, vertices_xy,
+                                vertex_count, offs);
+    });
+  });
+}
+
+static void run_batch_impl(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                           RtdlBatchJobResult* results, double* seconds_out,
+                           double* family_seconds, unsigned long long* visits);
+
+int rtdl_mi355x_run_batch(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                          RtdlBatchJobResult* results, double* seconds_out, char* error_out,
+                          size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    run_batch_impl(jobs, job_count, morton_sort, results, seconds_out, nullptr, nullptr);
+  });
+}
+
+int rtdl_mi355x_run_batch_profiled(const RtdlBatchJob* jobs, size_t job_count,
+                                   uint32_t morton_sort, RtdlBatchJobResult* results,
+                                   double* family_seconds, uint64_t* visits, char* error_out,
+                                   size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    run_batch_impl(jobs, job_count, morton_sort, results, nullptr, family_seconds,
+                   reinterpret_cast<unsigned long long*>(visits));
+  });
+}
+
+static void run_batch_impl(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                           RtdlBatchJobResult* results, double* seconds_out,
+                           double* family_seconds, unsigned long long* visits) {
+  {
+    require(job_count == 0 || (jobs != nullptr && results != nullptr),
+            "jobs and results must not be null when job_count is nonzero");
+    std::vector<HostJob> host(job_count);
+    for (size_t i = 0; i < job_count; ++i) {
+      const RtdlBatchJob& b = jobs[i];
+      Prepared* p = prepared_from(b.scene);
+      require(b.instance < p->scene.n_instances || (p->scene.n_instances == 0 && b.instance == 0),
+              "job instance out of range");
+      const bool point_kind = b.kind <= kJobBoundedKnn;
+      const bool ray_kind = b.kind >= kJobRayAnyHit && b.kind <= kJobRayClosest;
+      require(b.kind <= kJobPipPositive, "unknown job kind");
+      require(!point_kind || p->type == SceneType::Points2D || p->type == SceneType::Points3D,
+              "neighbour jobs need a point scene");
+      require(!ray_kind || p->type == SceneType::Triangles3D, "ray jobs need a triangle scene");
+      require(b.kind != kJobPipPositive || p->type == SceneType::Polygons, "PIP jobs need a polygon scene");
+      require(!ray_kind || b.query_stride == 7, "ray jobs need query_stride 7");
+      require(b.kind != kJobPipPositive || b.query_stride == 2, "PIP jobs need query_stride 2");
+      require(!point_kind || b.query_stride == 2 || b.query_stride == 3,
+              "neighbour jobs need query_stride 2 or 3");
+      HostJob& h = host[i];
+      h.scene = p->scene.view(b.instance);
+      h.kind = b.kind;
+      h.nq = b.query_count;
+      h.queries = b.queries;
+      h.qstride = b.query_stride;
+      h.radius = b.radius;
+      h.k = b.k;
+      h.rows = b.rows;
+      h.row_capacity = b.row_capacity;
+      require(h.row_capacity >= job_row_capacity(b.kind, b.query_count, b.k, 0) || b.kind == kJobPipPositive,
+              "row_capacity is below the rows the job can emit");
+    }
+    static thread_local std::unique_ptr<BatchEngine> engine;
+    hipStream_t s = thread_stream();
+    if (!engine) engine = std::make_unique<BatchEngine>(s);
+    engine->enable_timing(family_seconds != nullptr);
+    engine->enable_stats(visits != nullptr);
+    hipEvent_t start, stop;
+    RTDL_HIP(hipEventCreate(&start));
+    RTDL_HIP(hipEventCreate(&stop));
+    RTDL_HIP(hipEventRecord(start, s));
+    engine->enqueue(host, morton_sort != 0);
+    RTDL_HIP(hipEventRecord(stop, s));
+    RTDL_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    RTDL_HIP(hipEventElapsedTime(&ms, start, stop));
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    if (seconds_out) *seconds_out = ms * 1e-3;
+    if (family_seconds) engine->family_seconds(family_seconds);
+    if (visits) {
+      RTDL_HIP(hipMemcpy(visits, engine->device_stats(), 2 * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+    }
+    const JobCounters* c = engine->counters();
+    for (size_t i = 0; i < job_count; ++i) {
+      results[i].rows = c[i].rows;
+      results[i].status = c[i].status;
+      results[i].ambiguous = c[i].ambiguous;
+    }
+  }
+}
+
+int rtdl_mi355x_serve_replay(void* const* scenes, size_t scene_count,
+                             const RtdlServeRequest* requests, size_t request_count,
+                             const RtdlServeConfig* config, RtdlServeResult* results,
+                             RtdlServeStats* stats, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(config != nullptr, "config must not be null");
+    require(request_count == 0 || (requests != nullptr && results != nullptr),
+            "requests and results must not be null when request_count is nonzero");
+    require(scene_count == 0 || scenes != nullptr, "scenes must not be null");
+    serve_replay(scenes, scene_count, requests, request_count, *config, results, stats);
+  });
 }
 
 int rtdl_mi355x_selftest_fp64(const double* a, const double* b, size_t n, double* sqrt_out,

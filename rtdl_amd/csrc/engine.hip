@@ -1,0 +1,543 @@
+// Batched execution engine: persistent kernels over a device work queue.
+//
+// A batch holds many independent jobs (a query set against one resident
+// scene instance).  Each job's queries are cut into 64-query work items; one
+// persistent launch per predicate family lets every wavefront pull items
+// from an atomic counter until the queue is empty, so uneven jobs balance
+// across the 256 CUs.  Optionally the batch's queries are first put in
+// (job, Morton code) order with the LDS radix sort so that the lanes of a
+// wavefront traverse neighbouring parts of the tree.  Rows are compacted per
+// job with a wavefront exclusive prefix sum and one atomic per wavefront.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+#include "engine.hpp"
+#include "primitives.hpp"
+#include "queries.hpp"
+#include "traverse.hpp"
+
+namespace rtdl {
+
+size_t job_row_bytes(uint32_t kind) {
+  switch (kind) {
+    case kJobFixedRadius: return sizeof(FrnRowRec);
+    case kJobKnn:
+    case kJobBoundedKnn: return sizeof(KnnRowRec);
+    case kJobRayAnyHit:
+    case kJobRayHitCount: return sizeof(PairRowRec);
+    case kJobRayClosest: return sizeof(ClosestRowRec);
+    default: return sizeof(PipRowRec);
+  }
+}
+
+uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_cap) {
+  switch (kind) {
+    case kJobFixedRadius:
+    case kJobKnn:
+    case kJobBoundedKnn: return (uint64_t)nq * k;
+    case kJobPipPositive: return (uint64_t)nq * pip_cap;
+    default: return nq;
+  }
+}
+
+namespace {
+
+__device__ inline uint32_t wave_exclusive_sum(uint32_t v, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  *total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+__device__ inline uint32_t wave_fetch(uint32_t* counter) {
+  uint32_t it = 0;
+  if ((threadIdx.x & 63) == 0) it = atomicAdd(counter, 1u);
+  return __shfl(it, 0, 64);
+}
+
+// Reserves `c` rows per lane in the job's row buffer; returns the lane's
+// first row.
+__device__ inline uint64_t wave_reserve(JobCounters* counters, uint32_t c) {
+  uint32_t total;
+  const uint32_t prefix = wave_exclusive_sum(c, &total);
+  uint32_t base = 0;
+  if ((threadIdx.x & 63) == 0 && total) base = atomicAdd(&counters->rows, total);
+  base = __shfl(base, 0, 64);
+  return (uint64_t)base + prefix;
+}
+
+__device__ inline uint32_t job_query(const DeviceJob& j, uint32_t pos) {
+  return j.order ? j.order[j.order_base + pos] : pos;
+}
+
+template <int K>
+__device__ inline void write_neighbor_row(const DeviceJob& j, uint64_t at, uint32_t qi, uint32_t id,
+                                          double dist, uint32_t rank) {
+  if (j.kind == kJobFixedRadius) {
+    static_cast<FrnRowRec*>(j.rows)[at] = FrnRowRec{qi, id, dist};
+  } else {
+    static_cast<KnnRowRec*>(j.rows)[at] = KnnRowRec{qi, id, dist, rank, 0u};
+  }
+}
+
+template <int K, bool STATS>
+__global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __restrict__ jobs,
+                                                           const WorkItem* __restrict__ items,
+                                                           uint32_t n_items, uint32_t* item_counter,
+                                                           uint4* amb, uint32_t* amb_count,
+                                                           unsigned long long* stats) {
+  const uint32_t lane = threadIdx.x & 63;
+  LaneStats st;
+  for (;;) {
+    const uint32_t it = wave_fetch(item_counter);
+    if (it >= n_items) break;
+    const WorkItem w = items[it];
+    const DeviceJob& j = jobs[w.job];
+    const bool active = lane < w.count;
+    uint32_t qi = 0;
+    double qx = 0.0, qy = 0.0, qz = 0.0;
+    PackedTopK<K> list;
+    list.init(1);
+    bool ambiguous = false;
+    uint32_t c = 0;
+    const uint32_t k = j.k;
+    if (active) {
+      qi = job_query(j, w.first + lane);
+      const float* qp = j.queries + (size_t)qi * j.qstride;
+      qx = qp[0];
+      qy = qp[1];
+      qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
+      ambiguous = collect_packed<K, STATS>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius,
+                                           k, list, st);
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (i >= K - (int)k && list.e[i] != ~0ull) ++c;
+    }
+    const uint64_t at0 = wave_reserve(j.counters, c);
+    if (active && c) {
+      if (at0 + c > j.row_capacity) {
+        atomicOr(&j.counters->status, kStatusRowOverflow);
+      } else {
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          if (i >= K - (int)k && list.e[i] != ~0ull) {
+            const uint32_t slot = (uint32_t)list.e[i];
+            const float4 p = j.scene.prims[slot];
+            const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
+            write_neighbor_row<K>(j, at0 + r, qi, __float_as_uint(p.w), dist, r + 1);
+            ++r;
+          }
+        }
+        if (ambiguous) {
+          amb[atomicAdd(amb_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
+          atomicAdd(&j.counters->ambiguous, 1u);
+        }
+      }
+    }
+  }
+  if (STATS) flush_lane_stats(stats, st);
+}
+
+// Recomputes the queries whose FP32 selection keys tied at the boundary with
+// the exact FP64 list and rewrites their rows in place.
+template <int K>
+__global__ __launch_bounds__(64) void batch_fix_kernel(const DeviceJob* __restrict__ jobs,
+                                                       const uint4* __restrict__ amb,
+                                                       const uint32_t* amb_count) {
+  const uint32_t n = *amb_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint4 a = amb[i];
+    const DeviceJob& j = jobs[a.x];
+    const float* qp = j.queries + (size_t)a.y * j.qstride;
+    const double qx = qp[0], qy = qp[1], qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
+    ExactTopK<K> list;
+    uint32_t inside = 0;
+    LaneStats st;
+    collect_exact<K, false, false>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius, j.k,
+                                   list, inside, st);
+    uint32_t r = 0;
+#pragma unroll
+    for (int e = 0; e < K; ++e) {
+      if (e >= K - (int)j.k && list.d[e] < __builtin_inf() && r < a.w) {
+        write_neighbor_row<K>(j, (uint64_t)a.z + r, a.y, list.id[e], list.d[e], r + 1);
+        ++r;
+      }
+    }
+  }
+}
+
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __restrict__ jobs,
+                                                         const WorkItem* __restrict__ items,
+                                                         uint32_t n_items, uint32_t* item_counter,
+                                                         unsigned long long* stats) {
+  const uint32_t lane = threadIdx.x & 63;
+  LaneStats st;
+  for (;;) {
+    const uint32_t it = wave_fetch(item_counter);
+    if (it >= n_items) break;
+    const WorkItem w = items[it];
+    const DeviceJob& j = jobs[w.job];
+    const bool active = lane < w.count;
+    RayResult res;
+    uint32_t qi = 0;
+    if (active) {
+      qi = job_query(j, w.first + lane);
+      const float* r = j.queries + (size_t)qi * j.qstride;
+      res = trace_ray<MODE, STATS>(j.scene, j.scene.root, j.scene.end, r[0], r[1], r[2], r[3],
+                                   r[4], r[5], r[6], st);
+    }
+    if (MODE != 2) {
+      if (active)
+        static_cast<PairRowRec*>(j.rows)[qi] = PairRowRec{qi, MODE == 0 ? (res.hits ? 1u : 0u) : res.hits};
+      if (lane == 0) atomicAdd(&j.counters->rows, w.count);
+    } else {
+      const uint32_t c = (active && res.best_t == res.best_t) ? 1u : 0u;
+      const uint64_t at = wave_reserve(j.counters, c);
+      if (c) {
+        if (at >= j.row_capacity)
+          atomicOr(&j.counters->status, kStatusRowOverflow);
+        else
+          static_cast<ClosestRowRec*>(j.rows)[at] = ClosestRowRec{qi, res.best_id, res.best_t};
+      }
+    }
+  }
+  if (STATS) flush_lane_stats(stats, st);
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restrict__ jobs,
+                                                        const WorkItem* __restrict__ items,
+                                                        uint32_t n_items, uint32_t* item_counter,
+                                                        double eps_sq, unsigned long long* stats) {
+  const uint32_t lane = threadIdx.x & 63;
+  LaneStats st;
+  for (;;) {
+    const uint32_t it = wave_fetch(item_counter);
+    if (it >= n_items) break;
+    const WorkItem w = items[it];
+    const DeviceJob& j = jobs[w.job];
+    const bool active = lane < w.count;
+    // (input index << 32 | slot), ascending; ~0 marks an empty entry
+    uint64_t hit[kPipHitCap];
+#pragma unroll
+    for (int i = 0; i < (int)kPipHitCap; ++i) hit[i] = ~0ull;
+    uint32_t found = 0;
+    uint32_t qi = 0;
+    if (active) {
+      qi = job_query(j, w.first + lane);
+      const float* p = j.queries + (size_t)qi * j.qstride;
+      walk_pip<STATS>(j.scene, j.scene.root, j.scene.end, p[0], p[1], true, eps_sq, st,
+                      [&](uint32_t slot) {
+                        if (found < kPipHitCap) {
+                          hit[kPipHitCap - 1] = ((uint64_t)j.scene.slot_to_input[slot] << 32) | slot;
+#pragma unroll
+                          for (int i = (int)kPipHitCap - 1; i > 0; --i) {
+                            const uint64_t a = hit[i - 1], b = hit[i];
+                            hit[i - 1] = a < b ? a : b;
+                            hit[i] = a < b ? b : a;
+                          }
+                        }
+                        ++found;
+                      });
+      if (found > kPipHitCap) atomicOr(&j.counters->status, kStatusPipOverflow);
+    }
+    const uint32_t c = active ? min(found, kPipHitCap) : 0u;
+    const uint64_t at0 = wave_reserve(j.counters, c);
+    if (c) {
+      if (at0 + c > j.row_capacity) {
+        atomicOr(&j.counters->status, kStatusRowOverflow);
+      } else {
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < (int)kPipHitCap; ++i) {
+          if (hit[i] != ~0ull) {
+            const uint32_t slot = (uint32_t)hit[i];
+            static_cast<PipRowRec*>(j.rows)[at0 + r] = PipRowRec{qi, j.scene.poly[slot].z, 1u};
+            ++r;
+          }
+        }
+      }
+    }
+  }
+  if (STATS) flush_lane_stats(stats, st);
+}
+
+__device__ inline uint32_t spread10b(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void batch_keys_kernel(const DeviceJob* __restrict__ jobs,
+                                  const uint64_t* __restrict__ starts, uint32_t n_jobs,
+                                  uint64_t total, uint64_t* __restrict__ keys,
+                                  uint32_t* __restrict__ vals) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total) return;
+  uint32_t lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (starts[mid] <= p) lo = mid; else hi = mid;
+  }
+  const DeviceJob& j = jobs[lo];
+  const uint32_t local = (uint32_t)(p - starts[lo]);
+  const float* q = j.queries + (size_t)local * j.qstride;
+  const float c[3] = {q[0], q[1], j.qstride >= 3 ? q[2] : 0.0f};
+  uint32_t code = 0;
+  if (j.scene.root != kEndNode) {
+    const BvhNode& r = j.scene.nodes[j.scene.root];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float e = r.hi[a] - r.lo[a];
+      float f = e > 0.0f ? (c[a] - r.lo[a]) / e * 1024.0f : 0.0f;
+      f = fminf(fmaxf(f, 0.0f), 1023.0f);
+      if (!(f == f)) f = 0.0f;
+      code |= spread10b((uint32_t)f) << (2 - a);
+    }
+  }
+  keys[p] = ((uint64_t)lo << 32) | code;
+  vals[p] = local;
+}
+
+__global__ void zero_words_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb,
+                                  unsigned long long* c, size_t nc) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0u;
+  if (i < nb) b[i] = 0u;
+  if (i < nc) c[i] = 0ull;
+}
+
+enum Family : int { kFamPoints = 0, kFamRayAny, kFamRayCount, kFamRayClosest, kFamPip, kFamilies };
+
+int family_of(uint32_t kind) {
+  switch (kind) {
+    case kJobFixedRadius:
+    case kJobKnn:
+    case kJobBoundedKnn: return kFamPoints;
+    case kJobRayAnyHit: return kFamRayAny;
+    case kJobRayHitCount: return kFamRayCount;
+    case kJobRayClosest: return kFamRayClosest;
+    default: return kFamPip;
+  }
+}
+
+template <int K>
+void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
+                   uint32_t n_items, uint32_t* counter, uint4* amb, uint32_t* amb_count,
+                   unsigned long long* st, hipStream_t s) {
+  if (stats)
+    batch_points_kernel<K, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+  else
+    batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+  RTDL_LAUNCH_CHECK();
+  batch_fix_kernel<K><<<64, 64, 0, s>>>(jobs, amb, amb_count);
+  RTDL_LAUNCH_CHECK();
+}
+
+template <int MODE>
+void launch_rays_batch(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
+                       uint32_t n_items, uint32_t* counter, unsigned long long* st, hipStream_t s) {
+  if (stats)
+    batch_rays_kernel<MODE, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, st);
+  else
+    batch_rays_kernel<MODE, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, st);
+  RTDL_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
+  RTDL_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+  int dev = 0;
+  RTDL_HIP(hipGetDevice(&dev));
+  RTDL_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
+  stats_.resize(2);
+  item_counters_.resize(kFamilies);
+  amb_count_.resize(1);
+  for (int f = 0; f < kFamilies; ++f) {
+    RTDL_HIP(hipEventCreate(&fam_start_[f]));
+    RTDL_HIP(hipEventCreate(&fam_stop_[f]));
+  }
+}
+
+void BatchEngine::family_seconds(double out[5]) const {
+  for (int f = 0; f < kFamilies; ++f) {
+    out[f] = 0.0;
+    if (!timed_[f]) continue;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, fam_start_[f], fam_stop_[f]) == hipSuccess) out[f] = ms * 1e-3;
+  }
+}
+
+BatchEngine::~BatchEngine() {
+  for (int f = 0; f < kFamilies; ++f) {
+    if (fam_start_[f]) (void)hipEventDestroy(fam_start_[f]);
+    if (fam_stop_[f]) (void)hipEventDestroy(fam_stop_[f]);
+  }
+  if (done_) (void)hipEventDestroy(done_);
+  if (host_jobs_) (void)hipHostFree(host_jobs_);
+  if (host_items_) (void)hipHostFree(host_items_);
+  if (host_starts_) (void)hipHostFree(host_starts_);
+  if (host_counters_) (void)hipHostFree(host_counters_);
+}
+
+void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
+  max_jobs = std::max<size_t>(max_jobs, 1);
+  max_queries = std::max<size_t>(max_queries, 1);
+  const size_t max_items = max_queries / 64 + max_jobs;
+  if (max_jobs > cap_jobs_) {
+    if (host_jobs_) RTDL_HIP(hipHostFree(host_jobs_));
+    if (host_starts_) RTDL_HIP(hipHostFree(host_starts_));
+    if (host_counters_) RTDL_HIP(hipHostFree(host_counters_));
+    RTDL_HIP(hipHostMalloc((void**)&host_jobs_, max_jobs * sizeof(DeviceJob), hipHostMallocDefault));
+    RTDL_HIP(hipHostMalloc((void**)&host_starts_, (max_jobs + 1) * sizeof(uint64_t), hipHostMallocDefault));
+    RTDL_HIP(hipHostMalloc((void**)&host_counters_, max_jobs * sizeof(JobCounters), hipHostMallocDefault));
+    jobs_.resize(max_jobs);
+    starts_.resize(max_jobs + 1);
+    counters_.resize(max_jobs);
+    cap_jobs_ = max_jobs;
+  }
+  if (max_items > cap_items_) {
+    if (host_items_) RTDL_HIP(hipHostFree(host_items_));
+    RTDL_HIP(hipHostMalloc((void**)&host_items_, max_items * sizeof(WorkItem), hipHostMallocDefault));
+    items_.resize(max_items);
+    cap_items_ = max_items;
+  }
+  if (max_queries > cap_queries_) {
+    keys_.resize(max_queries);
+    order_.resize(max_queries);
+    amb_.resize(max_queries);
+    sort_ws_.reserve(max_queries);
+    cap_queries_ = max_queries;
+  }
+}
+
+void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
+  const size_t n = jobs.size();
+  uint64_t total = 0;
+  for (const HostJob& j : jobs) total += j.nq;
+  require(total < 0xFFFFFFFFull, "batch exceeds 2^32 queries");
+  reserve(n, total);
+  // order jobs' items by predicate family; each family gets one launch
+  std::vector<std::vector<WorkItem>> fam_items(kFamilies);
+  uint32_t max_k[kFamilies] = {0, 0, 0, 0, 0};
+  uint64_t start = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const HostJob& h = jobs[i];
+    require(h.qstride == 2 || h.qstride == 3 || h.qstride == 7, "query stride must be 2, 3 or 7");
+    require(h.nq == 0 || h.queries != nullptr, "job queries must not be null");
+    require(h.nq == 0 || h.rows != nullptr, "job rows must not be null");
+    const int fam = family_of(h.kind);
+    if (fam == kFamPoints) require(h.k >= 1 && h.k <= 64, "batched neighbour jobs need 1 <= k <= 64");
+    max_k[fam] = std::max(max_k[fam], h.k);
+    DeviceJob& d = host_jobs_[i];
+    std::memset(&d, 0, sizeof(d));
+    d.scene = h.scene;
+    d.queries = h.queries;
+    d.order = morton_sort ? order_.get() : nullptr;
+    d.rows = h.rows;
+    d.counters = counters_.get() + i;
+    d.row_capacity = h.row_capacity;
+    d.radius = h.kind == kJobKnn ? __builtin_inf() : h.radius;
+    d.kind = h.kind;
+    d.nq = h.nq;
+    d.k = h.k;
+    d.qstride = h.qstride;
+    d.order_base = (uint32_t)start;
+    host_starts_[i] = start;
+    for (uint32_t f = 0; f < h.nq; f += 64)
+      fam_items[fam].push_back(WorkItem{(uint32_t)i, f, std::min<uint32_t>(64, h.nq - f), 0});
+    start += h.nq;
+  }
+  host_starts_[n] = start;
+  size_t item_off[kFamilies + 1] = {0};
+  for (int f = 0; f < kFamilies; ++f) {
+    std::memcpy(host_items_ + item_off[f], fam_items[f].data(), fam_items[f].size() * sizeof(WorkItem));
+    item_off[f + 1] = item_off[f] + fam_items[f].size();
+  }
+  hipStream_t s = stream_;
+  if (n) {
+    RTDL_HIP(hipMemcpyAsync(jobs_.get(), host_jobs_, n * sizeof(DeviceJob), hipMemcpyHostToDevice, s));
+    RTDL_HIP(hipMemcpyAsync(starts_.get(), host_starts_, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  }
+  if (item_off[kFamilies])
+    RTDL_HIP(hipMemcpyAsync(items_.get(), host_items_, item_off[kFamilies] * sizeof(WorkItem),
+                            hipMemcpyHostToDevice, s));
+  const size_t zero_n = std::max<size_t>(n * 4, kFamilies + 1);
+  zero_words_kernel<<<blocks_for(zero_n, 256), 256, 0, s>>>(
+      reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kFamilies,
+      stats_.get(), 2);
+  RTDL_LAUNCH_CHECK();
+  fill_u32(amb_count_.get(), 1, 0u, s);
+  if (morton_sort && total > 1) {
+    batch_keys_kernel<<<blocks_for(total, 256), 256, 0, s>>>(jobs_.get(), starts_.get(), (uint32_t)n,
+                                                              total, keys_.get(), order_.get());
+    RTDL_LAUNCH_CHECK();
+    int job_bits = 0;
+    while ((1ull << job_bits) < n) ++job_bits;
+    radix_sort_pairs_async(keys_.get(), order_.get(), total, 30 + 2 + job_bits, sort_ws_, s);
+  } else if (morton_sort && total == 1) {
+    fill_u32(order_.get(), 1, 0u, s);
+  }
+  const unsigned max_grid = (unsigned)cus_ * 8;
+  auto grid_for = [&](size_t items) {
+    return (unsigned)std::max<size_t>(1, std::min<size_t>((items + 3) / 4, max_grid));
+  };
+  unsigned long long* st = stats_on_ ? stats_.get() : nullptr;
+  const DeviceJob* dj = jobs_.get();
+  for (int f = 0; f < kFamilies; ++f) timed_[f] = false;
+  for (int f = 0; f < kFamilies; ++f) {
+    const uint32_t n_items = (uint32_t)(item_off[f + 1] - item_off[f]);
+    if (!n_items) continue;
+    const WorkItem* it = items_.get() + item_off[f];
+    uint32_t* ctr = item_counters_.get() + f;
+    const unsigned grid = grid_for(n_items);
+    if (timing_on_) {
+      RTDL_HIP(hipEventRecord(fam_start_[f], s));
+      timed_[f] = true;
+    }
+    if (f == kFamPoints) {
+      switch (list_capacity(max_k[f])) {
+        case 1: launch_points<1>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 2: launch_points<2>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 4: launch_points<4>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 8: launch_points<8>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 16: launch_points<16>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 32: launch_points<32>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        default: launch_points<64>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+      }
+    } else if (f == kFamRayAny) {
+      launch_rays_batch<0>(stats_on_, grid, dj, it, n_items, ctr, st, s);
+    } else if (f == kFamRayCount) {
+      launch_rays_batch<1>(stats_on_, grid, dj, it, n_items, ctr, st, s);
+    } else if (f == kFamRayClosest) {
+      launch_rays_batch<2>(stats_on_, grid, dj, it, n_items, ctr, st, s);
+    } else {
+      const double eps_sq = std::pow(1.0e-12, 2.0);  // reference.py:757 `eps ** 2`
+      if (stats_on_)
+        batch_pip_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, eps_sq, st);
+      else
+        batch_pip_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, eps_sq, st);
+      RTDL_LAUNCH_CHECK();
+    }
+    if (timing_on_) RTDL_HIP(hipEventRecord(fam_stop_[f], s));
+  }
+  if (n)
+    RTDL_HIP(hipMemcpyAsync(host_counters_, counters_.get(), n * sizeof(JobCounters),
+                            hipMemcpyDeviceToHost, s));
+  RTDL_HIP(hipEventRecord(done_, s));
+}
+
+}  // namespace rtdl

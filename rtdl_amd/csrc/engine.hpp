@@ -1,0 +1,157 @@
+// Batched execution: many independent jobs (query sets against resident
+// scenes or scene instances) in one persistent launch per predicate family
+// over a device work queue, with rows compacted per job by wavefront
+// prefix sums.  Everything is enqueued asynchronously on one stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "primitives.hpp"
+#include "scene.hpp"
+
+namespace rtdl {
+
+enum JobKind : uint32_t {
+  kJobFixedRadius = 0,
+  kJobKnn = 1,
+  kJobBoundedKnn = 2,
+  kJobRayAnyHit = 3,
+  kJobRayHitCount = 4,
+  kJobRayClosest = 5,
+  kJobPipPositive = 6,
+};
+
+// Row records written by batch jobs (query ids are job-local indices).
+struct FrnRowRec {
+  uint32_t query_id;
+  uint32_t neighbor_id;
+  double distance;
+};
+struct KnnRowRec {
+  uint32_t query_id;
+  uint32_t neighbor_id;
+  double distance;
+  uint32_t neighbor_rank;
+  uint32_t pad;
+};
+struct PairRowRec {
+  uint32_t id;
+  uint32_t value;
+};
+struct ClosestRowRec {
+  uint32_t ray_id;
+  uint32_t triangle_id;
+  double t;
+};
+struct PipRowRec {
+  uint32_t point_id;
+  uint32_t polygon_id;
+  uint32_t contains;
+};
+
+size_t job_row_bytes(uint32_t kind);
+// Upper bound of rows a job can emit (PIP: `pip_cap` per point).
+uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_cap);
+
+constexpr uint32_t kPipHitCap = 16;   // containing polygons kept per point in a batch
+constexpr uint32_t kStatusRowOverflow = 1u;
+constexpr uint32_t kStatusPipOverflow = 2u;
+
+struct JobCounters {
+  uint32_t rows;
+  uint32_t status;
+  uint32_t ambiguous;
+  uint32_t pad;
+};
+
+struct DeviceJob {
+  SceneView scene;        // root / end delimit the job's instance subtree
+  const float* queries;   // FP32 queries, `qstride` floats each
+  const uint32_t* order;  // batch traversal order (job-local index per position) or null
+  void* rows;
+  JobCounters* counters;
+  uint64_t row_capacity;
+  double radius;          // +inf for KNN
+  uint32_t kind;
+  uint32_t nq;
+  uint32_t k;
+  uint32_t qstride;       // 2 (2-D points), 3 (3-D points) or 7 (rays)
+  uint32_t order_base;
+  uint32_t pad;
+};
+
+struct WorkItem {
+  uint32_t job;
+  uint32_t first;  // first position in the job
+  uint32_t count;  // <= 64
+  uint32_t pad;
+};
+
+struct HostJob {
+  SceneView scene;
+  uint32_t kind = kJobFixedRadius;
+  uint32_t nq = 0;
+  const float* queries = nullptr;
+  uint32_t qstride = 3;
+  double radius = 0.0;
+  uint32_t k = 1;
+  void* rows = nullptr;
+  uint64_t row_capacity = 0;
+};
+
+// One engine drives one batch at a time on its stream; reuse it only after
+// the previous batch has completed (done_event / stream sync).
+class BatchEngine {
+ public:
+  explicit BatchEngine(hipStream_t stream);
+  ~BatchEngine();
+  BatchEngine(const BatchEngine&) = delete;
+  BatchEngine& operator=(const BatchEngine&) = delete;
+
+  void reserve(size_t max_jobs, size_t max_queries);
+  // Enqueues the whole batch; returns immediately.
+  void enqueue(const std::vector<HostJob>& jobs, bool morton_sort);
+  // Valid once the batch has completed.
+  const JobCounters* counters() const { return host_counters_; }
+  hipStream_t stream() const { return stream_; }
+  hipEvent_t done_event() const { return done_; }
+  // node / primitive visit counters of the last batch (when stats enabled)
+  void enable_stats(bool on) { stats_on_ = on; }
+  unsigned long long* device_stats() { return stats_.get(); }
+  // per-family kernel times of the last batch, from events on the stream
+  // (points incl. the exact fix-up, ray any-hit, ray hit-count, ray closest,
+  // PIP); valid once the batch has completed
+  void enable_timing(bool on) { timing_on_ = on; }
+  void family_seconds(double out[5]) const;
+
+ private:
+  hipStream_t stream_;
+  hipEvent_t done_ = nullptr;
+  size_t cap_jobs_ = 0, cap_queries_ = 0, cap_items_ = 0;
+  DeviceJob* host_jobs_ = nullptr;       // pinned
+  WorkItem* host_items_ = nullptr;       // pinned
+  uint64_t* host_starts_ = nullptr;      // pinned
+  JobCounters* host_counters_ = nullptr; // pinned
+  DeviceArray<DeviceJob> jobs_;
+  DeviceArray<WorkItem> items_;
+  DeviceArray<uint64_t> starts_;
+  DeviceArray<JobCounters> counters_;
+  DeviceArray<uint32_t> item_counters_;  // one per launch
+  DeviceArray<uint64_t> keys_;
+  DeviceArray<uint32_t> order_;
+  DeviceArray<uint4> amb_;               // (job, query, row base, count)
+  DeviceArray<uint32_t> amb_count_;
+  DeviceArray<unsigned long long> stats_;
+  SortWorkspace sort_ws_;
+  bool stats_on_ = false;
+  bool timing_on_ = false;
+  bool timed_[5] = {false, false, false, false, false};
+  hipEvent_t fam_start_[5] = {}, fam_stop_[5] = {};
+  int cus_ = 256;
+};
+
+}  // namespace rtdl

@@ -218,6 +218,33 @@ __global__ void permute_f64_kernel(const double* src, const uint32_t* perm, size
   if (i < n) dst[i] = src[perm[i]];
 }
 
+__global__ void widen_points_kernel(const float* in, size_t n, const uint64_t* offs,
+                                    uint32_t n_inst, double* coords, uint32_t* ids) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  coords[3 * i] = in[3 * i];
+  coords[3 * i + 1] = in[3 * i + 1];
+  coords[3 * i + 2] = in[3 * i + 2];
+  ids[i] = (uint32_t)(i - offs[owner_of(offs, n_inst, i)]);
+}
+
+__global__ void widen_polygons_kernel(const uint32_t* voff, const uint32_t* vcnt, size_t n,
+                                      const uint64_t* offs, uint32_t n_inst, uint4* refs) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  refs[i] = make_uint4(voff[i], vcnt[i], (uint32_t)(i - offs[owner_of(offs, n_inst, i)]), 0u);
+}
+
+__global__ void instance_ids_kernel(size_t n, const uint64_t* offs, uint32_t n_inst, uint32_t* ids) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ids[i] = (uint32_t)(i - offs[owner_of(offs, n_inst, i)]);
+}
+
+__global__ void widen_f32_kernel(const float* in, size_t n, double* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
 __global__ void fp64_selftest_kernel(const double* a, const double* b, size_t n, double* s,
                                      double* d) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -343,6 +370,28 @@ void permute_f64(const double* src, const uint32_t* perm, size_t n, double* dst,
                  hipStream_t stream) {
   if (!n) return;
   permute_f64_kernel<<<RTDL_GRID(n)>>>(src, perm, n, dst);
+  RTDL_LAUNCH_CHECK();
+}
+void widen_points(const float* in, size_t n, const uint64_t* offs, uint32_t n_inst, double* coords,
+                  uint32_t* ids, hipStream_t stream) {
+  if (!n) return;
+  widen_points_kernel<<<RTDL_GRID(n)>>>(in, n, offs, n_inst, coords, ids);
+  RTDL_LAUNCH_CHECK();
+}
+void widen_polygons(const uint32_t* voff, const uint32_t* vcnt, size_t n, const uint64_t* offs,
+                    uint32_t n_inst, uint4* refs, hipStream_t stream) {
+  if (!n) return;
+  widen_polygons_kernel<<<RTDL_GRID(n)>>>(voff, vcnt, n, offs, n_inst, refs);
+  RTDL_LAUNCH_CHECK();
+}
+void widen_f32(const float* in, size_t n, double* out, hipStream_t stream) {
+  if (!n) return;
+  widen_f32_kernel<<<RTDL_GRID(n)>>>(in, n, out);
+  RTDL_LAUNCH_CHECK();
+}
+void instance_ids(size_t n, const uint64_t* offs, uint32_t n_inst, uint32_t* ids, hipStream_t stream) {
+  if (!n) return;
+  instance_ids_kernel<<<RTDL_GRID(n)>>>(n, offs, n_inst, ids);
   RTDL_LAUNCH_CHECK();
 }
 void fp64_selftest(const double* a, const double* b, size_t n, double* out_sqrt, double* out_div,

@@ -64,6 +64,16 @@ void permute_u32(const uint32_t* src, const uint32_t* perm, size_t n, uint32_t* 
 void permute_f64(const double* src, const uint32_t* perm, size_t n, double* dst,
                  hipStream_t stream);
 
+// FP32 device inputs widened to the canonical FP64 form; ids are
+// instance-local indices (offs: n_inst + 1 device entries).
+void widen_points(const float* in, size_t n, const uint64_t* offs, uint32_t n_inst, double* coords,
+                  uint32_t* ids, hipStream_t stream);
+void widen_polygons(const uint32_t* voff, const uint32_t* vcnt, size_t n, const uint64_t* offs,
+                    uint32_t n_inst, uint4* refs, hipStream_t stream);
+void widen_f32(const float* in, size_t n, double* out, hipStream_t stream);
+// ids[i] = i - offs[instance of i]
+void instance_ids(size_t n, const uint64_t* offs, uint32_t n_inst, uint32_t* ids, hipStream_t stream);
+
 // FP64 self-test: out_sqrt[i] = sqrt(a[i]), out_div[i] = a[i] / b[i].
 void fp64_selftest(const double* a, const double* b, size_t n, double* out_sqrt, double* out_div,
                    hipStream_t stream);

@@ -184,6 +184,36 @@ void scan_impl(const Tin* in, uint64_t* out, size_t n, hipStream_t stream) {
   RTDL_HIP(hipStreamSynchronize(stream));
 }
 
+__global__ void scan_zero_kernel(uint64_t* out) { out[0] = 0; }
+
+template <class Tin>
+void scan_async_level(const Tin* in, uint64_t* out, size_t n, ScanWorkspace& ws, size_t level,
+                      hipStream_t stream) {
+  if (n == 0) {
+    scan_zero_kernel<<<1, 1, 0, stream>>>(out);
+    RTDL_LAUNCH_CHECK();
+    return;
+  }
+  require(level < ws.tiles.size(), "scan workspace too small");
+  const size_t tiles = (n + kScanTile - 1) / kScanTile;
+  require(tiles <= ws.tiles[level], "scan workspace too small");
+  scan_tile_kernel<Tin><<<(unsigned)tiles, kScanBlock, 0, stream>>>(in, out, n, ws.sums[level]);
+  RTDL_LAUNCH_CHECK();
+  if (tiles == 1) {
+    scan_total_kernel<<<1, 1, 0, stream>>>(out, n, ws.sums[level]);
+    RTDL_LAUNCH_CHECK();
+  } else {
+    scan_async_level<uint64_t>(ws.sums[level], ws.offs[level], tiles, ws, level + 1, stream);
+    scan_add_kernel<<<(unsigned)tiles, kScanBlock, 0, stream>>>(out, n, ws.offs[level], tiles);
+    RTDL_LAUNCH_CHECK();
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* out, size_t n, uint32_t value) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = value;
+}
+
 __global__ void iota_kernel(uint32_t* out, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (uint32_t)i;
@@ -227,6 +257,79 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* values, size_t n, int bits, hipS
 
 void exclusive_scan(const uint32_t* in, uint64_t* out, size_t n, hipStream_t stream) {
   scan_impl<uint32_t>(in, out, n, stream);
+}
+
+void ScanWorkspace::reserve(size_t max_n) {
+  if (max_n <= capacity_ && !tiles.empty()) return;
+  tiles.clear();
+  size_t n = max_n ? max_n : 1;
+  size_t total = 0;
+  for (;;) {
+    const size_t t = (n + kScanTile - 1) / kScanTile;
+    tiles.push_back(t);
+    total += 2 * t + 1;
+    if (t <= 1) break;
+    n = t;
+  }
+  storage_.resize(total);
+  sums.clear();
+  offs.clear();
+  uint64_t* p = storage_.get();
+  for (size_t t : tiles) {
+    sums.push_back(p);
+    p += t;
+    offs.push_back(p);
+    p += t + 1;
+  }
+  capacity_ = max_n;
+}
+
+void exclusive_scan_async(const uint32_t* in, uint64_t* out, size_t n, ScanWorkspace& ws,
+                          hipStream_t stream) {
+  require(n <= ws.capacity(), "scan workspace too small");
+  scan_async_level<uint32_t>(in, out, n, ws, 0, stream);
+}
+
+void SortWorkspace::reserve(size_t max_n) {
+  if (max_n <= capacity_ && keys_tmp.size()) return;
+  const size_t n = max_n ? max_n : 1;
+  const size_t tiles = (n + kSortTile - 1) / kSortTile;
+  keys_tmp.resize(n);
+  vals_tmp.resize(n);
+  hist.resize(256 * tiles);
+  offs.resize(256 * tiles + 1);
+  scan.reserve(256 * tiles);
+  capacity_ = max_n;
+}
+
+void radix_sort_pairs_async(uint64_t* keys, uint32_t* values, size_t n, int bits,
+                            SortWorkspace& ws, hipStream_t stream) {
+  if (n <= 1 || bits <= 0) return;
+  require(n <= ws.capacity(), "sort workspace too small");
+  const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  int passes = (bits + 7) / 8;
+  if (passes & 1) ++passes;  // an extra pass over zero digits is a stable copy
+  uint64_t* kin = keys;
+  uint32_t* vin = values;
+  uint64_t* kout = ws.keys_tmp.get();
+  uint32_t* vout = ws.vals_tmp.get();
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = 8 * pass;
+    radix_hist_kernel<<<tiles, kSortBlock, 0, stream>>>(kin, n, shift, ws.hist.get(), tiles);
+    RTDL_LAUNCH_CHECK();
+    exclusive_scan_async(ws.hist.get(), ws.offs.get(), (size_t)256 * tiles, ws.scan, stream);
+    radix_scatter_kernel<<<tiles, kSortBlock, 0, stream>>>(kin, vin, kout, vout, n, shift,
+                                                           ws.offs.get(), tiles);
+    RTDL_LAUNCH_CHECK();
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+}
+
+void fill_u32(uint32_t* out, size_t n, uint32_t value, hipStream_t stream) {
+  if (!n) return;
+  fill_u32_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(out, n, value);
+  RTDL_LAUNCH_CHECK();
 }
 
 void iota(uint32_t* out, size_t n, hipStream_t stream) {

@@ -5,6 +5,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
+
+#include "common.hpp"
 
 namespace rtdl {
 
@@ -22,5 +25,50 @@ void iota(uint32_t* out, size_t n, hipStream_t stream);
 
 // Returns true when keys[i] <= keys[i + 1] for all i (device reduction).
 bool is_nondecreasing(const uint32_t* keys, size_t n, hipStream_t stream);
+
+// ---- asynchronous variants (no host synchronisation, no allocation, no
+// memset nodes), safe to enqueue back to back and to capture in a hipGraph.
+// The workspace must stay alive until the stream has executed the work.
+
+// Device memory for exclusive_scan_async over up to `max_n` inputs.
+class ScanWorkspace {
+ public:
+  void reserve(size_t max_n);
+  size_t capacity() const { return capacity_; }
+  // per level: tile sums and their exclusive prefix
+  std::vector<uint64_t*> sums, offs;
+  std::vector<size_t> tiles;
+
+ private:
+  size_t capacity_ = 0;
+  DeviceArray<uint64_t> storage_;
+};
+
+void exclusive_scan_async(const uint32_t* in, uint64_t* out, size_t n, ScanWorkspace& ws,
+                          hipStream_t stream);
+
+// Device memory for radix_sort_pairs_async over up to `max_n` pairs.
+class SortWorkspace {
+ public:
+  void reserve(size_t max_n);
+  size_t capacity() const { return capacity_; }
+  DeviceArray<uint64_t> keys_tmp;
+  DeviceArray<uint32_t> vals_tmp;
+  DeviceArray<uint32_t> hist;
+  DeviceArray<uint64_t> offs;
+  ScanWorkspace scan;
+
+ private:
+  size_t capacity_ = 0;
+};
+
+// Sorts on the low `bits` bits; an even number of 8-bit passes is always
+// used so that the result lands in keys/values without a copy.
+void radix_sort_pairs_async(uint64_t* keys, uint32_t* values, size_t n, int bits,
+                            SortWorkspace& ws, hipStream_t stream);
+
+// out[i] = value for i in [0, n) (a fill kernel; graph-safe replacement for
+// hipMemsetAsync).
+void fill_u32(uint32_t* out, size_t n, uint32_t value, hipStream_t stream);
 
 }  // namespace rtdl

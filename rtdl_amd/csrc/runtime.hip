@@ -291,6 +291,71 @@ void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* ver
   read_root_box(p);
 }
 
+void prepare_point_instances(Prepared& p, const float* coords, size_t n,
+                             const std::vector<uint64_t>& offsets) {
+  require(n == 0 || coords != nullptr, "coords must not be null when count is nonzero");
+  require(offsets.size() >= 2 && offsets.front() == 0 && offsets.back() == n,
+          "instance offsets must start at 0 and end at the point count");
+  hipStream_t s = p.stream;
+  p.type = SceneType::Points3D;
+  auto d_offs = upload(offsets.data(), offsets.size(), s);
+  DeviceArray<double> c64(3 * n);
+  DeviceArray<uint32_t> ids(n);
+  widen_points(coords, n, d_offs.get(), (uint32_t)(offsets.size() - 1), c64.get(), ids.get(), s);
+  build_scene(p.scene, PrimKind::Points, 3, n, c64.get(), ids.get(), nullptr, nullptr, 0, offsets,
+              kPointLeaf, s);
+  read_root_box(p);
+}
+
+void prepare_triangle_instances(Prepared& p, const float* coords, size_t n,
+                                const std::vector<uint64_t>& offsets) {
+  require(n == 0 || coords != nullptr, "coords must not be null when count is nonzero");
+  require(offsets.size() >= 2 && offsets.front() == 0 && offsets.back() == n,
+          "instance offsets must start at 0 and end at the triangle count");
+  hipStream_t s = p.stream;
+  p.type = SceneType::Triangles3D;
+  auto d_offs = upload(offsets.data(), offsets.size(), s);
+  DeviceArray<double> c64(9 * n);
+  DeviceArray<uint32_t> ids(n);
+  widen_f32(coords, 9 * n, c64.get(), s);
+  instance_ids(n, d_offs.get(), (uint32_t)(offsets.size() - 1), ids.get(), s);
+  build_scene(p.scene, PrimKind::Triangles, 3, n, c64.get(), ids.get(), nullptr, nullptr, 0, offsets,
+              kTriangleLeaf, s);
+  read_root_box(p);
+}
+
+void prepare_polygon_instances(Prepared& p, const uint32_t* voff, const uint32_t* vcnt, size_t n,
+                               const float* verts_xy, size_t n_verts,
+                               const std::vector<uint64_t>& offsets) {
+  require(n == 0 || (voff != nullptr && vcnt != nullptr),
+          "polygon arrays must not be null when count is nonzero");
+  require(n_verts == 0 || verts_xy != nullptr, "vertex array must not be null");
+  require(offsets.size() >= 2 && offsets.front() == 0 && offsets.back() == n,
+          "instance offsets must start at 0 and end at the polygon count");
+  hipStream_t s = p.stream;
+  std::vector<uint32_t> h_off(n), h_cnt(n);
+  if (n) {
+    RTDL_HIP(hipMemcpyAsync(h_off.data(), voff, n * 4, hipMemcpyDeviceToHost, s));
+    RTDL_HIP(hipMemcpyAsync(h_cnt.data(), vcnt, n * 4, hipMemcpyDeviceToHost, s));
+    RTDL_HIP(hipStreamSynchronize(s));
+  }
+  for (size_t i = 0; i < n; ++i) {
+    require(h_cnt[i] >= 3, "polygon " + std::to_string(i) + " needs at least 3 vertices");
+    require((uint64_t)h_off[i] + h_cnt[i] <= n_verts,
+            "polygon " + std::to_string(i) + " vertex range exceeds the vertex count");
+  }
+  p.type = SceneType::Polygons;
+  p.n_polygons = n;
+  auto d_offs = upload(offsets.data(), offsets.size(), s);
+  p.refs_input.resize(n ? n : 1);
+  widen_polygons(voff, vcnt, n, d_offs.get(), (uint32_t)(offsets.size() - 1), p.refs_input.get(), s);
+  DeviceArray<double> v64(2 * n_verts);
+  widen_f32(verts_xy, 2 * n_verts, v64.get(), s);
+  build_scene(p.scene, PrimKind::Polygons, 2, n, nullptr, nullptr, p.refs_input.get(), v64.get(),
+              n_verts, offsets, kPolygonLeaf, s);
+  read_root_box(p);
+}
+
 // -------------------------------------------------------------- neighbours --
 
 void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborMode mode,
@@ -332,8 +397,19 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
   if (list_path) {
     cand_d.resize((size_t)nq * k);
     cand_id.resize((size_t)nq * k);
-    launch_collect(view, q.get(), order.get(), nq, search_radius, k, false, cand_d.get(),
-                   cand_id.get(), count.get(), nullptr, nullptr, 256, s);
+    DeviceArray<uint32_t> amb_list(nq), amb_count(1);
+    RTDL_HIP(hipMemsetAsync(amb_count.get(), 0, sizeof(uint32_t), s));
+    launch_collect_packed(view, q.get(), order.get(), nq, search_radius, k, cand_d.get(),
+                          cand_id.get(), count.get(), amb_list.get(), amb_count.get(), nullptr,
+                          256, s);
+    uint32_t n_amb = 0;
+    RTDL_HIP(hipMemcpyAsync(&n_amb, amb_count.get(), sizeof(n_amb), hipMemcpyDeviceToHost, s));
+    RTDL_HIP(hipStreamSynchronize(s));
+    p.last_ambiguous = n_amb;
+    // queries whose FP32 selection keys tie at the boundary are recomputed
+    // with the exact FP64 list (same slots, same counts)
+    launch_collect_exact(view, q.get(), amb_list.get(), n_amb, search_radius, k, false,
+                         cand_d.get(), cand_id.get(), count.get(), nullptr, s);
   } else {
     // Keep the 64 nearest per query and count everything inside the radius;
     // queries with more than 64 candidates are collected in full and sorted.
@@ -341,8 +417,8 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
     DeviceArray<uint32_t> total(nq);
     DeviceArray<double> list_d((size_t)nq * kMaxListK);
     DeviceArray<uint32_t> list_id((size_t)nq * kMaxListK);
-    launch_collect(view, q.get(), order.get(), nq, search_radius, kMaxListK, true, list_d.get(),
-                   list_id.get(), count.get(), total.get(), nullptr, 256, s);
+    launch_collect_exact(view, q.get(), order.get(), nq, search_radius, kMaxListK, true,
+                         list_d.get(), list_id.get(), count.get(), total.get(), s);
     std::vector<uint32_t> host_total(nq);
     RTDL_HIP(hipMemcpyAsync(host_total.data(), total.get(), nq * sizeof(uint32_t),
                             hipMemcpyDeviceToHost, s));

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "queries.hpp"
 #include "scene.hpp"
@@ -43,6 +44,7 @@ struct Prepared {
   float root_hi[3] = {0, 0, 0};
   DeviceArray<uint4> refs_input;  // polygons: refs in input order
   size_t n_polygons = 0;
+  uint32_t last_ambiguous = 0;  // queries of the last call recomputed exactly
   bool owns_stream = true;
   // own_stream = false borrows the calling thread's stream (per-call path).
   explicit Prepared(bool own_stream = true);
@@ -56,6 +58,15 @@ void prepare_points(Prepared& p, const void* raw, size_t n, int dim);
 void prepare_triangles(Prepared& p, const void* raw, size_t n);
 void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* verts_xy,
                       size_t n_doubles);
+
+// Multi-instance builders from device FP32 data (ids are instance-local).
+void prepare_point_instances(Prepared& p, const float* coords, size_t n,
+                             const std::vector<uint64_t>& offsets);
+void prepare_triangle_instances(Prepared& p, const float* coords, size_t n,
+                                const std::vector<uint64_t>& offsets);
+void prepare_polygon_instances(Prepared& p, const uint32_t* voff, const uint32_t* vcnt, size_t n,
+                               const float* verts_xy, size_t n_verts,
+                               const std::vector<uint64_t>& offsets);
 
 // Query pipelines; rows are malloc'ed host arrays (freed by free_rows).
 enum class NeighborMode : int { FixedRadius = 0, Knn = 1, BoundedKnn = 2 };

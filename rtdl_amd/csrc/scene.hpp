@@ -33,7 +33,8 @@ struct SceneView {
   uint32_t end;  // traversal stops when it reaches this node (escape of root)
   uint32_t n_prims;
   uint32_t n_nodes;
-  float m;       // >= every |coordinate| in the scene, rounded up
+  float m;             // >= every |coordinate| in the scene, rounded up
+  const float* m_dev;  // when set, M is read from device memory instead
 };
 
 // A set of instances built into one LBVH: instance i owns primitives
@@ -76,6 +77,7 @@ class Scene {
     v.n_prims = (uint32_t)n_prims;
     v.n_nodes = (uint32_t)nodes.size();
     v.m = m;
+    v.m_dev = nullptr;
     return v;
   }
   size_t device_bytes() const {

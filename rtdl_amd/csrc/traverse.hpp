@@ -1,0 +1,441 @@
+// Per-lane traversal + refine bodies shared by the per-call kernels and the
+// batched / served persistent kernels.
+//
+// All traversals walk the rope-linked LBVH in preorder (no stack).  Boxes and
+// primitives are prefiltered in FP32 against conservative bounds
+// (frn_threshold2, segment_box); every surviving candidate is decided in FP64
+// with the reference's operation order (the library is compiled with
+// -ffp-contract=off).
+#pragma once
+
+#include "common.hpp"
+#include "scene.hpp"
+
+namespace rtdl {
+
+struct LaneStats {
+  unsigned long long nodes = 0;
+  unsigned long long prims = 0;
+};
+
+__device__ inline void flush_lane_stats(unsigned long long* stats, const LaneStats& st) {
+  unsigned long long nodes = st.nodes, prims = st.prims;
+  for (int d = 32; d >= 1; d >>= 1) {
+    nodes += __shfl_xor(nodes, d, 64);
+    prims += __shfl_xor(prims, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&stats[0], nodes);
+    atomicAdd(&stats[1], prims);
+  }
+}
+
+__device__ inline float scene_m(const SceneView& s) { return s.m_dev ? *s.m_dev : s.m; }
+
+// -------------------------------------------------------- point candidates --
+
+// FP64 squared distance of slot `slot` from (qx, qy, qz), reference.py:882-890:
+// dx = search - query, then dx*dx + dy*dy + dz*dz left to right.
+__device__ inline double point_d2(const SceneView& s, uint32_t slot, const float4& p, double qx,
+                                  double qy, double qz) {
+  double px, py, pz;
+  if (s.exact) {
+    px = s.exact[3 * (size_t)slot];
+    py = s.exact[3 * (size_t)slot + 1];
+    pz = s.exact[3 * (size_t)slot + 2];
+  } else {
+    px = p.x;
+    py = p.y;
+    pz = p.z;
+  }
+  const double dx = px - qx, dy = py - qy, dz = pz - qz;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Visits every leaf primitive whose FP32 distance passes `thr2` (which the
+// visitor may shrink).  visit(slot, p, d2f) -> void.
+template <bool STATS, class Visit>
+__device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t end, const float qf[3],
+                                   float& thr2, LaneStats& st, Visit&& visit) {
+  uint32_t node = root;
+  // a preorder walk visits each node at most once; the guard only bounds the
+  // loop should the tree ever be malformed
+  for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
+    const BvhNode nd = s.nodes[node];
+    if (STATS) ++st.nodes;
+    const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
+    const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
+    if (!(box_dist2(qf, lo, hi) <= thr2)) {
+      node = nd.escape;
+      continue;
+    }
+    if (!node_is_leaf(nd.link)) {
+      node = nd.link;
+      continue;
+    }
+    const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t slot = first + j;
+      const float4 p = s.prims[slot];
+      if (STATS) ++st.prims;
+      const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
+      const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+      if (d2f <= thr2) visit(slot, p, d2f);
+    }
+    node = nd.escape;
+  }
+}
+
+// Exact list: K (distance, id) entries in registers, k of them live in slots
+// [K - k, K); slots below hold a -1 sentinel, so the k-th smallest is always
+// slot K - 1 and every index is static.
+template <int K>
+struct ExactTopK {
+  double d[K];
+  uint32_t id[K];
+
+  __device__ inline void init(uint32_t k) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      d[i] = (i < K - (int)k) ? -1.0 : __builtin_inf();
+      id[i] = 0xFFFFFFFFu;
+    }
+  }
+  __device__ inline bool accepts(double dist, uint32_t ident) const {
+    return key_less(dist, ident, d[K - 1], id[K - 1]);
+  }
+  __device__ inline void insert(double dist, uint32_t ident) {
+    d[K - 1] = dist;
+    id[K - 1] = ident;
+#pragma unroll
+    for (int i = K - 1; i > 0; --i) {
+      const bool sw = key_less(d[i], id[i], d[i - 1], id[i - 1]);
+      const double td = d[i];
+      const uint32_t ti = id[i];
+      d[i] = sw ? d[i - 1] : td;
+      id[i] = sw ? id[i - 1] : ti;
+      d[i - 1] = sw ? td : d[i - 1];
+      id[i - 1] = sw ? ti : id[i - 1];
+    }
+  }
+  __device__ inline bool full() const { return d[K - 1] < __builtin_inf(); }
+};
+
+// Exact k nearest (distance, id) within `radius`; with COUNT_ALL the radius
+// never shrinks and `inside` counts every candidate within it.
+template <int K, bool COUNT_ALL, bool STATS>
+__device__ inline void collect_exact(const SceneView& s, uint32_t root, uint32_t end, double qx,
+                                     double qy, double qz, double radius, uint32_t k,
+                                     ExactTopK<K>& list, uint32_t& inside, LaneStats& st) {
+  const float qf[3] = {(float)qx, (float)qy, (float)qz};
+  const float m = coord_bound(scene_m(s), qx, qy, qz);
+  const double radius_sq = radius * radius;  // reference.py:357
+  float thr2 = frn_threshold2(radius, m);
+  list.init(k);
+  inside = 0;
+  walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float) {
+    const double d2 = point_d2(s, slot, p, qx, qy, qz);
+    if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
+    ++inside;
+    const double dist = sqrt(d2);
+    const uint32_t ident = __float_as_uint(p.w);
+    if (list.accepts(dist, ident)) {
+      list.insert(dist, ident);
+      if (!COUNT_ALL && list.full()) thr2 = frn_threshold2(fmin(radius, list.d[K - 1]), m);
+    }
+  });
+}
+
+// Packed list: each entry is (FP32 bits of fl32(d2) << 32) | slot, so the
+// whole (key, slot) order is one 64-bit compare and an entry costs two
+// registers.  fl32 is monotone, so a strictly smaller key means a strictly
+// smaller FP64 distance (one FP32 ulp of d2 is far above FP64 sqrt
+// resolution); only equal keys can be ordered differently from the exact
+// (distance, id) order.  The list therefore reports `ambiguous` when a key
+// equal to the final k-th key was dropped or when two kept keys are equal;
+// such queries are recomputed with the exact list.
+template <int K>
+struct PackedTopK {
+  uint64_t e[K];
+
+  __device__ inline void init(uint32_t k) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) e[i] = (i < K - (int)k) ? 0ull : ~0ull;
+  }
+  __device__ inline uint64_t insert(uint64_t v) {  // returns the evicted entry
+    const uint64_t out = e[K - 1];
+    e[K - 1] = v;
+#pragma unroll
+    for (int i = K - 1; i > 0; --i) {
+      const uint64_t a = e[i - 1], b = e[i];
+      e[i - 1] = a < b ? a : b;
+      e[i] = a < b ? b : a;
+    }
+    return out;
+  }
+  __device__ inline bool full() const { return e[K - 1] != ~0ull; }
+};
+
+__device__ inline uint64_t pack_key(double d2, uint32_t slot) {
+  return ((uint64_t)__float_as_uint(__double2float_rn(d2)) << 32) | slot;
+}
+__device__ inline uint32_t key_bits(uint64_t e) { return (uint32_t)(e >> 32); }
+
+template <int K, bool STATS>
+__device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_t end, double qx,
+                                      double qy, double qz, double radius, uint32_t k,
+                                      PackedTopK<K>& list, LaneStats& st) {
+  const float qf[3] = {(float)qx, (float)qy, (float)qz};
+  const float m = coord_bound(scene_m(s), qx, qy, qz);
+  const double radius_sq = radius * radius;
+  float thr2 = frn_threshold2(radius, m);
+  list.init(k);
+  uint32_t drop_min = 0xFFFFFFFFu;  // smallest key bits of any dropped candidate
+  walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float) {
+    const double d2 = point_d2(s, slot, p, qx, qy, qz);
+    if (d2 > radius_sq) return;
+    const uint64_t v = pack_key(d2, slot);
+    if (v < list.e[K - 1]) {
+      const uint64_t out = list.insert(v);
+      if (out != ~0ull) drop_min = min(drop_min, key_bits(out));
+      if (list.full()) {
+        // every candidate whose key could equal the k-th key stays inside
+        const double kth = (double)__uint_as_float(key_bits(list.e[K - 1]));
+        thr2 = frn_threshold2(fmin(radius, sqrt(kth) * 1.0000002384185791015625), m);
+      }
+    } else {
+      drop_min = min(drop_min, key_bits(v));
+    }
+  });
+  bool ambiguous = list.full() && drop_min == key_bits(list.e[K - 1]);
+#pragma unroll
+  for (int i = 1; i < K; ++i) {
+    if (i > K - (int)k && list.e[i] != ~0ull && key_bits(list.e[i]) == key_bits(list.e[i - 1]))
+      ambiguous = true;
+  }
+  return ambiguous;
+}
+
+// ------------------------------------------------------------------- rays --
+
+// reference.py:810-846, Moller-Trumbore in this operation order.
+__device__ inline bool mt_hit(double ox, double oy, double oz, double dx, double dy, double dz,
+                              double tmax, const double v[9], double* t_out) {
+  const double e1x = v[3] - v[0];
+  const double e1y = v[4] - v[1];
+  const double e1z = v[5] - v[2];
+  const double e2x = v[6] - v[0];
+  const double e2y = v[7] - v[1];
+  const double e2z = v[8] - v[2];
+  const double px = dy * e2z - dz * e2y;
+  const double py = dz * e2x - dx * e2z;
+  const double pz = dx * e2y - dy * e2x;
+  const double det = e1x * px + e1y * py + e1z * pz;
+  if (fabs(det) <= 1.0e-8) return false;
+  const double inv = 1.0 / det;
+  const double tx = ox - v[0];
+  const double ty = oy - v[1];
+  const double tz = oz - v[2];
+  const double u = (tx * px + ty * py + tz * pz) * inv;
+  if (u < 0.0 || u > 1.0) return false;
+  const double qx = ty * e1z - tz * e1y;
+  const double qy = tz * e1x - tx * e1z;
+  const double qz = tx * e1y - ty * e1x;
+  const double vv = (dx * qx + dy * qy + dz * qz) * inv;
+  if (vv < 0.0 || (u + vv) > 1.0) return false;
+  const double t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+  if (t >= 0.0 && t <= tmax) {
+    *t_out = t;
+    return true;
+  }
+  return false;
+}
+
+// Conservative FP32 test of the segment o + s * D, s in [0, slim], against
+// the box grown by `pad` on every side.  `pad` (2^-18 of the coordinate
+// scale) dominates every FP32 rounding of o, D, the box and the slab
+// parameters; the parameters get a further (1 + 2^-20) factor.
+__device__ inline bool segment_box(const float o[3], const float D[3], const float inv[3],
+                                   const float lo[3], const float hi[3], float pad, float slim) {
+  float tn = -9.5367431640625e-07f;
+  float tf = slim * 1.00000095367431640625f + 9.5367431640625e-07f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float l = lo[a] - pad, h = hi[a] + pad;
+    if (D[a] == 0.0f) {
+      if (o[a] < l || o[a] > h) return false;
+      continue;
+    }
+    const float t0 = (l - o[a]) * inv[a], t1 = (h - o[a]) * inv[a];
+    const float a0 = fminf(t0, t1);
+    const float a1 = fmaxf(t0, t1);
+    tn = fmaxf(tn, a0 - fabsf(a0) * 9.5367431640625e-07f);
+    tf = fminf(tf, a1 + fabsf(a1) * 9.5367431640625e-07f);
+  }
+  return tn <= tf;
+}
+
+struct RayResult {
+  uint32_t hits = 0;
+  double best_t = __builtin_nan("");
+  uint32_t best_id = 0;
+};
+
+// MODE 0 any-hit (stops at the first hit), 1 hit count, 2 closest hit.
+template <int MODE, bool STATS>
+__device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_t end, double ox,
+                                      double oy, double oz, double dx, double dy, double dz,
+                                      double tmax, LaneStats& st) {
+  RayResult res;
+  // t in [0, tmax] is impossible for a negative or NaN tmax (reference.py:844)
+  if (!(tmax >= 0.0) || root == kEndNode) return res;
+  const bool bounded = tmax < __builtin_inf();
+  const float o32[3] = {(float)ox, (float)oy, (float)oz};
+  float D32[3] = {0.0f, 0.0f, 0.0f};
+  float inv[3] = {0.0f, 0.0f, 0.0f};
+  float pad = __builtin_inff();
+  if (bounded) {
+    const double D[3] = {dx * tmax, dy * tmax, dz * tmax};
+    const double omax = fmax(fmax(fabs(ox), fabs(oy)), fabs(oz));
+    const double dmax = fmax(fmax(fabs(D[0]), fabs(D[1])), fabs(D[2]));
+    pad = __double2float_ru(((double)scene_m(s) + omax + dmax) * 3.814697265625e-06);  // 2^-18
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      D32[a] = (float)D[a];
+      inv[a] = D32[a] != 0.0f ? 1.0f / D32[a] : 0.0f;
+    }
+  }
+  float slim = 1.0f;
+  uint32_t node = root;
+  bool done = false;
+  for (uint32_t guard = s.n_nodes + 1; node != end && !done && guard; --guard) {
+    const BvhNode nd = s.nodes[node];
+    if (STATS) ++st.nodes;
+    const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
+    const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
+    if (!segment_box(o32, D32, inv, lo, hi, pad, slim)) {
+      node = nd.escape;
+      continue;
+    }
+    if (!node_is_leaf(nd.link)) {
+      node = nd.link;
+      continue;
+    }
+    const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t slot = first + j;
+      if (STATS) ++st.prims;
+      double v[9];
+      const float4 a = s.prims[3 * (size_t)slot];
+      const uint32_t ident = __float_as_uint(a.w);
+      if (s.exact) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) v[e] = s.exact[9 * (size_t)slot + e];
+      } else {
+        const float4 b = s.prims[3 * (size_t)slot + 1];
+        const float4 c = s.prims[3 * (size_t)slot + 2];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z;
+        v[3] = b.x; v[4] = b.y; v[5] = b.z;
+        v[6] = c.x; v[7] = c.y; v[8] = c.z;
+      }
+      double th;
+      if (!mt_hit(ox, oy, oz, dx, dy, dz, tmax, v, &th)) continue;
+      ++res.hits;
+      if (MODE == 0) {
+        done = true;
+        break;
+      }
+      if (MODE == 2) {
+        if (!(res.best_t == res.best_t) || th < res.best_t || (th == res.best_t && ident < res.best_id)) {
+          res.best_t = th;
+          res.best_id = ident;
+          if (bounded && tmax > 0.0) slim = fminf(1.0f, __double2float_ru(th / tmax));
+        }
+      }
+    }
+    node = nd.escape;
+  }
+  return res;
+}
+
+// -------------------------------------------------------- point in polygon --
+
+// reference.py:745-768 (`x ** 2` and `x ** 0.5` are evaluated as the
+// correctly rounded x * x and sqrt(x); libm pow can differ from them by one
+// ulp, which can only matter for a point at the 1e-12 tolerance boundary).
+__device__ inline bool point_on_segment(double px, double py, double ax, double ay, double bx,
+                                        double by, double eps_sq) {
+  const double eps = 1.0e-12;
+  const double ex = bx - ax, ey = by - ay;
+  const double length_sq = ex * ex + ey * ey;
+  if (length_sq <= eps_sq) return fabs(px - ax) <= eps && fabs(py - ay) <= eps;
+  const double cross = (px - ax) * (by - ay) - (py - ay) * (bx - ax);
+  const double length = sqrt(length_sq);
+  if (fabs(cross) > eps * length) return false;
+  const double dot = (px - ax) * (bx - ax) + (py - ay) * (by - ay);
+  const double along_eps = eps * length;
+  if (dot < -along_eps) return false;
+  if (dot - length_sq > along_eps) return false;
+  return true;
+}
+
+// reference.py:717-742
+__device__ inline bool point_in_polygon(double x, double y, const double* v, uint32_t n,
+                                        double eps_sq) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t j = (i + 1 == n) ? 0 : i + 1;
+    if (point_on_segment(x, y, v[2 * i], v[2 * i + 1], v[2 * j], v[2 * j + 1], eps_sq)) return true;
+  }
+  bool inside = false;
+  uint32_t j = n - 1;
+  for (uint32_t i = 0; i < n; ++i) {
+    const double xi = v[2 * i], yi = v[2 * i + 1], xj = v[2 * j], yj = v[2 * j + 1];
+    if ((yi > y) != (yj > y)) {
+      double den = yj - yi;
+      if (den == 0.0) den = 1.0e-20;
+      if (x <= (xj - xi) * (y - yi) / den + xi) inside = !inside;
+    }
+    j = i;
+  }
+  return inside;
+}
+
+// Calls hit(slot) for every polygon that contains (x, y) under the
+// reference's rules.  Boxes are grown by more than the on-edge tolerance
+// (1e-12 from the boundary) plus FP32 rounding of the point.
+template <bool STATS, class Hit>
+__device__ inline void walk_pip(const SceneView& s, uint32_t root, uint32_t end, double x, double y,
+                                bool positive_only, double eps_sq, LaneStats& st, Hit&& hit) {
+  const float pf0 = (float)x, pf1 = (float)y;
+  const float m = coord_bound(scene_m(s), x, y, 0.0);
+  const float pad = m * 9.5367431640625e-07f + 1.0e-11f;
+  uint32_t node = root;
+  for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
+    const BvhNode nd = s.nodes[node];
+    if (STATS) ++st.nodes;
+    const bool in = pf0 >= nd.lo[0] - pad && pf0 <= nd.hi[0] + pad && pf1 >= nd.lo[1] - pad &&
+                    pf1 <= nd.hi[1] + pad;
+    if (!in) {
+      node = nd.escape;
+      continue;
+    }
+    if (!node_is_leaf(nd.link)) {
+      node = nd.link;
+      continue;
+    }
+    const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t slot = first + j;
+      if (STATS) ++st.prims;
+      if (positive_only) {  // reference.py:122-125 exact bbox prefilter
+        const double* bb = s.poly_box + 4 * (size_t)slot;
+        if (x < bb[0] || x > bb[2] || y < bb[1] || y > bb[3]) continue;
+      }
+      const uint4 ref = s.poly[slot];
+      if (point_in_polygon(x, y, s.poly_verts + 2 * (size_t)ref.x, ref.y, eps_sq)) hit(slot);
+    }
+    node = nd.escape;
+  }
+}
+
+}  // namespace rtdl

@@ -146,6 +146,14 @@ _SIGNATURES = {
     "rtdl_mi355x_run_prepared_point_in_polygon": ([_VP, _VP, _SZ, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_destroy_prepared": ([_VP], None),
     "rtdl_mi355x_selftest_fp64": ([_VP, _VP, _SZ, _VP, _VP, *_ERR], ctypes.c_int),
+    # batched / served execution (structs passed by address; see rtdl_amd.batch)
+    "rtdl_mi355x_run_batch": ([_VP, _SZ, ctypes.c_uint32, _VP, ctypes.POINTER(ctypes.c_double), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_run_batch_profiled": ([_VP, _SZ, ctypes.c_uint32, _VP, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_uint64), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_prepare_point_instances": ([_VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_prepare_triangle_instances": ([_VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_prepare_polygon_instances": ([_VP, _VP, _SZ, _VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_serve_replay": ([_VP, _SZ, _VP, _SZ, _VP, _VP, _VP, *_ERR], ctypes.c_int),
 }
 
 

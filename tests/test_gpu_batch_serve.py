@@ -1,0 +1,252 @@
+"""GPU parity of the batched and served paths against the oracle.
+
+Batch rows carry job-local query ids and are compacted per job in
+wavefront order, so they are compared after the canonical (query, id)
+ordering the parity contract prescribes; every value must match the oracle
+bit for bit.  The serving test also checks that a request's rows are the
+same when it runs alone and when it runs inside a mixed stream.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import native as oracle
+from rtdl_amd import batch as B
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def _canon(rows, keys):
+    order = np.lexsort(tuple(rows[k] for k in reversed(keys)))
+    return rows[order]
+
+
+def _check_nn(rows, want, ranked):
+    got = _canon(rows, ["query_id", "neighbor_id"])
+    w = want[np.lexsort((want["nid"], want["qid"]))]
+    assert len(got) == len(w), (len(got), len(w))
+    assert np.array_equal(got["query_id"], w["qid"])
+    assert np.array_equal(got["neighbor_id"], w["nid"])
+    assert np.array_equal(got["distance"], w["dist"])
+    if ranked:
+        assert np.array_equal(got["neighbor_rank"], w["rank"])
+
+
+def _points3(rng, n, kind="uniform", centres=None):
+    p = cases.points(rng, n, 3, kind=kind, centres=centres)
+    return p, np.stack([p["x"], p["y"], p["z"]], axis=1)
+
+
+def _prepared_points(s3):
+    import rtdl_amd as rt
+    from rtdl_amd import mi355x_runtime as m
+
+    arr, dim = m.pack_points({"id": np.arange(len(s3), dtype=np.uint32), "x": s3[:, 0], "y": s3[:, 1],
+                              "z": s3[:, 2]})
+    return B.Instances(m._call_prepare("rtdl_mi355x_prepare_points_3d", m._ptr(arr), len(arr)),
+                       np.array([0, len(arr)]), "points")
+
+
+@pytest.mark.parametrize("morton", [True, False])
+def test_batch_mixed_jobs_match_oracle(gpu_ready, morton):
+    _say(f"case: batch mixed jobs morton={morton}")
+    rng = np.random.default_rng(77)
+    centres = rng.random((64, 3))
+    sdict, s3 = _points3(rng, 60000, "mixture", centres)
+    scene = _prepared_points(s3)
+    tris = cases.triangles(rng, 30000, sigma=0.01)
+    tri_scene = B.prepare_triangle_instances(_dev(cases.tri_matrix(tris)), [0, 30000])
+    polys = cases.star_polygons(rng, 800, rmin=0.01, rmax=0.04)
+    refs = polys["refs"]
+    poly_scene = B.prepare_polygon_instances(
+        torch.from_numpy(refs["vertex_offset"].astype(np.int32)).cuda(),
+        torch.from_numpy(refs["vertex_count"].astype(np.int32)).cuda(),
+        _dev(polys["vertices_xy"].reshape(-1, 2)), [0, len(refs)])
+    jobs, wants = [], []
+    for kind, k, radius, n in [(B.JOB_FIXED_RADIUS, 64, 0.03, 3000), (B.JOB_FIXED_RADIUS, 5, 0.05, 100),
+                               (B.JOB_KNN, 8, 0.0, 2500), (B.JOB_KNN, 16, 0.0, 777),
+                               (B.JOB_BOUNDED_KNN, 6, 0.02, 1000)]:
+        qd, q3 = _points3(rng, n, "mixture", centres)
+        jobs.append(B.BatchJob(scene, kind, _dev(q3), radius=radius, k=k))
+        mode = {B.JOB_FIXED_RADIUS: "frn", B.JOB_KNN: "knn", B.JOB_BOUNDED_KNN: "bounded_knn"}[kind]
+        wants.append(("nn", kind != B.JOB_FIXED_RADIUS,
+                      oracle.neighbors(mode, qd, sdict, radius=radius, k=k)))
+    for kind, mode in [(B.JOB_RAY_ANYHIT, "any"), (B.JOB_RAY_HITCOUNT, "count"), (B.JOB_RAY_CLOSEST, "closest")]:
+        r = cases.rays(rng, 4000, tmax=0.3)
+        rm = cases.ray_matrix(r)
+        jobs.append(B.BatchJob(tri_scene, kind, _dev(rm)))
+        wants.append(("ray", mode, oracle.rays_3d(mode, rm, cases.tri_matrix(tris), tris["id"])))
+    pts = cases.points(rng, 20000, 2)
+    jobs.append(B.BatchJob(poly_scene, B.JOB_PIP_POSITIVE, _dev(np.stack([pts["x"], pts["y"]], axis=1))))
+    wants.append(("pip", None, oracle.pip_pairs(pts["x"], pts["y"], refs["vertex_offset"], refs["vertex_count"],
+                                                polys["vertices_xy"], positive_only=True)))
+    results, seconds = B.run_batch(jobs, morton_sort=morton)
+    assert seconds > 0
+    for job, res, (what, extra, want) in zip(jobs, results, wants):
+        assert res["status"] == 0, res
+        rows = B.job_rows(job, res)
+        if what == "nn":
+            _check_nn(rows, want, extra)
+        elif what == "ray":
+            if extra == "closest":
+                tid, tt, has = want
+                got = _canon(rows, ["ray_id"])
+                assert np.array_equal(got["ray_id"], np.flatnonzero(has))
+                assert np.array_equal(got["triangle_id"], tid[has])
+                assert np.array_equal(got["t"], tt[has])
+            else:
+                got = _canon(rows, ["ray_id"])
+                assert np.array_equal(got["ray_id"], np.arange(len(want)))
+                assert np.array_equal(got[got.dtype.names[1]], want)
+        else:
+            got = _canon(rows, ["point_id", "polygon_id"])
+            assert np.array_equal(got["point_id"], want[:, 0])
+            assert np.array_equal(got["polygon_id"], refs["id"][want[:, 1]])
+
+
+def test_multi_instance_points_and_polygons(gpu_ready):
+    _say("case: multi-instance FRN (2-D and 3-D instances) and PIP instances")
+    rng = np.random.default_rng(78)
+    sizes = [1000, 37, 5000, 1, 2500, 0, 800]
+    dims = [2, 3, 3, 2, 2, 3, 3]
+    coords, offs, inst = [], [0], []
+    for n, d in zip(sizes, dims):
+        p = rng.random((n, 3)).astype(np.float32)
+        if d == 2:
+            p[:, 2] = 0
+        coords.append(p)
+        offs.append(offs[-1] + n)
+        inst.append(p)
+    scene = B.prepare_point_instances(_dev(np.concatenate(coords)), offs)
+    jobs, wants = [], []
+    for i, (n, d) in enumerate(zip(sizes, dims)):
+        q = rng.random((700, 3)).astype(np.float32)
+        if d == 2:
+            q[:, 2] = 0
+        radius = 0.05
+        jobs.append(B.BatchJob(scene, B.JOB_FIXED_RADIUS, _dev(q[:, :d]), instance=i, radius=radius, k=16))
+        qd = {"id": np.arange(700, dtype=np.uint32), "x": q[:, 0].astype(np.float64), "y": q[:, 1].astype(np.float64)}
+        sd = {"id": np.arange(n, dtype=np.uint32), "x": inst[i][:, 0].astype(np.float64),
+              "y": inst[i][:, 1].astype(np.float64)}
+        if d == 3:
+            qd["z"] = q[:, 2].astype(np.float64)
+            sd["z"] = inst[i][:, 2].astype(np.float64)
+        wants.append(oracle.neighbors("frn", qd, sd, radius=radius, k=16))
+    results, _ = B.run_batch(jobs)
+    for job, res, want in zip(jobs, results, wants):
+        assert res["status"] == 0
+        _check_nn(B.job_rows(job, res), want, False)
+
+    polys = [cases.star_polygons(rng, 300) for _ in range(3)]
+    voff, vcnt, verts, poffs = [], [], [], [0]
+    base = 0
+    for pset in polys:
+        voff.append(pset["refs"]["vertex_offset"] + base)
+        vcnt.append(pset["refs"]["vertex_count"])
+        verts.append(pset["vertices_xy"].reshape(-1, 2))
+        base += len(verts[-1])
+        poffs.append(poffs[-1] + len(pset["refs"]))
+    pscene = B.prepare_polygon_instances(torch.from_numpy(np.concatenate(voff).astype(np.int32)).cuda(),
+                                         torch.from_numpy(np.concatenate(vcnt).astype(np.int32)).cuda(),
+                                         _dev(np.concatenate(verts)), poffs)
+    jobs, wants = [], []
+    for i, pset in enumerate(polys):
+        pts = cases.points(rng, 30000, 2)
+        jobs.append(B.BatchJob(pscene, B.JOB_PIP_POSITIVE, _dev(np.stack([pts["x"], pts["y"]], 1)), instance=i))
+        r = pset["refs"]
+        wants.append(oracle.pip_pairs(pts["x"], pts["y"], r["vertex_offset"], r["vertex_count"],
+                                      pset["vertices_xy"], positive_only=True))
+    results, _ = B.run_batch(jobs)
+    for job, res, want in zip(jobs, results, wants):
+        assert res["status"] == 0
+        got = _canon(B.job_rows(job, res), ["point_id", "polygon_id"])
+        assert np.array_equal(got["point_id"], want[:, 0])
+        assert np.array_equal(got["polygon_id"], want[:, 1])
+
+
+def test_serve_replay_results_same_alone_and_in_mix(gpu_ready):
+    _say("case: serve replay, alone vs in mix")
+    rng = np.random.default_rng(79)
+    sdict, s3 = _points3(rng, 40000)
+    pscene = _prepared_points(s3)
+    tris = cases.triangles(rng, 20000, sigma=0.01)
+    tscene = B.prepare_triangle_instances(_dev(cases.tri_matrix(tris)), [0, 20000])
+    polys = cases.star_polygons(rng, 500, rmin=0.01, rmax=0.04)
+    refs = polys["refs"]
+    gscene = B.prepare_polygon_instances(torch.from_numpy(refs["vertex_offset"].astype(np.int32)).cuda(),
+                                         torch.from_numpy(refs["vertex_count"].astype(np.int32)).cuda(),
+                                         _dev(polys["vertices_xy"].reshape(-1, 2)), [0, len(refs)])
+    n_req = 400
+    kinds = rng.choice([B.JOB_FIXED_RADIUS, B.JOB_KNN, B.JOB_RAY_ANYHIT, B.JOB_PIP_POSITIVE], n_req)
+    sizes = np.floor(np.exp(rng.uniform(0, np.log(3000), n_req))).astype(np.int64)
+    arrivals = np.cumsum(rng.exponential(1 / 20000.0, n_req))
+    pq = rng.random((int(sizes.sum()), 3)).astype(np.float32)
+    rq = np.concatenate([cases.ray_matrix(cases.rays(rng, int(sizes.sum()), tmax=0.3))], axis=0).astype(np.float32)
+    gq = rng.random((int(sizes.sum()), 2)).astype(np.float32)
+    reqs = np.zeros(n_req, dtype=B.SERVE_REQUEST_DTYPE)
+    off = 0
+    scene_of = {B.JOB_FIXED_RADIUS: 0, B.JOB_KNN: 0, B.JOB_RAY_ANYHIT: 1, B.JOB_PIP_POSITIVE: 2}
+    for i in range(n_req):
+        reqs[i] = (kinds[i], scene_of[kinds[i]], off, sizes[i], 1 if i % 40 == 3 else 0, arrivals[i])
+        off += sizes[i]
+    arenas = dict(point_queries=_dev(pq), ray_queries=_dev(rq), pip_queries=_dev(gq))
+    scenes = [pscene, tscene, gscene]
+    kw = dict(radius=0.02, k_max=64, k=8, slots=4, **arenas)
+    results, stats, captured = B.serve_replay(scenes, reqs, **kw)
+    assert np.all(results["status"] == 0)
+    assert np.all(results["done_s"] >= results["arrival_s"])
+    assert stats["queries"] == sizes.sum()
+    assert stats["batches"] >= 2
+    for i, rows in captured.items():
+        alone = reqs[i:i + 1].copy()
+        alone["arrival_s"] = 0.0
+        res1, _, cap1 = B.serve_replay(scenes, alone, **kw)
+        assert res1["rows"][0] == results["rows"][i]
+        kind = int(reqs["kind"][i])
+        keys = {B.JOB_FIXED_RADIUS: ["query_id", "neighbor_id"], B.JOB_KNN: ["query_id", "neighbor_id"],
+                B.JOB_RAY_ANYHIT: ["ray_id"], B.JOB_PIP_POSITIVE: ["point_id", "polygon_id"]}[kind]
+        a, b = _canon(rows, keys), _canon(cap1[0], keys)
+        assert a.tobytes() == b.tobytes()
+        o, n = int(reqs["query_offset"][i]), int(reqs["query_count"][i])
+        if kind in (B.JOB_FIXED_RADIUS, B.JOB_KNN):
+            q = pq[o:o + n].astype(np.float64)
+            qd = {"id": np.arange(n, dtype=np.uint32), "x": q[:, 0], "y": q[:, 1], "z": q[:, 2]}
+            mode = "frn" if kind == B.JOB_FIXED_RADIUS else "knn"
+            want = oracle.neighbors(mode, qd, sdict, radius=0.02, k=64 if mode == "frn" else 8)
+            _check_nn(rows, want, mode == "knn")
+        elif kind == B.JOB_RAY_ANYHIT:
+            want = oracle.rays_3d("any", rq[o:o + n].astype(np.float64), cases.tri_matrix(tris), tris["id"])
+            assert np.array_equal(a["any_hit"], want)
+        else:
+            q = gq[o:o + n].astype(np.float64)
+            want = oracle.pip_pairs(q[:, 0], q[:, 1], refs["vertex_offset"], refs["vertex_count"],
+                                    polys["vertices_xy"], positive_only=True)
+            assert np.array_equal(a["point_id"], want[:, 0])
+            assert np.array_equal(a["polygon_id"], refs["id"][want[:, 1]])
+
+
+def test_serve_admission_rejects_oversized_requests(gpu_ready):
+    _say("case: serve admission budget")
+    rng = np.random.default_rng(80)
+    _, s3 = _points3(rng, 1000)
+    scene = _prepared_points(s3)
+    reqs = np.zeros(3, dtype=B.SERVE_REQUEST_DTYPE)
+    reqs[0] = (B.JOB_FIXED_RADIUS, 0, 0, 10, 0, 0.0)
+    reqs[1] = (B.JOB_FIXED_RADIUS, 0, 10, 5000, 0, 0.0)  # 5000 * 64 rows * 16 B > budget / slots
+    reqs[2] = (B.JOB_FIXED_RADIUS, 0, 0, 10, 0, 0.0)
+    pq = _dev(rng.random((6000, 3)))
+    res, stats, _ = B.serve_replay([scene], reqs, radius=0.05, k_max=64, k=1, point_queries=pq,
+                                   slots=2, memory_budget_bytes=2 * 1024 * 1024)
+    assert res["status"].tolist() == [0, B.STATUS_REJECTED, 0]
