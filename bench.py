@@ -52,56 +52,67 @@ def log(msg: str) -> None:
 # --------------------------------------------------------------- scenes --
 
 
-def make_scenes(torch, B, rng, max_prims: float, n_scenes: int = 16):
-    """Builds the resident scenes on the device; returns a list of dicts."""
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(SEED)
-    centres3 = torch.rand(64, 3, device="cuda", generator=gen)
-    centres2 = torch.rand(64, 2, device="cuda", generator=gen)
+def _points(rng, count, dim, dist, centres):
+    if dist == "uniform":
+        return rng.random((count, dim), dtype=np.float32)
+    pick = rng.integers(0, 64, count)
+    return (centres[pick, :dim] + 0.02 * rng.standard_normal((count, dim), dtype=np.float32)).astype(np.float32)
+
+
+def star_polygons(rng, n, dist, centres):
+    """n star polygons with 8-32 vertices at increasing angles (one jittered
+    vertex per equal sector, so no sort is needed); radii U[0.005, 0.02]
+    scaled by sqrt(1000 / n)."""
+    counts = rng.integers(8, 33, n).astype(np.int64)
+    offsets = np.cumsum(counts) - counts
+    total = int(counts.sum())
+    seg = np.repeat(np.arange(n, dtype=np.int32), counts)
+    j = (np.arange(total, dtype=np.int64) - offsets[seg]).astype(np.float32)
+    ang = (j + np.float32(0.9) * rng.random(total, dtype=np.float32)) * (
+        np.float32(2.0 * math.pi) / counts.astype(np.float32))[seg]
+    rad = (np.float32(0.005) + np.float32(0.015) * rng.random(total, dtype=np.float32)) * np.float32(
+        math.sqrt(1000.0 / n))
+    cen = _points(rng, n, 2, dist, centres)
+    verts = np.empty((total, 2), dtype=np.float32)
+    verts[:, 0] = cen[seg, 0] + rad * np.cos(ang)
+    verts[:, 1] = cen[seg, 1] + rad * np.sin(ang)
+    return offsets.astype(np.int32), counts.astype(np.int32), verts
+
+
+def make_scenes(B, rng, max_prims: float, n_scenes: int = 16):
+    """Generates the scenes on the host (seeded) and builds them resident on
+    the device; returns a list of dicts holding the host copy as well."""
+    from rtdl_amd.device import DeviceBuffer
+
+    centres = rng.random((64, 3), dtype=np.float32)
     scenes = []
     for i in range(n_scenes):
         n = int(round(1e5 * (max_prims / 1e5) ** (i / (n_scenes - 1))))
         kind = KINDS[i % 4]
         dist = "uniform" if (i // 4) % 2 == 0 else "mixture"
-
-        def pts(count, dim):
-            if dist == "uniform":
-                return torch.rand(count, dim, device="cuda", generator=gen)
-            c = centres3 if dim == 3 else centres2
-            pick = torch.randint(0, 64, (count,), device="cuda", generator=gen)
-            return c[pick] + 0.02 * torch.randn(count, dim, device="cuda", generator=gen)
-
+        if kind in ("frn", "knn"):
+            host = _points(rng, n, 3, dist, centres)
+        elif kind == "ray":
+            c = _points(rng, n, 3, dist, centres)
+            host = (c[:, None, :] + 0.005 * rng.standard_normal((n, 3, 3), dtype=np.float32))
+            host = host.reshape(n, 9).astype(np.float32)
+        else:
+            host = star_polygons(rng, n, dist, centres)
         t0 = time.perf_counter()
         if kind in ("frn", "knn"):
-            data = pts(n, 3).float().contiguous()
-            handle = B.prepare_point_instances(data, [0, n])
+            dev = DeviceBuffer.from_numpy(host)
+            handle = B.prepare_point_instances(dev, [0, n])
         elif kind == "ray":
-            c = pts(n, 3)
-            data = (c[:, None, :] + 0.005 * torch.randn(n, 3, 3, device="cuda", generator=gen))
-            data = data.reshape(n, 9).float().contiguous()
-            handle = B.prepare_triangle_instances(data, [0, n])
+            dev = DeviceBuffer.from_numpy(host)
+            handle = B.prepare_triangle_instances(dev, [0, n])
         else:
-            counts = torch.randint(8, 33, (n,), device="cuda", generator=gen, dtype=torch.int64)
-            offsets = torch.cumsum(counts, 0) - counts
-            total = int(counts.sum())
-            seg = torch.repeat_interleave(torch.arange(n, device="cuda"), counts)
-            key = seg.double() + torch.rand(total, device="cuda", generator=gen, dtype=torch.float64)
-            key, _ = torch.sort(key)
-            ang = (key - seg.double()) * 2.0 * math.pi
-            scale = math.sqrt(1000.0 / n)
-            rad = (0.005 + 0.015 * torch.rand(total, device="cuda", generator=gen, dtype=torch.float64)) * scale
-            cen = pts(n, 2).double()[seg]
-            verts = torch.stack([cen[:, 0] + rad * torch.cos(ang), cen[:, 1] + rad * torch.sin(ang)], 1)
-            verts = verts.float().contiguous()
-            voff = offsets.to(torch.int32).contiguous()
-            vcnt = counts.to(torch.int32).contiguous()
-            handle = B.prepare_polygon_instances(voff, vcnt, verts, [0, n])
-            data = (voff, vcnt, verts)
-        torch.cuda.synchronize()
-        scenes.append({"index": i, "kind": kind, "dist": dist, "n": n, "handle": handle, "data": data,
+            dev = tuple(DeviceBuffer.from_numpy(a) for a in host)
+            handle = B.prepare_polygon_instances(*dev, [0, n])
+        scenes.append({"index": i, "kind": kind, "dist": dist, "n": n, "handle": handle, "host": host,
                        "build_s": time.perf_counter() - t0})
-        log(f"scene {i:2d}: {kind:3s} {dist:7s} n={n:>9d} built in {scenes[-1]['build_s']:.2f}s")
-    return scenes, centres3, centres2
+        del dev  # the scene keeps its own copy
+        log(f"scene {i:2d}: {kind:3s} {dist:7s} n={n:>9d} uploaded + built in {scenes[-1]['build_s']:.2f}s")
+    return scenes, centres
 
 
 # ------------------------------------------------------- request stream --
@@ -118,17 +129,19 @@ def make_stream(rng, seconds: float, rate: float, scenes, t_offset: float = 0.0)
     return {"arrival": arrivals, "kind": kinds, "size": sizes, "scene": scene_idx}
 
 
-def make_arenas(torch, stream, scenes, centres3, centres2, seed: int):
-    """Query arenas on the device; returns (arenas, per-request offsets)."""
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
+def make_arenas(stream, scenes, centres, seed: int):
+    """Query arenas (host copies + device buffers); returns (host arenas,
+    device arenas, per-request offsets, totals)."""
+    from rtdl_amd.device import DeviceBuffer
+
+    rng = np.random.default_rng(seed)
     n = len(stream["kind"])
     offsets = np.zeros(n, dtype=np.int64)
     # neighbour requests share the point arena, drawn per request from its
-    # scene's distribution; rays and PIP points are uniform in the unit box
-    dists = np.array([scenes[s]["dist"] == "mixture" for s in stream["scene"]])
-    arenas = {}
-    totals = {}
+    # scene's distribution; rays are uniform in the unit box, PIP points
+    # follow their scene's distribution
+    dists = np.array([scenes[s]["dist"] == "mixture" for s in stream["scene"]], dtype=bool)
+    host, dev, totals = {}, {}, {}
     for group, kinds in (("point", (0, 1)), ("ray", (2,)), ("pip", (3,))):
         sel = np.isin(stream["kind"], kinds)
         idx = np.flatnonzero(sel)
@@ -136,28 +149,21 @@ def make_arenas(torch, stream, scenes, centres3, centres2, seed: int):
         offsets[idx] = np.concatenate([[0], np.cumsum(sizes)[:-1]]) if len(idx) else []
         total = int(sizes.sum())
         totals[group] = total
-        if group == "point":
-            q = torch.rand(total, 3, device="cuda", generator=gen)
+        if group in ("point", "pip"):
+            dim = 3 if group == "point" else 2
+            q = rng.random((total, dim), dtype=np.float32)
             mix = np.repeat(dists[idx], sizes)
-            m = torch.from_numpy(mix).cuda()
-            pick = torch.randint(0, 64, (total,), device="cuda", generator=gen)
-            g = centres3[pick] + 0.02 * torch.randn(total, 3, device="cuda", generator=gen)
-            q = torch.where(m[:, None], g, q)
-        elif group == "ray":
-            o = torch.rand(total, 3, device="cuda", generator=gen)
-            d = torch.randn(total, 3, device="cuda", generator=gen)
-            d = d / d.norm(dim=1, keepdim=True)
-            q = torch.cat([o, d, torch.full((total, 1), TMAX, device="cuda")], 1)
+            pick = rng.integers(0, 64, total)
+            g = centres[pick, :dim] + 0.02 * rng.standard_normal((total, dim), dtype=np.float32)
+            q = np.where(mix[:, None], g, q).astype(np.float32)
         else:
-            q = torch.rand(total, 2, device="cuda", generator=gen)
-            mix = np.repeat(dists[idx], sizes)
-            m = torch.from_numpy(mix).cuda()
-            pick = torch.randint(0, 64, (total,), device="cuda", generator=gen)
-            g = centres2[pick] + 0.02 * torch.randn(total, 2, device="cuda", generator=gen)
-            q = torch.where(m[:, None], g, q)
-        arenas[group] = q.float().contiguous()
-    torch.cuda.synchronize()
-    return arenas, offsets, totals
+            o = rng.random((total, 3))
+            d = rng.standard_normal((total, 3))
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            q = np.concatenate([o, d, np.full((total, 1), TMAX)], 1).astype(np.float32)
+        host[group] = q
+        dev[group] = DeviceBuffer.from_numpy(q if total else np.zeros((1, q.shape[1]), np.float32))
+    return host, dev, offsets, totals
 
 
 JOB_OF_KIND = None
@@ -180,7 +186,7 @@ def requests_array(B, stream, offsets, scene_slot, capture_every: int = 0):
 # ------------------------------------------------------------- roofline --
 
 
-def roofline_leg(torch, B, scenes, stream, offsets, arenas, limit_queries: int):
+def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int):
     """One batch per predicate family over the first requests of the stream.
     Kernel seconds from events on the launch stream (timed run), node and
     primitive visits from a separate counting run (untimed)."""
@@ -232,52 +238,52 @@ def roofline_leg(torch, B, scenes, stream, offsets, arenas, limit_queries: int):
 # ------------------------------------------------------------ baselines --
 
 
-def host_scene_records(torch, rtm, s):
+def host_scene_records(rtm, s):
     if "records" in s:
         return s["records"]
     if s["kind"] in ("frn", "knn"):
-        d = s["data"].double().cpu().numpy()
+        d = s["host"].astype(np.float64)
         rec = np.zeros(len(d), dtype=rtm.POINT3D_DTYPE)
         rec["id"] = np.arange(len(d))
         rec["x"], rec["y"], rec["z"] = d[:, 0], d[:, 1], d[:, 2]
     elif s["kind"] == "ray":
-        d = s["data"].double().cpu().numpy()
+        d = s["host"].astype(np.float64)
         rec = np.zeros(len(d), dtype=rtm.TRIANGLE3D_DTYPE)
         rec["id"] = np.arange(len(d))
         for j, f in enumerate(["x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2"]):
             rec[f] = d[:, j]
     else:
-        voff, vcnt, verts = s["data"]
+        voff, vcnt, verts = s["host"]
         refs = np.zeros(len(voff), dtype=rtm.POLYGON_REF_DTYPE)
         refs["id"] = np.arange(len(voff))
-        refs["vertex_offset"] = voff.cpu().numpy()
-        refs["vertex_count"] = vcnt.cpu().numpy()
-        rec = (refs, verts.double().cpu().numpy().reshape(-1))
+        refs["vertex_offset"] = voff
+        refs["vertex_count"] = vcnt
+        rec = (refs, verts.astype(np.float64).reshape(-1))
     s["records"] = rec
     return rec
 
 
-def host_queries(rtm, arenas, kind, o, n):
+def host_queries(rtm, host, kind, o, n):
     if kind in (0, 1):
-        q = arenas["point"][o:o + n].double().cpu().numpy()
+        q = host["point"][o:o + n].astype(np.float64)
         rec = np.zeros(n, dtype=rtm.POINT3D_DTYPE)
         rec["id"] = np.arange(n)
         rec["x"], rec["y"], rec["z"] = q[:, 0], q[:, 1], q[:, 2]
     elif kind == 2:
-        q = arenas["ray"][o:o + n].double().cpu().numpy()
+        q = host["ray"][o:o + n].astype(np.float64)
         rec = np.zeros(n, dtype=rtm.RAY3D_DTYPE)
         rec["id"] = np.arange(n)
         for j, f in enumerate(["ox", "oy", "oz", "dx", "dy", "dz", "tmax"]):
             rec[f] = q[:, j]
     else:
-        q = arenas["pip"][o:o + n].double().cpu().numpy()
+        q = host["pip"][o:o + n].astype(np.float64)
         rec = np.zeros(n, dtype=rtm.POINT2D_DTYPE)
         rec["id"] = np.arange(n)
         rec["x"], rec["y"] = q[:, 0], q[:, 1]
     return rec
 
 
-def plain_per_call_leg(torch, rtm, scenes, stream, offsets, arenas, budget_s: float):
+def plain_per_call_leg(rtm, scenes, stream, offsets, host, budget_s: float):
     """Plain per-call C-ABI: host records in, scene built per call, host rows out."""
     p = rtm._ptr
     done_q, spent, calls = 0, 0.0, 0
@@ -287,8 +293,8 @@ def plain_per_call_leg(torch, rtm, scenes, stream, offsets, arenas, budget_s: fl
         k = int(stream["kind"][i])
         n = int(stream["size"][i])
         s = scenes[int(stream["scene"][i])]
-        scene_rec = host_scene_records(torch, rtm, s)
-        q = host_queries(rtm, arenas, k, int(offsets[i]), n)
+        scene_rec = host_scene_records(rtm, s)
+        q = host_queries(rtm, host, k, int(offsets[i]), n)
         t0 = time.perf_counter()
         if k == 0:
             rtm._call_rows("rtdl_mi355x_run_fixed_radius_neighbors_3d", rtm.FRN_ROW_DTYPE, p(q), n,
@@ -310,7 +316,7 @@ def plain_per_call_leg(torch, rtm, scenes, stream, offsets, arenas, budget_s: fl
             "queries_per_s": done_q / spent if spent else 0.0}
 
 
-def per_request_resident_leg(torch, B, scenes, stream, offsets, arenas, budget_s: float):
+def per_request_resident_leg(B, scenes, stream, offsets, arenas, budget_s: float):
     """Resident scenes, one synchronous single-job batch per request."""
     kind_map = {0: B.JOB_FIXED_RADIUS, 1: B.JOB_KNN, 2: B.JOB_RAY_ANYHIT, 3: B.JOB_PIP_POSITIVE}
     group_of = {0: "point", 1: "point", 2: "ray", 3: "pip"}
@@ -333,7 +339,7 @@ def per_request_resident_leg(torch, B, scenes, stream, offsets, arenas, budget_s
             "queries_per_s": done_q / spent if spent else 0.0}
 
 
-def cpu_baseline_leg(torch, rtm, scenes, stream, offsets, arenas, budget_s: float):
+def cpu_baseline_leg(rtm, scenes, stream, offsets, host, budget_s: float):
     """The C oracle (FP64, multi-threaded over all host cores) on a bounded
     sample of the same requests, grouped per scene (one index per scene)."""
     from oracle import native as oracle
@@ -357,9 +363,9 @@ def cpu_baseline_leg(torch, rtm, scenes, stream, offsets, arenas, budget_s: floa
         if True:
             s = scenes[sidx]
             k = int(stream["kind"][reqs[0]])
-            qs = [host_queries(rtm, arenas, k, int(offsets[i]), int(stream["size"][i])) for i in reqs]
+            qs = [host_queries(rtm, host, k, int(offsets[i]), int(stream["size"][i])) for i in reqs]
             q = np.concatenate(qs)
-            rec = host_scene_records(torch, rtm, s)
+            rec = host_scene_records(rtm, s)
             t0 = time.perf_counter()
             if k in (0, 1):
                 qd = {"id": np.arange(len(q), dtype=np.uint32), "x": q["x"], "y": q["y"], "z": q["z"]}
@@ -418,15 +424,13 @@ def main() -> int:
               "design note only; run with --gpus 1", file=sys.stderr)
         return 2
 
-    import torch
-
     from rtdl_amd import batch as B
     from rtdl_amd import mi355x_runtime as rtm
 
     info = rtm.mi355x_context_probe()
     log(f"device {info}")
     rng = np.random.default_rng(SEED)
-    scenes, c3, c2 = make_scenes(torch, B, rng, args.max_prims)
+    scenes, centres = make_scenes(B, rng, args.max_prims)
     scene_slot = {s["index"]: s["index"] for s in scenes}
     handles = [s["handle"] for s in scenes]
 
@@ -443,24 +447,20 @@ def main() -> int:
         f"{per_slot * args.slots / (1 << 30):.1f} GiB of row buffers resident")
 
     if len(warm["kind"]):
-        wa, wo, _ = make_arenas(torch, warm, scenes, c3, c2, SEED + 1)
+        _, wa, wo, _ = make_arenas(warm, scenes, centres, SEED + 1)
         wreq = requests_array(B, warm, wo, scene_slot)
+        log(f"warmup replay: {len(wreq)} requests")
         B.serve_replay(handles, wreq, point_queries=wa["point"], ray_queries=wa["ray"],
                        pip_queries=wa["pip"], **serve_kw)
         del wa
-        torch.cuda.synchronize()
         log("warmup replay done")
 
-    arenas, offsets, totals = make_arenas(torch, timed, scenes, c3, c2, SEED + 2)
+    host, arenas, offsets, totals = make_arenas(timed, scenes, centres, SEED + 2)
     reqs = requests_array(B, timed, offsets, scene_slot)
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()
-    free, _ = torch.cuda.mem_get_info()
-    log(f"{free / (1 << 30):.1f} GiB free before the timed replay")
+    log(f"timed replay: {len(reqs)} requests")
     t0 = time.perf_counter()
     results, stats, _ = B.serve_replay(handles, reqs, point_queries=arenas["point"],
                                        ray_queries=arenas["ray"], pip_queries=arenas["pip"], **serve_kw)
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ok = results["status"] == 0
     lat = (results["done_s"] - results["arrival_s"])[ok] * 1e3
@@ -477,7 +477,8 @@ def main() -> int:
             per_kind[KINDS[k]] = {"requests": int(sel.sum()), "queries": int(timed["size"][sel].sum()),
                                   "p50_ms": float(np.percentile(lk, 50)), "p99_ms": float(np.percentile(lk, 99))}
 
-    roof = roofline_leg(torch, B, scenes, timed, offsets, arenas, args.roofline_queries)
+    log(f"timed replay wall (host clock) {wall:.3f}s")
+    roof = roofline_leg(B, scenes, timed, offsets, arenas, args.roofline_queries)
     dominant = max(roof, key=lambda k: roof[k]["kernel_s"] / max(1, roof[k]["queries"])
                    * max(1, int(timed["size"][timed["kind"] == KINDS.index(k)].sum())))
     d = roof[dominant]
@@ -485,13 +486,13 @@ def main() -> int:
     baselines = {}
     cpu = None
     if not args.skip_baselines:
-        baselines["plain_per_call"] = plain_per_call_leg(torch, rtm, scenes, timed, offsets, arenas,
+        baselines["plain_per_call"] = plain_per_call_leg(rtm, scenes, timed, offsets, host,
                                                          args.percall_seconds)
         log(f"plain per-call: {baselines['plain_per_call']}")
-        baselines["per_request_resident"] = per_request_resident_leg(torch, B, scenes, timed, offsets,
+        baselines["per_request_resident"] = per_request_resident_leg(B, scenes, timed, offsets,
                                                                       arenas, args.resident_seconds)
         log(f"per-request resident: {baselines['per_request_resident']}")
-        cpu = cpu_baseline_leg(torch, rtm, scenes, timed, offsets, arenas, args.cpu_seconds)
+        cpu = cpu_baseline_leg(rtm, scenes, timed, offsets, host, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
     plain = baselines.get("plain_per_call", {}).get("queries_per_s")

@@ -379,6 +379,19 @@ int rtdl_mi355x_serve_replay(void* const* scenes, size_t scene_count,
                              const RtdlServeConfig* config, RtdlServeResult* results,
                              RtdlServeStats* stats, char* error_out, size_t error_size);
 
+/* ---- device buffers (no counterpart in the reference) ----
+ * Plumbing for callers that keep query arenas, instance inputs and row
+ * buffers resident in HBM across batches (the batched and served entry
+ * points above take device pointers).  Allocation is hipMalloc on the
+ * current device; copies are synchronous with respect to the host and to
+ * every batch the library has completed. */
+int rtdl_mi355x_device_alloc(size_t bytes, void** ptr_out, char* error_out, size_t error_size);
+int rtdl_mi355x_device_free(void* ptr, char* error_out, size_t error_size);
+int rtdl_mi355x_copy_to_device(void* dst, const void* src, size_t bytes, char* error_out,
+                               size_t error_size);
+int rtdl_mi355x_copy_to_host(void* dst, const void* src, size_t bytes, char* error_out,
+                             size_t error_size);
+
 /* ---- diagnostics (no counterpart in the reference) ---- */
 
 /* Evaluates sqrt(a[i]) and a[i] / b[i] in FP64 on the device with the same

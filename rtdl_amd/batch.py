@@ -4,9 +4,11 @@
 scenes or instances of a multi-instance scene) in one persistent launch per
 predicate family; ``serve_replay`` replays a timed request stream against
 resident scenes with continuous batching and memory-budget admission.  Both
-sit on the C-ABI (``rtdl_mi355x_run_batch`` / ``rtdl_mi355x_serve_replay``);
-torch tensors on the GPU carry device memory (plumbing only: every
-computation runs in the library's HIP kernels).
+sit on the C-ABI (``rtdl_mi355x_run_batch`` / ``rtdl_mi355x_serve_replay``).
+Device inputs and row buffers are ``rtdl_amd.device.DeviceBuffer`` objects
+(allocated by the library itself) or any object exposing ``data_ptr()``,
+``shape`` and ``numel()`` for device memory; every computation runs in the
+library's HIP kernels.
 """
 from __future__ import annotations
 
@@ -16,6 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import mi355x_runtime as rt_native
+from .device import DeviceBuffer
 
 JOB_FIXED_RADIUS = 0
 JOB_KNN = 1
@@ -101,6 +104,17 @@ def _ptr(t) -> int | None:
     return int(t.data_ptr()) if t.numel() else None
 
 
+def _sync_foreign() -> None:
+    """Device arrays made by torch are written on torch's streams; when torch
+    is in use in this process, its work is completed before a native call.
+    (torch is never imported here: buffers from the library need no sync.)"""
+    import sys
+
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        torch.cuda.synchronize()
+
+
 class Instances:
     """A multi-instance scene: one LBVH with one subtree per instance."""
 
@@ -130,7 +144,8 @@ class Instances:
 def prepare_point_instances(coords, instance_offsets) -> Instances:
     """coords: float32 device tensor (n, 3) (z = 0 for 2-D instances)."""
     offs = np.ascontiguousarray(instance_offsets, dtype=np.uint64)
-    assert coords.dtype.is_floating_point and coords.shape[-1] == 3
+    assert coords.shape[-1] == 3
+    _sync_foreign()
     handle = ctypes.c_void_p()
     err = ctypes.create_string_buffer(4096)
     st = _lib().rtdl_mi355x_prepare_point_instances(
@@ -143,6 +158,7 @@ def prepare_triangle_instances(coords, instance_offsets) -> Instances:
     """coords: float32 device tensor (n, 9): three vertices per triangle."""
     offs = np.ascontiguousarray(instance_offsets, dtype=np.uint64)
     assert coords.shape[-1] == 9
+    _sync_foreign()
     handle = ctypes.c_void_p()
     err = ctypes.create_string_buffer(4096)
     st = _lib().rtdl_mi355x_prepare_triangle_instances(
@@ -152,8 +168,9 @@ def prepare_triangle_instances(coords, instance_offsets) -> Instances:
 
 
 def prepare_polygon_instances(vertex_offsets, vertex_counts, vertices_xy, instance_offsets) -> Instances:
-    """vertex_offsets / vertex_counts: int32 device tensors; vertices_xy: float32 (nv, 2)."""
+    """vertex_offsets / vertex_counts: int32 device arrays; vertices_xy: float32 (nv, 2)."""
     offs = np.ascontiguousarray(instance_offsets, dtype=np.uint64)
+    _sync_foreign()
     handle = ctypes.c_void_p()
     err = ctypes.create_string_buffer(4096)
     st = _lib().rtdl_mi355x_prepare_polygon_instances(
@@ -194,15 +211,12 @@ def row_capacity(kind: int, nq: int, k: int) -> int:
 
 
 def _pack_jobs(jobs: list[BatchJob]):
-    import torch
-
     arr = (_BatchJob * max(1, len(jobs)))()
     for i, j in enumerate(jobs):
         nq = int(j.queries.shape[0])
         cap = j.row_capacity or row_capacity(j.kind, nq, j.k)
         if j.rows is None:
-            j.rows = torch.empty(max(1, cap * ROW_DTYPES[j.kind].itemsize), dtype=torch.uint8,
-                                 device=j.queries.device)
+            j.rows = DeviceBuffer((max(1, cap * ROW_DTYPES[j.kind].itemsize),), np.uint8)
             j.row_capacity = cap
         arr[i].scene = _raw(_handle_of(j.scene))
         arr[i].instance = j.instance
@@ -219,12 +233,10 @@ def _pack_jobs(jobs: list[BatchJob]):
 
 def run_batch(jobs: list[BatchJob], *, morton_sort: bool = True):
     """Runs the jobs as one batch; returns (per-job result dicts, device seconds)."""
-    import torch
-
     arr, res = _pack_jobs(jobs)
     seconds = ctypes.c_double()
     err = ctypes.create_string_buffer(4096)
-    torch.cuda.synchronize()
+    _sync_foreign()
     st = _lib().rtdl_mi355x_run_batch(ctypes.addressof(arr), len(jobs), int(bool(morton_sort)),
                                       ctypes.addressof(res), ctypes.byref(seconds), err, 4096)
     _check(st, err, "rtdl_mi355x_run_batch")
@@ -239,13 +251,11 @@ FAMILIES = ("neighbors", "ray_any_hit", "ray_hit_count", "ray_closest_hit", "pip
 def run_batch_profiled(jobs: list[BatchJob], *, morton_sort: bool = True, count_visits: bool = False):
     """Like run_batch, returning per-family kernel seconds and, when
     `count_visits`, the (node records, primitive records) visited."""
-    import torch
-
     arr, res = _pack_jobs(jobs)
     fam = (ctypes.c_double * 5)()
     visits = (ctypes.c_uint64 * 2)()
     err = ctypes.create_string_buffer(4096)
-    torch.cuda.synchronize()
+    _sync_foreign()
     st = _lib().rtdl_mi355x_run_batch_profiled(ctypes.addressof(arr), len(jobs), int(bool(morton_sort)),
                                                ctypes.addressof(res), fam,
                                                visits if count_visits else None, err, 4096)
@@ -259,8 +269,11 @@ def run_batch_profiled(jobs: list[BatchJob], *, morton_sort: bool = True, count_
 def job_rows(job: BatchJob, result: dict) -> np.ndarray:
     """Copies a job's rows to the host as a numpy structured array."""
     dtype = ROW_DTYPES[job.kind]
-    n = int(result["rows"])
-    raw = job.rows[: n * dtype.itemsize].cpu().numpy()
+    n = min(int(result["rows"]), int(job.row_capacity))
+    if isinstance(job.rows, DeviceBuffer):
+        raw = job.rows.download(n * dtype.itemsize)
+    else:
+        raw = job.rows[: n * dtype.itemsize].cpu().numpy()
     return raw.view(dtype).copy()
 
 
@@ -271,8 +284,6 @@ def serve_replay(scenes, requests: np.ndarray, *, radius: float, k_max: int, k: 
     """Replays `requests` (SERVE_REQUEST_DTYPE) against `scenes`.
 
     Returns (results as SERVE_RESULT_DTYPE, stats dict, captured rows by request index)."""
-    import torch
-
     reqs = np.ascontiguousarray(requests, dtype=SERVE_REQUEST_DTYPE)
     handles = (ctypes.c_void_p * max(1, len(scenes)))(*[_raw(_handle_of(s)) for s in scenes])
     cfg = _ServeConfig()
@@ -290,7 +301,7 @@ def serve_replay(scenes, requests: np.ndarray, *, radius: float, k_max: int, k: 
     results = np.zeros(len(reqs), dtype=SERVE_RESULT_DTYPE)
     stats = _ServeStats()
     err = ctypes.create_string_buffer(4096)
-    torch.cuda.synchronize()
+    _sync_foreign()
     st = _lib().rtdl_mi355x_serve_replay(ctypes.addressof(handles), len(scenes), reqs.ctypes.data,
                                          len(reqs), ctypes.addressof(cfg), results.ctypes.data,
                                          ctypes.addressof(stats), err, 4096)

@@ -526,4 +526,42 @@ int rtdl_mi355x_selftest_fp64(const double* a, const double* b, size_t n, double
   });
 }
 
+int rtdl_mi355x_device_alloc(size_t bytes, void** ptr_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(ptr_out != nullptr, "ptr_out must not be null");
+    *ptr_out = nullptr;
+    void* p = nullptr;
+    RTDL_HIP(hipMalloc(&p, bytes ? bytes : 1));
+    *ptr_out = p;
+  });
+}
+
+int rtdl_mi355x_device_free(void* ptr, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    if (ptr) RTDL_HIP(hipFree(ptr));
+  });
+}
+
+int rtdl_mi355x_copy_to_device(void* dst, const void* src, size_t bytes, char* error_out,
+                               size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    if (!bytes) return;
+    require(dst != nullptr && src != nullptr, "null pointer with nonzero byte count");
+    hipStream_t s = thread_stream();
+    RTDL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    RTDL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int rtdl_mi355x_copy_to_host(void* dst, const void* src, size_t bytes, char* error_out,
+                             size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    if (!bytes) return;
+    require(dst != nullptr && src != nullptr, "null pointer with nonzero byte count");
+    hipStream_t s = thread_stream();
+    RTDL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    RTDL_HIP(hipStreamSynchronize(s));
+  });
+}
+
 }  // extern "C"

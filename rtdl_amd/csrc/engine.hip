@@ -44,32 +44,41 @@ uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_c
 
 namespace {
 
+// The skeleton of every persistent loop below is free of lane-dependent
+// branches: each wave-wide step is an atomic that all 64 lanes execute (lanes
+// other than 0 add zero) followed by readfirstlane, whose SGPR result is
+// uniform by construction.  A `if (lane == 0) atomicAdd(...)` at the end of
+// the body and another at the head of the loop let the compiler thread the
+// two branches across the latch, which split the loop so that lane 0 left to
+// fetch the next item while lanes 1-63 kept re-reading the old one and never
+// exited; a shuffle-broadcast item index (divergent to the compiler) allows
+// the same split.  Every caller runs with all 64 lanes active, so the first
+// active lane is lane 0.
+__device__ inline uint32_t lane0_add(uint32_t* counter, uint32_t value) {
+  const uint32_t mine = (threadIdx.x & 63) == 0 ? value : 0u;
+  return __builtin_amdgcn_readfirstlane(atomicAdd(counter, mine));
+}
+
 __device__ inline uint32_t wave_exclusive_sum(uint32_t v, uint32_t* total) {
   const uint32_t lane = threadIdx.x & 63;
   uint32_t x = v;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= (uint32_t)d) x += y;
+    x += lane >= (uint32_t)d ? y : 0u;
   }
-  *total = __shfl(x, 63, 64);
+  *total = __builtin_amdgcn_readlane(x, 63);
   return x - v;
 }
 
-__device__ inline uint32_t wave_fetch(uint32_t* counter) {
-  uint32_t it = 0;
-  if ((threadIdx.x & 63) == 0) it = atomicAdd(counter, 1u);
-  return __shfl(it, 0, 64);
-}
+__device__ inline uint32_t wave_fetch(uint32_t* counter) { return lane0_add(counter, 1u); }
 
 // Reserves `c` rows per lane in the job's row buffer; returns the lane's
 // first row.
 __device__ inline uint64_t wave_reserve(JobCounters* counters, uint32_t c) {
   uint32_t total;
   const uint32_t prefix = wave_exclusive_sum(c, &total);
-  uint32_t base = 0;
-  if ((threadIdx.x & 63) == 0 && total) base = atomicAdd(&counters->rows, total);
-  base = __shfl(base, 0, 64);
+  const uint32_t base = lane0_add(&counters->rows, total);
   return (uint64_t)base + prefix;
 }
 
@@ -147,30 +156,57 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
 }
 
 // Recomputes the queries whose FP32 selection keys tied at the boundary with
-// the exact FP64 list and rewrites their rows in place.
-template <int K>
-__global__ __launch_bounds__(64) void batch_fix_kernel(const DeviceJob* __restrict__ jobs,
-                                                       const uint4* __restrict__ amb,
-                                                       const uint32_t* amb_count) {
+// the exact (FP64 distance, id) order and rewrites their rows in place.  The
+// per-lane list lives in LDS (slot-major, so the 64 lanes of a wavefront hit
+// consecutive words) instead of registers: a 64-entry FP64 list does not fit
+// a lane's registers, and these queries are rare.
+constexpr int kFixBlock = 64;
+
+__global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* __restrict__ jobs,
+                                                             const uint4* __restrict__ amb,
+                                                             const uint32_t* amb_count) {
+  __shared__ double list_d[kMaxListK * kFixBlock];
+  __shared__ uint32_t list_id[kMaxListK * kFixBlock];
+  const uint32_t lane = threadIdx.x;
+  double* ld = list_d + lane;       // entry e at ld[e * kFixBlock]
+  uint32_t* li = list_id + lane;
   const uint32_t n = *amb_count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint4 a = amb[i];
     const DeviceJob& j = jobs[a.x];
+    const uint32_t k = j.k;  // 1 <= k <= kMaxListK, checked on the host
     const float* qp = j.queries + (size_t)a.y * j.qstride;
     const double qx = qp[0], qy = qp[1], qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
-    ExactTopK<K> list;
-    uint32_t inside = 0;
+    const float qf[3] = {(float)qx, (float)qy, (float)qz};
+    const float m = coord_bound(scene_m(j.scene), qx, qy, qz);
+    const double radius_sq = j.radius * j.radius;  // reference.py:357
+    float thr2 = frn_threshold2(j.radius, m);
+    uint32_t count = 0;  // entries [0, count) sorted by (distance, id)
     LaneStats st;
-    collect_exact<K, false, false>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius, j.k,
-                                   list, inside, st);
-    uint32_t r = 0;
-#pragma unroll
-    for (int e = 0; e < K; ++e) {
-      if (e >= K - (int)j.k && list.d[e] < __builtin_inf() && r < a.w) {
-        write_neighbor_row<K>(j, (uint64_t)a.z + r, a.y, list.id[e], list.d[e], r + 1);
-        ++r;
-      }
-    }
+    walk_points<false>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
+                       [&](uint32_t slot, const float4& p, float) {
+                         const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
+                         if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
+                         const double dist = sqrt(d2);
+                         const uint32_t ident = __float_as_uint(p.w);
+                         if (count == k &&
+                             !key_less(dist, ident, ld[(k - 1) * kFixBlock], li[(k - 1) * kFixBlock]))
+                           return;
+                         uint32_t pos = count < k ? count++ : k - 1;
+                         while (pos > 0 && key_less(dist, ident, ld[(pos - 1) * kFixBlock],
+                                                    li[(pos - 1) * kFixBlock])) {
+                           ld[pos * kFixBlock] = ld[(pos - 1) * kFixBlock];
+                           li[pos * kFixBlock] = li[(pos - 1) * kFixBlock];
+                           --pos;
+                         }
+                         ld[pos * kFixBlock] = dist;
+                         li[pos * kFixBlock] = ident;
+                         if (count == k)
+                           thr2 = frn_threshold2(fmin(j.radius, ld[(k - 1) * kFixBlock]), m);
+                       });
+    const uint32_t rows = min(count, a.w);
+    for (uint32_t r = 0; r < rows; ++r)
+      write_neighbor_row<0>(j, (uint64_t)a.z + r, a.y, li[r * kFixBlock], ld[r * kFixBlock], r + 1);
   }
 }
 
@@ -198,7 +234,7 @@ __global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __rest
     if (MODE != 2) {
       if (active)
         static_cast<PairRowRec*>(j.rows)[qi] = PairRowRec{qi, MODE == 0 ? (res.hits ? 1u : 0u) : res.hits};
-      if (lane == 0) atomicAdd(&j.counters->rows, w.count);
+      (void)lane0_add(&j.counters->rows, w.count);
     } else {
       const uint32_t c = (active && res.best_t == res.best_t) ? 1u : 0u;
       const uint64_t at = wave_reserve(j.counters, c);
@@ -342,7 +378,7 @@ void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkI
   else
     batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
   RTDL_LAUNCH_CHECK();
-  batch_fix_kernel<K><<<64, 64, 0, s>>>(jobs, amb, amb_count);
+  batch_fix_kernel<<<64, kFixBlock, 0, s>>>(jobs, amb, amb_count);
   RTDL_LAUNCH_CHECK();
 }
 
@@ -457,8 +493,9 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     d.qstride = h.qstride;
     d.order_base = (uint32_t)start;
     host_starts_[i] = start;
-    for (uint32_t f = 0; f < h.nq; f += 64)
-      fam_items[fam].push_back(WorkItem{(uint32_t)i, f, std::min<uint32_t>(64, h.nq - f), 0});
+    for (uint64_t f = 0; f < h.nq; f += 64)
+      fam_items[fam].push_back(
+          WorkItem{(uint32_t)i, (uint32_t)f, (uint32_t)std::min<uint64_t>(64, h.nq - f), 0});
     start += h.nq;
   }
   host_starts_[n] = start;

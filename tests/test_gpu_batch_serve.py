@@ -13,11 +13,10 @@ import pytest
 
 from oracle import native as oracle
 from rtdl_amd import batch as B
+from rtdl_amd.device import DeviceBuffer
 from tests import cases
 
 pytestmark = pytest.mark.gpu
-
-torch = pytest.importorskip("torch")
 
 
 def _say(msg):
@@ -25,7 +24,11 @@ def _say(msg):
 
 
 def _dev(a):
-    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    return DeviceBuffer.from_numpy(a, dtype=np.float32)
+
+
+def _dev_i32(a):
+    return DeviceBuffer.from_numpy(a, dtype=np.int32)
 
 
 def _canon(rows, keys):
@@ -67,12 +70,13 @@ def test_batch_mixed_jobs_match_oracle(gpu_ready, morton):
     sdict, s3 = _points3(rng, 60000, "mixture", centres)
     scene = _prepared_points(s3)
     tris = cases.triangles(rng, 30000, sigma=0.01)
+    _say("  prepare_triangle_instances")
     tri_scene = B.prepare_triangle_instances(_dev(cases.tri_matrix(tris)), [0, 30000])
     polys = cases.star_polygons(rng, 800, rmin=0.01, rmax=0.04)
+    _say("  prepare_polygon_instances")
     refs = polys["refs"]
     poly_scene = B.prepare_polygon_instances(
-        torch.from_numpy(refs["vertex_offset"].astype(np.int32)).cuda(),
-        torch.from_numpy(refs["vertex_count"].astype(np.int32)).cuda(),
+        _dev_i32(refs["vertex_offset"]), _dev_i32(refs["vertex_count"]),
         _dev(polys["vertices_xy"].reshape(-1, 2)), [0, len(refs)])
     jobs, wants = [], []
     for kind, k, radius, n in [(B.JOB_FIXED_RADIUS, 64, 0.03, 3000), (B.JOB_FIXED_RADIUS, 5, 0.05, 100),
@@ -92,6 +96,7 @@ def test_batch_mixed_jobs_match_oracle(gpu_ready, morton):
     jobs.append(B.BatchJob(poly_scene, B.JOB_PIP_POSITIVE, _dev(np.stack([pts["x"], pts["y"]], axis=1))))
     wants.append(("pip", None, oracle.pip_pairs(pts["x"], pts["y"], refs["vertex_offset"], refs["vertex_count"],
                                                 polys["vertices_xy"], positive_only=True)))
+    _say("  run_batch: 9 jobs over 4 families")
     results, seconds = B.run_batch(jobs, morton_sort=morton)
     assert seconds > 0
     for job, res, (what, extra, want) in zip(jobs, results, wants):
@@ -129,6 +134,7 @@ def test_multi_instance_points_and_polygons(gpu_ready):
         coords.append(p)
         offs.append(offs[-1] + n)
         inst.append(p)
+    _say("  prepare_point_instances (7 instances)")
     scene = B.prepare_point_instances(_dev(np.concatenate(coords)), offs)
     jobs, wants = [], []
     for i, (n, d) in enumerate(zip(sizes, dims)):
@@ -144,6 +150,7 @@ def test_multi_instance_points_and_polygons(gpu_ready):
             qd["z"] = q[:, 2].astype(np.float64)
             sd["z"] = inst[i][:, 2].astype(np.float64)
         wants.append(oracle.neighbors("frn", qd, sd, radius=radius, k=16))
+    _say("  run_batch: 7 fixed-radius jobs, k_max 16")
     results, _ = B.run_batch(jobs)
     for job, res, want in zip(jobs, results, wants):
         assert res["status"] == 0
@@ -158,8 +165,8 @@ def test_multi_instance_points_and_polygons(gpu_ready):
         verts.append(pset["vertices_xy"].reshape(-1, 2))
         base += len(verts[-1])
         poffs.append(poffs[-1] + len(pset["refs"]))
-    pscene = B.prepare_polygon_instances(torch.from_numpy(np.concatenate(voff).astype(np.int32)).cuda(),
-                                         torch.from_numpy(np.concatenate(vcnt).astype(np.int32)).cuda(),
+    _say("  prepare_polygon_instances (3 instances)")
+    pscene = B.prepare_polygon_instances(_dev_i32(np.concatenate(voff)), _dev_i32(np.concatenate(vcnt)),
                                          _dev(np.concatenate(verts)), poffs)
     jobs, wants = [], []
     for i, pset in enumerate(polys):
@@ -168,6 +175,7 @@ def test_multi_instance_points_and_polygons(gpu_ready):
         r = pset["refs"]
         wants.append(oracle.pip_pairs(pts["x"], pts["y"], r["vertex_offset"], r["vertex_count"],
                                       pset["vertices_xy"], positive_only=True))
+    _say("  run_batch: 3 PIP jobs")
     results, _ = B.run_batch(jobs)
     for job, res, want in zip(jobs, results, wants):
         assert res["status"] == 0
@@ -182,11 +190,11 @@ def test_serve_replay_results_same_alone_and_in_mix(gpu_ready):
     sdict, s3 = _points3(rng, 40000)
     pscene = _prepared_points(s3)
     tris = cases.triangles(rng, 20000, sigma=0.01)
+    _say("  prepare_triangle_instances")
     tscene = B.prepare_triangle_instances(_dev(cases.tri_matrix(tris)), [0, 20000])
     polys = cases.star_polygons(rng, 500, rmin=0.01, rmax=0.04)
     refs = polys["refs"]
-    gscene = B.prepare_polygon_instances(torch.from_numpy(refs["vertex_offset"].astype(np.int32)).cuda(),
-                                         torch.from_numpy(refs["vertex_count"].astype(np.int32)).cuda(),
+    gscene = B.prepare_polygon_instances(_dev_i32(refs["vertex_offset"]), _dev_i32(refs["vertex_count"]),
                                          _dev(polys["vertices_xy"].reshape(-1, 2)), [0, len(refs)])
     n_req = 400
     kinds = rng.choice([B.JOB_FIXED_RADIUS, B.JOB_KNN, B.JOB_RAY_ANYHIT, B.JOB_PIP_POSITIVE], n_req)
@@ -204,6 +212,7 @@ def test_serve_replay_results_same_alone_and_in_mix(gpu_ready):
     arenas = dict(point_queries=_dev(pq), ray_queries=_dev(rq), pip_queries=_dev(gq))
     scenes = [pscene, tscene, gscene]
     kw = dict(radius=0.02, k_max=64, k=8, slots=4, **arenas)
+    _say("  serve_replay: 400 mixed requests")
     results, stats, captured = B.serve_replay(scenes, reqs, **kw)
     assert np.all(results["status"] == 0)
     assert np.all(results["done_s"] >= results["arrival_s"])
@@ -212,6 +221,7 @@ def test_serve_replay_results_same_alone_and_in_mix(gpu_ready):
     for i, rows in captured.items():
         alone = reqs[i:i + 1].copy()
         alone["arrival_s"] = 0.0
+        _say(f"  serve_replay: request {i} alone")
         res1, _, cap1 = B.serve_replay(scenes, alone, **kw)
         assert res1["rows"][0] == results["rows"][i]
         kind = int(reqs["kind"][i])
