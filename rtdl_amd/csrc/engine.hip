@@ -56,7 +56,11 @@ namespace {
 // active lane is lane 0.
 __device__ inline uint32_t lane0_add(uint32_t* counter, uint32_t value) {
   const uint32_t mine = (threadIdx.x & 63) == 0 ? value : 0u;
-  return __builtin_amdgcn_readfirstlane(atomicAdd(counter, mine));
+  // counters live in device global memory; the global-address-space atomic
+  // lets the compiler combine the wave's 64 adds into a single atomic
+  auto* g = (__attribute__((address_space(1))) uint32_t*)counter;
+  return __builtin_amdgcn_readfirstlane(
+      __hip_atomic_fetch_add(g, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 __device__ inline uint32_t wave_exclusive_sum(uint32_t v, uint32_t* total) {
@@ -355,19 +359,36 @@ __global__ void zero_words_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb
   if (i < nc) c[i] = 0ull;
 }
 
-enum Family : int { kFamPoints = 0, kFamRayAny, kFamRayCount, kFamRayClosest, kFamPip, kFamilies };
+// One persistent launch per family.  Neighbour jobs are split by the list
+// size their k needs (1, 2, 4, ..., 64), so a small-k job never pays for the
+// registers of a larger one in the same batch.
+enum Family : int {
+  kFamPoints1 = 0,  // ... kFamPoints1 + 6 for K = 64
+  kFamRayAny = kPointBuckets,
+  kFamRayCount,
+  kFamRayClosest,
+  kFamPip,
+};
+static_assert(kFamPip + 1 == kLaunchFamilies, "family count");
 
-int family_of(uint32_t kind) {
+int family_of(uint32_t kind, uint32_t k) {
   switch (kind) {
     case kJobFixedRadius:
     case kJobKnn:
-    case kJobBoundedKnn: return kFamPoints;
+    case kJobBoundedKnn: {
+      int b = 0;
+      while ((1u << b) < k) ++b;  // list_capacity(k) == 1 << b
+      return kFamPoints1 + b;
+    }
     case kJobRayAnyHit: return kFamRayAny;
     case kJobRayHitCount: return kFamRayCount;
     case kJobRayClosest: return kFamRayClosest;
     default: return kFamPip;
   }
 }
+
+// report group of a launch family: neighbours, any-hit, hit-count, closest, PIP
+int group_of(int family) { return family < kPointBuckets ? 0 : family - kPointBuckets + 1; }
 
 template <int K>
 void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
@@ -377,8 +398,6 @@ void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkI
     batch_points_kernel<K, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
   else
     batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
-  RTDL_LAUNCH_CHECK();
-  batch_fix_kernel<<<64, kFixBlock, 0, s>>>(jobs, amb, amb_count);
   RTDL_LAUNCH_CHECK();
 }
 
@@ -400,25 +419,28 @@ BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
   RTDL_HIP(hipGetDevice(&dev));
   RTDL_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
   stats_.resize(2);
-  item_counters_.resize(kFamilies);
+  item_counters_.resize(kLaunchFamilies);
   amb_count_.resize(1);
-  for (int f = 0; f < kFamilies; ++f) {
+  for (int f = 0; f < kLaunchFamilies; ++f) {
     RTDL_HIP(hipEventCreate(&fam_start_[f]));
     RTDL_HIP(hipEventCreate(&fam_stop_[f]));
   }
 }
 
 void BatchEngine::family_seconds(double out[5]) const {
-  for (int f = 0; f < kFamilies; ++f) {
-    out[f] = 0.0;
+  for (int g = 0; g < 5; ++g) out[g] = 0.0;
+  for (int f = 0; f < kLaunchFamilies; ++f) {
     if (!timed_[f]) continue;
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, fam_start_[f], fam_stop_[f]) == hipSuccess) out[f] = ms * 1e-3;
+    if (hipEventElapsedTime(&ms, fam_start_[f], fam_stop_[f]) == hipSuccess)
+      out[group_of(f)] += ms * 1e-3;
+    else
+      (void)hipGetLastError();
   }
 }
 
 BatchEngine::~BatchEngine() {
-  for (int f = 0; f < kFamilies; ++f) {
+  for (int f = 0; f < kLaunchFamilies; ++f) {
     if (fam_start_[f]) (void)hipEventDestroy(fam_start_[f]);
     if (fam_stop_[f]) (void)hipEventDestroy(fam_stop_[f]);
   }
@@ -467,17 +489,15 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   require(total < 0xFFFFFFFFull, "batch exceeds 2^32 queries");
   reserve(n, total);
   // order jobs' items by predicate family; each family gets one launch
-  std::vector<std::vector<WorkItem>> fam_items(kFamilies);
-  uint32_t max_k[kFamilies] = {0, 0, 0, 0, 0};
+  std::vector<std::vector<WorkItem>> fam_items(kLaunchFamilies);
   uint64_t start = 0;
   for (size_t i = 0; i < n; ++i) {
     const HostJob& h = jobs[i];
     require(h.qstride == 2 || h.qstride == 3 || h.qstride == 7, "query stride must be 2, 3 or 7");
     require(h.nq == 0 || h.queries != nullptr, "job queries must not be null");
     require(h.nq == 0 || h.rows != nullptr, "job rows must not be null");
-    const int fam = family_of(h.kind);
-    if (fam == kFamPoints) require(h.k >= 1 && h.k <= 64, "batched neighbour jobs need 1 <= k <= 64");
-    max_k[fam] = std::max(max_k[fam], h.k);
+    if (h.kind <= kJobBoundedKnn) require(h.k >= 1 && h.k <= 64, "batched neighbour jobs need 1 <= k <= 64");
+    const int fam = family_of(h.kind, h.k);
     DeviceJob& d = host_jobs_[i];
     std::memset(&d, 0, sizeof(d));
     d.scene = h.scene;
@@ -499,8 +519,8 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     start += h.nq;
   }
   host_starts_[n] = start;
-  size_t item_off[kFamilies + 1] = {0};
-  for (int f = 0; f < kFamilies; ++f) {
+  size_t item_off[kLaunchFamilies + 1] = {0};
+  for (int f = 0; f < kLaunchFamilies; ++f) {
     std::memcpy(host_items_ + item_off[f], fam_items[f].data(), fam_items[f].size() * sizeof(WorkItem));
     item_off[f + 1] = item_off[f] + fam_items[f].size();
   }
@@ -509,12 +529,12 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     RTDL_HIP(hipMemcpyAsync(jobs_.get(), host_jobs_, n * sizeof(DeviceJob), hipMemcpyHostToDevice, s));
     RTDL_HIP(hipMemcpyAsync(starts_.get(), host_starts_, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
   }
-  if (item_off[kFamilies])
-    RTDL_HIP(hipMemcpyAsync(items_.get(), host_items_, item_off[kFamilies] * sizeof(WorkItem),
+  if (item_off[kLaunchFamilies])
+    RTDL_HIP(hipMemcpyAsync(items_.get(), host_items_, item_off[kLaunchFamilies] * sizeof(WorkItem),
                             hipMemcpyHostToDevice, s));
-  const size_t zero_n = std::max<size_t>(n * 4, kFamilies + 1);
+  const size_t zero_n = std::max<size_t>(n * 4, kLaunchFamilies + 1);
   zero_words_kernel<<<blocks_for(zero_n, 256), 256, 0, s>>>(
-      reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kFamilies,
+      reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kLaunchFamilies,
       stats_.get(), 2);
   RTDL_LAUNCH_CHECK();
   fill_u32(amb_count_.get(), 1, 0u, s);
@@ -534,8 +554,11 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   };
   unsigned long long* st = stats_on_ ? stats_.get() : nullptr;
   const DeviceJob* dj = jobs_.get();
-  for (int f = 0; f < kFamilies; ++f) timed_[f] = false;
-  for (int f = 0; f < kFamilies; ++f) {
+  int last_points = -1;  // the exact fix-up runs once, after the last neighbour launch
+  for (int f = 0; f < kPointBuckets; ++f)
+    if (item_off[f + 1] > item_off[f]) last_points = f;
+  for (int f = 0; f < kLaunchFamilies; ++f) timed_[f] = false;
+  for (int f = 0; f < kLaunchFamilies; ++f) {
     const uint32_t n_items = (uint32_t)(item_off[f + 1] - item_off[f]);
     if (!n_items) continue;
     const WorkItem* it = items_.get() + item_off[f];
@@ -545,15 +568,19 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
       RTDL_HIP(hipEventRecord(fam_start_[f], s));
       timed_[f] = true;
     }
-    if (f == kFamPoints) {
-      switch (list_capacity(max_k[f])) {
-        case 1: launch_points<1>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 2: launch_points<2>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 4: launch_points<4>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 8: launch_points<8>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 16: launch_points<16>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 32: launch_points<32>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+    if (f < kPointBuckets) {
+      switch (f) {
+        case 0: launch_points<1>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 1: launch_points<2>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 2: launch_points<4>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 3: launch_points<8>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 4: launch_points<16>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 5: launch_points<32>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
         default: launch_points<64>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+      }
+      if (f == last_points) {
+        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
+        RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamRayAny) {
       launch_rays_batch<0>(stats_on_, grid, dj, it, n_items, ctr, st, s);

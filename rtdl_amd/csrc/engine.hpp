@@ -58,6 +58,8 @@ size_t job_row_bytes(uint32_t kind);
 uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_cap);
 
 constexpr uint32_t kPipHitCap = 16;   // containing polygons kept per point in a batch
+constexpr int kPointBuckets = 7;      // neighbour list sizes 1, 2, 4, ..., 64
+constexpr int kLaunchFamilies = kPointBuckets + 4;  // + ray any / count / closest, PIP
 constexpr uint32_t kStatusRowOverflow = 1u;
 constexpr uint32_t kStatusPipOverflow = 2u;
 
@@ -149,8 +151,8 @@ class BatchEngine {
   SortWorkspace sort_ws_;
   bool stats_on_ = false;
   bool timing_on_ = false;
-  bool timed_[5] = {false, false, false, false, false};
-  hipEvent_t fam_start_[5] = {}, fam_stop_[5] = {};
+  bool timed_[kLaunchFamilies] = {};
+  hipEvent_t fam_start_[kLaunchFamilies] = {}, fam_stop_[kLaunchFamilies] = {};
   int cus_ = 256;
 };
 

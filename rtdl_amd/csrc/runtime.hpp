@@ -82,6 +82,11 @@ template <class F>
 int guarded(char* error_out, size_t error_size, F&& body) {
   if (error_out && error_size) error_out[0] = '\0';
   try {
+    // the launch checks read hipGetLastError, which also reports the status
+    // of any earlier call on this thread whose result was not examined (a
+    // cleanup call, another library); start every entry point from a clean
+    // record so that a launch is only blamed for its own error
+    (void)hipGetLastError();
     body();
     return 0;
   } catch (const std::exception& e) {

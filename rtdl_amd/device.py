@@ -84,6 +84,21 @@ class DeviceBuffer:
     def numel(self) -> int:
         return int(np.prod(self.shape, dtype=np.int64))
 
+    def close(self) -> None:
+        if getattr(self, "_ptr", None) is not None:
+            ptr, self._ptr = self._ptr, None
+            err = ctypes.create_string_buffer(1024)
+            _check(rt_native._library().rtdl_mi355x_device_free(ptr, err, 1024), err,
+                   "rtdl_mi355x_device_free")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception as exc:  # a finaliser cannot raise; report instead
+            import sys
+
+            print(f"rtdl_amd.device: {exc}", file=sys.stderr)
+
     def __getitem__(self, rows) -> "DeviceView":
         """A view of a contiguous range of rows (first axis): buf[a:b]."""
         if not isinstance(rows, slice) or rows.step not in (None, 1):
@@ -95,7 +110,8 @@ class DeviceBuffer:
 
 
 class DeviceView:
-    """Rows [a, b) of a DeviceBuffer; keeps the buffer alive."""
+    """Rows [a, b) of a DeviceBuffer; keeps the buffer alive and never frees
+    anything itself (only the DeviceBuffer owns the allocation)."""
 
     def __init__(self, base: DeviceBuffer, ptr: int, shape: tuple):
         self.base = base
@@ -108,18 +124,6 @@ class DeviceView:
 
     def numel(self) -> int:
         return int(np.prod(self.shape, dtype=np.int64))
-
-    def close(self) -> None:
-        if getattr(self, "_ptr", None) is not None:
-            err = ctypes.create_string_buffer(1024)
-            rt_native._library().rtdl_mi355x_device_free(self._ptr, err, 1024)
-            self._ptr = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
 
 def to_device(array, dtype=None) -> DeviceBuffer:
