@@ -265,10 +265,11 @@ enum {
   RTDL_JOB_PIP_POSITIVE = 6
 };
 
-/* status bits of a job / request */
-#define RTDL_STATUS_ROW_OVERFLOW 1u
-#define RTDL_STATUS_PIP_OVERFLOW 2u
-#define RTDL_STATUS_REJECTED 4u
+/* status bits of a job / request.  A PIP job's default row budget is 16 rows
+ * per point on average (row_capacity = 16 * query_count); one point may have
+ * any number of containing polygons within that budget. */
+#define RTDL_STATUS_ROW_OVERFLOW 1u /* rows beyond row_capacity were dropped */
+#define RTDL_STATUS_REJECTED 4u     /* served request larger than a slot's row region */
 
 typedef struct RtdlBatchJob {
   void* scene;           /* prepared handle */

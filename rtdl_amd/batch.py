@@ -29,9 +29,8 @@ JOB_RAY_CLOSEST = 5
 JOB_PIP_POSITIVE = 6
 
 STATUS_ROW_OVERFLOW = 1
-STATUS_PIP_OVERFLOW = 2
 STATUS_REJECTED = 4
-PIP_HIT_CAP = 16
+PIP_HIT_CAP = 16  # default PIP row budget per point (average); one point may exceed it
 
 ROW_DTYPES = {
     JOB_FIXED_RADIUS: rt_native.FRN_ROW_DTYPE,

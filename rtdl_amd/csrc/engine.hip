@@ -253,11 +253,15 @@ __global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __rest
   if (STATS) flush_lane_stats(stats, st);
 }
 
+// A point inside more than kPipHitCap polygons reserves all its rows, writes
+// the first kPipHitCap from registers and is queued for batch_pip_over_kernel,
+// which walks it again and writes every hit, so the row set stays exact.
 template <bool STATS>
 __global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restrict__ jobs,
                                                         const WorkItem* __restrict__ items,
                                                         uint32_t n_items, uint32_t* item_counter,
-                                                        double eps_sq, unsigned long long* stats) {
+                                                        double eps_sq, uint4* over, uint32_t* over_count,
+                                                        unsigned long long* stats) {
   const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
   for (;;) {
@@ -288,9 +292,8 @@ __global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restr
                         }
                         ++found;
                       });
-      if (found > kPipHitCap) atomicOr(&j.counters->status, kStatusPipOverflow);
     }
-    const uint32_t c = active ? min(found, kPipHitCap) : 0u;
+    const uint32_t c = active ? found : 0u;
     const uint64_t at0 = wave_reserve(j.counters, c);
     if (c) {
       if (at0 + c > j.row_capacity) {
@@ -305,10 +308,31 @@ __global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restr
             ++r;
           }
         }
+        if (c > kPipHitCap) over[atomicAdd(over_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
       }
     }
   }
   if (STATS) flush_lane_stats(stats, st);
+}
+
+// Writes every containing polygon of the queued points (see batch_pip_kernel).
+__global__ __launch_bounds__(64) void batch_pip_over_kernel(const DeviceJob* __restrict__ jobs,
+                                                            const uint4* __restrict__ over,
+                                                            const uint32_t* over_count, double eps_sq) {
+  const uint32_t n = *over_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint4 o = over[i];
+    const DeviceJob& j = jobs[o.x];
+    const float* p = j.queries + (size_t)o.y * j.qstride;
+    PipRowRec* rows = static_cast<PipRowRec*>(j.rows) + o.z;
+    uint32_t r = 0;
+    LaneStats st;
+    walk_pip<false>(j.scene, j.scene.root, j.scene.end, p[0], p[1], true, eps_sq, st,
+                    [&](uint32_t slot) {
+                      if (r < o.w) rows[r] = PipRowRec{o.y, j.scene.poly[slot].z, 1u};
+                      ++r;
+                    });
+  }
 }
 
 __device__ inline uint32_t spread10b(uint32_t v) {
@@ -420,7 +444,7 @@ BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
   RTDL_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
   stats_.resize(2);
   item_counters_.resize(kLaunchFamilies);
-  amb_count_.resize(1);
+  amb_count_.resize(2);  // [0] ambiguous neighbour queries, [1] PIP points past the hit cap
   for (int f = 0; f < kLaunchFamilies; ++f) {
     RTDL_HIP(hipEventCreate(&fam_start_[f]));
     RTDL_HIP(hipEventCreate(&fam_stop_[f]));
@@ -477,6 +501,7 @@ void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
     keys_.resize(max_queries);
     order_.resize(max_queries);
     amb_.resize(max_queries);
+    pip_over_.resize(max_queries);
     sort_ws_.reserve(max_queries);
     cap_queries_ = max_queries;
   }
@@ -537,7 +562,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
       reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kLaunchFamilies,
       stats_.get(), 2);
   RTDL_LAUNCH_CHECK();
-  fill_u32(amb_count_.get(), 1, 0u, s);
+  fill_u32(amb_count_.get(), 2, 0u, s);
   if (morton_sort && total > 1) {
     batch_keys_kernel<<<blocks_for(total, 256), 256, 0, s>>>(jobs_.get(), starts_.get(), (uint32_t)n,
                                                               total, keys_.get(), order_.get());
@@ -590,10 +615,15 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
       launch_rays_batch<2>(stats_on_, grid, dj, it, n_items, ctr, st, s);
     } else {
       const double eps_sq = std::pow(1.0e-12, 2.0);  // reference.py:757 `eps ** 2`
+      uint32_t* over_count = amb_count_.get() + 1;
       if (stats_on_)
-        batch_pip_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, eps_sq, st);
+        batch_pip_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
+                                                    over_count, st);
       else
-        batch_pip_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, eps_sq, st);
+        batch_pip_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
+                                                     over_count, st);
+      RTDL_LAUNCH_CHECK();
+      batch_pip_over_kernel<<<64, 64, 0, s>>>(dj, pip_over_.get(), over_count, eps_sq);
       RTDL_LAUNCH_CHECK();
     }
     if (timing_on_) RTDL_HIP(hipEventRecord(fam_stop_[f], s));

@@ -57,11 +57,10 @@ size_t job_row_bytes(uint32_t kind);
 // Upper bound of rows a job can emit (PIP: `pip_cap` per point).
 uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_cap);
 
-constexpr uint32_t kPipHitCap = 16;   // containing polygons kept per point in a batch
+constexpr uint32_t kPipHitCap = 16;   // containing polygons held in registers per point
 constexpr int kPointBuckets = 7;      // neighbour list sizes 1, 2, 4, ..., 64
 constexpr int kLaunchFamilies = kPointBuckets + 4;  // + ray any / count / closest, PIP
 constexpr uint32_t kStatusRowOverflow = 1u;
-constexpr uint32_t kStatusPipOverflow = 2u;
 
 struct JobCounters {
   uint32_t rows;
@@ -146,6 +145,7 @@ class BatchEngine {
   DeviceArray<uint64_t> keys_;
   DeviceArray<uint32_t> order_;
   DeviceArray<uint4> amb_;               // (job, query, row base, count)
+  DeviceArray<uint4> pip_over_;          // (job, point, row base, hit count)
   DeviceArray<uint32_t> amb_count_;
   DeviceArray<unsigned long long> stats_;
   SortWorkspace sort_ws_;

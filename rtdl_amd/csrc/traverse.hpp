@@ -181,14 +181,72 @@ __device__ inline uint64_t pack_key(double d2, uint32_t slot) {
 }
 __device__ inline uint32_t key_bits(uint64_t e) { return (uint32_t)(e >> 32); }
 
+// Squared-distance bound for a k-nearest search without a radius: the k-th
+// smallest distance of the subtree is at most the largest of any k of its
+// primitives.  A greedy descent (nearer child box first; the right child of
+// an internal node is its left child's escape) reaches a leaf near the query,
+// and the k primitives that follow it in tree order give a tight bound.  The
+// bound is widened by 2^-48 so that every point whose FP64 distance equals
+// the k-th distance still competes (ties are decided by id).  Returns +inf
+// when fewer than k primitives follow the leaf.
+template <bool STATS>
+__device__ inline double knn_seed_bound2(const SceneView& s, uint32_t root, uint32_t end,
+                                         const float qf[3], double qx, double qy, double qz,
+                                         uint32_t k, LaneStats& st) {
+  if (root == kEndNode) return __builtin_inf();
+  uint32_t node = root;
+  BvhNode nd = s.nodes[node];
+  for (uint32_t guard = s.n_nodes + 1; !node_is_leaf(nd.link) && guard; --guard) {
+    const uint32_t l = nd.link;
+    const BvhNode ln = s.nodes[l];
+    const uint32_t r = ln.escape;
+    const BvhNode rn = s.nodes[r];
+    if (STATS) st.nodes += 2;
+    const float dl = box_dist2(qf, ln.lo, ln.hi), dr = box_dist2(qf, rn.lo, rn.hi);
+    if (dr < dl) {
+      node = r;
+      nd = rn;
+    } else {
+      node = l;
+      nd = ln;
+    }
+  }
+  uint32_t got = 0;
+  double worst = 0.0;
+  for (uint32_t guard = s.n_nodes + 1; node != end && got < k && guard; --guard) {
+    nd = s.nodes[node];
+    if (STATS) ++st.nodes;
+    if (!node_is_leaf(nd.link)) {
+      node = nd.link;
+      continue;
+    }
+    const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
+    for (uint32_t j = 0; j < cnt && got < k; ++j) {
+      const float4 p = s.prims[first + j];
+      if (STATS) ++st.prims;
+      worst = fmax(worst, point_d2(s, first + j, p, qx, qy, qz));
+      ++got;
+    }
+    node = nd.escape;
+  }
+  return got == k ? worst * 1.0000000000000035527136788005009293556213378906250 : __builtin_inf();
+}
+
 template <int K, bool STATS>
 __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_t end, double qx,
                                       double qy, double qz, double radius, uint32_t k,
                                       PackedTopK<K>& list, LaneStats& st) {
   const float qf[3] = {(float)qx, (float)qy, (float)qz};
   const float m = coord_bound(scene_m(s), qx, qy, qz);
-  const double radius_sq = radius * radius;
+  double radius_sq = radius * radius;
   float thr2 = frn_threshold2(radius, m);
+  if (!(radius < __builtin_inf())) {  // k nearest without a radius: seed a bound first
+    const double b2 = knn_seed_bound2<STATS>(s, root, end, qf, qx, qy, qz, k, st);
+    if (b2 < radius_sq) {
+      radius_sq = b2;
+      thr2 = frn_threshold2(sqrt(b2), m);
+    }
+  }
   list.init(k);
   uint32_t drop_min = 0xFFFFFFFFu;  // smallest key bits of any dropped candidate
   walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float) {

@@ -263,3 +263,62 @@ def test_serve_admission_rejects_oversized_requests(gpu_ready):
     res, stats, _ = B.serve_replay([scene], reqs, radius=0.05, k_max=64, k=1, point_queries=pq,
                                    slots=2, memory_budget_bytes=2 * 1024 * 1024)
     assert res["status"].tolist() == [0, B.STATUS_REJECTED, 0]
+
+
+def test_batch_pip_points_inside_many_polygons(gpu_ready):
+    _say("case: batched PIP, points inside up to 40 polygons (past the 16-hit register list)")
+    rng = np.random.default_rng(81)
+    # 40 nested squares around (0.5, 0.5) plus random star polygons
+    verts, voff, vcnt = [], [], []
+    for i in range(40):
+        h = 0.01 + 0.01 * i
+        voff.append(sum(vcnt))
+        vcnt.append(4)
+        verts += [(0.5 - h, 0.5 - h), (0.5 + h, 0.5 - h), (0.5 + h, 0.5 + h), (0.5 - h, 0.5 + h)]
+    stars = cases.star_polygons(rng, 300, rmin=0.01, rmax=0.05)
+    sv = stars["vertices_xy"].reshape(-1, 2)
+    base = len(verts)
+    for r in stars["refs"]:
+        voff.append(base + int(r["vertex_offset"]))
+        vcnt.append(int(r["vertex_count"]))
+    verts = np.concatenate([np.asarray(verts, dtype=np.float64), sv])
+    # FP32 on the device: the oracle gets the same FP32-rounded values
+    verts32 = verts.astype(np.float32)
+    voff = np.asarray(voff, dtype=np.int64)
+    vcnt = np.asarray(vcnt, dtype=np.int64)
+    scene = B.prepare_polygon_instances(_dev_i32(voff), _dev_i32(vcnt), _dev(verts32), [0, len(voff)])
+    pts = np.concatenate([0.5 + 0.05 * rng.standard_normal((3000, 2)), rng.random((2000, 2))]).astype(np.float32)
+    # the default budget is 16 rows per point on average; these points average more
+    job = B.BatchJob(scene, B.JOB_PIP_POSITIVE, _dev(pts), row_capacity=64 * len(pts))
+    _say("  run_batch: 1 PIP job, 5000 points")
+    (res,), _ = B.run_batch([job])
+    want = oracle.pip_pairs(pts[:, 0].astype(np.float64), pts[:, 1].astype(np.float64), voff, vcnt,
+                            verts32.astype(np.float64).reshape(-1), positive_only=True)
+    per_point = np.bincount(want[:, 0], minlength=len(pts))
+    assert per_point.max() > 16  # the case exercises the overflow pass
+    assert res["status"] == 0
+    got = _canon(B.job_rows(job, res), ["point_id", "polygon_id"])
+    assert np.array_equal(got["point_id"], want[:, 0])
+    assert np.array_equal(got["polygon_id"], want[:, 1])
+
+
+def test_batch_knn_lattice_ties(gpu_ready):
+    _say("case: batched KNN on a lattice with many equal distances")
+    rng = np.random.default_rng(82)
+    g = np.stack(np.meshgrid(np.arange(14), np.arange(14), np.arange(14), indexing="ij"), -1).reshape(-1, 3)
+    order = rng.permutation(len(g))
+    g = g[order]  # prepared points carry ids 0..n-1 in input order
+    s3 = (g / 16.0).astype(np.float64)
+    sdict = {"id": np.arange(len(g), dtype=np.uint32), "x": s3[:, 0], "y": s3[:, 1], "z": s3[:, 2]}
+    scene = _prepared_points(s3)
+    jobs, wants = [], []
+    for k in (1, 6, 8, 27, 64):
+        qg = rng.integers(0, 28, (700, 3)) / 32.0
+        qd = {"id": np.arange(len(qg), dtype=np.uint32), "x": qg[:, 0], "y": qg[:, 1], "z": qg[:, 2]}
+        jobs.append(B.BatchJob(scene, B.JOB_KNN, _dev(qg), k=k))
+        wants.append(oracle.neighbors("knn", qd, sdict, k=k))
+    _say("  run_batch: 5 KNN jobs")
+    results, _ = B.run_batch(jobs)
+    for job, res, want in zip(jobs, results, wants):
+        assert res["status"] == 0
+        _check_nn(B.job_rows(job, res), want, True)

@@ -185,6 +185,23 @@ def test_knn_random_bit_exact(gpu_ready, dim, k):
     _assert_nn_equal(got, want, ranked=True)
 
 
+@pytest.mark.parametrize("k", [5, 13, 64])
+def test_knn_lattice_ties_bit_exact(gpu_ready, k):
+    """Points on a 1/16 lattice (exact in FP32 and FP64) and queries on the
+    lattice and half-lattice: many neighbours share one distance, so the
+    (distance, id) tie rule and the k-th-distance boundary decide the rows."""
+    _say(f"case: knn lattice ties k={k}")
+    rng = np.random.default_rng(3000 + k)
+    g = np.stack(np.meshgrid(np.arange(12), np.arange(12), np.arange(12), indexing="ij"), -1).reshape(-1, 3)
+    ids = rng.permutation(len(g)).astype(np.uint32)  # ids unrelated to position
+    s = {"id": ids, "x": g[:, 0] / 16.0, "y": g[:, 1] / 16.0, "z": g[:, 2] / 16.0}
+    qg = rng.integers(0, 24, (900, 3)) / 32.0
+    q = {"id": np.arange(len(qg), dtype=np.uint32), "x": qg[:, 0], "y": qg[:, 1], "z": qg[:, 2]}
+    got = m.knn_rows_mi355x(q, s, k=k)
+    want = oracle.neighbors("knn", q, s, k=k)
+    _assert_nn_equal(got, want, ranked=True)
+
+
 def test_knn_short_result_is_not_padded(gpu_ready):
     _say("case: knn short result")
     q = {"id": np.array([10], np.uint32), "x": np.array([1.0]), "y": np.array([1.0])}
