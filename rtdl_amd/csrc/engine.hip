@@ -187,27 +187,29 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
     float thr2 = frn_threshold2(j.radius, m);
     uint32_t count = 0;  // entries [0, count) sorted by (distance, id)
     LaneStats st;
-    walk_points<false>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
-                       [&](uint32_t slot, const float4& p, float) {
-                         const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
-                         if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
-                         const double dist = sqrt(d2);
-                         const uint32_t ident = __float_as_uint(p.w);
-                         if (count == k &&
-                             !key_less(dist, ident, ld[(k - 1) * kFixBlock], li[(k - 1) * kFixBlock]))
-                           return;
-                         uint32_t pos = count < k ? count++ : k - 1;
-                         while (pos > 0 && key_less(dist, ident, ld[(pos - 1) * kFixBlock],
-                                                    li[(pos - 1) * kFixBlock])) {
-                           ld[pos * kFixBlock] = ld[(pos - 1) * kFixBlock];
-                           li[pos * kFixBlock] = li[(pos - 1) * kFixBlock];
-                           --pos;
-                         }
-                         ld[pos * kFixBlock] = dist;
-                         li[pos * kFixBlock] = ident;
-                         if (count == k)
-                           thr2 = frn_threshold2(fmin(j.radius, ld[(k - 1) * kFixBlock]), m);
-                       });
+    if (j.scene.root == kEndNode) continue;
+    auto visit = [&](uint32_t slot, const float4& p, float) {
+      const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
+      if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
+      const double dist = sqrt(d2);
+      const uint32_t ident = __float_as_uint(p.w);
+      if (count == k && !key_less(dist, ident, ld[(k - 1) * kFixBlock], li[(k - 1) * kFixBlock]))
+        return;
+      uint32_t pos = count < k ? count++ : k - 1;
+      while (pos > 0 &&
+             key_less(dist, ident, ld[(pos - 1) * kFixBlock], li[(pos - 1) * kFixBlock])) {
+        ld[pos * kFixBlock] = ld[(pos - 1) * kFixBlock];
+        li[pos * kFixBlock] = li[(pos - 1) * kFixBlock];
+        --pos;
+      }
+      ld[pos * kFixBlock] = dist;
+      li[pos * kFixBlock] = ident;
+      if (count == k) thr2 = frn_threshold2(fmin(j.radius, ld[(k - 1) * kFixBlock]), m);
+    };
+    // near subtree first, as in the register-list walks (traverse.hpp)
+    const uint32_t seed = near_subtree<false>(j.scene, j.scene.root, qf, seed_levels(k), st);
+    walk_points<false>(j.scene, seed, j.scene.nodes[seed].escape, qf, thr2, st, visit);
+    walk_points<false>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit, seed);
     const uint32_t rows = min(count, a.w);
     for (uint32_t r = 0; r < rows; ++r)
       write_neighbor_row<0>(j, (uint64_t)a.z + r, a.y, li[r * kFixBlock], ld[r * kFixBlock], r + 1);

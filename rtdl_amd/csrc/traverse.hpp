@@ -56,13 +56,18 @@ __device__ inline double point_d2(const SceneView& s, uint32_t slot, const float
 // visitor may shrink).  visit(slot, p, d2f) -> void.
 template <bool STATS, class Visit>
 __device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t end, const float qf[3],
-                                   float& thr2, LaneStats& st, Visit&& visit) {
+                                   float& thr2, LaneStats& st, Visit&& visit,
+                                   uint32_t skip = kEndNode) {
   uint32_t node = root;
   // a preorder walk visits each node at most once; the guard only bounds the
   // loop should the tree ever be malformed
   for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
     const BvhNode nd = s.nodes[node];
     if (STATS) ++st.nodes;
+    if (node == skip) {  // a subtree the caller has already walked
+      node = nd.escape;
+      continue;
+    }
     const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
     const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
     if (!(box_dist2(qf, lo, hi) <= thr2)) {
@@ -84,6 +89,47 @@ __device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t e
     }
     node = nd.escape;
   }
+}
+
+// Root of a small subtree around the query: a greedy descent (nearer child
+// box first; the right child of an internal node is its left child's escape)
+// reaches a leaf near the query, and the node `levels` steps above that leaf
+// is returned.  Neighbour searches walk this subtree first, so the candidate
+// list fills with near points and the pruning threshold shrinks early; the
+// main walk then skips it, so every primitive is still offered exactly once.
+template <bool STATS>
+__device__ inline uint32_t near_subtree(const SceneView& s, uint32_t root, const float qf[3],
+                                        uint32_t levels, LaneStats& st) {
+  constexpr int kMaxUp = 7;
+  uint32_t up[kMaxUp + 1];  // up[i]: i steps above the current node
+#pragma unroll
+  for (int i = 0; i <= kMaxUp; ++i) up[i] = root;
+  BvhNode nd = s.nodes[root];
+  if (STATS) ++st.nodes;
+  for (uint32_t guard = s.n_nodes + 1; !node_is_leaf(nd.link) && guard; --guard) {
+    const uint32_t l = nd.link;
+    const BvhNode ln = s.nodes[l];
+    const uint32_t r = ln.escape;
+    const BvhNode rn = s.nodes[r];
+    if (STATS) st.nodes += 2;
+    const bool right = box_dist2(qf, rn.lo, rn.hi) < box_dist2(qf, ln.lo, ln.hi);
+#pragma unroll
+    for (int i = kMaxUp; i > 0; --i) up[i] = up[i - 1];
+    up[0] = right ? r : l;
+    nd = right ? rn : ln;
+  }
+  uint32_t pick = up[0];
+#pragma unroll
+  for (int i = 1; i <= kMaxUp; ++i)
+    if ((uint32_t)i == levels) pick = up[i];
+  return pick;
+}
+
+// levels above the leaf for a k-neighbour seed: about k primitives
+__device__ inline uint32_t seed_levels(uint32_t k) {
+  uint32_t l = 1;
+  while (l < 7 && (1u << l) < k) ++l;
+  return l;
 }
 
 // Exact list: K (distance, id) entries in registers, k of them live in slots
@@ -133,7 +179,8 @@ __device__ inline void collect_exact(const SceneView& s, uint32_t root, uint32_t
   float thr2 = frn_threshold2(radius, m);
   list.init(k);
   inside = 0;
-  walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float) {
+  if (root == kEndNode) return;
+  auto visit = [&](uint32_t slot, const float4& p, float) {
     const double d2 = point_d2(s, slot, p, qx, qy, qz);
     if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
     ++inside;
@@ -143,7 +190,10 @@ __device__ inline void collect_exact(const SceneView& s, uint32_t root, uint32_t
       list.insert(dist, ident);
       if (!COUNT_ALL && list.full()) thr2 = frn_threshold2(fmin(radius, list.d[K - 1]), m);
     }
-  });
+  };
+  const uint32_t seed = near_subtree<STATS>(s, root, qf, seed_levels(k), st);
+  walk_points<STATS>(s, seed, s.nodes[seed].escape, qf, thr2, st, visit);
+  walk_points<STATS>(s, root, end, qf, thr2, st, visit, seed);
 }
 
 // Packed list: each entry is (FP32 bits of fl32(d2) << 32) | slot, so the
@@ -181,75 +231,18 @@ __device__ inline uint64_t pack_key(double d2, uint32_t slot) {
 }
 __device__ inline uint32_t key_bits(uint64_t e) { return (uint32_t)(e >> 32); }
 
-// Squared-distance bound for a k-nearest search without a radius: the k-th
-// smallest distance of the subtree is at most the largest of any k of its
-// primitives.  A greedy descent (nearer child box first; the right child of
-// an internal node is its left child's escape) reaches a leaf near the query,
-// and the k primitives that follow it in tree order give a tight bound.  The
-// bound is widened by 2^-48 so that every point whose FP64 distance equals
-// the k-th distance still competes (ties are decided by id).  Returns +inf
-// when fewer than k primitives follow the leaf.
-template <bool STATS>
-__device__ inline double knn_seed_bound2(const SceneView& s, uint32_t root, uint32_t end,
-                                         const float qf[3], double qx, double qy, double qz,
-                                         uint32_t k, LaneStats& st) {
-  if (root == kEndNode) return __builtin_inf();
-  uint32_t node = root;
-  BvhNode nd = s.nodes[node];
-  for (uint32_t guard = s.n_nodes + 1; !node_is_leaf(nd.link) && guard; --guard) {
-    const uint32_t l = nd.link;
-    const BvhNode ln = s.nodes[l];
-    const uint32_t r = ln.escape;
-    const BvhNode rn = s.nodes[r];
-    if (STATS) st.nodes += 2;
-    const float dl = box_dist2(qf, ln.lo, ln.hi), dr = box_dist2(qf, rn.lo, rn.hi);
-    if (dr < dl) {
-      node = r;
-      nd = rn;
-    } else {
-      node = l;
-      nd = ln;
-    }
-  }
-  uint32_t got = 0;
-  double worst = 0.0;
-  for (uint32_t guard = s.n_nodes + 1; node != end && got < k && guard; --guard) {
-    nd = s.nodes[node];
-    if (STATS) ++st.nodes;
-    if (!node_is_leaf(nd.link)) {
-      node = nd.link;
-      continue;
-    }
-    const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
-    for (uint32_t j = 0; j < cnt && got < k; ++j) {
-      const float4 p = s.prims[first + j];
-      if (STATS) ++st.prims;
-      worst = fmax(worst, point_d2(s, first + j, p, qx, qy, qz));
-      ++got;
-    }
-    node = nd.escape;
-  }
-  return got == k ? worst * 1.0000000000000035527136788005009293556213378906250 : __builtin_inf();
-}
-
 template <int K, bool STATS>
 __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_t end, double qx,
                                       double qy, double qz, double radius, uint32_t k,
                                       PackedTopK<K>& list, LaneStats& st) {
   const float qf[3] = {(float)qx, (float)qy, (float)qz};
   const float m = coord_bound(scene_m(s), qx, qy, qz);
-  double radius_sq = radius * radius;
+  const double radius_sq = radius * radius;
   float thr2 = frn_threshold2(radius, m);
-  if (!(radius < __builtin_inf())) {  // k nearest without a radius: seed a bound first
-    const double b2 = knn_seed_bound2<STATS>(s, root, end, qf, qx, qy, qz, k, st);
-    if (b2 < radius_sq) {
-      radius_sq = b2;
-      thr2 = frn_threshold2(sqrt(b2), m);
-    }
-  }
   list.init(k);
   uint32_t drop_min = 0xFFFFFFFFu;  // smallest key bits of any dropped candidate
-  walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float) {
+  if (root == kEndNode) return false;
+  auto visit = [&](uint32_t slot, const float4& p, float) {
     const double d2 = point_d2(s, slot, p, qx, qy, qz);
     if (d2 > radius_sq) return;
     const uint64_t v = pack_key(d2, slot);
@@ -264,7 +257,10 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
     } else {
       drop_min = min(drop_min, key_bits(v));
     }
-  });
+  };
+  const uint32_t seed = near_subtree<STATS>(s, root, qf, seed_levels(k), st);
+  walk_points<STATS>(s, seed, s.nodes[seed].escape, qf, thr2, st, visit);
+  walk_points<STATS>(s, root, end, qf, thr2, st, visit, seed);
   bool ambiguous = list.full() && drop_min == key_bits(list.e[K - 1]);
 #pragma unroll
   for (int i = 1; i < K; ++i) {
