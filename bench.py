@@ -540,7 +540,8 @@ def main() -> int:
         if baselines.get("per_request_resident", {}).get("queries_per_s") else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": f"batch_{'points' if dominant in ('frn', 'knn') else dominant}_kernel ({dominant})",
+            "kernel": {"frn": "batch_frn_kernel + batch_frn_heavy_kernel", "knn": "batch_points_kernel",
+                       "ray": "batch_rays_kernel", "pip": "batch_pip_kernel"}[dominant] + f" ({dominant})",
             "achieved": d["achieved_GBps"],
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

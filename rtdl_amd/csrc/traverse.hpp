@@ -91,19 +91,22 @@ __device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t e
   }
 }
 
-// Root of a small subtree around the query: a greedy descent (nearer child
-// box first; the right child of an internal node is its left child's escape)
-// reaches a leaf near the query, and the node `levels` steps above that leaf
-// is returned.  Neighbour searches walk this subtree first, so the candidate
-// list fills with near points and the pruning threshold shrinks early; the
-// main walk then skips it, so every primitive is still offered exactly once.
-template <bool STATS>
-__device__ inline uint32_t near_subtree(const SceneView& s, uint32_t root, const float qf[3],
-                                        uint32_t levels, LaneStats& st) {
-  constexpr int kMaxUp = 7;
-  uint32_t up[kMaxUp + 1];  // up[i]: i steps above the current node
+// Neighbour walk in expanding rings around the query.  A greedy descent
+// (nearer child box first; the right child of an internal node is its left
+// child's escape) finds the query's leaf and its ancestors up[1..7].  The leaf
+// is walked first, then each ancestor's subtree without the part already
+// walked, then the rest of the tree: candidates arrive roughly nearest first,
+// so a shrinking threshold (a full k list) prunes early, and every primitive
+// is still offered to `visit` exactly once.
+template <bool STATS, class Visit>
+__device__ inline void walk_near_first(const SceneView& s, uint32_t root, uint32_t end,
+                                       const float qf[3], float& thr2, LaneStats& st,
+                                       Visit&& visit) {
+  if (root == kEndNode) return;
+  constexpr int kUp = 7;
+  uint32_t up[kUp + 1];  // up[i]: i steps above the leaf; the root repeats above it
 #pragma unroll
-  for (int i = 0; i <= kMaxUp; ++i) up[i] = root;
+  for (int i = 0; i <= kUp; ++i) up[i] = root;
   BvhNode nd = s.nodes[root];
   if (STATS) ++st.nodes;
   for (uint32_t guard = s.n_nodes + 1; !node_is_leaf(nd.link) && guard; --guard) {
@@ -114,22 +117,25 @@ __device__ inline uint32_t near_subtree(const SceneView& s, uint32_t root, const
     if (STATS) st.nodes += 2;
     const bool right = box_dist2(qf, rn.lo, rn.hi) < box_dist2(qf, ln.lo, ln.hi);
 #pragma unroll
-    for (int i = kMaxUp; i > 0; --i) up[i] = up[i - 1];
+    for (int i = kUp; i > 0; --i) up[i] = up[i - 1];
     up[0] = right ? r : l;
     nd = right ? rn : ln;
   }
-  uint32_t pick = up[0];
+  uint32_t done = kEndNode;  // root of the subtree walked so far
+#pragma unroll 1
+  for (int ring = 0; ring <= kUp + 1; ++ring) {
+    uint32_t from = root;
+    if (ring <= kUp) {
 #pragma unroll
-  for (int i = 1; i <= kMaxUp; ++i)
-    if ((uint32_t)i == levels) pick = up[i];
-  return pick;
-}
-
-// levels above the leaf for a k-neighbour seed: about k primitives
-__device__ inline uint32_t seed_levels(uint32_t k) {
-  uint32_t l = 1;
-  while (l < 7 && (1u << l) < k) ++l;
-  return l;
+      for (int i = 0; i <= kUp; ++i)
+        if (i == ring) from = up[i];
+    }
+    if (from == done) continue;
+    const uint32_t stop = from == root ? end : s.nodes[from].escape;
+    walk_points<STATS>(s, from, stop, qf, thr2, st, visit, done);
+    if (from == root) break;
+    done = from;
+  }
 }
 
 // Exact list: K (distance, id) entries in registers, k of them live in slots
@@ -191,9 +197,7 @@ __device__ inline void collect_exact(const SceneView& s, uint32_t root, uint32_t
       if (!COUNT_ALL && list.full()) thr2 = frn_threshold2(fmin(radius, list.d[K - 1]), m);
     }
   };
-  const uint32_t seed = near_subtree<STATS>(s, root, qf, seed_levels(k), st);
-  walk_points<STATS>(s, seed, s.nodes[seed].escape, qf, thr2, st, visit);
-  walk_points<STATS>(s, root, end, qf, thr2, st, visit, seed);
+  walk_near_first<STATS>(s, root, end, qf, thr2, st, visit);
 }
 
 // Packed list: each entry is (FP32 bits of fl32(d2) << 32) | slot, so the
@@ -258,9 +262,7 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
       drop_min = min(drop_min, key_bits(v));
     }
   };
-  const uint32_t seed = near_subtree<STATS>(s, root, qf, seed_levels(k), st);
-  walk_points<STATS>(s, seed, s.nodes[seed].escape, qf, thr2, st, visit);
-  walk_points<STATS>(s, root, end, qf, thr2, st, visit, seed);
+  walk_near_first<STATS>(s, root, end, qf, thr2, st, visit);
   bool ambiguous = list.full() && drop_min == key_bits(list.e[K - 1]);
 #pragma unroll
   for (int i = 1; i < K; ++i) {
