@@ -540,7 +540,7 @@ def main() -> int:
         if baselines.get("per_request_resident", {}).get("queries_per_s") else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": {"frn": "batch_frn_kernel + batch_frn_heavy_kernel", "knn": "batch_points_kernel",
+            "kernel": {"frn": "batch_points_kernel", "knn": "batch_points_kernel",
                        "ray": "batch_rays_kernel", "pip": "batch_pip_kernel"}[dominant] + f" ({dominant})",
             "achieved": d["achieved_GBps"],
             "peak": HBM_PEAK / 1e9,

@@ -159,110 +159,6 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
   if (STATS) flush_lane_stats(stats, st);
 }
 
-// Fixed-radius jobs.  Their rows carry no rank, and most queries have at most
-// k_max neighbours within the radius, so they need no top-k list: a counting
-// walk (no list, high occupancy) decides.  A query with c <= k_max neighbours
-// reserves c rows with the wavefront prefix sum and writes them on a second
-// walk; a query with more reserves k_max rows and is queued for
-// batch_frn_heavy_kernel, which selects the k_max nearest with the register
-// list (and may queue it again for the exact fix-up).
-template <bool STATS>
-__global__ __launch_bounds__(256) void batch_frn_kernel(const DeviceJob* __restrict__ jobs,
-                                                        const WorkItem* __restrict__ items,
-                                                        uint32_t n_items, uint32_t* item_counter,
-                                                        uint4* heavy, uint32_t* heavy_count,
-                                                        unsigned long long* stats) {
-  const uint32_t lane = threadIdx.x & 63;
-  LaneStats st;
-  for (;;) {
-    const uint32_t it = wave_fetch(item_counter);
-    if (it >= n_items) break;
-    const WorkItem w = items[it];
-    const DeviceJob& j = jobs[w.job];
-    const bool active = lane < w.count;
-    const uint32_t k = j.k;
-    uint32_t qi = 0, c = 0;
-    double qx = 0.0, qy = 0.0, qz = 0.0;
-    float qf[3] = {0.0f, 0.0f, 0.0f};
-    float m = 0.0f;
-    const double radius_sq = j.radius * j.radius;  // reference.py:357
-    if (active) {
-      qi = job_query(j, w.first + lane);
-      const float* qp = j.queries + (size_t)qi * j.qstride;
-      qx = qp[0];
-      qy = qp[1];
-      qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
-      qf[0] = (float)qx;
-      qf[1] = (float)qy;
-      qf[2] = (float)qz;
-      m = coord_bound(scene_m(j.scene), qx, qy, qz);
-      float thr2 = frn_threshold2(j.radius, m);
-      walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
-                         [&](uint32_t slot, const float4& p, float) {
-                           if (point_d2(j.scene, slot, p, qx, qy, qz) > radius_sq) return;
-                           if (++c > k) thr2 = -1.0f;  // heavy: no box passes any more
-                         });
-    }
-    const bool is_heavy = c > k;
-    const uint32_t n_rows = is_heavy ? k : c;
-    const uint64_t at0 = wave_reserve(j.counters, n_rows);
-    if (active && n_rows) {
-      if (at0 + n_rows > j.row_capacity) {
-        atomicOr(&j.counters->status, kStatusRowOverflow);
-      } else if (is_heavy) {
-        heavy[atomicAdd(heavy_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, n_rows);
-      } else {
-        FrnRowRec* rows = static_cast<FrnRowRec*>(j.rows) + at0;
-        uint32_t r = 0;
-        float thr2 = frn_threshold2(j.radius, m);
-        walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
-                           [&](uint32_t slot, const float4& p, float) {
-                             const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
-                             if (d2 > radius_sq) return;
-                             if (r < n_rows) rows[r] = FrnRowRec{qi, __float_as_uint(p.w), sqrt(d2)};
-                             ++r;
-                           });
-      }
-    }
-  }
-  if (STATS) flush_lane_stats(stats, st);
-}
-
-template <int K, bool STATS>
-__global__ __launch_bounds__(256) void batch_frn_heavy_kernel(const DeviceJob* __restrict__ jobs,
-                                                              const uint4* __restrict__ heavy,
-                                                              const uint32_t* heavy_count, uint4* amb,
-                                                              uint32_t* amb_count,
-                                                              unsigned long long* stats) {
-  LaneStats st;
-  const uint32_t n = *heavy_count;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint4 h = heavy[i];
-    const DeviceJob& j = jobs[h.x];
-    const float* qp = j.queries + (size_t)h.y * j.qstride;
-    const double qx = qp[0], qy = qp[1], qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
-    PackedTopK<K> list;
-    const bool ambiguous =
-        collect_packed<K, STATS>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius, j.k, list, st);
-    uint32_t r = 0;
-#pragma unroll
-    for (int e = 0; e < K; ++e) {
-      if (e >= K - (int)j.k && list.e[e] != ~0ull && r < h.w) {
-        const uint32_t slot = (uint32_t)list.e[e];
-        const float4 p = j.scene.prims[slot];
-        static_cast<FrnRowRec*>(j.rows)[h.z + r] =
-            FrnRowRec{h.y, __float_as_uint(p.w), sqrt(point_d2(j.scene, slot, p, qx, qy, qz))};
-        ++r;
-      }
-    }
-    if (ambiguous) {
-      amb[atomicAdd(amb_count, 1u)] = make_uint4(h.x, h.y, h.z, r);
-      atomicAdd(&j.counters->ambiguous, 1u);
-    }
-  }
-  if (STATS) flush_lane_stats(stats, st);
-}
-
 // Recomputes the queries whose FP32 selection keys tied at the boundary with
 // the exact (FP64 distance, id) order and rewrites their rows in place.  The
 // per-lane list lives in LDS (slot-major, so the 64 lanes of a wavefront hit
@@ -310,7 +206,7 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
       li[pos * kFixBlock] = ident;
       if (count == k) thr2 = frn_threshold2(fmin(j.radius, ld[(k - 1) * kFixBlock]), m);
     };
-    walk_near_first<false>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit);
+    walk_neighbors<false>(j.scene, j.scene.root, j.scene.end, qf, j.radius, thr2, st, visit);
     const uint32_t rows = min(count, a.w);
     for (uint32_t r = 0; r < rows; ++r)
       write_neighbor_row<0>(j, (uint64_t)a.z + r, a.y, li[r * kFixBlock], ld[r * kFixBlock], r + 1);
@@ -490,9 +386,8 @@ __global__ void zero_words_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb
 // size their k needs (1, 2, 4, ..., 64), so a small-k job never pays for the
 // registers of a larger one in the same batch.
 enum Family : int {
-  kFamPoints1 = 0,                  // KNN / bounded KNN, list size 1 ... 64
-  kFamFrn1 = kPointBuckets,         // fixed radius, k_max bucket 1 ... 64
-  kFamRayAny = 2 * kPointBuckets,
+  kFamPoints1 = 0,  // ... kFamPoints1 + 6 for K = 64
+  kFamRayAny = kPointBuckets,
   kFamRayCount,
   kFamRayClosest,
   kFamPip,
@@ -506,7 +401,7 @@ int family_of(uint32_t kind, uint32_t k) {
     case kJobBoundedKnn: {
       int b = 0;
       while ((1u << b) < k) ++b;  // list_capacity(k) == 1 << b
-      return (kind == kJobFixedRadius ? kFamFrn1 : kFamPoints1) + b;
+      return kFamPoints1 + b;
     }
     case kJobRayAnyHit: return kFamRayAny;
     case kJobRayHitCount: return kFamRayCount;
@@ -516,7 +411,7 @@ int family_of(uint32_t kind, uint32_t k) {
 }
 
 // report group of a launch family: neighbours, any-hit, hit-count, closest, PIP
-int group_of(int family) { return family < kFamRayAny ? 0 : family - kFamRayAny + 1; }
+int group_of(int family) { return family < kPointBuckets ? 0 : family - kPointBuckets + 1; }
 
 template <int K>
 void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
@@ -526,17 +421,6 @@ void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkI
     batch_points_kernel<K, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
   else
     batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
-  RTDL_LAUNCH_CHECK();
-}
-
-template <int K>
-void launch_frn_heavy(bool stats, unsigned grid, const DeviceJob* jobs, const uint4* heavy,
-                      const uint32_t* heavy_count, uint4* amb, uint32_t* amb_count,
-                      unsigned long long* st, hipStream_t s) {
-  if (stats)
-    batch_frn_heavy_kernel<K, true><<<grid, 256, 0, s>>>(jobs, heavy, heavy_count, amb, amb_count, st);
-  else
-    batch_frn_heavy_kernel<K, false><<<grid, 256, 0, s>>>(jobs, heavy, heavy_count, amb, amb_count, st);
   RTDL_LAUNCH_CHECK();
 }
 
@@ -559,8 +443,7 @@ BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
   RTDL_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
   stats_.resize(2);
   item_counters_.resize(kLaunchFamilies);
-  // [0] ambiguous neighbour queries, [1] PIP points past the hit cap, [2] heavy FRN queries
-  amb_count_.resize(3);
+  amb_count_.resize(2);  // [0] ambiguous neighbour queries, [1] PIP points past the hit cap
   for (int f = 0; f < kLaunchFamilies; ++f) {
     RTDL_HIP(hipEventCreate(&fam_start_[f]));
     RTDL_HIP(hipEventCreate(&fam_stop_[f]));
@@ -618,7 +501,6 @@ void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
     order_.resize(max_queries);
     amb_.resize(max_queries);
     pip_over_.resize(max_queries);
-    frn_heavy_.resize(max_queries);
     sort_ws_.reserve(max_queries);
     cap_queries_ = max_queries;
   }
@@ -697,7 +579,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   unsigned long long* st = stats_on_ ? stats_.get() : nullptr;
   const DeviceJob* dj = jobs_.get();
   int last_points = -1;  // the exact fix-up runs once, after the last neighbour launch
-  for (int f = 0; f < kFamRayAny; ++f)
+  for (int f = 0; f < kPointBuckets; ++f)
     if (item_off[f + 1] > item_off[f]) last_points = f;
   for (int f = 0; f < kLaunchFamilies; ++f) timed_[f] = false;
   for (int f = 0; f < kLaunchFamilies; ++f) {
@@ -720,23 +602,9 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
         case 5: launch_points<32>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
         default: launch_points<64>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
       }
-    } else if (f < kFamRayAny) {
-      uint32_t* heavy_count = amb_count_.get() + 2;
-      fill_u32(heavy_count, 1, 0u, s);
-      if (stats_on_)
-        batch_frn_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, frn_heavy_.get(), heavy_count, st);
-      else
-        batch_frn_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, frn_heavy_.get(), heavy_count, st);
-      RTDL_LAUNCH_CHECK();
-      const unsigned heavy_grid = (unsigned)cus_ * 2;
-      switch (f - kFamFrn1) {
-        case 0: launch_frn_heavy<1>(stats_on_, heavy_grid, dj, frn_heavy_.get(), heavy_count, amb_.get(), amb_count_.get(), st, s); break;
-        case 1: launch_frn_heavy<2>(stats_on_, heavy_grid, dj, frn_heavy_.get(), heavy_count, amb_.get(), amb_count_.get(), st, s); break;
-        case 2: launch_frn_heavy<4>(stats_on_, heavy_grid, dj, frn_heavy_.get(), heavy_count, amb_.get(), amb_count_.get(), st, s); break;
-        case 3: launch_frn_heavy<8>(stats_on_, heavy_grid, dj, frn_heavy_.get(), heavy_count, amb_.get(), amb_count_.get(), st, s); break;
-        case 4: launch_frn_heavy<16>(stats_on_, heavy_grid, dj, frn_heavy_.get(), heavy_count, amb_.get(), amb_count_.get(), st, s); break;
-        case 5: launch_frn_heavy<32>(stats_on_, heavy_grid, dj, frn_heavy_.get(), heavy_count, amb_.get(), amb_count_.get(), st, s); break;
-        default: launch_frn_heavy<64>(stats_on_, heavy_grid, dj, frn_heavy_.get(), heavy_count, amb_.get(), amb_count_.get(), st, s); break;
+      if (f == last_points) {
+        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
+        RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamRayAny) {
       launch_rays_batch<0>(stats_on_, grid, dj, it, n_items, ctr, st, s);
@@ -755,10 +623,6 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
                                                      over_count, st);
       RTDL_LAUNCH_CHECK();
       batch_pip_over_kernel<<<64, 64, 0, s>>>(dj, pip_over_.get(), over_count, eps_sq);
-      RTDL_LAUNCH_CHECK();
-    }
-    if (f == last_points) {
-      batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
       RTDL_LAUNCH_CHECK();
     }
     if (timing_on_) RTDL_HIP(hipEventRecord(fam_stop_[f], s));

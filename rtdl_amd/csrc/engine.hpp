@@ -59,8 +59,7 @@ uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_c
 
 constexpr uint32_t kPipHitCap = 16;   // containing polygons held in registers per point
 constexpr int kPointBuckets = 7;      // neighbour list sizes 1, 2, 4, ..., 64
-// KNN buckets, fixed-radius buckets, ray any / count / closest, PIP
-constexpr int kLaunchFamilies = 2 * kPointBuckets + 4;
+constexpr int kLaunchFamilies = kPointBuckets + 4;  // + ray any / count / closest, PIP
 constexpr uint32_t kStatusRowOverflow = 1u;
 
 struct JobCounters {
@@ -147,7 +146,6 @@ class BatchEngine {
   DeviceArray<uint32_t> order_;
   DeviceArray<uint4> amb_;               // (job, query, row base, count)
   DeviceArray<uint4> pip_over_;          // (job, point, row base, hit count)
-  DeviceArray<uint4> frn_heavy_;         // (job, query, row base, k_max)
   DeviceArray<uint32_t> amb_count_;
   DeviceArray<unsigned long long> stats_;
   SortWorkspace sort_ws_;

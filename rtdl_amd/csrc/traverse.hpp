@@ -138,6 +138,20 @@ __device__ inline void walk_near_first(const SceneView& s, uint32_t root, uint32
   }
 }
 
+// The walk a neighbour search uses.  With a radius, the radius bounds the walk
+// and a plain preorder walk is cheapest; a search for the k nearest without a
+// radius starts from the query's leaf and expands outwards, so its list fills
+// with near points and prunes early.
+template <bool STATS, class Visit>
+__device__ inline void walk_neighbors(const SceneView& s, uint32_t root, uint32_t end,
+                                      const float qf[3], double radius, float& thr2, LaneStats& st,
+                                      Visit&& visit) {
+  if (radius < __builtin_inf())
+    walk_points<STATS>(s, root, end, qf, thr2, st, visit);
+  else
+    walk_near_first<STATS>(s, root, end, qf, thr2, st, visit);
+}
+
 // Exact list: K (distance, id) entries in registers, k of them live in slots
 // [K - k, K); slots below hold a -1 sentinel, so the k-th smallest is always
 // slot K - 1 and every index is static.
@@ -197,7 +211,7 @@ __device__ inline void collect_exact(const SceneView& s, uint32_t root, uint32_t
       if (!COUNT_ALL && list.full()) thr2 = frn_threshold2(fmin(radius, list.d[K - 1]), m);
     }
   };
-  walk_near_first<STATS>(s, root, end, qf, thr2, st, visit);
+  walk_neighbors<STATS>(s, root, end, qf, radius, thr2, st, visit);
 }
 
 // Packed list: each entry is (FP32 bits of fl32(d2) << 32) | slot, so the
@@ -262,7 +276,7 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
       drop_min = min(drop_min, key_bits(v));
     }
   };
-  walk_near_first<STATS>(s, root, end, qf, thr2, st, visit);
+  walk_neighbors<STATS>(s, root, end, qf, radius, thr2, st, visit);
   bool ambiguous = list.full() && drop_min == key_bits(list.e[K - 1]);
 #pragma unroll
   for (int i = 1; i < K; ++i) {

@@ -69,7 +69,7 @@ def test_persistent_loops_fetch_in_the_outermost_loop(tmp_path):
     funcs = _functions(out.read_text())
     checked = 0
     for name, body in funcs.items():
-        if not re.search(r"batch_(points|rays|pip|frn)_kernel", name):
+        if not re.search(r"batch_(points|rays|pip)_kernel", name):
             continue
         depths = _readfirstlane_depths(body)
         assert depths, f"{name}: no readfirstlane broadcast of the fetched item"
