@@ -422,6 +422,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--roofline-queries", type=int, default=2 << 20)
     ap.add_argument("--skip-baselines", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (the reference kernel "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -431,6 +434,16 @@ def main() -> int:
 
     from rtdl_amd import batch as B
     from rtdl_amd import mi355x_runtime as rtm
+
+    if args.copy_calibration_mib:
+        from rtdl_amd.device import DeviceBuffer
+
+        n = args.copy_calibration_mib << 20
+        src, dst = DeviceBuffer((n,), np.uint8), DeviceBuffer((n,), np.uint8)
+        dst.copy_from(src)  # first touch
+        dst.copy_from(src)  # the dispatch the counters are read from
+        print(json.dumps({"copy_bytes": n}), flush=True)
+        return 0
 
     info = rtm.mi355x_context_probe()
     log(f"device {info}")

@@ -392,6 +392,9 @@ int rtdl_mi355x_copy_to_device(void* dst, const void* src, size_t bytes, char* e
                                size_t error_size);
 int rtdl_mi355x_copy_to_host(void* dst, const void* src, size_t bytes, char* error_out,
                              size_t error_size);
+/* device-to-device copy on the calling thread's stream (synchronous) */
+int rtdl_mi355x_copy_device(void* dst, const void* src, size_t bytes, char* error_out,
+                            size_t error_size);
 
 /* ---- diagnostics (no counterpart in the reference) ---- */
 

@@ -553,6 +553,17 @@ int rtdl_mi355x_copy_to_device(void* dst, const void* src, size_t bytes, char* e
   });
 }
 
+int rtdl_mi355x_copy_device(void* dst, const void* src, size_t bytes, char* error_out,
+                            size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    if (!bytes) return;
+    require(dst != nullptr && src != nullptr, "null pointer with nonzero byte count");
+    hipStream_t s = thread_stream();
+    RTDL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+    RTDL_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int rtdl_mi355x_copy_to_host(void* dst, const void* src, size_t bytes, char* error_out,
                              size_t error_size) {
   return guarded(error_out, error_size, [&] {

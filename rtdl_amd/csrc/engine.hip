@@ -75,7 +75,34 @@ __device__ inline uint32_t wave_exclusive_sum(uint32_t v, uint32_t* total) {
   return x - v;
 }
 
-__device__ inline uint32_t wave_fetch(uint32_t* counter) { return lane0_add(counter, 1u); }
+// Work queue of one persistent launch, split by XCD.  Workgroups are handed
+// to the eight XCDs in turn, and each XCD has its own L2, so XCD x first takes
+// items from the x-th eighth of the queue (consecutive items are neighbouring
+// Morton ranges of a job, so its L2 keeps serving the same part of the tree)
+// and then helps with the other eighths.  Each call makes one attempt:
+// kItemRetry means "this eighth is exhausted, call again"; kItemDone means
+// all eight are.  Every value is wave-uniform.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kItemDone = 0xFFFFFFFFu;
+constexpr uint32_t kItemRetry = 0xFFFFFFFEu;
+
+struct ItemQueue {
+  uint32_t* counters;  // kXcds counters
+  uint32_t n;
+  uint32_t src;
+  uint32_t tries = 0;
+  __device__ ItemQueue(uint32_t* c, uint32_t n_items) : counters(c), n(n_items), src(blockIdx.x % kXcds) {}
+  __device__ inline uint32_t next() {
+    if (tries >= kXcds) return kItemDone;
+    const uint32_t lo = (uint32_t)(((uint64_t)n * src) / kXcds);
+    const uint32_t hi = (uint32_t)(((uint64_t)n * (src + 1)) / kXcds);
+    const uint32_t it = lo + lane0_add(counters + src, 1u);
+    if (it < hi) return it;
+    src = (src + 1) % kXcds;
+    ++tries;
+    return kItemRetry;
+  }
+};
 
 // Reserves `c` rows per lane in the job's row buffer; returns the lane's
 // first row.
@@ -108,9 +135,11 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
                                                            unsigned long long* stats) {
   const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
+  ItemQueue queue(item_counter, n_items);
   for (;;) {
-    const uint32_t it = wave_fetch(item_counter);
-    if (it >= n_items) break;
+    const uint32_t it = queue.next();
+    if (it == kItemDone) break;
+    if (it == kItemRetry) continue;
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
     const bool active = lane < w.count;
@@ -220,9 +249,11 @@ __global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __rest
                                                          unsigned long long* stats) {
   const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
+  ItemQueue queue(item_counter, n_items);
   for (;;) {
-    const uint32_t it = wave_fetch(item_counter);
-    if (it >= n_items) break;
+    const uint32_t it = queue.next();
+    if (it == kItemDone) break;
+    if (it == kItemRetry) continue;
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
     const bool active = lane < w.count;
@@ -263,9 +294,11 @@ __global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restr
                                                         unsigned long long* stats) {
   const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
+  ItemQueue queue(item_counter, n_items);
   for (;;) {
-    const uint32_t it = wave_fetch(item_counter);
-    if (it >= n_items) break;
+    const uint32_t it = queue.next();
+    if (it == kItemDone) break;
+    if (it == kItemRetry) continue;
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
     const bool active = lane < w.count;
@@ -442,7 +475,7 @@ BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
   RTDL_HIP(hipGetDevice(&dev));
   RTDL_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
   stats_.resize(2);
-  item_counters_.resize(kLaunchFamilies);
+  item_counters_.resize(kLaunchFamilies * kXcds);
   amb_count_.resize(2);  // [0] ambiguous neighbour queries, [1] PIP points past the hit cap
   for (int f = 0; f < kLaunchFamilies; ++f) {
     RTDL_HIP(hipEventCreate(&fam_start_[f]));
@@ -556,9 +589,9 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   if (item_off[kLaunchFamilies])
     RTDL_HIP(hipMemcpyAsync(items_.get(), host_items_, item_off[kLaunchFamilies] * sizeof(WorkItem),
                             hipMemcpyHostToDevice, s));
-  const size_t zero_n = std::max<size_t>(n * 4, kLaunchFamilies + 1);
+  const size_t zero_n = std::max<size_t>(n * 4, kLaunchFamilies * kXcds + 1);
   zero_words_kernel<<<blocks_for(zero_n, 256), 256, 0, s>>>(
-      reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kLaunchFamilies,
+      reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kLaunchFamilies * kXcds,
       stats_.get(), 2);
   RTDL_LAUNCH_CHECK();
   fill_u32(amb_count_.get(), 2, 0u, s);
@@ -586,7 +619,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     const uint32_t n_items = (uint32_t)(item_off[f + 1] - item_off[f]);
     if (!n_items) continue;
     const WorkItem* it = items_.get() + item_off[f];
-    uint32_t* ctr = item_counters_.get() + f;
+    uint32_t* ctr = item_counters_.get() + f * kXcds;
     const unsigned grid = grid_for(n_items);
     if (timing_on_) {
       RTDL_HIP(hipEventRecord(fam_start_[f], s));

@@ -141,7 +141,7 @@ class BatchEngine {
   DeviceArray<WorkItem> items_;
   DeviceArray<uint64_t> starts_;
   DeviceArray<JobCounters> counters_;
-  DeviceArray<uint32_t> item_counters_;  // one per launch
+  DeviceArray<uint32_t> item_counters_;  // eight per launch (one per XCD)
   DeviceArray<uint64_t> keys_;
   DeviceArray<uint32_t> order_;
   DeviceArray<uint4> amb_;               // (job, query, row base, count)

@@ -399,6 +399,45 @@ __global__ void roots_kernel(uint32_t n_inst, uint32_t m, const uint64_t* leaf_o
   ends[i] = node == kEndNode ? kEndNode : nodes[node].escape;
 }
 
+// Depth-first preorder position of every node.  The nodes before n in
+// preorder are its ancestors and the subtrees left of its path; those subtrees
+// cover the leaves [0, lo(n)) with 2*lo(n) - t nodes for t right turns, so
+// pre(n) = 2*lo(n) + (number of left turns on the path from the root).
+__global__ void preorder_kernel(uint32_t m, const uint32_t* parent, const uint32_t* left,
+                                const uint32_t* range_lo, uint32_t* pre) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= 2 * m - 1) return;
+  const uint32_t lo = n < m - 1 ? range_lo[n] : n - (m - 1);
+  uint32_t turns = 0;
+  uint32_t c = n;
+  for (uint32_t guard = 2 * m; c != 0 && guard; --guard) {  // the root is node 0
+    const uint32_t p = parent[c];
+    if (left[p] == c) ++turns;
+    c = p;
+  }
+  pre[n] = 2 * lo + turns;
+}
+
+// Stores the nodes in preorder, so a walk that descends reads consecutive
+// records (the left child of an internal node is the next record).
+__global__ void permute_nodes_kernel(uint32_t n_nodes, const BvhNode* in, const uint32_t* pre,
+                                     BvhNode* out) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  BvhNode nd = in[n];
+  if (!node_is_leaf(nd.link)) nd.link = pre[nd.link];
+  if (nd.escape != kEndNode) nd.escape = pre[nd.escape];
+  out[pre[n]] = nd;
+}
+
+__global__ void remap_roots_kernel(uint32_t n_inst, const uint32_t* pre, uint32_t* roots,
+                                   uint32_t* ends) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inst) return;
+  if (roots[i] != kEndNode) roots[i] = pre[roots[i]];
+  if (ends[i] != kEndNode) ends[i] = pre[ends[i]];
+}
+
 }  // namespace
 
 void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* coords,
@@ -564,6 +603,22 @@ void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* c
       n_inst, (uint32_t)m, d_leaf_offsets.get(), parent.get(), range_lo.get(), range_hi.get(),
       scene.nodes.get(), scene.roots.get(), scene.ends.get());
   RTDL_LAUNCH_CHECK();
+  // 7. store the nodes in depth-first preorder
+  {
+    DeviceArray<uint32_t> pre(n_nodes);
+    DeviceArray<BvhNode> ordered(n_nodes);
+    preorder_kernel<<<blocks_for(n_nodes, 256), 256, 0, stream>>>((uint32_t)m, parent.get(), left.get(),
+                                                                   range_lo.get(), pre.get());
+    RTDL_LAUNCH_CHECK();
+    permute_nodes_kernel<<<blocks_for(n_nodes, 256), 256, 0, stream>>>((uint32_t)n_nodes, scene.nodes.get(),
+                                                                        pre.get(), ordered.get());
+    RTDL_LAUNCH_CHECK();
+    remap_roots_kernel<<<blocks_for(n_inst, 64), 64, 0, stream>>>(n_inst, pre.get(), scene.roots.get(),
+                                                                  scene.ends.get());
+    RTDL_LAUNCH_CHECK();
+    RTDL_HIP(hipStreamSynchronize(stream));  // before the old nodes return to the pool
+    scene.nodes = std::move(ordered);
+  }
   RTDL_HIP(hipMemcpyAsync(scene.host_roots.data(), scene.roots.get(), n_inst * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, stream));
   RTDL_HIP(hipMemcpyAsync(scene.host_ends.data(), scene.ends.get(), n_inst * sizeof(uint32_t),

@@ -68,6 +68,14 @@ class DeviceBuffer:
                    err, "rtdl_mi355x_copy_to_host")
         return out
 
+    def copy_from(self, other: "DeviceBuffer") -> None:
+        """Device-to-device copy of `other` (same byte size) into this buffer."""
+        if other.nbytes != self.nbytes:
+            raise ValueError(f"copy of {other.nbytes} bytes into a {self.nbytes}-byte buffer")
+        err = ctypes.create_string_buffer(1024)
+        _check(rt_native._library().rtdl_mi355x_copy_device(self.ptr, other.ptr, self.nbytes, err, 1024),
+               err, "rtdl_mi355x_copy_device")
+
     def to_numpy(self) -> np.ndarray:
         return self.download().view(self.dtype).reshape(self.shape)
 

@@ -159,6 +159,7 @@ _SIGNATURES = {
     "rtdl_mi355x_device_free": ([_VP, *_ERR], ctypes.c_int),
     "rtdl_mi355x_copy_to_device": ([_VP, _VP, _SZ, *_ERR], ctypes.c_int),
     "rtdl_mi355x_copy_to_host": ([_VP, _VP, _SZ, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_copy_device": ([_VP, _VP, _SZ, *_ERR], ctypes.c_int),
 }
 
 
