@@ -170,26 +170,18 @@ struct ExactTopK {
   __device__ inline bool accepts(double dist, uint32_t ident) const {
     return key_less(dist, ident, d[K - 1], id[K - 1]);
   }
-  // Same early-stopping sink as PackedTopK::insert.
   __device__ inline void insert(double dist, uint32_t ident) {
     d[K - 1] = dist;
     id[K - 1] = ident;
-    constexpr int kChunk = 8;
 #pragma unroll
-    for (int top = K - 1; top > 0; top -= kChunk) {
-      bool moved = false;
-#pragma unroll
-      for (int i = top; i > 0 && i > top - kChunk; --i) {
-        const bool sw = key_less(d[i], id[i], d[i - 1], id[i - 1]);
-        const double td = d[i];
-        const uint32_t ti = id[i];
-        d[i] = sw ? d[i - 1] : td;
-        id[i] = sw ? id[i - 1] : ti;
-        d[i - 1] = sw ? td : d[i - 1];
-        id[i - 1] = sw ? ti : id[i - 1];
-        moved = moved || sw;
-      }
-      if (K > kChunk && !moved) break;
+    for (int i = K - 1; i > 0; --i) {
+      const bool sw = key_less(d[i], id[i], d[i - 1], id[i - 1]);
+      const double td = d[i];
+      const uint32_t ti = id[i];
+      d[i] = sw ? d[i - 1] : td;
+      id[i] = sw ? id[i - 1] : ti;
+      d[i - 1] = sw ? td : d[i - 1];
+      id[i - 1] = sw ? ti : id[i - 1];
     }
   }
   __device__ inline bool full() const { return d[K - 1] < __builtin_inf(); }
@@ -238,27 +230,14 @@ struct PackedTopK {
 #pragma unroll
     for (int i = 0; i < K; ++i) e[i] = (i < K - (int)k) ? 0ull : ~0ull;
   }
-  // Places v (smaller than the last entry) and returns the evicted entry.  The
-  // new entry sinks from the end in chunks of 8 compare-exchanges; the list
-  // is otherwise sorted, so once a chunk finds it in place no later chunk
-  // would move anything and the sinking stops.  Typical candidates are just
-  // below the k-th entry and stop within a chunk or two.
-  __device__ inline uint64_t insert(uint64_t v) {
+  __device__ inline uint64_t insert(uint64_t v) {  // returns the evicted entry
     const uint64_t out = e[K - 1];
     e[K - 1] = v;
-    constexpr int kChunk = 8;
 #pragma unroll
-    for (int top = K - 1; top > 0; top -= kChunk) {
-      bool moved = false;
-#pragma unroll
-      for (int i = top; i > 0 && i > top - kChunk; --i) {
-        const uint64_t a = e[i - 1], b = e[i];
-        const bool sw = b < a;
-        e[i - 1] = sw ? b : a;
-        e[i] = sw ? a : b;
-        moved = moved || sw;
-      }
-      if (K > kChunk && !moved) break;
+    for (int i = K - 1; i > 0; --i) {
+      const uint64_t a = e[i - 1], b = e[i];
+      e[i - 1] = a < b ? a : b;
+      e[i] = a < b ? b : a;
     }
     return out;
   }
