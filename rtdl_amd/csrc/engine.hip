@@ -127,13 +127,20 @@ __device__ inline void write_neighbor_row(const DeviceJob& j, uint64_t at, uint3
   }
 }
 
+// One wavefront per block: the cooperative neighbour lists of the 64 queries
+// of a work item live in this block's LDS.
+constexpr int kPointsBlock = 64;
+
 template <int K, bool STATS>
-__global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __restrict__ jobs,
-                                                           const WorkItem* __restrict__ items,
-                                                           uint32_t n_items, uint32_t* item_counter,
-                                                           uint4* amb, uint32_t* amb_count,
-                                                           unsigned long long* stats) {
-  const uint32_t lane = threadIdx.x & 63;
+__global__ __launch_bounds__(kPointsBlock) void batch_points_kernel(const DeviceJob* __restrict__ jobs,
+                                                                   const WorkItem* __restrict__ items,
+                                                                   uint32_t n_items, uint32_t* item_counter,
+                                                                   uint4* amb, uint32_t* amb_count,
+                                                                   unsigned long long* stats) {
+  __shared__ uint64_t lds[coop_lds_words<K>()];
+  uint64_t* rows = lds;             // query q's list at rows[q * K]
+  uint64_t* stage = lds + 64 * K;   // candidate stages
+  const uint32_t lane = threadIdx.x;
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -143,47 +150,56 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
     const bool active = lane < w.count;
+    const uint32_t k = j.k;  // 1 <= k <= K, checked on the host
     uint32_t qi = 0;
     double qx = 0.0, qy = 0.0, qz = 0.0;
-    PackedTopK<K> list;
-    list.init(1);
-    bool ambiguous = false;
-    uint32_t c = 0;
-    const uint32_t k = j.k;
     if (active) {
       qi = job_query(j, w.first + lane);
       const float* qp = j.queries + (size_t)qi * j.qstride;
       qx = qp[0];
       qy = qp[1];
       qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
-      ambiguous = collect_packed<K, STATS>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius,
-                                           k, list, st);
-#pragma unroll
-      for (int i = 0; i < K; ++i)
-        if (i >= K - (int)k && list.e[i] != ~0ull) ++c;
     }
+    const CoopResult res = collect_coop<K, STATS>(j.scene, j.scene.root, j.scene.end, active, qx, qy,
+                                                  qz, j.radius, k, rows, stage, st);
+    uint32_t c = active ? res.live : 0u;
     const uint64_t at0 = wave_reserve(j.counters, c);
-    if (active && c) {
-      if (at0 + c > j.row_capacity) {
-        atomicOr(&j.counters->status, kStatusRowOverflow);
-      } else {
-        uint32_t r = 0;
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-          if (i >= K - (int)k && list.e[i] != ~0ull) {
-            const uint32_t slot = (uint32_t)list.e[i];
-            const float4 p = j.scene.prims[slot];
-            const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
-            write_neighbor_row<K>(j, at0 + r, qi, __float_as_uint(p.w), dist, r + 1);
-            ++r;
-          }
-        }
-        if (ambiguous) {
-          amb[atomicAdd(amb_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
-          atomicAdd(&j.counters->ambiguous, 1u);
-        }
+    if (c && at0 + c > j.row_capacity) {
+      atomicOr(&j.counters->status, kStatusRowOverflow);
+      c = 0;
+    }
+    // rows of one query at a time, one entry per lane; live entries are
+    // [K - k, K - k + c) of the sorted list
+    for (uint64_t todo = __ballot(c != 0); todo; todo &= todo - 1) {
+      const uint32_t q = (uint32_t)__builtin_ctzll(todo);
+      const uint32_t cq = __builtin_amdgcn_readlane(c, q);
+      const uint64_t at = readlane64(at0, q);
+      const uint32_t qiq = __builtin_amdgcn_readlane(qi, q);
+      const double x = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, qx), q));
+      const double y = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, qy), q));
+      const double z = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, qz), q));
+      const uint64_t e = lane < (uint32_t)K ? rows[q * K + lane] : ~0ull;
+      const uint32_t r = lane - ((uint32_t)K - k);  // wraps for the sentinels
+      const bool live = lane < (uint32_t)K && r < cq;
+      // two kept keys that tie, or a dropped key equal to the k-th one, may be
+      // ordered differently by the exact (distance, id) order
+      const uint32_t below_bits = key_bits(lane_up1(e));
+      bool tie = live && r > 0 && key_bits(e) == below_bits;
+      const uint64_t kth_q = readlane64(e, K - 1);
+      const uint32_t drop_q = __builtin_amdgcn_readlane(res.drop_min, q);
+      const bool ambiguous = __ballot(tie) != 0 || (kth_q != ~0ull && drop_q == key_bits(kth_q));
+      if (live) {
+        const uint32_t slot = (uint32_t)e;
+        const float4 p = j.scene.prims[slot];
+        const double dist = sqrt(point_d2(j.scene, slot, p, x, y, z));
+        write_neighbor_row<K>(j, at + r, qiq, __float_as_uint(p.w), dist, r + 1);
+      }
+      if (ambiguous && lane == 0) {
+        amb[atomicAdd(amb_count, 1u)] = make_uint4(w.job, qiq, (uint32_t)at, cq);
+        atomicAdd(&j.counters->ambiguous, 1u);
       }
     }
+    __builtin_amdgcn_wave_barrier();  // the next item rewrites rows
   }
   if (STATS) flush_lane_stats(stats, st);
 }
@@ -451,9 +467,9 @@ void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkI
                    uint32_t n_items, uint32_t* counter, uint4* amb, uint32_t* amb_count,
                    unsigned long long* st, hipStream_t s) {
   if (stats)
-    batch_points_kernel<K, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+    batch_points_kernel<K, true><<<grid, kPointsBlock, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
   else
-    batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+    batch_points_kernel<K, false><<<grid, kPointsBlock, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
   RTDL_LAUNCH_CHECK();
 }
 
@@ -620,7 +636,10 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     if (!n_items) continue;
     const WorkItem* it = items_.get() + item_off[f];
     uint32_t* ctr = item_counters_.get() + f * kXcds;
-    const unsigned grid = grid_for(n_items);
+    // neighbour families run one wavefront per block
+    const unsigned grid = f < kPointBuckets
+                              ? (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 32))
+                              : grid_for(n_items);
     if (timing_on_) {
       RTDL_HIP(hipEventRecord(fam_start_[f], s));
       timed_[f] = true;

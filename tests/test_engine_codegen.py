@@ -44,8 +44,11 @@ def _functions(asm: str) -> dict[str, list[str]]:
     return funcs
 
 
-def _readfirstlane_depths(body: list[str]) -> set[int]:
-    depths, depth = set(), 0
+def _fetch_broadcast_depths(body: list[str]) -> set[int]:
+    """Loop depths of the v_readfirstlane broadcasts of a returning global
+    atomic's result (the work-item fetch and the row reservation).  Other
+    broadcasts (of values every lane already agrees on) may sit at any depth."""
+    depths, depth, pending = set(), 0, {}
     for line in body:
         if re.match(r"^(\.LBB\d+_\d+|; %bb\.\d+):", line):
             d = re.search(r"Depth=(\d+)", line)
@@ -53,7 +56,12 @@ def _readfirstlane_depths(body: list[str]) -> set[int]:
         h = re.search(r"This (?:Inner )?Loop Header: Depth=(\d+)", line)
         if h:
             depth = int(h.group(1))
-        if "v_readfirstlane_b32" in line:
+        a = re.match(r"\s*global_atomic_\w+\s+(v\d+),.*\b(sc0|glc)\b", line)
+        if a:
+            pending[a.group(1)] = True
+            continue
+        r = re.match(r"\s*v_readfirstlane_b32\s+s\d+,\s*(v\d+)", line)
+        if r and pending.pop(r.group(1), False):
             depths.add(depth)
     return depths
 
@@ -71,7 +79,7 @@ def test_persistent_loops_fetch_in_the_outermost_loop(tmp_path):
     for name, body in funcs.items():
         if not re.search(r"batch_(points|rays|pip)_kernel", name):
             continue
-        depths = _readfirstlane_depths(body)
+        depths = _fetch_broadcast_depths(body)
         assert depths, f"{name}: no readfirstlane broadcast of the fetched item"
         assert max(depths) <= 1, f"{name}: item broadcast inside a nested loop (depths {sorted(depths)})"
         checked += 1
