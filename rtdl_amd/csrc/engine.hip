@@ -127,20 +127,13 @@ __device__ inline void write_neighbor_row(const DeviceJob& j, uint64_t at, uint3
   }
 }
 
-// One wavefront per block: the cooperative neighbour lists of the 64 queries
-// of a work item live in this block's LDS.
-constexpr int kPointsBlock = 64;
-
 template <int K, bool STATS>
-__global__ __launch_bounds__(kPointsBlock) void batch_points_kernel(const DeviceJob* __restrict__ jobs,
-                                                                   const WorkItem* __restrict__ items,
-                                                                   uint32_t n_items, uint32_t* item_counter,
-                                                                   uint4* amb, uint32_t* amb_count,
-                                                                   unsigned long long* stats) {
-  __shared__ uint64_t lds[coop_lds_words<K>()];
-  uint64_t* rows = lds;             // query q's list at rows[q * K]
-  uint64_t* stage = lds + 64 * K;   // candidate stages
-  const uint32_t lane = threadIdx.x;
+__global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __restrict__ jobs,
+                                                           const WorkItem* __restrict__ items,
+                                                           uint32_t n_items, uint32_t* item_counter,
+                                                           uint4* amb, uint32_t* amb_count,
+                                                           unsigned long long* stats) {
+  const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -150,56 +143,47 @@ __global__ __launch_bounds__(kPointsBlock) void batch_points_kernel(const Device
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
     const bool active = lane < w.count;
-    const uint32_t k = j.k;  // 1 <= k <= K, checked on the host
     uint32_t qi = 0;
     double qx = 0.0, qy = 0.0, qz = 0.0;
+    PackedTopK<K> list;
+    list.init(1);
+    bool ambiguous = false;
+    uint32_t c = 0;
+    const uint32_t k = j.k;
     if (active) {
       qi = job_query(j, w.first + lane);
       const float* qp = j.queries + (size_t)qi * j.qstride;
       qx = qp[0];
       qy = qp[1];
       qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
+      ambiguous = collect_packed<K, STATS>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius,
+                                           k, list, st);
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (i >= K - (int)k && list.e[i] != ~0ull) ++c;
     }
-    const CoopResult res = collect_coop<K, STATS>(j.scene, j.scene.root, j.scene.end, active, qx, qy,
-                                                  qz, j.radius, k, rows, stage, st);
-    uint32_t c = active ? res.live : 0u;
     const uint64_t at0 = wave_reserve(j.counters, c);
-    if (c && at0 + c > j.row_capacity) {
-      atomicOr(&j.counters->status, kStatusRowOverflow);
-      c = 0;
-    }
-    // rows of one query at a time, one entry per lane; live entries are
-    // [K - k, K - k + c) of the sorted list
-    for (uint64_t todo = __ballot(c != 0); todo; todo &= todo - 1) {
-      const uint32_t q = (uint32_t)__builtin_ctzll(todo);
-      const uint32_t cq = __builtin_amdgcn_readlane(c, q);
-      const uint64_t at = readlane64(at0, q);
-      const uint32_t qiq = __builtin_amdgcn_readlane(qi, q);
-      const double x = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, qx), q));
-      const double y = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, qy), q));
-      const double z = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, qz), q));
-      const uint64_t e = lane < (uint32_t)K ? rows[q * K + lane] : ~0ull;
-      const uint32_t r = lane - ((uint32_t)K - k);  // wraps for the sentinels
-      const bool live = lane < (uint32_t)K && r < cq;
-      // two kept keys that tie, or a dropped key equal to the k-th one, may be
-      // ordered differently by the exact (distance, id) order
-      const uint32_t below_bits = key_bits(lane_up1(e));
-      bool tie = live && r > 0 && key_bits(e) == below_bits;
-      const uint64_t kth_q = readlane64(e, K - 1);
-      const uint32_t drop_q = __builtin_amdgcn_readlane(res.drop_min, q);
-      const bool ambiguous = __ballot(tie) != 0 || (kth_q != ~0ull && drop_q == key_bits(kth_q));
-      if (live) {
-        const uint32_t slot = (uint32_t)e;
-        const float4 p = j.scene.prims[slot];
-        const double dist = sqrt(point_d2(j.scene, slot, p, x, y, z));
-        write_neighbor_row<K>(j, at + r, qiq, __float_as_uint(p.w), dist, r + 1);
-      }
-      if (ambiguous && lane == 0) {
-        amb[atomicAdd(amb_count, 1u)] = make_uint4(w.job, qiq, (uint32_t)at, cq);
-        atomicAdd(&j.counters->ambiguous, 1u);
+    if (active && c) {
+      if (at0 + c > j.row_capacity) {
+        atomicOr(&j.counters->status, kStatusRowOverflow);
+      } else {
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          if (i >= K - (int)k && list.e[i] != ~0ull) {
+            const uint32_t slot = (uint32_t)list.e[i];
+            const float4 p = j.scene.prims[slot];
+            const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
+            write_neighbor_row<K>(j, at0 + r, qi, __float_as_uint(p.w), dist, r + 1);
+            ++r;
+          }
+        }
+        if (ambiguous) {
+          amb[atomicAdd(amb_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
+          atomicAdd(&j.counters->ambiguous, 1u);
+        }
       }
     }
-    __builtin_amdgcn_wave_barrier();  // the next item rewrites rows
   }
   if (STATS) flush_lane_stats(stats, st);
 }
@@ -255,6 +239,272 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
     const uint32_t rows = min(count, a.w);
     for (uint32_t r = 0; r < rows; ++r)
       write_neighbor_row<0>(j, (uint64_t)a.z + r, a.y, li[r * kFixBlock], ld[r * kFixBlock], r + 1);
+  }
+}
+
+// ---------------------------------------------------------- fixed radius --
+//
+// Fixed-radius jobs need no top-k list.  frn_count_kernel walks each query's
+// radius once, counting the neighbours inside it with a histogram of their
+// distances over kFrnBuckets equal slices of [0, radius].  A query with
+// c <= k neighbours reserves c rows, which frn_emit_kernel writes in walk
+// order and frn_sort_kernel sorts into the (distance, id) order
+// (reference.py:362-366).  A query with more keeps its k nearest: the first
+// bucket b whose cumulative count reaches k holds the k-th distance, so the
+// emit walk parks only the neighbours of buckets 0..b (at most kFrnCandMax)
+// and the sort writes the first k of them.  Every decision uses the FP64
+// distance in the reference's operation order and equal distances share a
+// bucket, so the rows are exact by construction.  A query whose buckets 0..b
+// hold more than kFrnCandMax neighbours (or that finds the parking space
+// full) is recomputed by batch_fix_kernel instead.
+constexpr uint32_t kFrnBuckets = 32;
+constexpr uint32_t kFrnCandMax = 128;
+constexpr int kFrnBlock = 64;
+
+// Distance bucket in [0, kFrnBuckets): dist * scale is monotone in dist.
+__device__ inline uint32_t frn_bucket(double dist, double scale) {
+  const double x = dist * scale;
+  return x < (double)kFrnBuckets ? (x >= 0.0 ? (uint32_t)x : 0u) : kFrnBuckets - 1;
+}
+
+__device__ inline uint64_t lane0_add64(unsigned long long* counter, uint32_t value) {
+  const unsigned long long mine = (threadIdx.x & 63) == 0 ? value : 0ull;
+  auto* g = (__attribute__((address_space(1))) unsigned long long*)counter;
+  const unsigned long long old = __hip_atomic_fetch_add(g, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)old);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(old >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline void load_query3(const DeviceJob& j, uint32_t qi, double& qx, double& qy, double& qz) {
+  const float* qp = j.queries + (size_t)qi * j.qstride;
+  qx = qp[0];
+  qy = qp[1];
+  qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(kFrnBlock) void frn_count_kernel(
+    const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
+    uint32_t* item_counter, FrnPlan* __restrict__ plans, unsigned long long* cand_count,
+    uint64_t cand_cap, uint4* amb, uint32_t* amb_count, unsigned long long* stats) {
+  __shared__ uint32_t hist[kFrnBuckets * kFrnBlock];  // bucket b of lane l at hist[b * 64 + l]
+  const uint32_t lane = threadIdx.x;
+  uint32_t* my_hist = hist + lane;
+  LaneStats st;
+  ItemQueue queue(item_counter, n_items);
+  for (;;) {
+    const uint32_t it = queue.next();
+    if (it == kItemDone) break;
+    if (it == kItemRetry) continue;
+    const WorkItem w = items[it];
+    const DeviceJob& j = jobs[w.job];
+    const bool active = lane < w.count;
+    const uint32_t k = j.k;
+    const double radius_sq = j.radius * j.radius;  // reference.py:357
+    const double scale = (double)kFrnBuckets / j.radius;
+    for (uint32_t b = 0; b < kFrnBuckets; ++b) my_hist[64 * b] = 0u;
+    uint32_t qi = 0, count = 0;
+    if (active) {
+      qi = job_query(j, w.first + lane);
+      double qx, qy, qz;
+      load_query3(j, qi, qx, qy, qz);
+      const float qf[3] = {(float)qx, (float)qy, (float)qz};
+      float thr2 = frn_threshold2(j.radius, coord_bound(scene_m(j.scene), qx, qy, qz));
+      walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
+                         [&](uint32_t slot, const float4& p, float) {
+                           const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
+                           if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
+                           ++count;
+                           ++my_hist[64 * frn_bucket(sqrt(d2), scale)];
+                         });
+    }
+    const bool heavy = count > k;
+    uint32_t n_rows = heavy ? k : count;
+    uint32_t bucket = kFrnBuckets - 1, n_cand = 0;
+    if (heavy) {
+      uint32_t cum = 0;
+      for (uint32_t b = 0; b < kFrnBuckets; ++b) {
+        cum += my_hist[64 * b];
+        if (cum >= k) {
+          bucket = b;
+          n_cand = cum;
+          break;
+        }
+      }
+    }
+    bool refine = heavy && n_cand > kFrnCandMax;
+    const uint64_t row_at = wave_reserve(j.counters, n_rows);
+    if (n_rows && row_at + n_rows > j.row_capacity) {
+      atomicOr(&j.counters->status, kStatusRowOverflow);
+      n_rows = 0;
+      refine = false;
+    }
+    const uint32_t want = (n_rows && heavy && !refine) ? n_cand : 0u;
+    uint32_t total;
+    const uint32_t prefix = wave_exclusive_sum(want, &total);
+    const uint64_t cand_at = lane0_add64(cand_count, total) + prefix;
+    if (want && cand_at + want > cand_cap) refine = true;
+    if (n_rows && refine) {
+      amb[atomicAdd(amb_count, 1u)] = make_uint4(w.job, qi, (uint32_t)row_at, n_rows);
+      atomicAdd(&j.counters->ambiguous, 1u);
+    }
+    if (active)
+      plans[(size_t)it * 64 + lane] =
+          FrnPlan{row_at, cand_at, refine ? 0u : n_rows, heavy ? n_cand : 0u, bucket, qi};
+  }
+  if (STATS) flush_lane_stats(stats, st);
+}
+
+// Writes one query's rows (or parks its candidates) on a second radius walk;
+// the walk meets the same candidates in the same FP64 arithmetic as the count.
+__device__ inline void frn_emit_query(const DeviceJob& j, const FrnPlan& pl, FrnCand* cands, LaneStats& st) {
+  double qx, qy, qz;
+  load_query3(j, pl.qi, qx, qy, qz);
+  const float qf[3] = {(float)qx, (float)qy, (float)qz};
+  const double radius_sq = j.radius * j.radius;  // reference.py:357
+  const double scale = (double)kFrnBuckets / j.radius;
+  const bool parked = pl.n_cand != 0;
+  double bound = j.radius;
+  if (parked) {
+    // bucket(dist) <= b means dist * scale < b + 1, so dist < (b + 1) / scale
+    const double edge = (double)(pl.bucket + 1) / scale;
+    bound = fmin(bound, edge * 1.0000000000009094947);  // 1 + 2^-40
+  }
+  float thr2 = frn_threshold2(bound, coord_bound(scene_m(j.scene), qx, qy, qz));
+  uint32_t r = 0;
+  FrnRowRec* rows = static_cast<FrnRowRec*>(j.rows) + pl.row_at;
+  FrnCand* park = cands + pl.cand_at;
+  walk_points<false>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
+                     [&](uint32_t slot, const float4& p, float) {
+                       const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
+                       if (d2 > radius_sq) return;
+                       const double dist = sqrt(d2);
+                       const uint32_t id = __float_as_uint(p.w);
+                       if (!parked) {
+                         if (r < pl.n_rows) rows[r] = FrnRowRec{pl.qi, id, dist};
+                         ++r;
+                       } else if (frn_bucket(dist, scale) <= pl.bucket) {
+                         if (r < pl.n_cand) park[r] = FrnCand{dist, id, 0u};
+                         ++r;
+                       }
+                     });
+}
+
+__global__ __launch_bounds__(256) void frn_emit_kernel(const DeviceJob* __restrict__ jobs,
+                                                       const WorkItem* __restrict__ items,
+                                                       uint32_t n_items, uint32_t* item_counter,
+                                                       const FrnPlan* __restrict__ plans,
+                                                       FrnCand* __restrict__ cands) {
+  const uint32_t lane = threadIdx.x & 63;
+  LaneStats st;
+  ItemQueue queue(item_counter, n_items);
+  for (;;) {
+    const uint32_t it = queue.next();
+    if (it == kItemDone) break;
+    if (it == kItemRetry) continue;
+    const WorkItem w = items[it];
+    const DeviceJob& j = jobs[w.job];
+    const FrnPlan pl = lane < w.count ? plans[(size_t)it * 64 + lane] : FrnPlan{0, 0, 0, 0, 0, 0};
+    // every lane returns to the item fetch together; lanes without rows idle
+    if (pl.n_rows) frn_emit_query(j, pl, cands, st);
+  }
+}
+
+
+__device__ inline uint64_t shfl_xor64(uint64_t x, uint32_t m) {
+  const uint32_t lo = __shfl_xor((uint32_t)x, (int)m, 64);
+  const uint32_t hi = __shfl_xor((uint32_t)(x >> 32), (int)m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// (distance bits, id) order; distances are non-negative, so their bit
+// patterns order like the values.
+__device__ inline bool dist_id_less(uint64_t da, uint32_t ia, uint64_t db, uint32_t ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+// One bitonic compare-exchange of element `idx` (held by this lane) with
+// element idx ^ stride (held by lane ^ stride).
+__device__ inline void bitonic_lanes(uint64_t& d, uint32_t& id, uint32_t idx, uint32_t size,
+                                     uint32_t stride) {
+  const uint64_t od = shfl_xor64(d, stride);
+  const uint32_t oi = __shfl_xor(id, (int)stride, 64);
+  const bool keep_min = ((idx & stride) == 0) == ((idx & size) == 0);
+  const bool take = keep_min ? dist_id_less(od, oi, d, id) : dist_id_less(d, id, od, oi);
+  d = take ? od : d;
+  id = take ? oi : id;
+}
+
+// Sorts n <= 128 (distance bits, id) pairs held two per lane (element lane in
+// (d0, i0), element lane + 64 in (d1, i1); pads are ~0) with a bitonic
+// network over the next power of two >= n.  Every lane must take part.
+__device__ inline void wave_sort_dist_id(uint64_t& d0, uint32_t& i0, uint64_t& d1, uint32_t& i1,
+                                         uint32_t n) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t size_max = 2;
+  while (size_max < n) size_max <<= 1;
+  for (uint32_t size = 2; size <= size_max; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride == 64) {  // element lane vs lane + 64, one ascending block
+        const bool sw = dist_id_less(d1, i1, d0, i0);
+        const uint64_t td = d0;
+        const uint32_t ti = i0;
+        d0 = sw ? d1 : d0;
+        i0 = sw ? i1 : i0;
+        d1 = sw ? td : d1;
+        i1 = sw ? ti : i1;
+      } else {
+        bitonic_lanes(d0, i0, lane, size, stride);
+        if (size_max > 64) bitonic_lanes(d1, i1, lane + 64, size, stride);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kFrnBlock) void frn_sort_kernel(const DeviceJob* __restrict__ jobs,
+                                                            const WorkItem* __restrict__ items,
+                                                            uint32_t n_items, uint32_t* item_counter,
+                                                            const FrnPlan* __restrict__ plans,
+                                                            const FrnCand* __restrict__ cands) {
+  const uint32_t lane = threadIdx.x;
+  ItemQueue queue(item_counter, n_items);
+  for (;;) {
+    const uint32_t it = queue.next();
+    if (it == kItemDone) break;
+    if (it == kItemRetry) continue;
+    const WorkItem w = items[it];
+    const DeviceJob& j = jobs[w.job];
+    FrnRowRec* rows = static_cast<FrnRowRec*>(j.rows);
+    for (uint32_t q = 0; q < w.count; ++q) {  // one query at a time, all lanes
+      const FrnPlan pl = plans[(size_t)it * 64 + q];
+      if (pl.n_rows == 0 || (pl.n_cand == 0 && pl.n_rows == 1)) continue;
+      uint64_t d0 = ~0ull, d1 = ~0ull;
+      uint32_t i0 = ~0u, i1 = ~0u;
+      uint32_t n;
+      if (pl.n_cand == 0) {  // rows written in walk order, sorted in place
+        n = pl.n_rows;
+        if (lane < n) {
+          const FrnRowRec r = rows[pl.row_at + lane];
+          d0 = __builtin_bit_cast(uint64_t, r.distance);
+          i0 = r.neighbor_id;
+        }
+      } else {  // parked candidates; the first n_rows of them are the rows
+        n = pl.n_cand;
+        if (lane < n) {
+          const FrnCand c = cands[pl.cand_at + lane];
+          d0 = __builtin_bit_cast(uint64_t, c.distance);
+          i0 = c.id;
+        }
+        if (lane + 64 < n) {
+          const FrnCand c = cands[pl.cand_at + lane + 64];
+          d1 = __builtin_bit_cast(uint64_t, c.distance);
+          i1 = c.id;
+        }
+      }
+      wave_sort_dist_id(d0, i0, d1, i1, n);
+      if (lane < pl.n_rows) rows[pl.row_at + lane] = FrnRowRec{pl.qi, i0, __builtin_bit_cast(double, d0)};
+    }
   }
 }
 
@@ -436,16 +686,23 @@ __global__ void zero_words_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb
 // registers of a larger one in the same batch.
 enum Family : int {
   kFamPoints1 = 0,  // ... kFamPoints1 + 6 for K = 64
-  kFamRayAny = kPointBuckets,
+  kFamFrn = kPointBuckets,  // fixed radius: count, emit, sort
+  kFamRayAny,
   kFamRayCount,
   kFamRayClosest,
   kFamPip,
 };
 static_assert(kFamPip + 1 == kLaunchFamilies, "family count");
+// work queues: one per family, plus the fixed-radius emit and sort launches
+constexpr int kItemCounterSets = kLaunchFamilies + 2;
+constexpr int kQueueFrnEmit = kLaunchFamilies;
+constexpr int kQueueFrnSort = kLaunchFamilies + 1;
+// parking space for fixed-radius candidates, per fixed-radius query of a batch
+constexpr uint64_t kFrnCandPerQuery = 80;
 
 int family_of(uint32_t kind, uint32_t k) {
   switch (kind) {
-    case kJobFixedRadius:
+    case kJobFixedRadius: return kFamFrn;
     case kJobKnn:
     case kJobBoundedKnn: {
       int b = 0;
@@ -460,16 +717,16 @@ int family_of(uint32_t kind, uint32_t k) {
 }
 
 // report group of a launch family: neighbours, any-hit, hit-count, closest, PIP
-int group_of(int family) { return family < kPointBuckets ? 0 : family - kPointBuckets + 1; }
+int group_of(int family) { return family <= kFamFrn ? 0 : family - kFamFrn; }
 
 template <int K>
 void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
                    uint32_t n_items, uint32_t* counter, uint4* amb, uint32_t* amb_count,
                    unsigned long long* st, hipStream_t s) {
   if (stats)
-    batch_points_kernel<K, true><<<grid, kPointsBlock, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+    batch_points_kernel<K, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
   else
-    batch_points_kernel<K, false><<<grid, kPointsBlock, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+    batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
   RTDL_LAUNCH_CHECK();
 }
 
@@ -491,8 +748,9 @@ BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
   RTDL_HIP(hipGetDevice(&dev));
   RTDL_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
   stats_.resize(2);
-  item_counters_.resize(kLaunchFamilies * kXcds);
-  amb_count_.resize(2);  // [0] ambiguous neighbour queries, [1] PIP points past the hit cap
+  item_counters_.resize(kItemCounterSets * kXcds);
+  amb_count_.resize(2);  // [0] neighbour queries for the exact fix-up, [1] PIP points past the hit cap
+  frn_cand_count_.resize(1);
   for (int f = 0; f < kLaunchFamilies; ++f) {
     RTDL_HIP(hipEventCreate(&fam_start_[f]));
     RTDL_HIP(hipEventCreate(&fam_stop_[f]));
@@ -543,6 +801,7 @@ void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
     if (host_items_) RTDL_HIP(hipHostFree(host_items_));
     RTDL_HIP(hipHostMalloc((void**)&host_items_, max_items * sizeof(WorkItem), hipHostMallocDefault));
     items_.resize(max_items);
+    frn_plan_.resize(max_items * 64);
     cap_items_ = max_items;
   }
   if (max_queries > cap_queries_) {
@@ -592,6 +851,13 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     start += h.nq;
   }
   host_starts_[n] = start;
+  uint64_t frn_queries = 0;
+  for (const HostJob& h : jobs)
+    if (h.kind == kJobFixedRadius) frn_queries += h.nq;
+  if (frn_queries) {
+    const size_t want = (size_t)std::max<uint64_t>(frn_queries * kFrnCandPerQuery, 1u << 16);
+    if (frn_cand_.size() < want) frn_cand_.resize(want);
+  }
   size_t item_off[kLaunchFamilies + 1] = {0};
   for (int f = 0; f < kLaunchFamilies; ++f) {
     std::memcpy(host_items_ + item_off[f], fam_items[f].data(), fam_items[f].size() * sizeof(WorkItem));
@@ -605,12 +871,13 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   if (item_off[kLaunchFamilies])
     RTDL_HIP(hipMemcpyAsync(items_.get(), host_items_, item_off[kLaunchFamilies] * sizeof(WorkItem),
                             hipMemcpyHostToDevice, s));
-  const size_t zero_n = std::max<size_t>(n * 4, kLaunchFamilies * kXcds + 1);
+  const size_t zero_n = std::max<size_t>(n * 4, kItemCounterSets * kXcds + 1);
   zero_words_kernel<<<blocks_for(zero_n, 256), 256, 0, s>>>(
-      reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kLaunchFamilies * kXcds,
+      reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kItemCounterSets * kXcds,
       stats_.get(), 2);
   RTDL_LAUNCH_CHECK();
   fill_u32(amb_count_.get(), 2, 0u, s);
+  fill_u32(reinterpret_cast<uint32_t*>(frn_cand_count_.get()), 2, 0u, s);
   if (morton_sort && total > 1) {
     batch_keys_kernel<<<blocks_for(total, 256), 256, 0, s>>>(jobs_.get(), starts_.get(), (uint32_t)n,
                                                               total, keys_.get(), order_.get());
@@ -628,7 +895,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   unsigned long long* st = stats_on_ ? stats_.get() : nullptr;
   const DeviceJob* dj = jobs_.get();
   int last_points = -1;  // the exact fix-up runs once, after the last neighbour launch
-  for (int f = 0; f < kPointBuckets; ++f)
+  for (int f = 0; f <= kFamFrn; ++f)
     if (item_off[f + 1] > item_off[f]) last_points = f;
   for (int f = 0; f < kLaunchFamilies; ++f) timed_[f] = false;
   for (int f = 0; f < kLaunchFamilies; ++f) {
@@ -636,10 +903,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     if (!n_items) continue;
     const WorkItem* it = items_.get() + item_off[f];
     uint32_t* ctr = item_counters_.get() + f * kXcds;
-    // neighbour families run one wavefront per block
-    const unsigned grid = f < kPointBuckets
-                              ? (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 32))
-                              : grid_for(n_items);
+    const unsigned grid = grid_for(n_items);
     if (timing_on_) {
       RTDL_HIP(hipEventRecord(fam_start_[f], s));
       timed_[f] = true;
@@ -654,6 +918,27 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
         case 5: launch_points<32>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
         default: launch_points<64>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
       }
+      if (f == last_points) {
+        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
+        RTDL_LAUNCH_CHECK();
+      }
+    } else if (f == kFamFrn) {
+      const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 16));
+      if (stats_on_)
+        frn_count_kernel<true><<<g64, kFrnBlock, 0, s>>>(dj, it, n_items, ctr, frn_plan_.get(),
+                                                          frn_cand_count_.get(), frn_cand_.size(),
+                                                          amb_.get(), amb_count_.get(), st);
+      else
+        frn_count_kernel<false><<<g64, kFrnBlock, 0, s>>>(dj, it, n_items, ctr, frn_plan_.get(),
+                                                           frn_cand_count_.get(), frn_cand_.size(),
+                                                           amb_.get(), amb_count_.get(), st);
+      RTDL_LAUNCH_CHECK();
+      frn_emit_kernel<<<grid, 256, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueFrnEmit * kXcds,
+                                           frn_plan_.get(), frn_cand_.get());
+      RTDL_LAUNCH_CHECK();
+      frn_sort_kernel<<<g64, kFrnBlock, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueFrnSort * kXcds,
+                                                frn_plan_.get(), frn_cand_.get());
+      RTDL_LAUNCH_CHECK();
       if (f == last_points) {
         batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
         RTDL_LAUNCH_CHECK();

@@ -59,7 +59,7 @@ uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_c
 
 constexpr uint32_t kPipHitCap = 16;   // containing polygons held in registers per point
 constexpr int kPointBuckets = 7;      // neighbour list sizes 1, 2, 4, ..., 64
-constexpr int kLaunchFamilies = kPointBuckets + 4;  // + ray any / count / closest, PIP
+constexpr int kLaunchFamilies = kPointBuckets + 5;  // + fixed radius, ray any / count / closest, PIP
 constexpr uint32_t kStatusRowOverflow = 1u;
 
 struct JobCounters {
@@ -82,6 +82,24 @@ struct DeviceJob {
   uint32_t k;
   uint32_t qstride;       // 2 (2-D points), 3 (3-D points) or 7 (rays)
   uint32_t order_base;
+  uint32_t pad;
+};
+
+// Per-query plan of a fixed-radius job between its count, emit and sort
+// launches (indexed by work item * 64 + lane).
+struct FrnPlan {
+  uint64_t row_at;   // first output row
+  uint64_t cand_at;  // first parked candidate (queries with more than k neighbours)
+  uint32_t n_rows;   // rows of the query (0: nothing to emit or sort here)
+  uint32_t n_cand;   // parked candidates; 0 when the rows are written directly
+  uint32_t bucket;   // last distance bucket parked
+  uint32_t qi;       // job-local query index
+};
+
+// A parked fixed-radius candidate.
+struct FrnCand {
+  double distance;
+  uint32_t id;
   uint32_t pad;
 };
 
@@ -147,6 +165,9 @@ class BatchEngine {
   DeviceArray<uint4> amb_;               // (job, query, row base, count)
   DeviceArray<uint4> pip_over_;          // (job, point, row base, hit count)
   DeviceArray<uint32_t> amb_count_;
+  DeviceArray<FrnPlan> frn_plan_;        // cap_items_ * 64
+  DeviceArray<FrnCand> frn_cand_;        // parked candidates of a batch
+  DeviceArray<unsigned long long> frn_cand_count_;
   DeviceArray<unsigned long long> stats_;
   SortWorkspace sort_ws_;
   bool stats_on_ = false;

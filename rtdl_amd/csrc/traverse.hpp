@@ -286,248 +286,6 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
   return ambiguous;
 }
 
-// ------------------------------------------- wavefront-cooperative lists --
-
-// One lane's neighbour walk, advanced one node per step() so that a wavefront
-// can keep all of its lanes in one loop (the cooperative list below needs
-// every lane at each insert).  The walk is a sequence of preorder segments:
-// with `near_first`, a greedy descent (nearer child box first; the right child
-// of an internal node is its left child's escape) records the query's leaf
-// and its ancestors up[1..7]; segment r walks the subtree of up[r] without the
-// subtree of segment r - 1, and the segment from the root runs to `end` and
-// is the last.  Without `near_first` the only segment is root .. end.  Either
-// way every primitive of the tree is offered once.
-struct RingWalk {
-  static constexpr int kUp = 7;
-  uint32_t up[kUp + 1];    // segment roots; the root repeats above the top
-  uint32_t upend[kUp + 1]; // where each segment's preorder walk stops
-  uint32_t node, stop, skip, from, root, end, guard;
-  int ring;
-  bool walking;
-
-  template <bool STATS>
-  __device__ inline void init(const SceneView& s, uint32_t root_, uint32_t end_, const float qf[3],
-                              bool near_first, bool active, LaneStats& st) {
-    root = root_;
-    end = end_;
-#pragma unroll
-    for (int i = 0; i <= kUp; ++i) {
-      up[i] = root;
-      upend[i] = end;
-    }
-    walking = active && root != kEndNode && root != end;
-    if (walking && near_first) {
-      BvhNode nd = s.nodes[root];
-      if (STATS) ++st.nodes;
-      for (uint32_t g = s.n_nodes + 1; !node_is_leaf(nd.link) && g; --g) {
-        const uint32_t l = nd.link;
-        const BvhNode ln = s.nodes[l];
-        const uint32_t r = ln.escape;
-        const BvhNode rn = s.nodes[r];
-        if (STATS) st.nodes += 2;
-        const bool right = box_dist2(qf, rn.lo, rn.hi) < box_dist2(qf, ln.lo, ln.hi);
-#pragma unroll
-        for (int i = kUp; i > 0; --i) {
-          up[i] = up[i - 1];
-          upend[i] = upend[i - 1];
-        }
-        up[0] = right ? r : l;
-        upend[0] = right ? rn.escape : ln.escape;
-        nd = right ? rn : ln;
-      }
-    }
-    ring = 0;
-    from = up[0];
-    stop = upend[0];
-    node = from;
-    skip = kEndNode;
-    // each node is visited at most once per segment; the guard only bounds
-    // the walk should the tree ever be malformed
-    const uint64_t bound = (uint64_t)(kUp + 2) * ((uint64_t)s.n_nodes + 1);
-    guard = bound < 0xFFFFFFFFull ? (uint32_t)bound : 0xFFFFFFFFu;
-  }
-
-  // visit(slot, p, d2f) for every leaf primitive whose FP32 distance passes thr2
-  template <bool STATS, class Visit>
-  __device__ inline void step(const SceneView& s, const float qf[3], float thr2, LaneStats& st,
-                              Visit&& visit) {
-    if (!guard--) {
-      walking = false;
-      return;
-    }
-    if (node == stop) {  // segment done
-      if (from == root) {
-        walking = false;
-        return;
-      }
-      skip = from;
-      ++ring;
-      from = root;
-      stop = end;
-#pragma unroll
-      for (int i = 1; i <= kUp; ++i) {
-        if (i == ring) {
-          from = up[i];
-          stop = upend[i];
-        }
-      }
-      node = from;
-      return;
-    }
-    const BvhNode nd = s.nodes[node];
-    if (STATS) ++st.nodes;
-    uint32_t next = nd.escape;
-    const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
-    const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
-    if (node != skip && box_dist2(qf, lo, hi) <= thr2) {
-      if (!node_is_leaf(nd.link)) {
-        next = nd.link;
-      } else {
-        const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
-        for (uint32_t j = 0; j < cnt; ++j) {
-          const uint32_t slot = first + j;
-          const float4 p = s.prims[slot];
-          if (STATS) ++st.prims;
-          const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
-          const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
-          if (d2f <= thr2) visit(slot, p, d2f);
-        }
-      }
-    }
-    node = next;
-  }
-};
-
-__device__ inline uint64_t readlane64(uint64_t x, uint32_t lane) {
-  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, lane);
-  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), lane);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// Entry of lane - 1 (lane 0 gets its own).
-__device__ inline uint64_t lane_up1(uint64_t x) {
-  const uint32_t lo = __shfl_up((uint32_t)x, 1, 64);
-  const uint32_t hi = __shfl_up((uint32_t)(x >> 32), 1, 64);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// Per-lane candidate stage depth of the cooperative collector: a node step
-// stages at most kMaxLeafSize entries, and the wavefront drains whenever a
-// lane holds more than kCoopStage - kMaxLeafSize.
-constexpr uint32_t kCoopStage = 12;
-
-// LDS words (uint64) one wavefront needs for collect_coop<K>.
-template <int K>
-constexpr uint32_t coop_lds_words() {
-  return 64u * K + 64u * kCoopStage;
-}
-
-// Results of collect_coop for the calling lane's query.
-struct CoopResult {
-  uint32_t live;      // entries of the list that hold a candidate (<= k)
-  uint32_t drop_min;  // smallest key bits of any candidate that left or never entered
-};
-
-// The k nearest candidates within `radius` (all of them when fewer) for the
-// query of every lane of a wavefront, in packed (fl32(d2) bits << 32 | slot)
-// keys; see PackedTopK for the order and the ambiguity rule.  Must be called
-// by all 64 lanes of the wavefront together; lanes with !active take part in
-// the inserts only.  `rows` (LDS, 64 * K words) receives each lane's sorted
-// list, lane q's at rows[q * K], entries [K - k, K) live; `stage` is LDS
-// scratch of 64 * kCoopStage words.
-//
-// Each lane walks its own query and parks in-radius candidates that beat its
-// current k-th key in its stage.  When the wavefront drains, the lists are
-// updated one query at a time by all lanes together: lane i holds entry i of
-// the row, the insert position is the first lane whose entry is larger (a
-// ballot), and the entries behind it move up one lane.  An insert costs a
-// handful of instructions for the whole wavefront instead of a K-step
-// compare-exchange chain per lane, and a lane's inserts no longer run once for
-// each of its neighbours' candidates that arrive at other node steps.
-template <int K, bool STATS>
-__device__ inline CoopResult collect_coop(const SceneView& s, uint32_t root, uint32_t end, bool active,
-                                          double qx, double qy, double qz, double radius, uint32_t k,
-                                          uint64_t* rows, uint64_t* stage, LaneStats& st) {
-  static_assert(K >= 1 && K <= 64, "one list entry per lane");
-  static_assert(kCoopStage > kMaxLeafSize, "a node step must fit a drained stage");
-  const uint32_t lane = threadIdx.x & 63;
-  const float qf[3] = {(float)qx, (float)qy, (float)qz};
-  const float m = coord_bound(scene_m(s), qx, qy, qz);
-  const double radius_sq = radius * radius;
-  float thr2 = frn_threshold2(radius, m);
-  // entries below K - k hold a 0 sentinel, so the k-th smallest is entry K - 1
-  // (k is the same for the whole wavefront: one job)
-  if (lane < (uint32_t)K) {
-    const uint64_t init = lane < (uint32_t)K - k ? 0ull : ~0ull;
-    for (uint32_t q = 0; q < 64; ++q) rows[q * K + lane] = init;
-  }
-  __builtin_amdgcn_wave_barrier();
-  uint64_t kth = ~0ull;  // this lane's entry K - 1
-  CoopResult res{0u, 0xFFFFFFFFu};
-  uint32_t staged = 0;
-  uint64_t* my_stage = stage + lane;  // entry r at my_stage[64 * r]
-
-  RingWalk walk;
-  walk.init<STATS>(s, root, end, qf, !(radius < __builtin_inf()), active, st);
-
-  auto visit = [&](uint32_t slot, const float4& p, float) {
-    const double d2 = point_d2(s, slot, p, qx, qy, qz);
-    if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
-    const uint64_t v = pack_key(d2, slot);
-    if (v < kth)
-      my_stage[64 * staged++] = v;
-    else
-      res.drop_min = min(res.drop_min, key_bits(v));
-  };
-
-  auto drain = [&]() {
-    __builtin_amdgcn_wave_barrier();  // stage and rows are read by other lanes
-    for (uint64_t pending = __ballot(staged != 0); pending; pending &= pending - 1) {
-      const uint32_t q = (uint32_t)__builtin_ctzll(pending);
-      const uint32_t n = __builtin_amdgcn_readlane(staged, q);
-      uint64_t mine = lane < (uint32_t)K ? rows[q * K + lane] : ~0ull;
-      uint32_t dropped = 0xFFFFFFFFu, filled = 0;
-      for (uint32_t r = 0; r < n; ++r) {
-        const uint64_t v = stage[64 * r + q];
-        const uint64_t larger = __ballot(lane < (uint32_t)K && v < mine);
-        if (larger == 0) {  // at or past the k-th key (the list changed since staging)
-          dropped = min(dropped, key_bits(v));
-          continue;
-        }
-        const uint32_t pos = (uint32_t)__builtin_ctzll(larger);
-        const uint64_t last = readlane64(mine, K - 1);
-        const uint64_t below = lane_up1(mine);
-        mine = lane < pos ? mine : (lane == pos ? v : below);
-        if (last == ~0ull)
-          ++filled;
-        else
-          dropped = min(dropped, key_bits(last));
-      }
-      if (lane < (uint32_t)K) rows[q * K + lane] = mine;
-      const uint64_t new_kth = readlane64(mine, K - 1);
-      if (lane == q) {
-        kth = new_kth;
-        res.live += filled;
-        res.drop_min = min(res.drop_min, dropped);
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    staged = 0;
-    if (kth != ~0ull) {
-      // every candidate whose key could equal the k-th key stays inside
-      const double kd2 = (double)__uint_as_float(key_bits(kth));
-      thr2 = frn_threshold2(fmin(radius, sqrt(kd2) * 1.0000002384185791015625), m);
-    }
-  };
-
-  while (__ballot(walk.walking) != 0) {
-    if (walk.walking) walk.step<STATS>(s, qf, thr2, st, visit);
-    if (__ballot(staged > kCoopStage - kMaxLeafSize) != 0) drain();
-  }
-  drain();
-  return res;
-}
-
 // ------------------------------------------------------------------- rays --
 
 // reference.py:810-846, Moller-Trumbore in this operation order.

@@ -322,3 +322,60 @@ def test_batch_knn_lattice_ties(gpu_ready):
     for job, res, want in zip(jobs, results, wants):
         assert res["status"] == 0
         _check_nn(B.job_rows(job, res), want, True)
+
+
+def _check_frn_order(rows):
+    """Each query's rows are contiguous and in (distance, neighbour id) order."""
+    q = rows["query_id"]
+    starts = np.flatnonzero(np.r_[True, q[1:] != q[:-1]])
+    assert len(np.unique(q)) == len(starts), "a query's rows are not contiguous"
+    d, n = rows["distance"], rows["neighbor_id"]
+    same = q[1:] == q[:-1]
+    ok = (d[1:] > d[:-1]) | ((d[1:] == d[:-1]) & (n[1:] > n[:-1]))
+    assert np.all(ok | ~same), "rows of a query are not in (distance, id) order"
+
+
+def test_batch_fixed_radius_dense_heavy_and_light(gpu_ready):
+    _say("case: batched fixed radius, queries with more and fewer neighbours than k")
+    rng = np.random.default_rng(91)
+    centres = rng.random((8, 3))
+    sdict, s3 = _points3(rng, 120000, "mixture", centres)
+    scene = _prepared_points(s3)
+    jobs, wants = [], []
+    for k, radius, kind in ((64, 0.02, "mixture"), (16, 0.01, "mixture"), (5, 0.03, "uniform"),
+                            (64, 0.002, "uniform")):
+        qd, q3 = _points3(rng, 3000, kind, centres)
+        jobs.append(B.BatchJob(scene, B.JOB_FIXED_RADIUS, _dev(q3), radius=radius, k=k))
+        wants.append(oracle.neighbors("frn", qd, sdict, radius=radius, k=k))
+    _say("  run_batch: 4 fixed-radius jobs")
+    results, _ = B.run_batch(jobs)
+    for job, res, want in zip(jobs, results, wants):
+        assert res["status"] == 0
+        rows = B.job_rows(job, res)
+        _check_frn_order(rows)
+        _check_nn(rows, want, False)
+
+
+def test_batch_fixed_radius_lattice_ties_and_duplicates(gpu_ready):
+    _say("case: batched fixed radius on a lattice with duplicate points")
+    rng = np.random.default_rng(92)
+    g = np.stack(np.meshgrid(np.arange(12), np.arange(12), np.arange(12), indexing="ij"), -1).reshape(-1, 3)
+    # 300 copies of one lattice site: more than the candidate parking of one query holds
+    g = np.concatenate([g, np.repeat(np.array([[5, 5, 5]]), 300, axis=0)])
+    g = g[rng.permutation(len(g))]
+    s3 = (g / 16.0).astype(np.float64)
+    sdict = {"id": np.arange(len(g), dtype=np.uint32), "x": s3[:, 0], "y": s3[:, 1], "z": s3[:, 2]}
+    scene = _prepared_points(s3)
+    jobs, wants = [], []
+    for k, radius in ((20, 2.5 / 16), (64, 3.2 / 16), (7, 1.0 / 16), (64, 0.0)):
+        qg = np.concatenate([rng.integers(0, 24, (600, 3)) / 32.0, np.array([[5, 5, 5], [5, 5, 6]]) / 16.0])
+        qd = {"id": np.arange(len(qg), dtype=np.uint32), "x": qg[:, 0], "y": qg[:, 1], "z": qg[:, 2]}
+        jobs.append(B.BatchJob(scene, B.JOB_FIXED_RADIUS, _dev(qg), radius=radius, k=k))
+        wants.append(oracle.neighbors("frn", qd, sdict, radius=radius, k=k))
+    _say("  run_batch: 4 fixed-radius lattice jobs")
+    results, _ = B.run_batch(jobs)
+    for job, res, want in zip(jobs, results, wants):
+        assert res["status"] == 0
+        rows = B.job_rows(job, res)
+        _check_frn_order(rows)
+        _check_nn(rows, want, False)
