@@ -260,6 +260,9 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
 constexpr uint32_t kFrnBuckets = 32;
 constexpr uint32_t kFrnCandMax = 128;
 constexpr int kFrnBlock = 64;
+// histogram counters are 16-bit, two per LDS word; a query with more
+// neighbours than one counter holds goes to the exact fix-up
+constexpr uint32_t kFrnCountMax = 0xFFFFu;
 
 // Distance bucket in [0, kFrnBuckets): dist * scale is monotone in dist.
 __device__ inline uint32_t frn_bucket(double dist, double scale) {
@@ -284,13 +287,14 @@ __device__ inline void load_query3(const DeviceJob& j, uint32_t qi, double& qx, 
 }
 
 template <bool STATS>
-__global__ __launch_bounds__(kFrnBlock) void frn_count_kernel(
+__global__ __launch_bounds__(256) void frn_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
     uint32_t* item_counter, FrnPlan* __restrict__ plans, unsigned long long* cand_count,
     uint64_t cand_cap, uint4* amb, uint32_t* amb_count, unsigned long long* stats) {
-  __shared__ uint32_t hist[kFrnBuckets * kFrnBlock];  // bucket b of lane l at hist[b * 64 + l]
-  const uint32_t lane = threadIdx.x;
-  uint32_t* my_hist = hist + lane;
+  // buckets 2w and 2w + 1 of thread t in the low and high half of hist[w * 256 + t]
+  __shared__ uint32_t hist[kFrnBuckets / 2 * 256];
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t* my_hist = hist + threadIdx.x;
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -303,7 +307,7 @@ __global__ __launch_bounds__(kFrnBlock) void frn_count_kernel(
     const uint32_t k = j.k;
     const double radius_sq = j.radius * j.radius;  // reference.py:357
     const double scale = (double)kFrnBuckets / j.radius;
-    for (uint32_t b = 0; b < kFrnBuckets; ++b) my_hist[64 * b] = 0u;
+    for (uint32_t w2 = 0; w2 < kFrnBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
     uint32_t qi = 0, count = 0;
     if (active) {
       qi = job_query(j, w.first + lane);
@@ -316,16 +320,19 @@ __global__ __launch_bounds__(kFrnBlock) void frn_count_kernel(
                            const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
                            if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
                            ++count;
-                           ++my_hist[64 * frn_bucket(sqrt(d2), scale)];
+                           const uint32_t b = frn_bucket(sqrt(d2), scale);
+                           // no result needed: a plain LDS add
+                           __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)),
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                          });
     }
     const bool heavy = count > k;
     uint32_t n_rows = heavy ? k : count;
     uint32_t bucket = kFrnBuckets - 1, n_cand = 0;
-    if (heavy) {
+    if (heavy && count <= kFrnCountMax) {
       uint32_t cum = 0;
       for (uint32_t b = 0; b < kFrnBuckets; ++b) {
-        cum += my_hist[64 * b];
+        cum += (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu;
         if (cum >= k) {
           bucket = b;
           n_cand = cum;
@@ -333,7 +340,7 @@ __global__ __launch_bounds__(kFrnBlock) void frn_count_kernel(
         }
       }
     }
-    bool refine = heavy && n_cand > kFrnCandMax;
+    bool refine = heavy && (count > kFrnCountMax || n_cand > kFrnCandMax);
     const uint64_t row_at = wave_reserve(j.counters, n_rows);
     if (n_rows && row_at + n_rows > j.row_capacity) {
       atomicOr(&j.counters->status, kStatusRowOverflow);
@@ -923,15 +930,15 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
         RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamFrn) {
-      const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 16));
+      const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 32));
       if (stats_on_)
-        frn_count_kernel<true><<<g64, kFrnBlock, 0, s>>>(dj, it, n_items, ctr, frn_plan_.get(),
-                                                          frn_cand_count_.get(), frn_cand_.size(),
-                                                          amb_.get(), amb_count_.get(), st);
+        frn_count_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, frn_plan_.get(),
+                                                     frn_cand_count_.get(), frn_cand_.size(),
+                                                     amb_.get(), amb_count_.get(), st);
       else
-        frn_count_kernel<false><<<g64, kFrnBlock, 0, s>>>(dj, it, n_items, ctr, frn_plan_.get(),
-                                                           frn_cand_count_.get(), frn_cand_.size(),
-                                                           amb_.get(), amb_count_.get(), st);
+        frn_count_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, frn_plan_.get(),
+                                                      frn_cand_count_.get(), frn_cand_.size(),
+                                                      amb_.get(), amb_count_.get(), st);
       RTDL_LAUNCH_CHECK();
       frn_emit_kernel<<<grid, 256, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueFrnEmit * kXcds,
                                            frn_plan_.get(), frn_cand_.get());
