@@ -570,7 +570,7 @@ def main() -> int:
         if baselines.get("per_request_resident", {}).get("queries_per_s") else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": {"frn": "batch_points_kernel", "knn": "batch_points_kernel",
+            "kernel": {"frn": "nbr_count/emit/sort_kernel", "knn": "nbr_count/emit/sort_kernel",
                        "ray": "batch_rays_kernel", "pip": "batch_pip_kernel"}[dominant] + f" ({dominant})",
             "achieved": d["achieved_GBps"],
             "peak": HBM_PEAK / 1e9,

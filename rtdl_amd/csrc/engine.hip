@@ -117,7 +117,6 @@ __device__ inline uint32_t job_query(const DeviceJob& j, uint32_t pos) {
   return j.order ? j.order[j.order_base + pos] : pos;
 }
 
-template <int K>
 __device__ inline void write_neighbor_row(const DeviceJob& j, uint64_t at, uint32_t qi, uint32_t id,
                                           double dist, uint32_t rank) {
   if (j.kind == kJobFixedRadius) {
@@ -127,72 +126,24 @@ __device__ inline void write_neighbor_row(const DeviceJob& j, uint64_t at, uint3
   }
 }
 
-template <int K, bool STATS>
-__global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __restrict__ jobs,
-                                                           const WorkItem* __restrict__ items,
-                                                           uint32_t n_items, uint32_t* item_counter,
-                                                           uint4* amb, uint32_t* amb_count,
-                                                           unsigned long long* stats) {
-  const uint32_t lane = threadIdx.x & 63;
-  LaneStats st;
-  ItemQueue queue(item_counter, n_items);
-  for (;;) {
-    const uint32_t it = queue.next();
-    if (it == kItemDone) break;
-    if (it == kItemRetry) continue;
-    const WorkItem w = items[it];
-    const DeviceJob& j = jobs[w.job];
-    const bool active = lane < w.count;
-    uint32_t qi = 0;
-    double qx = 0.0, qy = 0.0, qz = 0.0;
-    PackedTopK<K> list;
-    list.init(1);
-    bool ambiguous = false;
-    uint32_t c = 0;
-    const uint32_t k = j.k;
-    if (active) {
-      qi = job_query(j, w.first + lane);
-      const float* qp = j.queries + (size_t)qi * j.qstride;
-      qx = qp[0];
-      qy = qp[1];
-      qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
-      ambiguous = collect_packed<K, STATS>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius,
-                                           k, list, st);
-#pragma unroll
-      for (int i = 0; i < K; ++i)
-        if (i >= K - (int)k && list.e[i] != ~0ull) ++c;
-    }
-    const uint64_t at0 = wave_reserve(j.counters, c);
-    if (active && c) {
-      if (at0 + c > j.row_capacity) {
-        atomicOr(&j.counters->status, kStatusRowOverflow);
-      } else {
-        uint32_t r = 0;
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-          if (i >= K - (int)k && list.e[i] != ~0ull) {
-            const uint32_t slot = (uint32_t)list.e[i];
-            const float4 p = j.scene.prims[slot];
-            const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
-            write_neighbor_row<K>(j, at0 + r, qi, __float_as_uint(p.w), dist, r + 1);
-            ++r;
-          }
-        }
-        if (ambiguous) {
-          amb[atomicAdd(amb_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
-          atomicAdd(&j.counters->ambiguous, 1u);
-        }
-      }
-    }
+__device__ inline void read_neighbor_row(const DeviceJob& j, uint64_t at, double& dist, uint32_t& id) {
+  if (j.kind == kJobFixedRadius) {
+    const FrnRowRec r = static_cast<const FrnRowRec*>(j.rows)[at];
+    dist = r.distance;
+    id = r.neighbor_id;
+  } else {
+    const KnnRowRec r = static_cast<const KnnRowRec*>(j.rows)[at];
+    dist = r.distance;
+    id = r.neighbor_id;
   }
-  if (STATS) flush_lane_stats(stats, st);
 }
 
-// Recomputes the queries whose FP32 selection keys tied at the boundary with
-// the exact (FP64 distance, id) order and rewrites their rows in place.  The
-// per-lane list lives in LDS (slot-major, so the 64 lanes of a wavefront hit
-// consecutive words) instead of registers: a 64-entry FP64 list does not fit
-// a lane's registers, and these queries are rare.
+// Computes the rows of the neighbour queries whose candidates do not fit the
+// parking space of the count / emit / sort pipeline, with an exact sorted
+// (FP64 distance, id) list per lane, and writes them at the rows the count
+// pass reserved.  The list lives in LDS (slot-major, so the 64 lanes of a
+// wavefront hit consecutive words): a 64-entry FP64 list does not fit a
+// lane's registers, and these queries are rare.
 constexpr int kFixBlock = 64;
 
 __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* __restrict__ jobs,
@@ -238,36 +189,47 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
     walk_neighbors<false>(j.scene, j.scene.root, j.scene.end, qf, j.radius, thr2, st, visit);
     const uint32_t rows = min(count, a.w);
     for (uint32_t r = 0; r < rows; ++r)
-      write_neighbor_row<0>(j, (uint64_t)a.z + r, a.y, li[r * kFixBlock], ld[r * kFixBlock], r + 1);
+      write_neighbor_row(j, (uint64_t)a.z + r, a.y, li[r * kFixBlock], ld[r * kFixBlock], r + 1);
   }
 }
 
-// ---------------------------------------------------------- fixed radius --
+// ------------------------------------------------------------- neighbours --
 //
-// Fixed-radius jobs need no top-k list.  frn_count_kernel walks each query's
-// radius once, counting the neighbours inside it with a histogram of their
-// distances over kFrnBuckets equal slices of [0, radius].  A query with
-// c <= k neighbours reserves c rows, which frn_emit_kernel writes in walk
-// order and frn_sort_kernel sorts into the (distance, id) order
-// (reference.py:362-366).  A query with more keeps its k nearest: the first
-// bucket b whose cumulative count reaches k holds the k-th distance, so the
-// emit walk parks only the neighbours of buckets 0..b (at most kFrnCandMax)
-// and the sort writes the first k of them.  Every decision uses the FP64
-// distance in the reference's operation order and equal distances share a
-// bucket, so the rows are exact by construction.  A query whose buckets 0..b
-// hold more than kFrnCandMax neighbours (or that finds the parking space
-// full) is recomputed by batch_fix_kernel instead.
-constexpr uint32_t kFrnBuckets = 32;
-constexpr uint32_t kFrnCandMax = 128;
-constexpr int kFrnBlock = 64;
-// histogram counters are 16-bit, two per LDS word; a query with more
+// Fixed-radius, KNN and bounded-KNN jobs run without a top-k list, in three
+// launches.  nbr_count_kernel walks each query's ball near-first, counting the
+// neighbours inside it with a histogram of their distances over kNbrBuckets
+// equal slices of [0, radius].  Once k neighbours lie in buckets 0..b the
+// walk shrinks to bucket b's edge (farther points cannot be among the k
+// nearest), so it ends close to the k-th distance.  A KNN query has no
+// radius: a first near-first walk takes the largest distance of the first k
+// points it meets, which bounds the k-th distance from above.  A query with
+// c <= k neighbours reserves c rows, which nbr_emit_kernel writes in walk
+// order and nbr_sort_kernel sorts into the (distance, id) order
+// (reference.py:362-366, 382-390).  A query with more keeps its k nearest:
+// the first bucket b whose cumulative count reaches k holds the k-th
+// distance, the emit walk parks only the neighbours of buckets 0..b (at most
+// kNbrCandMax) and the sort writes the first k of them.  Every decision uses
+// the FP64 distance in the reference's operation order and equal distances
+// share a bucket, so the rows are exact by construction.  A query whose
+// buckets 0..b hold more than kNbrCandMax neighbours (or that finds the
+// parking space full) is computed by batch_fix_kernel instead.
+constexpr uint32_t kNbrBuckets = 32;
+constexpr uint32_t kNbrCandMax = 128;
+constexpr int kSortBlock = 64;
+// histogram counters are 16-bit, two per LDS word; a query that meets more
 // neighbours than one counter holds goes to the exact fix-up
-constexpr uint32_t kFrnCountMax = 0xFFFFu;
+constexpr uint32_t kNbrCountMax = 0xFFFFu;
 
-// Distance bucket in [0, kFrnBuckets): dist * scale is monotone in dist.
-__device__ inline uint32_t frn_bucket(double dist, double scale) {
+// Distance bucket in [0, kNbrBuckets): dist * scale is monotone in dist.
+__device__ inline uint32_t nbr_bucket(double dist, double scale) {
   const double x = dist * scale;
-  return x < (double)kFrnBuckets ? (x >= 0.0 ? (uint32_t)x : 0u) : kFrnBuckets - 1;
+  return x < (double)kNbrBuckets ? (x >= 0.0 ? (uint32_t)x : 0u) : kNbrBuckets - 1;
+}
+
+// Distance bound of buckets 0..b: bucket(dist) <= b means dist * scale < b + 1,
+// so dist < (b + 1) / scale; the factor covers the rounding of the quotient.
+__device__ inline double nbr_bucket_edge(uint32_t b, double scale) {
+  return (double)(b + 1) / scale * 1.0000000000009094947;  // 1 + 2^-40
 }
 
 __device__ inline uint64_t lane0_add64(unsigned long long* counter, uint32_t value) {
@@ -287,14 +249,15 @@ __device__ inline void load_query3(const DeviceJob& j, uint32_t qi, double& qx, 
 }
 
 template <bool STATS>
-__global__ __launch_bounds__(256) void frn_count_kernel(
+__global__ __launch_bounds__(256) void nbr_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
-    uint32_t* item_counter, FrnPlan* __restrict__ plans, unsigned long long* cand_count,
+    uint32_t* item_counter, NbrPlan* __restrict__ plans, unsigned long long* cand_count,
     uint64_t cand_cap, uint4* amb, uint32_t* amb_count, unsigned long long* stats) {
   // buckets 2w and 2w + 1 of thread t in the low and high half of hist[w * 256 + t]
-  __shared__ uint32_t hist[kFrnBuckets / 2 * 256];
+  __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
   const uint32_t lane = threadIdx.x & 63;
   uint32_t* my_hist = hist + threadIdx.x;
+  auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -305,42 +268,75 @@ __global__ __launch_bounds__(256) void frn_count_kernel(
     const DeviceJob& j = jobs[w.job];
     const bool active = lane < w.count;
     const uint32_t k = j.k;
-    const double radius_sq = j.radius * j.radius;  // reference.py:357
-    const double scale = (double)kFrnBuckets / j.radius;
-    for (uint32_t w2 = 0; w2 < kFrnBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
-    uint32_t qi = 0, count = 0;
+    for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
+    uint32_t qi = 0, count = 0, top = kNbrBuckets - 1;
+    bool shrunk = false;
+    double radius = j.radius, radius_sq = j.radius * j.radius;  // reference.py:357
     if (active) {
       qi = job_query(j, w.first + lane);
       double qx, qy, qz;
       load_query3(j, qi, qx, qy, qz);
       const float qf[3] = {(float)qx, (float)qy, (float)qz};
-      float thr2 = frn_threshold2(j.radius, coord_bound(scene_m(j.scene), qx, qy, qz));
-      walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
-                         [&](uint32_t slot, const float4& p, float) {
-                           const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
-                           if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
-                           ++count;
-                           const uint32_t b = frn_bucket(sqrt(d2), scale);
-                           // no result needed: a plain LDS add
-                           __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)),
-                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                         });
+      const float m = coord_bound(scene_m(j.scene), qx, qy, qz);
+      if (!(radius < __builtin_inf())) {
+        // KNN: the farthest of the first k points met bounds the k-th distance
+        uint32_t seen = 0;
+        double far = 0.0;
+        float t2 = __builtin_inff();
+        walk_near_first<STATS>(j.scene, j.scene.root, j.scene.end, qf, t2, st,
+                               [&](uint32_t slot, const float4& p, float) {
+                                 if (seen == k) return;
+                                 far = fmax(far, point_d2(j.scene, slot, p, qx, qy, qz));
+                                 if (++seen == k) t2 = -1.0f;  // the walk ends
+                               });
+        if (seen == k) {
+          radius_sq = far;
+          radius = sqrt(far);
+        }
+      }
+      const double scale = (double)kNbrBuckets / radius;
+      uint32_t next_check = k;
+      float thr2 = frn_threshold2(radius, m);
+      walk_near_first<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
+                             [&](uint32_t slot, const float4& p, float) {
+                               const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
+                               if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
+                               const uint32_t b = nbr_bucket(sqrt(d2), scale);
+                               if (b > top) return;  // beyond the k nearest
+                               ++count;
+                               // no result needed: a plain LDS add
+                               __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)),
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                               if (count < next_check) return;
+                               next_check = 2 * count;
+                               uint32_t cum = 0;
+                               for (uint32_t c = 0; c < top; ++c) {
+                                 cum += bucket_count(c);
+                                 if (cum >= k) {  // the k nearest lie in buckets 0..c
+                                   top = c;
+                                   shrunk = true;
+                                   thr2 = frn_threshold2(fmin(radius, nbr_bucket_edge(c, scale)), m);
+                                   break;
+                                 }
+                               }
+                             });
     }
-    const bool heavy = count > k;
+    // more than k neighbours: keep the k nearest, from buckets 0..bucket
+    const bool heavy = shrunk || count > k;
     uint32_t n_rows = heavy ? k : count;
-    uint32_t bucket = kFrnBuckets - 1, n_cand = 0;
-    if (heavy && count <= kFrnCountMax) {
+    uint32_t bucket = top, n_cand = 0;
+    if (heavy && count <= kNbrCountMax) {
       uint32_t cum = 0;
-      for (uint32_t b = 0; b < kFrnBuckets; ++b) {
-        cum += (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu;
+      for (uint32_t c = 0; c <= top; ++c) {
+        cum += bucket_count(c);
         if (cum >= k) {
-          bucket = b;
+          bucket = c;
           n_cand = cum;
           break;
         }
       }
     }
-    bool refine = heavy && (count > kFrnCountMax || n_cand > kFrnCandMax);
+    bool refine = heavy && (count > kNbrCountMax || n_cand > kNbrCandMax);
     const uint64_t row_at = wave_reserve(j.counters, n_rows);
     if (n_rows && row_at + n_rows > j.row_capacity) {
       atomicOr(&j.counters->status, kStatusRowOverflow);
@@ -357,52 +353,46 @@ __global__ __launch_bounds__(256) void frn_count_kernel(
       atomicAdd(&j.counters->ambiguous, 1u);
     }
     if (active)
-      plans[(size_t)it * 64 + lane] =
-          FrnPlan{row_at, cand_at, refine ? 0u : n_rows, heavy ? n_cand : 0u, bucket, qi};
+      plans[(size_t)it * 64 + lane] = NbrPlan{row_at,  cand_at,   radius, radius_sq, refine ? 0u : n_rows,
+                                              heavy ? n_cand : 0u, bucket, qi};
   }
   if (STATS) flush_lane_stats(stats, st);
 }
 
-// Writes one query's rows (or parks its candidates) on a second radius walk;
-// the walk meets the same candidates in the same FP64 arithmetic as the count.
-__device__ inline void frn_emit_query(const DeviceJob& j, const FrnPlan& pl, FrnCand* cands, LaneStats& st) {
+// Writes one query's rows (or parks its candidates) on a second walk of its
+// ball; the walk meets the same candidates in the same FP64 arithmetic as the
+// count.
+__device__ inline void nbr_emit_query(const DeviceJob& j, const NbrPlan& pl, NbrCand* cands, LaneStats& st) {
   double qx, qy, qz;
   load_query3(j, pl.qi, qx, qy, qz);
   const float qf[3] = {(float)qx, (float)qy, (float)qz};
-  const double radius_sq = j.radius * j.radius;  // reference.py:357
-  const double scale = (double)kFrnBuckets / j.radius;
+  const double scale = (double)kNbrBuckets / pl.radius;
   const bool parked = pl.n_cand != 0;
-  double bound = j.radius;
-  if (parked) {
-    // bucket(dist) <= b means dist * scale < b + 1, so dist < (b + 1) / scale
-    const double edge = (double)(pl.bucket + 1) / scale;
-    bound = fmin(bound, edge * 1.0000000000009094947);  // 1 + 2^-40
-  }
+  const double bound = parked ? fmin(pl.radius, nbr_bucket_edge(pl.bucket, scale)) : pl.radius;
   float thr2 = frn_threshold2(bound, coord_bound(scene_m(j.scene), qx, qy, qz));
   uint32_t r = 0;
-  FrnRowRec* rows = static_cast<FrnRowRec*>(j.rows) + pl.row_at;
-  FrnCand* park = cands + pl.cand_at;
+  NbrCand* park = cands + pl.cand_at;
   walk_points<false>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
                      [&](uint32_t slot, const float4& p, float) {
                        const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
-                       if (d2 > radius_sq) return;
+                       if (d2 > pl.radius_sq) return;
                        const double dist = sqrt(d2);
                        const uint32_t id = __float_as_uint(p.w);
-                       if (!parked) {
-                         if (r < pl.n_rows) rows[r] = FrnRowRec{pl.qi, id, dist};
+                       if (!parked) {  // ranks of more than one row are set by the sort
+                         if (r < pl.n_rows) write_neighbor_row(j, pl.row_at + r, pl.qi, id, dist, r + 1);
                          ++r;
-                       } else if (frn_bucket(dist, scale) <= pl.bucket) {
-                         if (r < pl.n_cand) park[r] = FrnCand{dist, id, 0u};
+                       } else if (nbr_bucket(dist, scale) <= pl.bucket) {
+                         if (r < pl.n_cand) park[r] = NbrCand{dist, id, 0u};
                          ++r;
                        }
                      });
 }
 
-__global__ __launch_bounds__(256) void frn_emit_kernel(const DeviceJob* __restrict__ jobs,
+__global__ __launch_bounds__(256) void nbr_emit_kernel(const DeviceJob* __restrict__ jobs,
                                                        const WorkItem* __restrict__ items,
                                                        uint32_t n_items, uint32_t* item_counter,
-                                                       const FrnPlan* __restrict__ plans,
-                                                       FrnCand* __restrict__ cands) {
+                                                       const NbrPlan* __restrict__ plans,
+                                                       NbrCand* __restrict__ cands) {
   const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
@@ -412,12 +402,11 @@ __global__ __launch_bounds__(256) void frn_emit_kernel(const DeviceJob* __restri
     if (it == kItemRetry) continue;
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
-    const FrnPlan pl = lane < w.count ? plans[(size_t)it * 64 + lane] : FrnPlan{0, 0, 0, 0, 0, 0};
+    const NbrPlan pl = lane < w.count ? plans[(size_t)it * 64 + lane] : NbrPlan{};
     // every lane returns to the item fetch together; lanes without rows idle
-    if (pl.n_rows) frn_emit_query(j, pl, cands, st);
+    if (pl.n_rows) nbr_emit_query(j, pl, cands, st);
   }
 }
-
 
 __device__ inline uint64_t shfl_xor64(uint64_t x, uint32_t m) {
   const uint32_t lo = __shfl_xor((uint32_t)x, (int)m, 64);
@@ -469,11 +458,11 @@ __device__ inline void wave_sort_dist_id(uint64_t& d0, uint32_t& i0, uint64_t& d
   }
 }
 
-__global__ __launch_bounds__(kFrnBlock) void frn_sort_kernel(const DeviceJob* __restrict__ jobs,
-                                                            const WorkItem* __restrict__ items,
-                                                            uint32_t n_items, uint32_t* item_counter,
-                                                            const FrnPlan* __restrict__ plans,
-                                                            const FrnCand* __restrict__ cands) {
+__global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* __restrict__ jobs,
+                                                             const WorkItem* __restrict__ items,
+                                                             uint32_t n_items, uint32_t* item_counter,
+                                                             const NbrPlan* __restrict__ plans,
+                                                             const NbrCand* __restrict__ cands) {
   const uint32_t lane = threadIdx.x;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -482,9 +471,8 @@ __global__ __launch_bounds__(kFrnBlock) void frn_sort_kernel(const DeviceJob* __
     if (it == kItemRetry) continue;
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
-    FrnRowRec* rows = static_cast<FrnRowRec*>(j.rows);
     for (uint32_t q = 0; q < w.count; ++q) {  // one query at a time, all lanes
-      const FrnPlan pl = plans[(size_t)it * 64 + q];
+      const NbrPlan pl = plans[(size_t)it * 64 + q];
       if (pl.n_rows == 0 || (pl.n_cand == 0 && pl.n_rows == 1)) continue;
       uint64_t d0 = ~0ull, d1 = ~0ull;
       uint32_t i0 = ~0u, i1 = ~0u;
@@ -492,25 +480,26 @@ __global__ __launch_bounds__(kFrnBlock) void frn_sort_kernel(const DeviceJob* __
       if (pl.n_cand == 0) {  // rows written in walk order, sorted in place
         n = pl.n_rows;
         if (lane < n) {
-          const FrnRowRec r = rows[pl.row_at + lane];
-          d0 = __builtin_bit_cast(uint64_t, r.distance);
-          i0 = r.neighbor_id;
+          double dist;
+          read_neighbor_row(j, pl.row_at + lane, dist, i0);
+          d0 = __builtin_bit_cast(uint64_t, dist);
         }
       } else {  // parked candidates; the first n_rows of them are the rows
         n = pl.n_cand;
         if (lane < n) {
-          const FrnCand c = cands[pl.cand_at + lane];
+          const NbrCand c = cands[pl.cand_at + lane];
           d0 = __builtin_bit_cast(uint64_t, c.distance);
           i0 = c.id;
         }
         if (lane + 64 < n) {
-          const FrnCand c = cands[pl.cand_at + lane + 64];
+          const NbrCand c = cands[pl.cand_at + lane + 64];
           d1 = __builtin_bit_cast(uint64_t, c.distance);
           i1 = c.id;
         }
       }
       wave_sort_dist_id(d0, i0, d1, i1, n);
-      if (lane < pl.n_rows) rows[pl.row_at + lane] = FrnRowRec{pl.qi, i0, __builtin_bit_cast(double, d0)};
+      if (lane < pl.n_rows)
+        write_neighbor_row(j, pl.row_at + lane, pl.qi, i0, __builtin_bit_cast(double, d0), lane + 1);
     }
   }
 }
@@ -688,53 +677,32 @@ __global__ void zero_words_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb
   if (i < nc) c[i] = 0ull;
 }
 
-// One persistent launch per family.  Neighbour jobs are split by the list
-// size their k needs (1, 2, 4, ..., 64), so a small-k job never pays for the
-// registers of a larger one in the same batch.
+// One persistent launch (or chain of launches) per family.
 enum Family : int {
-  kFamPoints1 = 0,  // ... kFamPoints1 + 6 for K = 64
-  kFamFrn = kPointBuckets,  // fixed radius: count, emit, sort
+  kFamNbr = 0,  // fixed radius, KNN, bounded KNN: count, emit, sort
   kFamRayAny,
   kFamRayCount,
   kFamRayClosest,
   kFamPip,
 };
 static_assert(kFamPip + 1 == kLaunchFamilies, "family count");
-// work queues: one per family, plus the fixed-radius emit and sort launches
+// work queues: one per family, plus the neighbour emit and sort launches
 constexpr int kItemCounterSets = kLaunchFamilies + 2;
-constexpr int kQueueFrnEmit = kLaunchFamilies;
-constexpr int kQueueFrnSort = kLaunchFamilies + 1;
-// parking space for fixed-radius candidates, per fixed-radius query of a batch
-constexpr uint64_t kFrnCandPerQuery = 80;
+constexpr int kQueueNbrEmit = kLaunchFamilies;
+constexpr int kQueueNbrSort = kLaunchFamilies + 1;
+// parking space for neighbour candidates, per neighbour query of a batch
+constexpr uint64_t kNbrCandPerQuery = 80;
 
-int family_of(uint32_t kind, uint32_t k) {
+int family_of(uint32_t kind) {
   switch (kind) {
-    case kJobFixedRadius: return kFamFrn;
+    case kJobFixedRadius:
     case kJobKnn:
-    case kJobBoundedKnn: {
-      int b = 0;
-      while ((1u << b) < k) ++b;  // list_capacity(k) == 1 << b
-      return kFamPoints1 + b;
-    }
+    case kJobBoundedKnn: return kFamNbr;
     case kJobRayAnyHit: return kFamRayAny;
     case kJobRayHitCount: return kFamRayCount;
     case kJobRayClosest: return kFamRayClosest;
     default: return kFamPip;
   }
-}
-
-// report group of a launch family: neighbours, any-hit, hit-count, closest, PIP
-int group_of(int family) { return family <= kFamFrn ? 0 : family - kFamFrn; }
-
-template <int K>
-void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
-                   uint32_t n_items, uint32_t* counter, uint4* amb, uint32_t* amb_count,
-                   unsigned long long* st, hipStream_t s) {
-  if (stats)
-    batch_points_kernel<K, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
-  else
-    batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
-  RTDL_LAUNCH_CHECK();
 }
 
 template <int MODE>
@@ -757,7 +725,7 @@ BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
   stats_.resize(2);
   item_counters_.resize(kItemCounterSets * kXcds);
   amb_count_.resize(2);  // [0] neighbour queries for the exact fix-up, [1] PIP points past the hit cap
-  frn_cand_count_.resize(1);
+  nbr_cand_count_.resize(1);
   for (int f = 0; f < kLaunchFamilies; ++f) {
     RTDL_HIP(hipEventCreate(&fam_start_[f]));
     RTDL_HIP(hipEventCreate(&fam_stop_[f]));
@@ -770,7 +738,7 @@ void BatchEngine::family_seconds(double out[5]) const {
     if (!timed_[f]) continue;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, fam_start_[f], fam_stop_[f]) == hipSuccess)
-      out[group_of(f)] += ms * 1e-3;
+      out[f] += ms * 1e-3;  // families are the report groups
     else
       (void)hipGetLastError();
   }
@@ -808,7 +776,7 @@ void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
     if (host_items_) RTDL_HIP(hipHostFree(host_items_));
     RTDL_HIP(hipHostMalloc((void**)&host_items_, max_items * sizeof(WorkItem), hipHostMallocDefault));
     items_.resize(max_items);
-    frn_plan_.resize(max_items * 64);
+    nbr_plan_.resize(max_items * 64);
     cap_items_ = max_items;
   }
   if (max_queries > cap_queries_) {
@@ -836,7 +804,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     require(h.nq == 0 || h.queries != nullptr, "job queries must not be null");
     require(h.nq == 0 || h.rows != nullptr, "job rows must not be null");
     if (h.kind <= kJobBoundedKnn) require(h.k >= 1 && h.k <= 64, "batched neighbour jobs need 1 <= k <= 64");
-    const int fam = family_of(h.kind, h.k);
+    const int fam = family_of(h.kind);
     DeviceJob& d = host_jobs_[i];
     std::memset(&d, 0, sizeof(d));
     d.scene = h.scene;
@@ -858,12 +826,12 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     start += h.nq;
   }
   host_starts_[n] = start;
-  uint64_t frn_queries = 0;
+  uint64_t nbr_queries = 0;
   for (const HostJob& h : jobs)
-    if (h.kind == kJobFixedRadius) frn_queries += h.nq;
-  if (frn_queries) {
-    const size_t want = (size_t)std::max<uint64_t>(frn_queries * kFrnCandPerQuery, 1u << 16);
-    if (frn_cand_.size() < want) frn_cand_.resize(want);
+    if (h.kind <= kJobBoundedKnn) nbr_queries += h.nq;
+  if (nbr_queries) {
+    const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
+    if (nbr_cand_.size() < want) nbr_cand_.resize(want);
   }
   size_t item_off[kLaunchFamilies + 1] = {0};
   for (int f = 0; f < kLaunchFamilies; ++f) {
@@ -884,7 +852,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
       stats_.get(), 2);
   RTDL_LAUNCH_CHECK();
   fill_u32(amb_count_.get(), 2, 0u, s);
-  fill_u32(reinterpret_cast<uint32_t*>(frn_cand_count_.get()), 2, 0u, s);
+  fill_u32(reinterpret_cast<uint32_t*>(nbr_cand_count_.get()), 2, 0u, s);
   if (morton_sort && total > 1) {
     batch_keys_kernel<<<blocks_for(total, 256), 256, 0, s>>>(jobs_.get(), starts_.get(), (uint32_t)n,
                                                               total, keys_.get(), order_.get());
@@ -901,9 +869,6 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   };
   unsigned long long* st = stats_on_ ? stats_.get() : nullptr;
   const DeviceJob* dj = jobs_.get();
-  int last_points = -1;  // the exact fix-up runs once, after the last neighbour launch
-  for (int f = 0; f <= kFamFrn; ++f)
-    if (item_off[f + 1] > item_off[f]) last_points = f;
   for (int f = 0; f < kLaunchFamilies; ++f) timed_[f] = false;
   for (int f = 0; f < kLaunchFamilies; ++f) {
     const uint32_t n_items = (uint32_t)(item_off[f + 1] - item_off[f]);
@@ -915,41 +880,25 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
       RTDL_HIP(hipEventRecord(fam_start_[f], s));
       timed_[f] = true;
     }
-    if (f < kPointBuckets) {
-      switch (f) {
-        case 0: launch_points<1>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 1: launch_points<2>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 2: launch_points<4>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 3: launch_points<8>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 4: launch_points<16>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        case 5: launch_points<32>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-        default: launch_points<64>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
-      }
-      if (f == last_points) {
-        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
-        RTDL_LAUNCH_CHECK();
-      }
-    } else if (f == kFamFrn) {
-      const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 32));
+    if (f == kFamNbr) {
       if (stats_on_)
-        frn_count_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, frn_plan_.get(),
-                                                     frn_cand_count_.get(), frn_cand_.size(),
+        nbr_count_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
+                                                     nbr_cand_count_.get(), nbr_cand_.size(),
                                                      amb_.get(), amb_count_.get(), st);
       else
-        frn_count_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, frn_plan_.get(),
-                                                      frn_cand_count_.get(), frn_cand_.size(),
+        nbr_count_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
+                                                      nbr_cand_count_.get(), nbr_cand_.size(),
                                                       amb_.get(), amb_count_.get(), st);
       RTDL_LAUNCH_CHECK();
-      frn_emit_kernel<<<grid, 256, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueFrnEmit * kXcds,
-                                           frn_plan_.get(), frn_cand_.get());
+      nbr_emit_kernel<<<grid, 256, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrEmit * kXcds,
+                                           nbr_plan_.get(), nbr_cand_.get());
       RTDL_LAUNCH_CHECK();
-      frn_sort_kernel<<<g64, kFrnBlock, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueFrnSort * kXcds,
-                                                frn_plan_.get(), frn_cand_.get());
+      const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 32));
+      nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrSort * kXcds,
+                                                 nbr_plan_.get(), nbr_cand_.get());
       RTDL_LAUNCH_CHECK();
-      if (f == last_points) {
-        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
-        RTDL_LAUNCH_CHECK();
-      }
+      batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
+      RTDL_LAUNCH_CHECK();
     } else if (f == kFamRayAny) {
       launch_rays_batch<0>(stats_on_, grid, dj, it, n_items, ctr, st, s);
     } else if (f == kFamRayCount) {

@@ -58,8 +58,7 @@ size_t job_row_bytes(uint32_t kind);
 uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_cap);
 
 constexpr uint32_t kPipHitCap = 16;   // containing polygons held in registers per point
-constexpr int kPointBuckets = 7;      // neighbour list sizes 1, 2, 4, ..., 64
-constexpr int kLaunchFamilies = kPointBuckets + 5;  // + fixed radius, ray any / count / closest, PIP
+constexpr int kLaunchFamilies = 5;  // neighbours, ray any / count / closest, PIP
 constexpr uint32_t kStatusRowOverflow = 1u;
 
 struct JobCounters {
@@ -85,19 +84,21 @@ struct DeviceJob {
   uint32_t pad;
 };
 
-// Per-query plan of a fixed-radius job between its count, emit and sort
+// Per-query plan of a neighbour job between its count, emit and sort
 // launches (indexed by work item * 64 + lane).
-struct FrnPlan {
+struct NbrPlan {
   uint64_t row_at;   // first output row
   uint64_t cand_at;  // first parked candidate (queries with more than k neighbours)
+  double radius;     // ball walked: the job's radius, or the KNN bound
+  double radius_sq;  // membership: squared distance <= radius_sq
   uint32_t n_rows;   // rows of the query (0: nothing to emit or sort here)
   uint32_t n_cand;   // parked candidates; 0 when the rows are written directly
   uint32_t bucket;   // last distance bucket parked
   uint32_t qi;       // job-local query index
 };
 
-// A parked fixed-radius candidate.
-struct FrnCand {
+// A parked neighbour candidate.
+struct NbrCand {
   double distance;
   uint32_t id;
   uint32_t pad;
@@ -165,9 +166,9 @@ class BatchEngine {
   DeviceArray<uint4> amb_;               // (job, query, row base, count)
   DeviceArray<uint4> pip_over_;          // (job, point, row base, hit count)
   DeviceArray<uint32_t> amb_count_;
-  DeviceArray<FrnPlan> frn_plan_;        // cap_items_ * 64
-  DeviceArray<FrnCand> frn_cand_;        // parked candidates of a batch
-  DeviceArray<unsigned long long> frn_cand_count_;
+  DeviceArray<NbrPlan> nbr_plan_;        // cap_items_ * 64
+  DeviceArray<NbrCand> nbr_cand_;        // parked candidates of a batch
+  DeviceArray<unsigned long long> nbr_cand_count_;
   DeviceArray<unsigned long long> stats_;
   SortWorkspace sort_ws_;
   bool stats_on_ = false;

@@ -379,3 +379,35 @@ def test_batch_fixed_radius_lattice_ties_and_duplicates(gpu_ready):
         rows = B.job_rows(job, res)
         _check_frn_order(rows)
         _check_nn(rows, want, False)
+
+
+def test_batch_knn_small_scene_and_duplicates(gpu_ready):
+    _say("case: batched KNN with k above the scene size and with duplicate points")
+    rng = np.random.default_rng(93)
+    jobs, wants = [], []
+    tiny = rng.random((5, 3))
+    tiny_d = {"id": np.arange(5, dtype=np.uint32), "x": tiny[:, 0], "y": tiny[:, 1], "z": tiny[:, 2]}
+    tiny_scene = _prepared_points(tiny)
+    base = rng.random((4000, 3))
+    dup = np.concatenate([base, np.repeat(base[:1], 250, axis=0), np.repeat(base[1:2], 40, axis=0)])
+    dup = dup[rng.permutation(len(dup))]
+    dup_d = {"id": np.arange(len(dup), dtype=np.uint32), "x": dup[:, 0], "y": dup[:, 1], "z": dup[:, 2]}
+    dup_scene = _prepared_points(dup)
+    for scene, sd, kind, k, radius in ((tiny_scene, tiny_d, B.JOB_KNN, 8, 0.0),
+                                       (tiny_scene, tiny_d, B.JOB_KNN, 64, 0.0),
+                                       (dup_scene, dup_d, B.JOB_KNN, 64, 0.0),
+                                       (dup_scene, dup_d, B.JOB_KNN, 3, 0.0),
+                                       (dup_scene, dup_d, B.JOB_BOUNDED_KNN, 64, 0.05)):
+        q = np.concatenate([rng.random((500, 3)), base[:2], base[:2] + 1e-3])
+        q = q.astype(np.float32).astype(np.float64)  # batched queries are FP32
+        qd = {"id": np.arange(len(q), dtype=np.uint32), "x": q[:, 0], "y": q[:, 1], "z": q[:, 2]}
+        jobs.append(B.BatchJob(scene, kind, _dev(q), radius=radius, k=k))
+        mode = "knn" if kind == B.JOB_KNN else "bounded_knn"
+        wants.append(oracle.neighbors(mode, qd, sd, radius=radius, k=k))
+    _say("  run_batch: 5 KNN jobs")
+    results, _ = B.run_batch(jobs)
+    for job, res, want in zip(jobs, results, wants):
+        assert res["status"] == 0
+        rows = B.job_rows(job, res)
+        _check_frn_order(rows)
+        _check_nn(rows, want, True)
