@@ -196,13 +196,15 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
 // ------------------------------------------------------------- neighbours --
 //
 // Fixed-radius, KNN and bounded-KNN jobs run without a top-k list, in three
-// launches.  nbr_count_kernel walks each query's ball near-first, counting the
-// neighbours inside it with a histogram of their distances over kNbrBuckets
-// equal slices of [0, radius].  Once k neighbours lie in buckets 0..b the
+// launches.  nbr_count_kernel walks each query's ball (near-first for the
+// k-nearest kinds), counting the neighbours inside it with a histogram of
+// their distances over kNbrBuckets equal slices of [0, radius].  Once k neighbours lie in buckets 0..b the
 // walk shrinks to bucket b's edge (farther points cannot be among the k
-// nearest), so it ends close to the k-th distance.  A KNN query has no
-// radius: a first near-first walk takes the largest distance of the first k
-// points it meets, which bounds the k-th distance from above.  A query with
+// nearest), so it ends close to the k-th distance; when bucket b still holds
+// more candidates than the parking takes, the walk counts again over
+// kNbrBuckets slices of bucket b's ball.  A KNN query has no radius: a first
+// near-first walk takes the largest distance of the first k points it meets,
+// which bounds the k-th distance from above.  A query with
 // c <= k neighbours reserves c rows, which nbr_emit_kernel writes in walk
 // order and nbr_sort_kernel sorts into the (distance, id) order
 // (reference.py:362-366, 382-390).  A query with more keeps its k nearest:
@@ -211,8 +213,8 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
 // kNbrCandMax) and the sort writes the first k of them.  Every decision uses
 // the FP64 distance in the reference's operation order and equal distances
 // share a bucket, so the rows are exact by construction.  A query whose
-// buckets 0..b hold more than kNbrCandMax neighbours (or that finds the
-// parking space full) is computed by batch_fix_kernel instead.
+// buckets 0..b still hold more than kNbrCandMax neighbours after four counts
+// (or that finds the parking space full) is computed by batch_fix_kernel.
 constexpr uint32_t kNbrBuckets = 32;
 constexpr uint32_t kNbrCandMax = 128;
 constexpr int kSortBlock = 64;
@@ -248,6 +250,109 @@ __device__ inline void load_query3(const DeviceJob& j, uint32_t qi, double& qx, 
   qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
 }
 
+// Result of the count walks of one query.
+struct NbrCount {
+  double radius;     // final ball
+  double radius_sq;  // membership: squared distance <= radius_sq
+  uint32_t count;    // neighbours counted in buckets 0..top
+  uint32_t bucket;   // more than k: the bucket holding the k-th distance
+  uint32_t n_cand;   // more than k: neighbours in buckets 0..bucket
+  bool heavy;        // more than k neighbours: keep the k nearest
+  bool refine;       // leave the query to the exact fix-up
+};
+
+template <bool STATS>
+__device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint32_t* my_hist,
+                                           LaneStats& st) {
+  auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
+  double qx, qy, qz;
+  load_query3(j, qi, qx, qy, qz);
+  const float qf[3] = {(float)qx, (float)qy, (float)qz};
+  const float m = coord_bound(scene_m(j.scene), qx, qy, qz);
+  const uint32_t k = j.k;
+  // fixed-radius balls hold few points more often than not: a plain preorder
+  // walk is cheapest; k-nearest searches want their near points first
+  const bool near_first = j.kind != kJobFixedRadius;
+  NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false};  // reference.py:357
+  if (!(r.radius < __builtin_inf())) {
+    // KNN: the farthest of the first k points met bounds the k-th distance
+    uint32_t seen = 0;
+    double far = 0.0;
+    float t2 = __builtin_inff();
+    walk_near_first<STATS>(j.scene, j.scene.root, j.scene.end, qf, t2, st,
+                           [&](uint32_t slot, const float4& p, float) {
+                             if (seen == k) return;
+                             far = fmax(far, point_d2(j.scene, slot, p, qx, qy, qz));
+                             if (++seen == k) t2 = -1.0f;  // the walk ends
+                           });
+    if (seen == k) {
+      r.radius_sq = far;
+      r.radius = sqrt(far);
+    }
+  }
+  const uint32_t step = k / 2 > 4u ? k / 2 : 4u;
+  for (uint32_t round = 0;; ++round) {
+    for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
+    const double scale = (double)kNbrBuckets / r.radius;
+    uint32_t count = 0, top = kNbrBuckets - 1, next_check = k;
+    bool shrunk = false;
+    float thr2 = frn_threshold2(r.radius, m);
+    auto visit = [&](uint32_t slot, const float4& p, float) {
+      const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
+      if (d2 > r.radius_sq) return;  // reference.py:361 inclusive radius
+      const uint32_t b = nbr_bucket(sqrt(d2), scale);
+      if (b > top) return;  // beyond the k nearest
+      ++count;
+      // no result needed: a plain LDS add
+      __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (count < next_check) return;
+      next_check = count + step;
+      uint32_t cum = 0;
+      for (uint32_t c = 0; c < top; ++c) {
+        cum += bucket_count(c);
+        if (cum >= k) {  // the k nearest lie in buckets 0..c: walk no farther
+          top = c;
+          shrunk = true;
+          thr2 = frn_threshold2(fmin(r.radius, nbr_bucket_edge(c, scale)), m);
+          break;
+        }
+      }
+    };
+    if (near_first)
+      walk_near_first<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit);
+    else
+      walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit);
+    r.count = count;
+    r.heavy = shrunk || count > k;
+    r.bucket = top;
+    r.n_cand = 0;
+    if (!r.heavy) return r;
+    if (count > kNbrCountMax) {
+      r.refine = true;
+      return r;
+    }
+    uint32_t cum = 0;
+    for (uint32_t c = 0; c <= top; ++c) {
+      cum += bucket_count(c);
+      if (cum >= k) {
+        r.bucket = c;
+        r.n_cand = cum;
+        break;
+      }
+    }
+    if (r.n_cand <= kNbrCandMax) return r;
+    if (round == 3) {
+      r.refine = true;
+      return r;
+    }
+    // too coarse: count again with kNbrBuckets slices of bucket b's ball,
+    // which holds the k nearest
+    r.radius = fmin(r.radius, nbr_bucket_edge(r.bucket, scale));
+    r.radius_sq = fmin(r.radius_sq, r.radius * r.radius);
+  }
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(256) void nbr_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
@@ -257,7 +362,6 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
   __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
   const uint32_t lane = threadIdx.x & 63;
   uint32_t* my_hist = hist + threadIdx.x;
-  auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -268,82 +372,21 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
     const DeviceJob& j = jobs[w.job];
     const bool active = lane < w.count;
     const uint32_t k = j.k;
-    for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
-    uint32_t qi = 0, count = 0, top = kNbrBuckets - 1;
-    bool shrunk = false;
-    double radius = j.radius, radius_sq = j.radius * j.radius;  // reference.py:357
+    uint32_t qi = 0;
+    NbrCount r{0.0, 0.0, 0u, 0u, 0u, false, false};
     if (active) {
       qi = job_query(j, w.first + lane);
-      double qx, qy, qz;
-      load_query3(j, qi, qx, qy, qz);
-      const float qf[3] = {(float)qx, (float)qy, (float)qz};
-      const float m = coord_bound(scene_m(j.scene), qx, qy, qz);
-      if (!(radius < __builtin_inf())) {
-        // KNN: the farthest of the first k points met bounds the k-th distance
-        uint32_t seen = 0;
-        double far = 0.0;
-        float t2 = __builtin_inff();
-        walk_near_first<STATS>(j.scene, j.scene.root, j.scene.end, qf, t2, st,
-                               [&](uint32_t slot, const float4& p, float) {
-                                 if (seen == k) return;
-                                 far = fmax(far, point_d2(j.scene, slot, p, qx, qy, qz));
-                                 if (++seen == k) t2 = -1.0f;  // the walk ends
-                               });
-        if (seen == k) {
-          radius_sq = far;
-          radius = sqrt(far);
-        }
-      }
-      const double scale = (double)kNbrBuckets / radius;
-      uint32_t next_check = k;
-      float thr2 = frn_threshold2(radius, m);
-      walk_near_first<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
-                             [&](uint32_t slot, const float4& p, float) {
-                               const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
-                               if (d2 > radius_sq) return;  // reference.py:361 inclusive radius
-                               const uint32_t b = nbr_bucket(sqrt(d2), scale);
-                               if (b > top) return;  // beyond the k nearest
-                               ++count;
-                               // no result needed: a plain LDS add
-                               __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)),
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                               if (count < next_check) return;
-                               next_check = 2 * count;
-                               uint32_t cum = 0;
-                               for (uint32_t c = 0; c < top; ++c) {
-                                 cum += bucket_count(c);
-                                 if (cum >= k) {  // the k nearest lie in buckets 0..c
-                                   top = c;
-                                   shrunk = true;
-                                   thr2 = frn_threshold2(fmin(radius, nbr_bucket_edge(c, scale)), m);
-                                   break;
-                                 }
-                               }
-                             });
+      r = nbr_count_query<STATS>(j, qi, my_hist, st);
     }
-    // more than k neighbours: keep the k nearest, from buckets 0..bucket
-    const bool heavy = shrunk || count > k;
-    uint32_t n_rows = heavy ? k : count;
-    uint32_t bucket = top, n_cand = 0;
-    if (heavy && count <= kNbrCountMax) {
-      uint32_t cum = 0;
-      for (uint32_t c = 0; c <= top; ++c) {
-        cum += bucket_count(c);
-        if (cum >= k) {
-          bucket = c;
-          n_cand = cum;
-          break;
-        }
-      }
-    }
-    bool refine = heavy && (count > kNbrCountMax || n_cand > kNbrCandMax);
+    uint32_t n_rows = r.heavy ? k : r.count;
+    bool refine = r.refine;
     const uint64_t row_at = wave_reserve(j.counters, n_rows);
     if (n_rows && row_at + n_rows > j.row_capacity) {
       atomicOr(&j.counters->status, kStatusRowOverflow);
       n_rows = 0;
       refine = false;
     }
-    const uint32_t want = (n_rows && heavy && !refine) ? n_cand : 0u;
+    const uint32_t want = (n_rows && r.heavy && !refine) ? r.n_cand : 0u;
     uint32_t total;
     const uint32_t prefix = wave_exclusive_sum(want, &total);
     const uint64_t cand_at = lane0_add64(cand_count, total) + prefix;
@@ -353,8 +396,8 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
       atomicAdd(&j.counters->ambiguous, 1u);
     }
     if (active)
-      plans[(size_t)it * 64 + lane] = NbrPlan{row_at,  cand_at,   radius, radius_sq, refine ? 0u : n_rows,
-                                              heavy ? n_cand : 0u, bucket, qi};
+      plans[(size_t)it * 64 + lane] = NbrPlan{row_at,   cand_at, r.radius, r.radius_sq, refine ? 0u : n_rows,
+                                              r.heavy ? r.n_cand : 0u, r.bucket, qi};
   }
   if (STATS) flush_lane_stats(stats, st);
 }
