@@ -138,10 +138,75 @@ __device__ inline void read_neighbor_row(const DeviceJob& j, uint64_t at, double
   }
 }
 
-// Computes the rows of the neighbour queries whose candidates do not fit the
-// parking space of the count / emit / sort pipeline, with an exact sorted
-// (FP64 distance, id) list per lane, and writes them at the rows the count
-// pass reserved.  The list lives in LDS (slot-major, so the 64 lanes of a
+// KNN and bounded-KNN jobs: each lane walks its query near-first with a
+// PackedTopK list in registers whose k-th key shrinks the walk; K is the list
+// size the job's k needs.  Queries whose FP32 keys tie at the boundary go to
+// batch_fix_kernel.
+template <int K, bool STATS>
+__global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __restrict__ jobs,
+                                                           const WorkItem* __restrict__ items,
+                                                           uint32_t n_items, uint32_t* item_counter,
+                                                           uint4* amb, uint32_t* amb_count,
+                                                           unsigned long long* stats) {
+  const uint32_t lane = threadIdx.x & 63;
+  LaneStats st;
+  ItemQueue queue(item_counter, n_items);
+  for (;;) {
+    const uint32_t it = queue.next();
+    if (it == kItemDone) break;
+    if (it == kItemRetry) continue;
+    const WorkItem w = items[it];
+    const DeviceJob& j = jobs[w.job];
+    const bool active = lane < w.count;
+    uint32_t qi = 0;
+    double qx = 0.0, qy = 0.0, qz = 0.0;
+    PackedTopK<K> list;
+    list.init(1);
+    bool ambiguous = false;
+    uint32_t c = 0;
+    const uint32_t k = j.k;
+    if (active) {
+      qi = job_query(j, w.first + lane);
+      const float* qp = j.queries + (size_t)qi * j.qstride;
+      qx = qp[0];
+      qy = qp[1];
+      qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
+      ambiguous = collect_packed<K, STATS>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius,
+                                           k, list, st);
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (i >= K - (int)k && list.e[i] != ~0ull) ++c;
+    }
+    const uint64_t at0 = wave_reserve(j.counters, c);
+    if (active && c) {
+      if (at0 + c > j.row_capacity) {
+        atomicOr(&j.counters->status, kStatusRowOverflow);
+      } else {
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          if (i >= K - (int)k && list.e[i] != ~0ull) {
+            const uint32_t slot = (uint32_t)list.e[i];
+            const float4 p = j.scene.prims[slot];
+            const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
+            write_neighbor_row(j, at0 + r, qi, __float_as_uint(p.w), dist, r + 1);
+            ++r;
+          }
+        }
+        if (ambiguous) {
+          amb[atomicAdd(amb_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
+          atomicAdd(&j.counters->ambiguous, 1u);
+        }
+      }
+    }
+  }
+  if (STATS) flush_lane_stats(stats, st);
+}
+
+// Computes the rows of the neighbour queries that the list kernels flag as
+// ambiguous, or whose candidates do not fit the parking space of the
+// fixed-radius pipeline, with an exact sorted (FP64 distance, id) list per
+// lane, and writes them at the rows reserved for them.  The list lives in LDS (slot-major, so the 64 lanes of a
 // wavefront hit consecutive words): a 64-entry FP64 list does not fit a
 // lane's registers, and these queries are rare.
 constexpr int kFixBlock = 64;
@@ -193,28 +258,25 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
   }
 }
 
-// ------------------------------------------------------------- neighbours --
+// ----------------------------------------------------------- fixed radius --
 //
-// Fixed-radius, KNN and bounded-KNN jobs run without a top-k list, in three
-// launches.  nbr_count_kernel walks each query's ball (near-first for the
-// k-nearest kinds), counting the neighbours inside it with a histogram of
-// their distances over kNbrBuckets equal slices of [0, radius].  Once k neighbours lie in buckets 0..b the
-// walk shrinks to bucket b's edge (farther points cannot be among the k
-// nearest), so it ends close to the k-th distance; when bucket b still holds
-// more candidates than the parking takes, the walk counts again over
-// kNbrBuckets slices of bucket b's ball.  A KNN query has no radius: a first
-// near-first walk takes the largest distance of the first k points it meets,
-// which bounds the k-th distance from above.  A query with
-// c <= k neighbours reserves c rows, which nbr_emit_kernel writes in walk
-// order and nbr_sort_kernel sorts into the (distance, id) order
-// (reference.py:362-366, 382-390).  A query with more keeps its k nearest:
-// the first bucket b whose cumulative count reaches k holds the k-th
-// distance, the emit walk parks only the neighbours of buckets 0..b (at most
-// kNbrCandMax) and the sort writes the first k of them.  Every decision uses
-// the FP64 distance in the reference's operation order and equal distances
-// share a bucket, so the rows are exact by construction.  A query whose
-// buckets 0..b still hold more than kNbrCandMax neighbours after four counts
-// (or that finds the parking space full) is computed by batch_fix_kernel.
+// Fixed-radius jobs run without a top-k list, in three launches.
+// nbr_count_kernel walks each query's ball, counting the neighbours inside it
+// with a histogram of their distances over kNbrBuckets equal slices of
+// [0, radius].  Once k neighbours lie in buckets 0..b the walk shrinks to
+// bucket b's edge (farther points cannot be among the k nearest); when bucket
+// b still holds more candidates than the parking takes, the walk counts again
+// over kNbrBuckets slices of bucket b's ball.  A query with c <= k neighbours
+// reserves c rows, which nbr_emit_kernel writes in walk order and
+// nbr_sort_kernel sorts into the (distance, id) order (reference.py:362-366).
+// A query with more keeps its k nearest: the first bucket b whose cumulative
+// count reaches k holds the k-th distance, the emit walk parks only the
+// neighbours of buckets 0..b (at most kNbrCandMax) and the sort writes the
+// first k of them.  Every decision uses the FP64 distance in the reference's
+// operation order and equal distances share a bucket, so the rows are exact
+// by construction.  A query whose buckets 0..b still hold more than
+// kNbrCandMax neighbours after four counts (or that finds the parking space
+// full) is computed by batch_fix_kernel.
 constexpr uint32_t kNbrBuckets = 32;
 constexpr uint32_t kNbrCandMax = 128;
 constexpr int kSortBlock = 64;
@@ -270,26 +332,7 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
   const float qf[3] = {(float)qx, (float)qy, (float)qz};
   const float m = coord_bound(scene_m(j.scene), qx, qy, qz);
   const uint32_t k = j.k;
-  // fixed-radius balls hold few points more often than not: a plain preorder
-  // walk is cheapest; k-nearest searches want their near points first
-  const bool near_first = j.kind != kJobFixedRadius;
   NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false};  // reference.py:357
-  if (!(r.radius < __builtin_inf())) {
-    // KNN: the farthest of the first k points met bounds the k-th distance
-    uint32_t seen = 0;
-    double far = 0.0;
-    float t2 = __builtin_inff();
-    walk_near_first<STATS>(j.scene, j.scene.root, j.scene.end, qf, t2, st,
-                           [&](uint32_t slot, const float4& p, float) {
-                             if (seen == k) return;
-                             far = fmax(far, point_d2(j.scene, slot, p, qx, qy, qz));
-                             if (++seen == k) t2 = -1.0f;  // the walk ends
-                           });
-    if (seen == k) {
-      r.radius_sq = far;
-      r.radius = sqrt(far);
-    }
-  }
   const uint32_t step = k / 2 > 4u ? k / 2 : 4u;
   for (uint32_t round = 0;; ++round) {
     for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
@@ -319,10 +362,7 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
         }
       }
     };
-    if (near_first)
-      walk_near_first<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit);
-    else
-      walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit);
+    walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit);
     r.count = count;
     r.heavy = shrunk || count > k;
     r.bucket = top;
@@ -720,32 +760,53 @@ __global__ void zero_words_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb
   if (i < nc) c[i] = 0ull;
 }
 
-// One persistent launch (or chain of launches) per family.
+// One persistent launch (or chain of launches) per family.  KNN jobs are
+// split by the list size their k needs (1, 2, 4, ..., 64), so a small-k job
+// never pays for the registers of a larger one in the same batch.
 enum Family : int {
-  kFamNbr = 0,  // fixed radius, KNN, bounded KNN: count, emit, sort
+  kFamPoints1 = 0,          // ... kFamPoints1 + 6 for K = 64
+  kFamNbr = kPointBuckets,  // fixed radius: count, emit, sort
   kFamRayAny,
   kFamRayCount,
   kFamRayClosest,
   kFamPip,
 };
 static_assert(kFamPip + 1 == kLaunchFamilies, "family count");
-// work queues: one per family, plus the neighbour emit and sort launches
+// work queues: one per family, plus the fixed-radius emit and sort launches
 constexpr int kItemCounterSets = kLaunchFamilies + 2;
 constexpr int kQueueNbrEmit = kLaunchFamilies;
 constexpr int kQueueNbrSort = kLaunchFamilies + 1;
-// parking space for neighbour candidates, per neighbour query of a batch
+// parking space for fixed-radius candidates, per fixed-radius query of a batch
 constexpr uint64_t kNbrCandPerQuery = 80;
 
-int family_of(uint32_t kind) {
+int family_of(uint32_t kind, uint32_t k) {
   switch (kind) {
-    case kJobFixedRadius:
+    case kJobFixedRadius: return kFamNbr;
     case kJobKnn:
-    case kJobBoundedKnn: return kFamNbr;
+    case kJobBoundedKnn: {
+      int b = 0;
+      while ((1u << b) < k) ++b;  // list_capacity(k) == 1 << b
+      return kFamPoints1 + b;
+    }
     case kJobRayAnyHit: return kFamRayAny;
     case kJobRayHitCount: return kFamRayCount;
     case kJobRayClosest: return kFamRayClosest;
     default: return kFamPip;
   }
+}
+
+// report group of a launch family: neighbours, any-hit, hit-count, closest, PIP
+int group_of(int family) { return family <= kFamNbr ? 0 : family - kFamNbr; }
+
+template <int K>
+void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
+                   uint32_t n_items, uint32_t* counter, uint4* amb, uint32_t* amb_count,
+                   unsigned long long* st, hipStream_t s) {
+  if (stats)
+    batch_points_kernel<K, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+  else
+    batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+  RTDL_LAUNCH_CHECK();
 }
 
 template <int MODE>
@@ -781,7 +842,7 @@ void BatchEngine::family_seconds(double out[5]) const {
     if (!timed_[f]) continue;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, fam_start_[f], fam_stop_[f]) == hipSuccess)
-      out[f] += ms * 1e-3;  // families are the report groups
+      out[group_of(f)] += ms * 1e-3;
     else
       (void)hipGetLastError();
   }
@@ -847,7 +908,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     require(h.nq == 0 || h.queries != nullptr, "job queries must not be null");
     require(h.nq == 0 || h.rows != nullptr, "job rows must not be null");
     if (h.kind <= kJobBoundedKnn) require(h.k >= 1 && h.k <= 64, "batched neighbour jobs need 1 <= k <= 64");
-    const int fam = family_of(h.kind);
+    const int fam = family_of(h.kind, h.k);
     DeviceJob& d = host_jobs_[i];
     std::memset(&d, 0, sizeof(d));
     d.scene = h.scene;
@@ -871,7 +932,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   host_starts_[n] = start;
   uint64_t nbr_queries = 0;
   for (const HostJob& h : jobs)
-    if (h.kind <= kJobBoundedKnn) nbr_queries += h.nq;
+    if (h.kind == kJobFixedRadius) nbr_queries += h.nq;
   if (nbr_queries) {
     const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
     if (nbr_cand_.size() < want) nbr_cand_.resize(want);
@@ -912,6 +973,9 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   };
   unsigned long long* st = stats_on_ ? stats_.get() : nullptr;
   const DeviceJob* dj = jobs_.get();
+  int last_nbr = -1;  // the exact fix-up runs once, after the last neighbour launch
+  for (int f = 0; f <= kFamNbr; ++f)
+    if (item_off[f + 1] > item_off[f]) last_nbr = f;
   for (int f = 0; f < kLaunchFamilies; ++f) timed_[f] = false;
   for (int f = 0; f < kLaunchFamilies; ++f) {
     const uint32_t n_items = (uint32_t)(item_off[f + 1] - item_off[f]);
@@ -923,7 +987,21 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
       RTDL_HIP(hipEventRecord(fam_start_[f], s));
       timed_[f] = true;
     }
-    if (f == kFamNbr) {
+    if (f < kPointBuckets) {
+      switch (f) {
+        case 0: launch_points<1>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 1: launch_points<2>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 2: launch_points<4>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 3: launch_points<8>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 4: launch_points<16>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        case 5: launch_points<32>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+        default: launch_points<64>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
+      }
+      if (f == last_nbr) {
+        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
+        RTDL_LAUNCH_CHECK();
+      }
+    } else if (f == kFamNbr) {
       if (stats_on_)
         nbr_count_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
                                                      nbr_cand_count_.get(), nbr_cand_.size(),
@@ -940,8 +1018,10 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
       nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrSort * kXcds,
                                                  nbr_plan_.get(), nbr_cand_.get());
       RTDL_LAUNCH_CHECK();
-      batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
-      RTDL_LAUNCH_CHECK();
+      if (f == last_nbr) {
+        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
+        RTDL_LAUNCH_CHECK();
+      }
     } else if (f == kFamRayAny) {
       launch_rays_batch<0>(stats_on_, grid, dj, it, n_items, ctr, st, s);
     } else if (f == kFamRayCount) {

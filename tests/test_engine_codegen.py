@@ -77,10 +77,10 @@ def test_persistent_loops_fetch_in_the_outermost_loop(tmp_path):
     funcs = _functions(out.read_text())
     checked = 0
     for name, body in funcs.items():
-        if not re.search(r"batch_(rays|pip)_kernel|nbr_(count|emit|sort)_kernel", name):
+        if not re.search(r"batch_(points|rays|pip)_kernel|nbr_(count|emit|sort)_kernel", name):
             continue
         depths = _fetch_broadcast_depths(body)
         assert depths, f"{name}: no readfirstlane broadcast of the fetched item"
         assert max(depths) <= 1, f"{name}: item broadcast inside a nested loop (depths {sorted(depths)})"
         checked += 1
-    assert checked >= 10
+    assert checked >= 20
