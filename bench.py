@@ -186,7 +186,7 @@ def requests_array(B, stream, offsets, scene_slot, capture_every: int = 0):
 # ------------------------------------------------------------- roofline --
 
 
-def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int):
+def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int, kinds=range(4)):
     """One batch per predicate family over the first requests of the stream.
     Kernel seconds from events on the launch stream (timed run), node and
     primitive visits from a separate counting run (untimed)."""
@@ -197,7 +197,7 @@ def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int):
     prim_bytes = {0: 12, 1: 12, 2: 36, 3: None}
     row_bytes = {0: 16, 1: 24, 2: 1, 3: 8}
     out = {}
-    for k in range(4):
+    for k in kinds:
         idx = np.flatnonzero(stream["kind"] == k)
         jobs, q_total = [], 0
         for i in idx:
@@ -439,6 +439,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--roofline-queries", type=int, default=2 << 20)
     ap.add_argument("--skip-baselines", action="store_true")
+    ap.add_argument("--roofline-only", choices=KINDS, default=None,
+                    help="only build the scenes and streams of a full run and measure this kind's "
+                         "roofline batch (the batch the bench line's roofline is quoted on), for "
+                         "profiling that batch alone")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -480,6 +484,15 @@ def main() -> int:
                    (args.max_batch_queries + 65536) * K_MAX * 24 + (64 << 20))
     log(f"admission budget {args.budget_gb:.0f} GiB over {args.slots} slots; "
         f"{per_slot * args.slots / (1 << 30):.1f} GiB of row buffers resident")
+
+    if args.roofline_only:
+        host, arenas, offsets, totals = make_arenas(timed, scenes, centres, SEED + 2)
+        roof = roofline_leg(B, scenes, timed, offsets, arenas, args.roofline_queries,
+                            kinds=[KINDS.index(args.roofline_only)])
+        print(json.dumps({"roofline_only": args.roofline_only, "steps": args.steps,
+                          "max_prims": args.max_prims, "roofline_queries": args.roofline_queries,
+                          **roof[args.roofline_only]}), flush=True)
+        return 0
 
     if len(warm["kind"]):
         _, wa, wo, _ = make_arenas(warm, scenes, centres, SEED + 1)
