@@ -242,19 +242,21 @@ def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int, kinds=r
 
 def measured_traffic(kind: str, measured: dict, args):
     """HBM bytes per launch of the roofline batch, from the committed
-    rocprofv3 --pmc measurement of this same workload (FETCH_SIZE + WRITE_SIZE,
-    each scaled by the counter's reading for a copy of known size), when that
-    measurement matches this run's batch; otherwise None."""
+    rocprofv3 --pmc measurements of this same workload (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the counter's reading for a copy of known size),
+    when one matches this run's batch; otherwise None."""
     path = os.path.join(ROOT, "profiles", "roofline_traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f)
+            recs = json.load(f)
     except (OSError, ValueError):
         return None
-    same = (rec.get("kind") == kind and rec.get("queries") == measured["queries"]
-            and rec.get("steps") == args.steps and rec.get("max_prims") == args.max_prims
-            and rec.get("roofline_queries") == args.roofline_queries)
-    return rec.get("bytes_per_launch") if same else None
+    for rec in recs if isinstance(recs, list) else [recs]:
+        if (rec.get("kind") == kind and rec.get("queries") == measured["queries"]
+                and rec.get("steps") == args.steps and rec.get("max_prims") == args.max_prims
+                and rec.get("roofline_queries") == args.roofline_queries):
+            return rec.get("bytes_per_launch")
+    return None
 
 
 # ------------------------------------------------------------ baselines --
