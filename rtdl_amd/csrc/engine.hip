@@ -148,6 +148,7 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
                                                            uint32_t n_items, uint32_t* item_counter,
                                                            uint4* amb, uint32_t* amb_count,
                                                            unsigned long long* stats) {
+  __shared__ uint32_t stack[kStackDepth * 256];  // walk_nearest's per-lane stacks
   const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
       qy = qp[1];
       qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
       ambiguous = collect_packed<K, STATS>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius,
-                                           k, list, st);
+                                           k, list, st, stack + threadIdx.x, 256);
 #pragma unroll
       for (int i = 0; i < K; ++i)
         if (i >= K - (int)k && list.e[i] != ~0ull) ++c;

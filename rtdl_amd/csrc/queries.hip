@@ -17,13 +17,15 @@ __global__ __launch_bounds__(256) void collect_packed_kernel(
     double radius, uint32_t k, double* __restrict__ list_d, uint32_t* __restrict__ list_id,
     uint32_t* __restrict__ count, uint32_t* __restrict__ amb_list, uint32_t* __restrict__ amb_count,
     unsigned long long* stats) {
+  __shared__ uint32_t stack[kStackDepth * 256];  // walk_nearest's per-lane stacks
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStats st;
   if (t < nq) {
     const uint32_t qi = order ? order[t] : t;
     const double qx = q[3 * (size_t)qi], qy = q[3 * (size_t)qi + 1], qz = q[3 * (size_t)qi + 2];
     PackedTopK<K> list;
-    const bool ambiguous = collect_packed<K, STATS>(s, s.root, s.end, qx, qy, qz, radius, k, list, st);
+    const bool ambiguous = collect_packed<K, STATS>(s, s.root, s.end, qx, qy, qz, radius, k, list, st,
+                                                    stack + threadIdx.x, blockDim.x);
     uint32_t c = 0;
 #pragma unroll
     for (int i = 0; i < K; ++i) {

@@ -152,6 +152,77 @@ __device__ inline void walk_neighbors(const SceneView& s, uint32_t root, uint32_
     walk_near_first<STATS>(s, root, end, qf, thr2, st, visit);
 }
 
+// Depth of the per-lane node stack of walk_nearest.
+constexpr uint32_t kStackDepth = 32;
+
+// Nearest-child-first walk for a k-nearest search: at an internal node both
+// children's boxes are tested, the walk descends into the nearer one and
+// pushes the farther one onto the lane's stack (LDS, entry d at
+// stack[d * stride]); a popped node is tested again against the threshold,
+// which has shrunk meanwhile.  Near primitives arrive first, so the list's
+// k-th key prunes most of the tree.  Returns false when the stack was full at
+// some push and a subtree was left out: the caller then walks the whole tree
+// again, meeting some primitives a second time.
+template <bool STATS, class Visit>
+__device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t end, const float qf[3],
+                                    float& thr2, LaneStats& st, Visit&& visit, uint32_t* stack,
+                                    uint32_t stride) {
+  if (root == kEndNode || root == end) return true;
+  bool complete = true;
+  uint32_t depth = 0;
+  BvhNode nd = s.nodes[root];
+  if (STATS) ++st.nodes;
+  if (!(box_dist2(qf, nd.lo, nd.hi) <= thr2)) return true;
+  // every node is pushed at most once; the guard only bounds the loop should
+  // the tree ever be malformed
+  for (uint32_t guard = s.n_nodes + 1; guard; --guard) {
+    bool leaf = true;
+    while (!node_is_leaf(nd.link)) {
+      const uint32_t l = nd.link;
+      const BvhNode ln = s.nodes[l];
+      const uint32_t r = ln.escape;  // the right child follows the left subtree
+      const BvhNode rn = s.nodes[r];
+      if (STATS) st.nodes += 2;
+      const float dl = box_dist2(qf, ln.lo, ln.hi), dr = box_dist2(qf, rn.lo, rn.hi);
+      const bool right_first = dr < dl;
+      const float dn = right_first ? dr : dl, df = right_first ? dl : dr;
+      if (df <= thr2) {
+        if (depth < kStackDepth)
+          stack[stride * depth++] = right_first ? l : r;
+        else
+          complete = false;
+      }
+      if (!(dn <= thr2)) {
+        leaf = false;
+        break;
+      }
+      nd = right_first ? rn : ln;
+    }
+    if (leaf) {
+      const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const uint32_t slot = first + j;
+        const float4 p = s.prims[slot];
+        if (STATS) ++st.prims;
+        const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
+        const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+        if (d2f <= thr2) visit(slot, p, d2f);
+      }
+    }
+    bool found = false;
+    while (depth) {
+      nd = s.nodes[stack[stride * --depth]];
+      if (STATS) ++st.nodes;
+      if (box_dist2(qf, nd.lo, nd.hi) <= thr2) {
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  return complete;
+}
+
 // Exact list: K (distance, id) entries in registers, k of them live in slots
 // [K - k, K); slots below hold a -1 sentinel, so the k-th smallest is always
 // slot K - 1 and every index is static.
@@ -249,10 +320,14 @@ __device__ inline uint64_t pack_key(double d2, uint32_t slot) {
 }
 __device__ inline uint32_t key_bits(uint64_t e) { return (uint32_t)(e >> 32); }
 
+// `stack` (LDS, kStackDepth entries at stride `stride`) selects the
+// nearest-child-first walk for a search without a radius; without one the
+// ring walk is used.
 template <int K, bool STATS>
 __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_t end, double qx,
                                       double qy, double qz, double radius, uint32_t k,
-                                      PackedTopK<K>& list, LaneStats& st) {
+                                      PackedTopK<K>& list, LaneStats& st, uint32_t* stack = nullptr,
+                                      uint32_t stride = 0) {
   const float qf[3] = {(float)qx, (float)qy, (float)qz};
   const float m = coord_bound(scene_m(s), qx, qy, qz);
   const double radius_sq = radius * radius;
@@ -276,7 +351,21 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
       drop_min = min(drop_min, key_bits(v));
     }
   };
-  walk_neighbors<STATS>(s, root, end, qf, radius, thr2, st, visit);
+  if (stack && !(radius < __builtin_inf())) {
+    if (!walk_nearest<STATS>(s, root, end, qf, thr2, st, visit, stack, stride)) {
+      // the stack overflowed: offer every primitive again, skipping the
+      // entries already in the list
+      walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float d2f) {
+        const uint64_t v = pack_key(point_d2(s, slot, p, qx, qy, qz), slot);
+        bool dup = false;
+#pragma unroll
+        for (int i = 0; i < K; ++i) dup = dup || list.e[i] == v;
+        if (!dup) visit(slot, p, d2f);
+      });
+    }
+  } else {
+    walk_neighbors<STATS>(s, root, end, qf, radius, thr2, st, visit);
+  }
   bool ambiguous = list.full() && drop_min == key_bits(list.e[K - 1]);
 #pragma unroll
   for (int i = 1; i < K; ++i) {
