@@ -528,6 +528,14 @@ def main() -> int:
                                   "p50_ms": float(np.percentile(lk, 50)), "p99_ms": float(np.percentile(lk, 99))}
 
     log(f"timed replay wall (host clock) {wall:.3f}s")
+    # capacity: the same requests, all arrived at time 0 (a saturated replay)
+    sat = reqs.copy()
+    sat["arrival_s"] = 0.0
+    _, sat_stats, _ = B.serve_replay(handles, sat, point_queries=arenas["point"], ray_queries=arenas["ray"],
+                                     pip_queries=arenas["pip"], **serve_kw)
+    capacity = served_q / sat_stats["wall_s"]
+    log(f"saturated replay (every request arrived at 0): {capacity/1e6:.2f} Mq/s, "
+        f"{sat_stats['batches']} batches")
     roof = roofline_leg(B, scenes, timed, offsets, arenas, args.roofline_queries)
     measured = {k: v for k, v in roof.items() if "kernel_s" in v}
     # the kernel family that takes the largest share of the stream's device time
@@ -579,6 +587,8 @@ def main() -> int:
         "served": {"wall_s": stats["wall_s"], "batches": stats["batches"], "rows": stats["rows"],
                    "rejected": int((~ok).sum()), "peak_inflight_bytes": stats["peak_inflight_bytes"],
                    "per_kind": per_kind},
+        "saturated_queries_per_s": capacity,
+        "offered_queries_per_s": int(timed["size"].sum()) / args.steps,
         "baselines": baselines,
         "speedup_vs_plain_per_call": (value / plain) if plain else None,
         "speedup_vs_per_request_resident": (value / baselines["per_request_resident"]["queries_per_s"])
