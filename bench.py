@@ -229,9 +229,14 @@ def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int, kinds=r
         else:
             pb = prim_bytes[k]
         w = q_total * probe_bytes[k] + 32 * nodes + pb * prims + row_bytes[k] * rows
+        # implementation-independent floor: every probe, every primitive of
+        # the scenes used and every row once
+        used = {int(stream["scene"][i]) for i in idx[:len(jobs)]}
+        n_prims = sum(scenes[i]["n"] for i in used)
+        w_min = q_total * probe_bytes[k] + pb * n_prims + row_bytes[k] * rows
         out[KINDS[k]] = {
             "queries": q_total, "jobs": len(jobs), "kernel_s": t, "node_visits": nodes,
-            "prim_tests": prims, "rows": rows, "algorithmic_bytes": w,
+            "prim_tests": prims, "rows": rows, "algorithmic_bytes": w, "w_min_bytes": w_min,
             "achieved_GBps": w / t / 1e9 if t > 0 else 0.0,
             "queries_per_s": q_total / t if t > 0 else 0.0,
         }
@@ -423,6 +428,32 @@ def cpu_baseline_leg(rtm, scenes, stream, offsets, host, budget_s: float):
                       f"group, included), C FP64 oracle, {spent:.1f} s"}
 
 
+def python_baseline_leg():
+    """The pure-Python FP64 restatement of the reference, single-threaded, on
+    the reference's own CPU-runnable case: 4,096 query and 4,096 search
+    points, 2-D, uniform in [0, 1)^2, radius 0.02, k_max = 4,096 (never
+    truncates)."""
+    from oracle import cpu_python_reference as ref
+    from rtdl_amd.records import Point
+
+    rng = np.random.default_rng(20261203)
+    q = rng.random((4096, 2))
+    sp = rng.random((4096, 2))
+    qs = [Point(i, float(x), float(y)) for i, (x, y) in enumerate(q)]
+    ss = [Point(i, float(x), float(y)) for i, (x, y) in enumerate(sp)]
+    t0 = time.perf_counter()
+    rows = ref.fixed_radius_neighbors_cpu(qs, ss, radius=0.02, k_max=4096)
+    dt = time.perf_counter() - t0
+    return {"value": len(qs) / dt, "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"fixed-radius 4096 x 4096 2-D points, r = 0.02: {len(rows)} rows in {dt:.1f} s"}
+
+
+def embree_status() -> str:
+    import ctypes.util
+
+    return "available" if ctypes.util.find_library("embree4") else "unavailable (no libembree4 on this host)"
+
+
 # ----------------------------------------------------------------- main --
 
 
@@ -554,6 +585,9 @@ def main() -> int:
         log(f"per-request resident: {baselines['per_request_resident']}")
         cpu = cpu_baseline_leg(rtm, scenes, timed, offsets, host, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
+        baselines["cpu_python_reference"] = python_baseline_leg()
+        log(f"python reference: {baselines['cpu_python_reference']}")
+        baselines["embree"] = embree_status()
 
     plain = baselines.get("plain_per_call", {}).get("queries_per_s")
     line = {

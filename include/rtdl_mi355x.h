@@ -309,6 +309,23 @@ int rtdl_mi355x_run_batch_profiled(const RtdlBatchJob* jobs, size_t job_count,
                                    double* family_seconds, uint64_t* visits, char* error_out,
                                    size_t error_size);
 
+/* A batch recorded once into an executable hipGraph and replayed with one
+ * call (no counterpart in the reference; for callers that run the same
+ * batch shape repeatedly, e.g. new query contents in the same device buffers
+ * every frame).  Create validates the jobs like rtdl_mi355x_run_batch,
+ * sizes every buffer and captures the whole batch (job descriptors,
+ * ordering sort, family launches, counter copy) on a stream of its own;
+ * launch replays it, blocks until it is done and fills results (job_count
+ * entries) and seconds_out (optional, device time).  The jobs' scenes,
+ * query and row buffers are read at every launch and must outlive the
+ * graph; their contents may change between launches.  Launches of one graph
+ * must not overlap. */
+int rtdl_mi355x_batch_graph_create(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                                   void** graph_out, char* error_out, size_t error_size);
+int rtdl_mi355x_batch_graph_launch(void* graph, RtdlBatchJobResult* results, double* seconds_out,
+                                   char* error_out, size_t error_size);
+int rtdl_mi355x_batch_graph_destroy(void* graph, char* error_out, size_t error_size);
+
 /* Multi-instance scenes for batched jobs: instance i owns the primitives
  * [instance_offsets[i], instance_offsets[i + 1]) of the device input and
  * gets its own LBVH subtree; neighbour / polygon ids are instance-local

@@ -244,6 +244,61 @@ def run_batch(jobs: list[BatchJob], *, morton_sort: bool = True):
     return out, seconds.value
 
 
+class BatchGraph:
+    """A batch recorded once into a hipGraph and replayed by `launch()`.
+
+    The jobs' scenes and device buffers are read at every launch and must
+    stay alive while the graph does (the graph keeps references to the
+    jobs); their contents may change between launches.  Rows, status and
+    counts of every launch are those `run_batch` would give for the same
+    jobs and contents."""
+
+    def __init__(self, jobs: list[BatchJob], *, morton_sort: bool = True):
+        self._jobs = list(jobs)
+        self._arr, self._res = _pack_jobs(self._jobs)
+        handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(4096)
+        _sync_foreign()
+        st = _lib().rtdl_mi355x_batch_graph_create(ctypes.addressof(self._arr), len(self._jobs),
+                                                   int(bool(morton_sort)), ctypes.byref(handle), err, 4096)
+        _check(st, err, "rtdl_mi355x_batch_graph_create")
+        self._handle = handle
+
+    def launch(self):
+        """Replays the batch; returns (per-job result dicts, device seconds)."""
+        if self._handle is None:
+            raise RuntimeError("BatchGraph is closed")
+        seconds = ctypes.c_double()
+        err = ctypes.create_string_buffer(4096)
+        _sync_foreign()
+        st = _lib().rtdl_mi355x_batch_graph_launch(self._handle, ctypes.addressof(self._res),
+                                                   ctypes.byref(seconds), err, 4096)
+        _check(st, err, "rtdl_mi355x_batch_graph_launch")
+        res = self._res
+        out = [{"rows": int(res[i].rows), "status": int(res[i].status), "ambiguous": int(res[i].ambiguous)}
+               for i in range(len(self._jobs))]
+        return out, seconds.value
+
+    def close(self):
+        if self._handle is not None:
+            err = ctypes.create_string_buffer(4096)
+            st = _lib().rtdl_mi355x_batch_graph_destroy(self._handle, err, 4096)
+            self._handle = None
+            _check(st, err, "rtdl_mi355x_batch_graph_destroy")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception as exc:  # noqa: BLE001 - interpreter shutdown or a failed destroy
+            print(f"BatchGraph: {exc}")
+
+
 FAMILIES = ("neighbors", "ray_any_hit", "ray_hit_count", "ray_closest_hit", "pip")
 
 

@@ -430,42 +430,112 @@ int rtdl_mi355x_run_batch_profiled(const RtdlBatchJob* jobs, size_t job_count,
   });
 }
 
+// Validated engine jobs of a batch (the handles and pointers are the caller's).
+static std::vector<HostJob> host_jobs_from(const RtdlBatchJob* jobs, size_t job_count) {
+  require(job_count == 0 || jobs != nullptr, "jobs must not be null when job_count is nonzero");
+  std::vector<HostJob> host(job_count);
+  for (size_t i = 0; i < job_count; ++i) {
+    const RtdlBatchJob& b = jobs[i];
+    Prepared* p = prepared_from(b.scene);
+    require(b.instance < p->scene.n_instances || (p->scene.n_instances == 0 && b.instance == 0),
+            "job instance out of range");
+    const bool point_kind = b.kind <= kJobBoundedKnn;
+    const bool ray_kind = b.kind >= kJobRayAnyHit && b.kind <= kJobRayClosest;
+    require(b.kind <= kJobPipPositive, "unknown job kind");
+    require(!point_kind || p->type == SceneType::Points2D || p->type == SceneType::Points3D,
+            "neighbour jobs need a point scene");
+    require(!ray_kind || p->type == SceneType::Triangles3D, "ray jobs need a triangle scene");
+    require(b.kind != kJobPipPositive || p->type == SceneType::Polygons, "PIP jobs need a polygon scene");
+    require(!ray_kind || b.query_stride == 7, "ray jobs need query_stride 7");
+    require(b.kind != kJobPipPositive || b.query_stride == 2, "PIP jobs need query_stride 2");
+    require(!point_kind || b.query_stride == 2 || b.query_stride == 3,
+            "neighbour jobs need query_stride 2 or 3");
+    HostJob& h = host[i];
+    h.scene = p->scene.view(b.instance);
+    h.kind = b.kind;
+    h.nq = b.query_count;
+    h.queries = b.queries;
+    h.qstride = b.query_stride;
+    h.radius = b.radius;
+    h.k = b.k;
+    h.rows = b.rows;
+    h.row_capacity = b.row_capacity;
+    require(h.row_capacity >= job_row_capacity(b.kind, b.query_count, b.k, 0) || b.kind == kJobPipPositive,
+            "row_capacity is below the rows the job can emit");
+  }
+  return host;
+}
+
+namespace {
+struct BatchGraph {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<BatchEngine> engine;
+  hipGraphExec_t exec = nullptr;
+  size_t job_count = 0;
+  ~BatchGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    engine.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+}  // namespace
+
+int rtdl_mi355x_batch_graph_create(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                                   void** graph_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(graph_out != nullptr, "graph_out must not be null");
+    *graph_out = nullptr;
+    const std::vector<HostJob> host = host_jobs_from(jobs, job_count);
+    auto g = std::make_unique<BatchGraph>();
+    RTDL_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    g->engine = std::make_unique<BatchEngine>(g->stream);
+    g->exec = g->engine->capture(host, morton_sort != 0);
+    g->job_count = job_count;
+    *graph_out = g.release();
+  });
+}
+
+int rtdl_mi355x_batch_graph_launch(void* graph, RtdlBatchJobResult* results, double* seconds_out,
+                                   char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(graph != nullptr, "graph must not be null");
+    BatchGraph* g = static_cast<BatchGraph*>(graph);
+    require(g->job_count == 0 || results != nullptr, "results must not be null");
+    hipEvent_t start, stop;
+    RTDL_HIP(hipEventCreate(&start));
+    RTDL_HIP(hipEventCreate(&stop));
+    RTDL_HIP(hipEventRecord(start, g->stream));
+    RTDL_HIP(hipGraphLaunch(g->exec, g->stream));
+    RTDL_HIP(hipEventRecord(stop, g->stream));
+    RTDL_HIP(hipStreamSynchronize(g->stream));
+    float ms = 0.0f;
+    RTDL_HIP(hipEventElapsedTime(&ms, start, stop));
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    if (seconds_out) *seconds_out = ms * 1e-3;
+    const JobCounters* c = g->engine->counters();
+    for (size_t i = 0; i < g->job_count; ++i) {
+      results[i].rows = c[i].rows;
+      results[i].status = c[i].status;
+      results[i].ambiguous = c[i].ambiguous;
+    }
+  });
+}
+
+int rtdl_mi355x_batch_graph_destroy(void* graph, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    BatchGraph* g = static_cast<BatchGraph*>(graph);
+    if (g) RTDL_HIP(hipStreamSynchronize(g->stream));
+    delete g;
+  });
+}
+
 static void run_batch_impl(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
                            RtdlBatchJobResult* results, double* seconds_out,
                            double* family_seconds, unsigned long long* visits) {
   {
-    require(job_count == 0 || (jobs != nullptr && results != nullptr),
-            "jobs and results must not be null when job_count is nonzero");
-    std::vector<HostJob> host(job_count);
-    for (size_t i = 0; i < job_count; ++i) {
-      const RtdlBatchJob& b = jobs[i];
-      Prepared* p = prepared_from(b.scene);
-      require(b.instance < p->scene.n_instances || (p->scene.n_instances == 0 && b.instance == 0),
-              "job instance out of range");
-      const bool point_kind = b.kind <= kJobBoundedKnn;
-      const bool ray_kind = b.kind >= kJobRayAnyHit && b.kind <= kJobRayClosest;
-      require(b.kind <= kJobPipPositive, "unknown job kind");
-      require(!point_kind || p->type == SceneType::Points2D || p->type == SceneType::Points3D,
-              "neighbour jobs need a point scene");
-      require(!ray_kind || p->type == SceneType::Triangles3D, "ray jobs need a triangle scene");
-      require(b.kind != kJobPipPositive || p->type == SceneType::Polygons, "PIP jobs need a polygon scene");
-      require(!ray_kind || b.query_stride == 7, "ray jobs need query_stride 7");
-      require(b.kind != kJobPipPositive || b.query_stride == 2, "PIP jobs need query_stride 2");
-      require(!point_kind || b.query_stride == 2 || b.query_stride == 3,
-              "neighbour jobs need query_stride 2 or 3");
-      HostJob& h = host[i];
-      h.scene = p->scene.view(b.instance);
-      h.kind = b.kind;
-      h.nq = b.query_count;
-      h.queries = b.queries;
-      h.qstride = b.query_stride;
-      h.radius = b.radius;
-      h.k = b.k;
-      h.rows = b.rows;
-      h.row_capacity = b.row_capacity;
-      require(h.row_capacity >= job_row_capacity(b.kind, b.query_count, b.k, 0) || b.kind == kJobPipPositive,
-              "row_capacity is below the rows the job can emit");
-    }
+    require(job_count == 0 || results != nullptr, "results must not be null when job_count is nonzero");
+    const std::vector<HostJob> host = host_jobs_from(jobs, job_count);
     static thread_local std::unique_ptr<BatchEngine> engine;
     hipStream_t s = thread_stream();
     if (!engine) engine = std::make_unique<BatchEngine>(s);

@@ -1047,7 +1047,44 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   if (n)
     RTDL_HIP(hipMemcpyAsync(host_counters_, counters_.get(), n * sizeof(JobCounters),
                             hipMemcpyDeviceToHost, s));
-  RTDL_HIP(hipEventRecord(done_, s));
+  if (!capturing_) RTDL_HIP(hipEventRecord(done_, s));
+}
+
+hipGraphExec_t BatchEngine::capture(const std::vector<HostJob>& jobs, bool morton_sort) {
+  // every allocation enqueue would make happens here, outside the capture
+  uint64_t total = 0, nbr_queries = 0;
+  for (const HostJob& h : jobs) {
+    total += h.nq;
+    if (h.kind == kJobFixedRadius) nbr_queries += h.nq;
+  }
+  require(total < 0xFFFFFFFFull, "batch exceeds 2^32 queries");
+  reserve(jobs.size(), total);
+  if (nbr_queries) {
+    const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
+    if (nbr_cand_.size() < want) nbr_cand_.resize(want);
+  }
+  enable_timing(false);
+  enable_stats(false);
+  RTDL_HIP(hipStreamSynchronize(stream_));
+  RTDL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  capturing_ = true;
+  try {
+    enqueue(jobs, morton_sort);
+  } catch (...) {
+    capturing_ = false;
+    hipGraph_t broken = nullptr;
+    (void)hipStreamEndCapture(stream_, &broken);
+    if (broken) (void)hipGraphDestroy(broken);
+    throw;
+  }
+  capturing_ = false;
+  hipGraph_t graph = nullptr;
+  RTDL_HIP(hipStreamEndCapture(stream_, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  RTDL_HIP(e);
+  return exec;
 }
 
 }  // namespace rtdl

@@ -136,6 +136,13 @@ class BatchEngine {
   void reserve(size_t max_jobs, size_t max_queries);
   // Enqueues the whole batch; returns immediately.
   void enqueue(const std::vector<HostJob>& jobs, bool morton_sort);
+  // Records the batch into an executable hipGraph instead of running it:
+  // buffers are sized first, then the stream is captured while the batch is
+  // enqueued, so every launch of the graph replays the whole batch (job
+  // descriptors, ordering sort, family launches, counter copy) with one
+  // call.  Timing and visit counting are off in a captured batch.  The
+  // engine then belongs to the graph: it must not enqueue other batches.
+  hipGraphExec_t capture(const std::vector<HostJob>& jobs, bool morton_sort);
   // Valid once the batch has completed.
   const JobCounters* counters() const { return host_counters_; }
   hipStream_t stream() const { return stream_; }
@@ -174,6 +181,7 @@ class BatchEngine {
   SortWorkspace sort_ws_;
   bool stats_on_ = false;
   bool timing_on_ = false;
+  bool capturing_ = false;
   bool timed_[kLaunchFamilies] = {};
   hipEvent_t fam_start_[kLaunchFamilies] = {}, fam_stop_[kLaunchFamilies] = {};
   int cus_ = 256;

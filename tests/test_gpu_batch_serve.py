@@ -411,3 +411,69 @@ def test_batch_knn_small_scene_and_duplicates(gpu_ready):
         rows = B.job_rows(job, res)
         _check_frn_order(rows)
         _check_nn(rows, want, True)
+
+
+def _rows_canon(job, res):
+    rows = B.job_rows(job, res)
+    names = rows.dtype.names
+    keys = [n for n in ("query_id", "ray_id", "point_id", "neighbor_id", "polygon_id", "triangle_id")
+            if n in names]
+    return _canon(rows, keys)
+
+
+def test_batch_graph_replays_match_eager_runs(gpu_ready):
+    _say("case: batch captured into a hipGraph, replayed between eager batches")
+    rng = np.random.default_rng(94)
+    centres = rng.random((64, 3))
+    _, s3 = _points3(rng, 40000, "mixture", centres)
+    scene = _prepared_points(s3)
+    tris = cases.triangles(rng, 20000, sigma=0.01)
+    tri_scene = B.prepare_triangle_instances(_dev(cases.tri_matrix(tris)), [0, 20000])
+    polys = cases.star_polygons(rng, 500, rmin=0.01, rmax=0.04)
+    refs = polys["refs"]
+    poly_scene = B.prepare_polygon_instances(
+        _dev_i32(refs["vertex_offset"]), _dev_i32(refs["vertex_count"]),
+        _dev(polys["vertices_xy"].reshape(-1, 2)), [0, len(refs)])
+
+    def queries():
+        _, q3 = _points3(rng, 3000, "mixture", centres)
+        o = rng.random((2000, 3))
+        d = rng.standard_normal((2000, 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        rays = np.concatenate([o, d, np.full((2000, 1), 0.3)], 1).astype(np.float32)
+        pts = rng.random((2500, 2)).astype(np.float32)
+        return q3.astype(np.float32), rays, pts
+
+    q3, rays, pts = queries()
+    qbuf, rbuf, pbuf = _dev(q3), _dev(rays), _dev(pts)
+    jobs = [B.BatchJob(scene, B.JOB_FIXED_RADIUS, qbuf, radius=0.02, k=32),
+            B.BatchJob(scene, B.JOB_KNN, qbuf, k=8),
+            B.BatchJob(scene, B.JOB_BOUNDED_KNN, qbuf, radius=0.01, k=4),
+            B.BatchJob(tri_scene, B.JOB_RAY_ANYHIT, rbuf),
+            B.BatchJob(tri_scene, B.JOB_RAY_CLOSEST, rbuf),
+            B.BatchJob(poly_scene, B.JOB_PIP_POSITIVE, pbuf)]
+    other = [B.BatchJob(scene, B.JOB_KNN, _dev(q3[::-1].copy()), k=5)]
+
+    def eager():
+        res, _ = B.run_batch(jobs)
+        return res, [_rows_canon(j, r) for j, r in zip(jobs, res)]
+
+    want_res, want_rows = eager()
+    with B.BatchGraph(jobs) as graph:
+        for rep in range(4):
+            if rep == 2:
+                # new query contents in the same buffers: the replay reads them
+                q3, rays, pts = queries()
+                qbuf.upload(q3)
+                rbuf.upload(rays)
+                pbuf.upload(pts)
+                want_res, want_rows = eager()
+            if rep == 1:
+                B.run_batch(other)  # eager work between replays
+            _say(f"  replay {rep}")
+            res, _ = graph.launch()
+            for j, (job, r, w, wr) in enumerate(zip(jobs, res, want_res, want_rows)):
+                assert r == w, (rep, j, r, w)
+                got = _rows_canon(job, r)
+                for name in got.dtype.names:
+                    assert np.array_equal(got[name], wr[name]), (rep, j, name)
