@@ -309,6 +309,20 @@ int rtdl_mi355x_run_batch_profiled(const RtdlBatchJob* jobs, size_t job_count,
                                    double* family_seconds, uint64_t* visits, char* error_out,
                                    size_t error_size);
 
+/* Process-wide build and launch choices (no counterpart in the reference;
+ * for the autotuner in rtdl_amd/autotune.py and callers that pin a tuned
+ * configuration).  Every setting gives the same rows; scenes built and
+ * graphs captured earlier keep the values they were made with. */
+typedef struct RtdlTuning {
+  uint32_t point_leaf;     /* primitives per LBVH leaf in point scenes, 1..8 (default 4) */
+  uint32_t triangle_leaf;  /* triangle scenes, 1..8 (default 2) */
+  uint32_t polygon_leaf;   /* polygon scenes, 1..8 (default 1) */
+  uint32_t blocks_per_cu;  /* persistent batch launches: 256-thread workgroups per CU, 1..64 (default 8) */
+} RtdlTuning;
+
+int rtdl_mi355x_get_tuning(RtdlTuning* out, char* error_out, size_t error_size);
+int rtdl_mi355x_set_tuning(const RtdlTuning* tuning, char* error_out, size_t error_size);
+
 /* A batch recorded once into an executable hipGraph and replayed with one
  * call (no counterpart in the reference; for callers that run the same
  * batch shape repeatedly, e.g. new query contents in the same device buffers

@@ -466,6 +466,26 @@ static std::vector<HostJob> host_jobs_from(const RtdlBatchJob* jobs, size_t job_
   return host;
 }
 
+int rtdl_mi355x_get_tuning(RtdlTuning* out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(out != nullptr, "out must not be null");
+    const Tuning t = current_tuning();
+    *out = RtdlTuning{t.point_leaf, t.triangle_leaf, t.polygon_leaf, t.blocks_per_cu};
+  });
+}
+
+int rtdl_mi355x_set_tuning(const RtdlTuning* tuning, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(tuning != nullptr, "tuning must not be null");
+    Tuning t;
+    t.point_leaf = tuning->point_leaf;
+    t.triangle_leaf = tuning->triangle_leaf;
+    t.polygon_leaf = tuning->polygon_leaf;
+    t.blocks_per_cu = tuning->blocks_per_cu;
+    set_tuning(t);
+  });
+}
+
 namespace {
 struct BatchGraph {
   hipStream_t stream = nullptr;

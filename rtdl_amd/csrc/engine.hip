@@ -968,7 +968,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   } else if (morton_sort && total == 1) {
     fill_u32(order_.get(), 1, 0u, s);
   }
-  const unsigned max_grid = (unsigned)cus_ * 8;
+  const unsigned max_grid = (unsigned)cus_ * current_tuning().blocks_per_cu;
   auto grid_for = [&](size_t items) {
     return (unsigned)std::max<size_t>(1, std::min<size_t>((items + 3) / 4, max_grid));
   };

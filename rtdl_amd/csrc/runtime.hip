@@ -150,10 +150,26 @@ Prepared* prepared_from(void* handle) {
 }
 
 namespace {
+std::mutex g_tuning_mutex;
+Tuning g_tuning;
+}  // namespace
 
-constexpr uint32_t kPointLeaf = 4;
-constexpr uint32_t kTriangleLeaf = 2;
-constexpr uint32_t kPolygonLeaf = 1;
+Tuning current_tuning() {
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  return g_tuning;
+}
+
+void set_tuning(const Tuning& t) {
+  require(t.point_leaf >= 1 && t.point_leaf <= kMaxLeafSize, "point_leaf must be in [1, 8]");
+  require(t.triangle_leaf >= 1 && t.triangle_leaf <= kMaxLeafSize, "triangle_leaf must be in [1, 8]");
+  require(t.polygon_leaf >= 1 && t.polygon_leaf <= kMaxLeafSize, "polygon_leaf must be in [1, 8]");
+  require(t.blocks_per_cu >= 1 && t.blocks_per_cu <= 64, "blocks_per_cu must be in [1, 64]");
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  g_tuning = t;
+}
+
+namespace {
+
 constexpr uint32_t kSortQueriesAbove = 2048;
 
 template <class T>
@@ -243,7 +259,7 @@ void prepare_points(Prepared& p, const void* raw, size_t n, int dim) {
   DeviceArray<uint32_t> ids(n);
   unpack_points(bytes.get(), n, dim, coords.get(), ids.get(), s);
   build_scene(p.scene, PrimKind::Points, dim, n, coords.get(), ids.get(), nullptr, nullptr, 0,
-              {0, (uint64_t)n}, kPointLeaf, s);
+              {0, (uint64_t)n}, current_tuning().point_leaf, s);
   read_root_box(p);
 }
 
@@ -257,7 +273,7 @@ void prepare_triangles(Prepared& p, const void* raw, size_t n) {
   DeviceArray<uint32_t> ids(n);
   unpack_triangles(bytes.get(), n, coords.get(), ids.get(), s);
   build_scene(p.scene, PrimKind::Triangles, 3, n, coords.get(), ids.get(), nullptr, nullptr, 0,
-              {0, (uint64_t)n}, kTriangleLeaf, s);
+              {0, (uint64_t)n}, current_tuning().triangle_leaf, s);
   read_root_box(p);
 }
 
@@ -287,7 +303,7 @@ void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* ver
   unpack_polygons(bytes.get(), n, p.refs_input.get(), s);
   auto verts = upload(verts_xy, n_doubles, s);
   build_scene(p.scene, PrimKind::Polygons, 2, n, nullptr, nullptr, p.refs_input.get(), verts.get(),
-              n_verts, {0, (uint64_t)n}, kPolygonLeaf, s);
+              n_verts, {0, (uint64_t)n}, current_tuning().polygon_leaf, s);
   read_root_box(p);
 }
 
@@ -303,7 +319,7 @@ void prepare_point_instances(Prepared& p, const float* coords, size_t n,
   DeviceArray<uint32_t> ids(n);
   widen_points(coords, n, d_offs.get(), (uint32_t)(offsets.size() - 1), c64.get(), ids.get(), s);
   build_scene(p.scene, PrimKind::Points, 3, n, c64.get(), ids.get(), nullptr, nullptr, 0, offsets,
-              kPointLeaf, s);
+              current_tuning().point_leaf, s);
   read_root_box(p);
 }
 
@@ -320,7 +336,7 @@ void prepare_triangle_instances(Prepared& p, const float* coords, size_t n,
   widen_f32(coords, 9 * n, c64.get(), s);
   instance_ids(n, d_offs.get(), (uint32_t)(offsets.size() - 1), ids.get(), s);
   build_scene(p.scene, PrimKind::Triangles, 3, n, c64.get(), ids.get(), nullptr, nullptr, 0, offsets,
-              kTriangleLeaf, s);
+              current_tuning().triangle_leaf, s);
   read_root_box(p);
 }
 
@@ -352,7 +368,7 @@ void prepare_polygon_instances(Prepared& p, const uint32_t* voff, const uint32_t
   DeviceArray<double> v64(2 * n_verts);
   widen_f32(verts_xy, 2 * n_verts, v64.get(), s);
   build_scene(p.scene, PrimKind::Polygons, 2, n, nullptr, nullptr, p.refs_input.get(), v64.get(),
-              n_verts, offsets, kPolygonLeaf, s);
+              n_verts, offsets, current_tuning().polygon_leaf, s);
   read_root_box(p);
 }
 

@@ -477,3 +477,45 @@ def test_batch_graph_replays_match_eager_runs(gpu_ready):
                 got = _rows_canon(job, r)
                 for name in got.dtype.names:
                     assert np.array_equal(got[name], wr[name]), (rep, j, name)
+
+
+def test_autotune_picks_the_fastest_and_every_candidate_gives_the_same_rows(gpu_ready):
+    from rtdl_amd import autotune as T
+
+    _say("case: autotuner over leaf sizes and launch sizes, rows checked per candidate")
+    rng = np.random.default_rng(95)
+    centres = rng.random((64, 3))
+    _, s3 = _points3(rng, 30000, "mixture", centres)
+    tris = cases.triangles(rng, 15000, sigma=0.01)
+    tri_m = cases.tri_matrix(tris)
+    _, q3 = _points3(rng, 4000, "mixture", centres)
+    o = rng.random((3000, 3))
+    d = rng.standard_normal((3000, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = np.concatenate([o, d, np.full((3000, 1), 0.3)], 1).astype(np.float32)
+    qbuf, rbuf = _dev(q3), _dev(rays)
+    pts_buf, tri_buf = _dev(s3), _dev(tri_m)
+
+    def build():
+        return (B.prepare_point_instances(pts_buf, [0, len(s3)]),
+                B.prepare_triangle_instances(tri_buf, [0, len(tri_m)]))
+
+    def jobs(scenes):
+        pts, tri = scenes
+        return [B.BatchJob(pts, B.JOB_FIXED_RADIUS, qbuf, radius=0.02, k=64),
+                B.BatchJob(pts, B.JOB_KNN, qbuf, k=8),
+                B.BatchJob(tri, B.JOB_RAY_CLOSEST, rbuf)]
+
+    original = T.get_tuning()
+    try:
+        out = T.autotune_batch(build, jobs, repeats=2,
+                               knobs={"blocks_per_cu": (4, 16), "point_leaf": (2, 8), "triangle_leaf": (1, 4)})
+        table = out["table"]
+        assert len(table) == 7
+        fastest = min(table, key=lambda r: r["seconds"])
+        assert out["best"] == T.Tuning(**{k: fastest[k] for k in ("point_leaf", "triangle_leaf",
+                                                                  "polygon_leaf", "blocks_per_cu")})
+        assert T.get_tuning() == out["best"]
+    finally:
+        T.set_tuning(original)
+    assert T.get_tuning() == original
