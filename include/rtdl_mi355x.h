@@ -309,6 +309,26 @@ int rtdl_mi355x_run_batch_profiled(const RtdlBatchJob* jobs, size_t job_count,
                                    double* family_seconds, uint64_t* visits, char* error_out,
                                    size_t error_size);
 
+/* Host copy of a prepared point or triangle scene (no counterpart in the
+ * reference; for checking the traversal visit counters against a replay of
+ * the walk on the host).  Call once with null buffers to get the sizes, then
+ * with buffers of node_count 32-byte nodes, prim_count * prim_float4s
+ * float4 records, prim_count * exact_doubles doubles (when exact_doubles is
+ * nonzero) and instance_count roots and ends. */
+typedef struct RtdlSceneExport {
+  uint64_t node_count;     /* BvhNode {float lo[3]; uint32 link; float hi[3]; uint32 escape} */
+  uint64_t prim_count;
+  uint64_t instance_count;
+  uint32_t prim_float4s;   /* float4 records per primitive: 1 (point), 3 (triangle) */
+  uint32_t exact_doubles;  /* FP64 coordinates per primitive kept for refine, 0 if FP32 is exact */
+  float m;                 /* bound on every |coordinate| of the scene */
+  uint32_t pad;
+} RtdlSceneExport;
+
+int rtdl_mi355x_scene_export(void* prepared, RtdlSceneExport* info, void* nodes, void* prims,
+                             double* exact, uint32_t* roots, uint32_t* ends, char* error_out,
+                             size_t error_size);
+
 /* Process-wide build and launch choices (no counterpart in the reference;
  * for the autotuner in rtdl_amd/autotune.py and callers that pin a tuned
  * configuration).  Every setting gives the same rows; scenes built and

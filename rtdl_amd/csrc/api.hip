@@ -466,6 +466,36 @@ static std::vector<HostJob> host_jobs_from(const RtdlBatchJob* jobs, size_t job_
   return host;
 }
 
+int rtdl_mi355x_scene_export(void* prepared, RtdlSceneExport* info, void* nodes, void* prims,
+                             double* exact, uint32_t* roots, uint32_t* ends, char* error_out,
+                             size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(info != nullptr, "info must not be null");
+    Prepared* p = prepared_from(prepared);
+    std::lock_guard<std::mutex> lock(p->lock);
+    const Scene& sc = p->scene;
+    require(sc.kind == PrimKind::Points || sc.kind == PrimKind::Triangles,
+            "scene export covers point and triangle scenes");
+    RtdlSceneExport out{};
+    out.node_count = sc.nodes.size();
+    out.prim_count = sc.n_prims;
+    out.instance_count = sc.host_roots.size();
+    out.prim_float4s = sc.kind == PrimKind::Triangles ? 3u : 1u;
+    out.exact_doubles = sc.fp32_exact ? 0u : (sc.kind == PrimKind::Triangles ? 9u : 3u);
+    out.m = sc.m;
+    *info = out;
+    if (nodes) RTDL_HIP(hipMemcpy(nodes, sc.nodes.get(), sc.nodes.bytes(), hipMemcpyDeviceToHost));
+    if (prims)
+      RTDL_HIP(hipMemcpy(prims, sc.prims.get(), out.prim_count * out.prim_float4s * sizeof(float4),
+                         hipMemcpyDeviceToHost));
+    if (exact && out.exact_doubles)
+      RTDL_HIP(hipMemcpy(exact, sc.exact.get(), out.prim_count * out.exact_doubles * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    if (roots) std::copy(sc.host_roots.begin(), sc.host_roots.end(), roots);
+    if (ends) std::copy(sc.host_ends.begin(), sc.host_ends.end(), ends);
+  });
+}
+
 int rtdl_mi355x_get_tuning(RtdlTuning* out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
     require(out != nullptr, "out must not be null");

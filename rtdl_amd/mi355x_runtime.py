@@ -157,6 +157,7 @@ _SIGNATURES = {
     "rtdl_mi355x_batch_graph_create": ([_VP, _SZ, ctypes.c_uint32, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_batch_graph_launch": ([_VP, _VP, ctypes.POINTER(ctypes.c_double), *_ERR], ctypes.c_int),
     "rtdl_mi355x_batch_graph_destroy": ([_VP, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_scene_export": ([_VP, _VP, _VP, _VP, _VP, _VP, _VP, *_ERR], ctypes.c_int),
     "rtdl_mi355x_get_tuning": ([_VP, *_ERR], ctypes.c_int),
     "rtdl_mi355x_set_tuning": ([_VP, *_ERR], ctypes.c_int),
     # device buffers (see rtdl_amd.device)

@@ -1,0 +1,62 @@
+"""The in-kernel visit counters (nodes fetched, primitives tested) that
+bench.py turns into algorithmic bytes equal a host replay of the same LBVH
+walks (tests/walk_replay.py) on more than 1,000 sampled queries."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from rtdl_amd import batch as B
+from rtdl_amd.device import DeviceBuffer
+from tests import cases
+from tests import walk_replay as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "mixture"])
+def test_knn_visit_counters_equal_a_host_replay(gpu_ready, kind):
+    _say(f"case: KNN k = 8 visit counters vs host replay ({kind})")
+    rng = np.random.default_rng(96)
+    centres = rng.random((64, 3))
+    pts = cases.points(rng, 60000, 3, kind=kind, centres=centres)
+    s3 = np.stack([pts["x"], pts["y"], pts["z"]], 1).astype(np.float32)
+    q = cases.points(rng, 1200, 3, kind=kind, centres=centres)
+    q3 = np.stack([q["x"], q["y"], q["z"]], 1).astype(np.float32)
+    scene = B.prepare_point_instances(DeviceBuffer.from_numpy(s3), [0, len(s3)])
+    job = B.BatchJob(scene, B.JOB_KNN, DeviceBuffer.from_numpy(q3), k=8)
+    res, _, visits = B.run_batch_profiled([job], count_visits=True)
+    assert res[0]["status"] == 0
+    sc = W.export_scene(scene.handle)
+    assert sc["exact"] is None  # FP32 device input: the FP32 copy is exact
+    nodes = prims = 0
+    for row in q3.astype(np.float64):
+        n, p = W.knn_counts(sc, row, 8)
+        nodes += n
+        prims += p
+    assert (nodes, prims) == visits, ((nodes, prims), visits)
+
+
+def test_ray_anyhit_visit_counters_equal_a_host_replay(gpu_ready):
+    _say("case: ray any-hit visit counters vs host replay")
+    rng = np.random.default_rng(97)
+    t = cases.triangles(rng, 40000, sigma=0.01)
+    r = cases.rays(rng, 1200, tmax=0.3)
+    rays = cases.ray_matrix(r).astype(np.float32)
+    rays[::7, 6] = np.inf  # some unbounded rays
+    scene = B.prepare_triangle_instances(DeviceBuffer.from_numpy(cases.tri_matrix(t).astype(np.float32)),
+                                         [0, len(t["id"])])
+    job = B.BatchJob(scene, B.JOB_RAY_ANYHIT, DeviceBuffer.from_numpy(rays))
+    res, _, visits = B.run_batch_profiled([job], count_visits=True)
+    assert res[0]["status"] == 0
+    sc = W.export_scene(scene.handle)
+    nodes = prims = 0
+    for row in rays.astype(np.float64):
+        n, p = W.ray_anyhit_counts(sc, row)
+        nodes += n
+        prims += p
+    assert (nodes, prims) == visits, ((nodes, prims), visits)

@@ -1,0 +1,296 @@
+"""Host replay of the library's LBVH walks (test infrastructure).
+
+It restates, step by step and with the same FP32 / FP64 arithmetic, the
+device walks whose in-kernel counters bench.py turns into algorithmic bytes:
+the KNN nearest-child-first walk with its packed k-list
+(rtdl_amd/csrc/traverse.hpp: walk_nearest, collect_packed) and the ray
+any-hit walk (trace_ray, mode 0).  A replay returns the nodes fetched and the
+primitives tested, which must equal the kernels' counters.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from rtdl_amd import mi355x_runtime as rt_native
+
+F32 = np.float32
+END = 0xFFFFFFFF
+LEAF = 0x80000000
+START_MASK = (1 << 28) - 1
+NODE_DTYPE = np.dtype([("lo", "<f4", (3,)), ("link", "<u4"), ("hi", "<f4", (3,)), ("escape", "<u4")])
+assert NODE_DTYPE.itemsize == 32
+
+
+class _Export(ctypes.Structure):
+    _fields_ = [("node_count", ctypes.c_uint64), ("prim_count", ctypes.c_uint64),
+                ("instance_count", ctypes.c_uint64), ("prim_float4s", ctypes.c_uint32),
+                ("exact_doubles", ctypes.c_uint32), ("m", ctypes.c_float), ("pad", ctypes.c_uint32)]
+
+
+def export_scene(handle) -> dict:
+    lib = rt_native._library()
+    info = _Export()
+    err = ctypes.create_string_buffer(1024)
+    if lib.rtdl_mi355x_scene_export(handle, ctypes.byref(info), None, None, None, None, None, err, 1024):
+        raise RuntimeError(err.value.decode())
+    nodes = np.zeros(info.node_count, dtype=NODE_DTYPE)
+    prims = np.zeros((info.prim_count * info.prim_float4s, 4), dtype=np.float32)
+    exact = np.zeros(max(1, info.prim_count * info.exact_doubles), dtype=np.float64)
+    roots = np.zeros(max(1, info.instance_count), dtype=np.uint32)
+    ends = np.zeros(max(1, info.instance_count), dtype=np.uint32)
+    if lib.rtdl_mi355x_scene_export(handle, ctypes.byref(info), nodes.ctypes.data, prims.ctypes.data,
+                                    exact.ctypes.data if info.exact_doubles else None, roots.ctypes.data,
+                                    ends.ctypes.data, err, 1024):
+        raise RuntimeError(err.value.decode())
+    return {"nodes": nodes, "prims": prims, "exact": exact if info.exact_doubles else None,
+            "roots": roots, "ends": ends, "m": F32(info.m), "prim_float4s": info.prim_float4s}
+
+
+# ------------------------------------------------------------ arithmetic --
+
+def f32_ru(x: float) -> np.float32:
+    """FP32 rounding of x towards +infinity."""
+    f = F32(x)
+    if float(f) < x:
+        f = np.nextafter(f, F32(np.inf))
+    return f
+
+
+def fma32(a: np.float32, b: np.float32, c: np.float32) -> np.float32:
+    """a * b + c rounded once to FP32.  The product of two FP32 values is
+    exact in FP64; the FP64 sum s carries an exact error term e (TwoSum), and
+    s + e rounds to FP32 like s unless s is exactly halfway between two FP32
+    values, where the sign of e decides."""
+    p = float(a) * float(b)
+    cc = float(c)
+    s = p + cc
+    bb = s - p
+    e = (p - (s - bb)) + (cc - bb)
+    f = F32(s)
+    if float(f) != s and e != 0.0:
+        lo = f if float(f) < s else np.nextafter(f, F32(-np.inf))
+        hi = f if float(f) > s else np.nextafter(f, F32(np.inf))
+        if s == (float(lo) + float(hi)) / 2:
+            f = hi if e > 0 else lo
+    return f
+
+
+def box_dist2(q, lo, hi) -> np.float32:
+    d2 = F32(0.0)
+    for a in range(3):
+        g = max(max(F32(lo[a] - q[a]), F32(q[a] - hi[a])), F32(0.0))
+        d2 = fma32(g, g, d2)
+    return d2
+
+
+def frn_threshold2(rho: float, m: np.float32) -> np.float32:
+    if not rho < 3.0e38:
+        return F32(np.inf)
+    slack = float(m) * 4.76837158203125e-07
+    t = (rho + slack) * 1.000003814697265625
+    return f32_ru(t * t)
+
+
+def coord_bound(m: np.float32, x: float, y: float, z: float) -> np.float32:
+    out = F32(m)
+    for v in (x, y, z):
+        out = max(out, f32_ru(abs(v)))
+    return out
+
+
+def leaf_range(link: int) -> range:
+    first = link & START_MASK
+    return range(first, first + ((link >> 28) & 7) + 1)
+
+
+# -------------------------------------------------------------------- KNN --
+
+def knn_counts(sc: dict, q, k: int, *, instance: int = 0, stack_depth: int = 32) -> tuple[int, int]:
+    """Nodes fetched and primitives tested by the batched KNN walk of query q
+    (FP64 coordinates of FP32 values) for a k-nearest search."""
+    nodes, prims, exact = sc["nodes"], sc["prims"], sc["exact"]
+    root, end = int(sc["roots"][instance]), int(sc["ends"][instance])
+    kk = 1
+    while kk < k:
+        kk <<= 1
+    qx, qy, qz = (float(v) for v in q)
+    qf = (F32(qx), F32(qy), F32(qz))
+    m = coord_bound(sc["m"], qx, qy, qz)
+    full_key = (1 << 64) - 1
+    lst = [0] * (kk - k) + [full_key] * k
+    state = {"thr2": F32(np.inf), "nodes": 0, "prims": 0}
+
+    def visit(slot: int) -> None:
+        if exact is not None:
+            px, py, pz = exact[3 * slot], exact[3 * slot + 1], exact[3 * slot + 2]
+        else:
+            px, py, pz = (float(v) for v in prims[slot, :3])
+        dx, dy, dz = px - qx, py - qy, pz - qz
+        d2 = dx * dx + dy * dy + dz * dz
+        key = (int(F32(d2).view(np.uint32)) << 32) | slot
+        if key < lst[-1]:
+            lst[-1] = key
+            lst.sort()
+            if lst[-1] != full_key:
+                kth = float(np.uint32(lst[-1] >> 32).view(np.float32))
+                state["thr2"] = frn_threshold2(math.sqrt(kth) * 1.0000002384185791015625, m)
+
+    def leaf_tests(link: int, skip_in_list: bool) -> None:
+        for slot in leaf_range(link):
+            state["prims"] += 1
+            p = prims[slot]
+            fx, fy, fz = F32(p[0] - qf[0]), F32(p[1] - qf[1]), F32(p[2] - qf[2])
+            d2f = fma32(fz, fz, fma32(fy, fy, F32(fx * fx)))
+            if d2f <= state["thr2"]:
+                if skip_in_list:
+                    if exact is not None:
+                        px, py, pz = exact[3 * slot], exact[3 * slot + 1], exact[3 * slot + 2]
+                    else:
+                        px, py, pz = (float(v) for v in p[:3])
+                    dx, dy, dz = px - qx, py - qy, pz - qz
+                    key = (int(F32(dx * dx + dy * dy + dz * dz).view(np.uint32)) << 32) | slot
+                    if key in lst:
+                        continue
+                visit(slot)
+
+    def box(n: int) -> np.float32:
+        return box_dist2(qf, nodes[n]["lo"], nodes[n]["hi"])
+
+    if root == END or root == end:
+        return 0, 0
+    complete = True
+    stack: list[int] = []
+    cur = root
+    state["nodes"] += 1
+    if box(cur) <= state["thr2"]:
+        while True:
+            leaf = True
+            while not (int(nodes[cur]["link"]) & LEAF):
+                left = int(nodes[cur]["link"])
+                right = int(nodes[left]["escape"])
+                state["nodes"] += 2
+                dl, dr = box(left), box(right)
+                right_first = dr < dl
+                dn, df = (dr, dl) if right_first else (dl, dr)
+                if df <= state["thr2"]:
+                    if len(stack) < stack_depth:
+                        stack.append(left if right_first else right)
+                    else:
+                        complete = False
+                if not (dn <= state["thr2"]):
+                    leaf = False
+                    break
+                cur = right if right_first else left
+            if leaf:
+                leaf_tests(int(nodes[cur]["link"]), False)
+            found = False
+            while stack:
+                cur = stack.pop()
+                state["nodes"] += 1
+                if box(cur) <= state["thr2"]:
+                    found = True
+                    break
+            if not found:
+                break
+    if not complete:  # the device walks the tree again in rope preorder
+        node = root
+        while node != end:
+            state["nodes"] += 1
+            if box(node) <= state["thr2"]:
+                link = int(nodes[node]["link"])
+                if not link & LEAF:
+                    node = link
+                    continue
+                leaf_tests(link, True)
+            node = int(nodes[node]["escape"])
+    return state["nodes"], state["prims"]
+
+
+# ------------------------------------------------------------------- rays --
+
+_C = F32(9.5367431640625e-07)
+
+
+def _segment_box(o, d, inv, lo, hi, pad, slim) -> bool:
+    tn = F32(-9.5367431640625e-07)
+    tf = F32(F32(slim * F32(1.00000095367431640625)) + _C)
+    for a in range(3):
+        lo_a, hi_a = F32(lo[a] - pad), F32(hi[a] + pad)
+        if d[a] == 0.0:
+            if o[a] < lo_a or o[a] > hi_a:
+                return False
+            continue
+        t0 = F32(F32(lo_a - o[a]) * inv[a])
+        t1 = F32(F32(hi_a - o[a]) * inv[a])
+        a0, a1 = min(t0, t1), max(t0, t1)
+        tn = max(tn, F32(a0 - F32(abs(a0) * _C)))
+        tf = min(tf, F32(a1 + F32(abs(a1) * _C)))
+    return tn <= tf
+
+
+def _mt_hit(ox, oy, oz, dx, dy, dz, tmax, v) -> bool:
+    e1x, e1y, e1z = v[3] - v[0], v[4] - v[1], v[5] - v[2]
+    e2x, e2y, e2z = v[6] - v[0], v[7] - v[1], v[8] - v[2]
+    px, py, pz = dy * e2z - dz * e2y, dz * e2x - dx * e2z, dx * e2y - dy * e2x
+    det = e1x * px + e1y * py + e1z * pz
+    if abs(det) <= 1.0e-8:
+        return False
+    inv = 1.0 / det
+    tx, ty, tz = ox - v[0], oy - v[1], oz - v[2]
+    u = (tx * px + ty * py + tz * pz) * inv
+    if u < 0.0 or u > 1.0:
+        return False
+    qx, qy, qz = ty * e1z - tz * e1y, tz * e1x - tx * e1z, tx * e1y - ty * e1x
+    vv = (dx * qx + dy * qy + dz * qz) * inv
+    if vv < 0.0 or (u + vv) > 1.0:
+        return False
+    t = (e2x * qx + e2y * qy + e2z * qz) * inv
+    return 0.0 <= t <= tmax
+
+
+def ray_anyhit_counts(sc: dict, ray, *, instance: int = 0) -> tuple[int, int]:
+    """Nodes fetched and primitives tested by the any-hit walk of one ray
+    (ox, oy, oz, dx, dy, dz, tmax as FP64 values of FP32 inputs)."""
+    nodes, prims, exact = sc["nodes"], sc["prims"], sc["exact"]
+    root, end = int(sc["roots"][instance]), int(sc["ends"][instance])
+    ox, oy, oz, dx, dy, dz, tmax = (float(v) for v in ray)
+    if not tmax >= 0.0 or root == END:
+        return 0, 0
+    o32 = (F32(ox), F32(oy), F32(oz))
+    d32, inv, pad = [F32(0)] * 3, [F32(0)] * 3, F32(np.inf)
+    if tmax < math.inf:
+        dd = (dx * tmax, dy * tmax, dz * tmax)
+        omax = max(max(abs(ox), abs(oy)), abs(oz))
+        dmax = max(max(abs(dd[0]), abs(dd[1])), abs(dd[2]))
+        pad = f32_ru((float(sc["m"]) + omax + dmax) * 3.814697265625e-06)
+        d32 = [F32(v) for v in dd]
+        inv = [F32(F32(1.0) / v) if v != 0 else F32(0) for v in d32]
+    slim = F32(1.0)
+    n_nodes = n_prims = 0
+    node, done = root, False
+    while node != end and not done:
+        nd = nodes[node]
+        n_nodes += 1
+        if not _segment_box(o32, d32, inv, nd["lo"], nd["hi"], pad, slim):
+            node = int(nd["escape"])
+            continue
+        link = int(nd["link"])
+        if not link & LEAF:
+            node = link
+            continue
+        for slot in leaf_range(link):
+            n_prims += 1
+            if exact is not None:
+                v = [float(x) for x in exact[9 * slot: 9 * slot + 9]]
+            else:
+                a, b, c = prims[3 * slot], prims[3 * slot + 1], prims[3 * slot + 2]
+                v = [float(a[0]), float(a[1]), float(a[2]), float(b[0]), float(b[1]), float(b[2]),
+                     float(c[0]), float(c[1]), float(c[2])]
+            if _mt_hit(ox, oy, oz, dx, dy, dz, tmax, v):
+                done = True
+                break
+        node = int(nd["escape"])
+    return n_nodes, n_prims
