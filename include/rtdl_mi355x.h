@@ -63,7 +63,17 @@ typedef struct RtdlRay3D {
   uint32_t id;
   double ox, oy, oz, dx, dy, dz, tmax;
 } RtdlRay3D; /* 60 bytes, packed */
+
+typedef struct RtdlRay2D {
+  uint32_t id;
+  double ox, oy, dx, dy, tmax;
+} RtdlRay2D; /* 44 bytes, packed (hiprt_runtime.py:137-148) */
 #pragma pack(pop)
+
+typedef struct RtdlTriangle {
+  uint32_t id;
+  double x0, y0, x1, y1, x2, y2;
+} RtdlTriangle; /* 56 bytes, natural alignment (hiprt_runtime.py:125-135) */
 
 /* ---- output rows (reference: src/rtdsl/hiprt_runtime.py:172-267,
  *      src/rtdsl/embree_runtime.py:420-479) ---- */
@@ -168,6 +178,13 @@ int rtdl_mi355x_run_ray_anyhit_3d(const RtdlRay3D* rays, size_t ray_count,
                                   RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
                                   char* error_out, size_t error_size);
 
+/* replaces rtdl_hiprt_run_ray_anyhit_2d (src/native/hiprt/rtdl_hiprt_api.cpp:555-572); the ray
+ * is the segment o -> o + d * tmax (reference.py:783-803) */
+int rtdl_mi355x_run_ray_anyhit_2d(const RtdlRay2D* rays, size_t ray_count,
+                                  const RtdlTriangle* triangles, size_t triangle_count,
+                                  RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
+                                  char* error_out, size_t error_size);
+
 /* replaces rtdl_hiprt_run_ray_hitcount_3d (src/native/hiprt/rtdl_hiprt_api.cpp:183-286) */
 int rtdl_mi355x_run_ray_hitcount_3d(const RtdlRay3D* rays, size_t ray_count,
                                     const RtdlTriangle3D* triangles, size_t triangle_count,
@@ -214,6 +231,15 @@ int rtdl_mi355x_run_prepared_bounded_knn_rows(void* prepared, const void* querie
                                               size_t query_count, double radius, uint32_t k_max,
                                               RtdlKnnNeighborRow** rows_out, size_t* row_count_out,
                                               char* error_out, size_t error_size);
+
+/* replace rtdl_hiprt_prepare_ray_anyhit_2d / _run_prepared_ray_anyhit_2d
+ * (src/native/hiprt/rtdl_hiprt_api.cpp:81-181); the handle is released with
+ * rtdl_mi355x_destroy_prepared. */
+int rtdl_mi355x_prepare_triangles_2d(const RtdlTriangle* triangles, size_t triangle_count,
+                                     void** prepared_out, char* error_out, size_t error_size);
+int rtdl_mi355x_run_prepared_ray_anyhit_2d(void* prepared, const RtdlRay2D* rays, size_t ray_count,
+                                           RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
+                                           char* error_out, size_t error_size);
 
 /* replaces rtdl_hiprt_prepare_ray_hitcount_3d / _run_prepared_ray_hitcount_3d
  * (src/native/hiprt/rtdl_hiprt_api.cpp:15-79); the prepared triangle scene serves

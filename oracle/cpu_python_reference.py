@@ -29,6 +29,7 @@ if _ROOT not in sys.path:
 from rtdl_amd.records import (  # noqa: E402
     Point3D,
     Polygon,
+    Ray2D,
     Ray3D,
     Triangle3D,
     normalize_records,
@@ -254,6 +255,65 @@ def ray_triangle_closest_hit_cpu(rays, triangles):
         if best_t is not None and best_id is not None:
             rows.append({"ray_id": int(ray.id), "triangle_id": best_id, "t": best_t})
     return tuple(rows)
+
+
+# --------------------------------------------------------------------------
+# visibility rows (reference: src/rtdsl/reference.py:275-346)
+# --------------------------------------------------------------------------
+
+
+def _dimension(record) -> int:
+    # reference: src/rtdsl/reference.py:269-274 (points and triangles)
+    return 3 if isinstance(record, (Point3D, Triangle3D)) else 2
+
+
+def _visibility_ray(ray_id, observer, target):
+    # reference: src/rtdsl/reference.py:275-293 (segment observer -> target, tmax 1)
+    if _dimension(observer) != _dimension(target):
+        raise ValueError("visibility_rows requires observers and targets with the same dimensionality")
+    if isinstance(observer, Point3D):
+        dx = target.x - observer.x
+        dy = target.y - observer.y
+        dz = target.z - observer.z
+        if dx == 0.0 and dy == 0.0 and dz == 0.0:
+            raise ValueError("visibility_rows requires distinct observer and target points")
+        return Ray3D(ray_id, observer.x, observer.y, observer.z, dx, dy, dz, 1.0)
+    dx = target.x - observer.x
+    dy = target.y - observer.y
+    if dx == 0.0 and dy == 0.0:
+        raise ValueError("visibility_rows requires distinct observer and target points")
+    return Ray2D(ray_id, observer.x, observer.y, dx, dy, 1.0)
+
+
+def visibility_ray_pairs(observers, targets, blockers):
+    # reference: src/rtdsl/reference.py:310-334 (observer-major ray ids)
+    if not observers or not targets:
+        return (), ()
+    dim = _dimension(observers[0])
+    if any(_dimension(p) != dim for p in (*observers, *targets)):
+        raise ValueError("visibility_rows requires all observers and targets to share one dimensionality")
+    if blockers and any(_dimension(t) != dim for t in blockers):
+        raise ValueError("visibility_rows blocker triangles must match observer/target dimensionality")
+    rays, pairs = [], []
+    for observer in observers:
+        for target in targets:
+            rid = len(rays)
+            rays.append(_visibility_ray(rid, observer, target))
+            pairs.append((rid, int(observer.id), int(target.id)))
+    return tuple(rays), tuple(pairs)
+
+
+def visibility_rows_from_any_hit(ray_pairs, any_hit_by_ray):
+    # reference: src/rtdsl/reference.py:337-346
+    return tuple({"observer_id": o, "target_id": t, "visible": 0 if any_hit_by_ray.get(r, 0) else 1}
+                 for r, o, t in ray_pairs)
+
+
+def visibility_rows_cpu(observers, targets, blockers):
+    # reference: src/rtdsl/reference.py:296-307
+    rays, pairs = visibility_ray_pairs(observers, targets, blockers)
+    hits = {int(r["ray_id"]): int(r["any_hit"]) for r in ray_triangle_any_hit_cpu(rays, blockers)}
+    return visibility_rows_from_any_hit(pairs, hits)
 
 
 # --------------------------------------------------------------------------

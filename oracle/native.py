@@ -126,6 +126,27 @@ def rays_3d(mode: str, rays: np.ndarray, tris: np.ndarray, tri_ids: np.ndarray, 
     return out_u
 
 
+def rays_2d_any(rays: np.ndarray, tris: np.ndarray, *, threads: int | None = None,
+                brute: bool = False) -> np.ndarray:
+    """2-D any-hit flags.  rays: (n,5) ox oy dx dy tmax; tris: (m,6) x0 y0 x1 y1 x2 y2."""
+    r2 = _f64(rays).reshape(-1, 5)
+    t2 = _f64(tris).reshape(-1, 6)
+    r = np.zeros((r2.shape[0], 7))
+    r[:, [0, 1, 3, 4, 6]] = r2
+    t = np.zeros((t2.shape[0], 9))
+    t[:, [0, 1, 3, 4, 6, 7]] = t2
+    n = r.shape[0]
+    out_u = np.zeros(n, dtype=np.uint32)
+    out_t = np.zeros(n, dtype=np.float64)
+    out_h = np.zeros(n, dtype=np.uint8)
+    ids = np.arange(t.shape[0], dtype=np.uint32)
+    library().oracle_rays_3d(
+        3, n, _dp(r), t.shape[0], _dp(t), _up(ids), int(threads or default_threads()), int(bool(brute)),
+        _up(out_u), _dp(out_t), out_h.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+    )
+    return out_u
+
+
 def pip_pairs(px, py, offsets, counts, vertices_xy, *, positive_only: bool,
               threads: int | None = None, brute: bool = False) -> np.ndarray:
     """Return an (n, 2) array of (point index, polygon index) with contains = 1."""

@@ -22,6 +22,8 @@
  *   _point_distance_sq          src/rtdsl/reference.py:882-890
  *   _finite_ray_triangle_hit_t_3d  src/rtdsl/reference.py:810-846
  *   ray any-hit / hit-count / closest-hit  src/rtdsl/reference.py:174-200,248-266
+ *   2-D any-hit _finite_ray_hits_triangle_2d  src/rtdsl/reference.py:783-803
+ *     (_segment_intersection :693-714, _point_in_triangle :893-915)
  *   pip_cpu / _point_in_polygon / _point_on_segment  src/rtdsl/reference.py:103-130,717-768
  */
 #include <math.h>
@@ -486,12 +488,63 @@ static int hit_t_3d(const double* r, const double* tri, double* t_out) {
   return 0;
 }
 
+/* ------------------------------------------------------- rays (2-D any) -- */
+
+/* reference: src/rtdsl/reference.py:693-714 (left = p -> e, right = q0 -> q1) */
+static int segments_cross_2d(double px, double py, double ex, double ey, double qx0, double qy0,
+                             double qx1, double qy1) {
+  double rx = ex - px;
+  double ry = ey - py;
+  double sx = qx1 - qx0;
+  double sy = qy1 - qy0;
+  double denom = rx * sy - ry * sx;
+  if (fabs(denom) < 1.0e-7) return 0;
+  double qpx = qx0 - px;
+  double qpy = qy0 - py;
+  double t = (qpx * sy - qpy * sx) / denom;
+  double u = (qpx * ry - qpy * rx) / denom;
+  return 0.0 <= t && t <= 1.0 && 0.0 <= u && u <= 1.0;
+}
+
+/* reference: src/rtdsl/reference.py:893-915 */
+static int point_in_triangle_2d(double x, double y, const double* v) {
+  double v0x = v[6] - v[0];
+  double v0y = v[7] - v[1];
+  double v1x = v[3] - v[0];
+  double v1y = v[4] - v[1];
+  double v2x = x - v[0];
+  double v2y = y - v[1];
+  double dot00 = v0x * v0x + v0y * v0y;
+  double dot01 = v0x * v1x + v0y * v1y;
+  double dot02 = v0x * v2x + v0y * v2y;
+  double dot11 = v1x * v1x + v1y * v1y;
+  double dot12 = v1x * v2x + v1y * v2y;
+  double denom = dot00 * dot11 - dot01 * dot01;
+  if (fabs(denom) < 1.0e-7) return 0;
+  double inv = 1.0 / denom;
+  double u = (dot11 * dot02 - dot01 * dot12) * inv;
+  double w = (dot00 * dot12 - dot01 * dot02) * inv;
+  return u >= 0.0 && w >= 0.0 && (u + w) <= 1.0;
+}
+
+/* reference: src/rtdsl/reference.py:783-803; r = ox oy - dx dy - tmax, tri = x y - (x3) */
+static int hits_2d(const double* r, const double* v) {
+  double ox = r[0], oy = r[1];
+  double ex = ox + r[3] * r[6];
+  double ey = oy + r[4] * r[6];
+  if (point_in_triangle_2d(ox, oy, v)) return 1;
+  if (point_in_triangle_2d(ex, ey, v)) return 1;
+  return segments_cross_2d(ox, oy, ex, ey, v[0], v[1], v[3], v[4]) ||
+         segments_cross_2d(ox, oy, ex, ey, v[3], v[4], v[6], v[7]) ||
+         segments_cross_2d(ox, oy, ex, ey, v[6], v[7], v[0], v[1]);
+}
+
 typedef struct {
   size_t nr, nt;
   const double* rays;  /* nr x 7: ox oy oz dx dy dz tmax */
   const double* tris;  /* nt x 9 */
   const uint32_t* tid; /* nt */
-  int mode;            /* 0 any, 1 count, 2 closest */
+  int mode;            /* 0 any, 1 count, 2 closest, 3 2-D any */
   int brute;
   grid_t grid;
   double pad;
@@ -532,6 +585,21 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
   uint32_t* stamp = c->stamp ? c->stamp[worker] : NULL;
   for (size_t i = b; i < e; ++i) {
     const double* r = c->rays + i * 7;
+    /* 2-D: the grid walks the segment o -> o + d * tmax as s in [0, 1] */
+    double seg[7];
+    int brute = c->brute;
+    const double* rg = r;
+    if (c->mode == 3) {
+      seg[0] = r[0];
+      seg[1] = r[1];
+      seg[2] = 0.0;
+      seg[3] = r[3] * r[6];
+      seg[4] = r[4] * r[6];
+      seg[5] = 0.0;
+      seg[6] = 1.0;
+      rg = seg;
+      if (!isfinite(seg[3]) || !isfinite(seg[4])) brute = 1;
+    }
     uint32_t count = 0;
     int any = 0;
     int has = 0;
@@ -539,8 +607,9 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
     uint32_t best_id = 0;
 #define VISIT(TRI)                                                                        \
   do {                                                                                    \
-    double t_;                                                                            \
-    if (hit_t_3d(r, c->tris + (size_t)(TRI) * 9, &t_)) {                                  \
+    double t_ = 0.0;                                                                      \
+    const double* v_ = c->tris + (size_t)(TRI) * 9;                                       \
+    if (c->mode == 3 ? hits_2d(r, v_) : hit_t_3d(r, v_, &t_)) {                           \
       any = 1;                                                                            \
       ++count;                                                                            \
       uint32_t id_ = c->tid[TRI];                                                         \
@@ -551,16 +620,16 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
       }                                                                                   \
     }                                                                                     \
   } while (0)
-    if (c->brute) {
+    if (brute) {
       for (size_t t = 0; t < c->nt; ++t) {
         VISIT(t);
-        if (any && c->mode == 0) break;
+        if (any && (c->mode == 0 || c->mode == 3)) break;
       }
     } else if (c->nt > 0) {
       const grid_t* g = &c->grid;
       double s0[3], s1[3];
       for (int a = 0; a < 3; ++a) {
-        double p0 = r[a], p1 = r[a] + r[3 + a] * r[6];
+        double p0 = rg[a], p1 = rg[a] + rg[3 + a] * rg[6];
         s0[a] = p0 < p1 ? p0 : p1;
         s1[a] = p0 < p1 ? p1 : p0;
         if (!isfinite(s0[a])) s0[a] = -INFINITY;
@@ -578,7 +647,7 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
           for (int64_t x = lo[0]; x <= hi[0] && !done; ++x) {
             double clo[3] = {g->lo[0] + x * g->cell, g->lo[1] + y * g->cell, g->lo[2] + z * g->cell};
             double chi[3] = {clo[0] + g->cell, clo[1] + g->cell, clo[2] + g->cell};
-            if (!seg_box(r, clo, chi, c->pad + g->cell * 1e-9)) continue;
+            if (!seg_box(rg, clo, chi, c->pad + g->cell * 1e-9)) continue;
             int64_t cell = (z * g->n[1] + y) * g->n[0] + x;
             for (int64_t it = g->start[cell]; it < g->start[cell + 1]; ++it) {
               uint32_t t = g->items[it];
@@ -587,7 +656,7 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
                 stamp[t] = (uint32_t)(i + 1);
               }
               VISIT(t);
-              if (any && c->mode == 0) {
+              if (any && (c->mode == 0 || c->mode == 3)) {
                 done = 1;
                 break;
               }
@@ -595,7 +664,7 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
           }
     }
 #undef VISIT
-    if (c->mode == 0)
+    if (c->mode == 0 || c->mode == 3)
       c->out_u[i] = (uint32_t)any;
     else if (c->mode == 1)
       c->out_u[i] = count;
@@ -607,7 +676,8 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
   }
 }
 
-/* mode 0: out_u = any_hit; 1: out_u = hit_count; 2: closest (out_u id, out_t, out_has). */
+/* mode 0: out_u = any_hit; 1: out_u = hit_count; 2: closest (out_u id, out_t, out_has);
+ * 3: out_u = 2-D any_hit (rays ox oy 0 dx dy 0 tmax, triangles with z = 0). */
 int oracle_rays_3d(int mode, size_t nr, const double* rays, size_t nt, const double* tris,
                    const uint32_t* tid, int threads, int brute, uint32_t* out_u, double* out_t,
                    uint8_t* out_has) {

@@ -74,5 +74,6 @@ from .mi355x_runtime import (
     run_mi355x,
 )
 from .records import Point, Point3D, Polygon, Ray2D, Ray3D, Triangle, Triangle3D
+from .visibility import visibility_pair_rows, visibility_rows
 
 __all__ = [name for name in dir() if not name.startswith("_")]

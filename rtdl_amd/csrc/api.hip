@@ -209,6 +209,20 @@ int rtdl_mi355x_run_ray_anyhit_3d(const RtdlRay3D* rays, size_t ray_count,
                        row_count_out, error_out, error_size);
 }
 
+int rtdl_mi355x_run_ray_anyhit_2d(const RtdlRay2D* rays, size_t ray_count,
+                                  const RtdlTriangle* triangles, size_t triangle_count,
+                                  RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
+                                  char* error_out, size_t error_size) {
+  static_assert(sizeof(RtdlRay2D) == 44 && sizeof(RtdlTriangle) == 56, "reference record layouts");
+  return guarded(error_out, error_size, [&] {
+    zero_outputs((void**)rows_out, row_count_out);
+    require((ray_count == 0 || rays != nullptr) && (triangle_count == 0 || triangles != nullptr),
+            "input pointers must not be null when counts are nonzero");
+    per_call([&](Prepared& p) { prepare_triangles_2d(p, triangles, triangle_count); },
+             [&](Prepared& p) { run_rays_2d(p, rays, ray_count, (void**)rows_out, row_count_out); });
+  });
+}
+
 int rtdl_mi355x_run_ray_hitcount_3d(const RtdlRay3D* rays, size_t ray_count,
                                     const RtdlTriangle3D* triangles, size_t triangle_count,
                                     RtdlRayHitCountRow** rows_out, size_t* row_count_out,
@@ -307,6 +321,22 @@ int rtdl_mi355x_prepare_triangles_3d(const RtdlTriangle3D* triangles, size_t tri
                                      void** prepared_out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
     make_prepared(prepared_out, [&](Prepared& p) { prepare_triangles(p, triangles, triangle_count); });
+  });
+}
+
+int rtdl_mi355x_prepare_triangles_2d(const RtdlTriangle* triangles, size_t triangle_count,
+                                     void** prepared_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    make_prepared(prepared_out, [&](Prepared& p) { prepare_triangles_2d(p, triangles, triangle_count); });
+  });
+}
+
+int rtdl_mi355x_run_prepared_ray_anyhit_2d(void* prepared, const RtdlRay2D* rays, size_t ray_count,
+                                           RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
+                                           char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs((void**)rows_out, row_count_out);
+    on_prepared(prepared, [&](Prepared& p) { run_rays_2d(p, rays, ray_count, (void**)rows_out, row_count_out); });
   });
 }
 

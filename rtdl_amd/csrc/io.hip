@@ -55,6 +55,38 @@ __global__ void unpack_rays_kernel(const uint8_t* raw, size_t n, double* rays, u
   for (int a = 0; a < 7; ++a) rays[7 * i + a] = f64_at(raw, b + 4 + 8 * a);
 }
 
+// RtdlTriangle (56 B, natural alignment: id, pad, x0 y0 x1 y1 x2 y2) -> 9
+// doubles with z = 0.
+__global__ void unpack_triangles_2d_kernel(const uint8_t* raw, size_t n, double* coords,
+                                           uint32_t* ids) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t b = i * 56;
+  ids[i] = u32_at(raw, b);
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    coords[9 * i + 3 * v] = f64_at(raw, b + 8 + 16 * v);
+    coords[9 * i + 3 * v + 1] = f64_at(raw, b + 16 + 16 * v);
+    coords[9 * i + 3 * v + 2] = 0.0;
+  }
+}
+
+// RtdlRay2D (44 B, packed: id, ox oy dx dy tmax) -> ox oy 0 dx dy 0 tmax.
+__global__ void unpack_rays_2d_kernel(const uint8_t* raw, size_t n, double* rays, uint32_t* ids) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t b = i * 44;
+  ids[i] = u32_at(raw, b);
+  double* r = rays + 7 * i;
+  r[0] = f64_at(raw, b + 4);
+  r[1] = f64_at(raw, b + 12);
+  r[2] = 0.0;
+  r[3] = f64_at(raw, b + 20);
+  r[4] = f64_at(raw, b + 28);
+  r[5] = 0.0;
+  r[6] = f64_at(raw, b + 36);
+}
+
 __global__ void unpack_polygons_kernel(const uint8_t* raw, size_t n, uint4* refs) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -277,6 +309,17 @@ void unpack_triangles(const void* raw, size_t n, double* coords, uint32_t* ids,
 void unpack_rays(const void* raw, size_t n, double* rays, uint32_t* ids, hipStream_t stream) {
   if (!n) return;
   unpack_rays_kernel<<<RTDL_GRID(n)>>>(static_cast<const uint8_t*>(raw), n, rays, ids);
+  RTDL_LAUNCH_CHECK();
+}
+void unpack_triangles_2d(const void* raw, size_t n, double* coords, uint32_t* ids,
+                         hipStream_t stream) {
+  if (!n) return;
+  unpack_triangles_2d_kernel<<<RTDL_GRID(n)>>>(static_cast<const uint8_t*>(raw), n, coords, ids);
+  RTDL_LAUNCH_CHECK();
+}
+void unpack_rays_2d(const void* raw, size_t n, double* rays, uint32_t* ids, hipStream_t stream) {
+  if (!n) return;
+  unpack_rays_2d_kernel<<<RTDL_GRID(n)>>>(static_cast<const uint8_t*>(raw), n, rays, ids);
   RTDL_LAUNCH_CHECK();
 }
 void unpack_polygons(const void* raw, size_t n, uint4* refs, hipStream_t stream) {

@@ -17,6 +17,10 @@ void unpack_points(const void* raw, size_t n, int dim, double* coords, uint32_t*
 void unpack_triangles(const void* raw, size_t n, double* coords, uint32_t* ids, hipStream_t stream);
 // RtdlRay3D (60 B, packed): 7 doubles per ray.
 void unpack_rays(const void* raw, size_t n, double* rays, uint32_t* ids, hipStream_t stream);
+// RtdlTriangle (56 B): 9 doubles per triangle with z = 0.
+void unpack_triangles_2d(const void* raw, size_t n, double* coords, uint32_t* ids, hipStream_t stream);
+// RtdlRay2D (44 B, packed): ox oy 0 dx dy 0 tmax per ray.
+void unpack_rays_2d(const void* raw, size_t n, double* rays, uint32_t* ids, hipStream_t stream);
 // RtdlPolygonRef (12 B): (vertex_offset, vertex_count, id, 0).
 void unpack_polygons(const void* raw, size_t n, uint4* refs, hipStream_t stream);
 // RtdlPoint (24 B) -> 2 doubles per point (PIP probes).

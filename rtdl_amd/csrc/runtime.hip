@@ -277,6 +277,20 @@ void prepare_triangles(Prepared& p, const void* raw, size_t n) {
   read_root_box(p);
 }
 
+void prepare_triangles_2d(Prepared& p, const void* raw, size_t n) {
+  require(n == 0 || raw != nullptr, "triangle pointer must not be null when count is nonzero");
+  require(n <= kMaxLeafStart, "too many triangles");
+  hipStream_t s = p.stream;
+  p.type = SceneType::Triangles2D;
+  auto bytes = upload_bytes(raw, n * 56, s);
+  DeviceArray<double> coords(9 * n);
+  DeviceArray<uint32_t> ids(n);
+  unpack_triangles_2d(bytes.get(), n, coords.get(), ids.get(), s);
+  build_scene(p.scene, PrimKind::Triangles, 3, n, coords.get(), ids.get(), nullptr, nullptr, 0,
+              {0, (uint64_t)n}, current_tuning().triangle_leaf, s);
+  read_root_box(p);
+}
+
 void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* verts_xy,
                       size_t n_doubles) {
   require(n == 0 || refs != nullptr, "polygon pointer must not be null when count is nonzero");
@@ -509,6 +523,37 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
 }
 
 // -------------------------------------------------------------------- rays --
+
+void run_rays_2d(Prepared& p, const void* raw_rays, size_t nr_in, void** rows_out, size_t* n_out) {
+  require(p.type == SceneType::Triangles2D, "prepared handle is not a 2-D triangle scene");
+  require(nr_in == 0 || raw_rays != nullptr, "ray pointer must not be null when ray count is nonzero");
+  require(nr_in < 0xFFFFFFFFull, "too many rays");
+  PhaseClock& clock = thread_clock();
+  const uint32_t nr = (uint32_t)nr_in;
+  hipStream_t s = p.stream;
+  if (nr == 0) {
+    *rows_out = host_rows(0, 8);
+    *n_out = 0;
+    return;
+  }
+  auto bytes = upload_bytes(raw_rays, (size_t)nr * 44, s);
+  DeviceArray<double> rays(7 * (size_t)nr);
+  DeviceArray<uint32_t> rid(nr);
+  unpack_rays_2d(bytes.get(), nr, rays.get(), rid.get(), s);
+  auto order = traversal_order(rays.get(), 7, nr, p, s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(0);
+  DeviceArray<uint32_t> out_u(nr);
+  launch_rays(p.scene.view(0), rays.get(), order.get(), nr, 3, out_u.get(), nullptr, nullptr, 256, s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(2);
+  DeviceArray<uint8_t> rows((size_t)nr * 8);
+  emit_pair_rows(nr, rid.get(), out_u.get(), rows.get(), s);
+  RTDL_HIP(hipStreamSynchronize(s));
+  clock.lap(3);
+  copy_rows_back(rows.get(), nr, 8, rows_out, n_out, s);
+  clock.lap(4);
+}
 
 void run_rays(Prepared& p, const void* raw_rays, size_t nr_in, int mode, void** rows_out,
               size_t* n_out) {

@@ -30,7 +30,7 @@ struct PhaseClock {
 };
 PhaseClock& thread_clock();
 
-enum class SceneType : int { Points2D = 0, Points3D = 1, Triangles3D = 2, Polygons = 3 };
+enum class SceneType : int { Points2D = 0, Points3D = 1, Triangles3D = 2, Polygons = 3, Triangles2D = 4 };
 
 // A prepared (device-resident) scene.  Calls on one handle are serialised by
 // `lock`; each handle owns its stream.
@@ -56,6 +56,8 @@ Prepared* prepared_from(void* handle);
 // Builders (raw = caller's record array on the host).
 void prepare_points(Prepared& p, const void* raw, size_t n, int dim);
 void prepare_triangles(Prepared& p, const void* raw, size_t n);
+// RtdlTriangle records (2-D): a triangle scene with z = 0, for 2-D rays.
+void prepare_triangles_2d(Prepared& p, const void* raw, size_t n);
 void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* verts_xy,
                       size_t n_doubles);
 
@@ -72,6 +74,8 @@ void prepare_polygon_instances(Prepared& p, const uint32_t* voff, const uint32_t
 enum class NeighborMode : int { FixedRadius = 0, Knn = 1, BoundedKnn = 2 };
 void run_neighbors(Prepared& p, const void* raw_queries, size_t nq, NeighborMode mode,
                    double radius, uint32_t k, void** rows_out, size_t* n_out);
+// RtdlRay2D records against a 2-D triangle scene: any-hit rows.
+void run_rays_2d(Prepared& p, const void* raw_rays, size_t nr, void** rows_out, size_t* n_out);
 void run_rays(Prepared& p, const void* raw_rays, size_t nr, int mode, void** rows_out,
               size_t* n_out);
 void run_pip(Prepared& p, const void* raw_points, size_t np, bool positive_only, void** rows_out,

@@ -410,6 +410,59 @@ __device__ inline bool mt_hit(double ox, double oy, double oz, double dx, double
   return false;
 }
 
+// 2-D ray any-hit (reference.py:783-803): the ray is the segment from o to
+// o + d * tmax; it hits a triangle when either end lies inside it or the
+// segment crosses one of its edges.  FP64 in the reference's operation order.
+
+// reference.py:693-714, left = p -> e, right = q0 -> q1
+__device__ inline bool segments_cross_2d(double px, double py, double ex, double ey, double qx0,
+                                         double qy0, double qx1, double qy1) {
+  const double rx = ex - px;
+  const double ry = ey - py;
+  const double sx = qx1 - qx0;
+  const double sy = qy1 - qy0;
+  const double denom = rx * sy - ry * sx;
+  if (fabs(denom) < 1.0e-7) return false;
+  const double qpx = qx0 - px;
+  const double qpy = qy0 - py;
+  const double t = (qpx * sy - qpy * sx) / denom;
+  const double u = (qpx * ry - qpy * rx) / denom;
+  return 0.0 <= t && t <= 1.0 && 0.0 <= u && u <= 1.0;
+}
+
+// reference.py:893-915, vertices (ax, ay), (bx, by), (cx, cy)
+__device__ inline bool point_in_triangle_2d(double x, double y, double ax, double ay, double bx,
+                                            double by, double cx, double cy) {
+  const double v0x = cx - ax;
+  const double v0y = cy - ay;
+  const double v1x = bx - ax;
+  const double v1y = by - ay;
+  const double v2x = x - ax;
+  const double v2y = y - ay;
+  const double dot00 = v0x * v0x + v0y * v0y;
+  const double dot01 = v0x * v1x + v0y * v1y;
+  const double dot02 = v0x * v2x + v0y * v2y;
+  const double dot11 = v1x * v1x + v1y * v1y;
+  const double dot12 = v1x * v2x + v1y * v2y;
+  const double denom = dot00 * dot11 - dot01 * dot01;
+  if (fabs(denom) < 1.0e-7) return false;
+  const double inv = 1.0 / denom;
+  const double u = (dot11 * dot02 - dot01 * dot12) * inv;
+  const double v = (dot00 * dot12 - dot01 * dot02) * inv;
+  return u >= 0.0 && v >= 0.0 && (u + v) <= 1.0;
+}
+
+// v: x0 y0 z0 x1 y1 z1 x2 y2 z2 (z unused)
+__device__ inline bool hit_2d(double ox, double oy, double dx, double dy, double tmax, const double v[9]) {
+  const double ex = ox + dx * tmax;
+  const double ey = oy + dy * tmax;
+  if (point_in_triangle_2d(ox, oy, v[0], v[1], v[3], v[4], v[6], v[7])) return true;
+  if (point_in_triangle_2d(ex, ey, v[0], v[1], v[3], v[4], v[6], v[7])) return true;
+  return segments_cross_2d(ox, oy, ex, ey, v[0], v[1], v[3], v[4]) ||
+         segments_cross_2d(ox, oy, ex, ey, v[3], v[4], v[6], v[7]) ||
+         segments_cross_2d(ox, oy, ex, ey, v[6], v[7], v[0], v[1]);
+}
+
 // Conservative FP32 test of the segment o + s * D, s in [0, slim], against
 // the box grown by `pad` on every side.  `pad` (2^-18 of the coordinate
 // scale) dominates every FP32 rounding of o, D, the box and the slab
@@ -440,15 +493,21 @@ struct RayResult {
   uint32_t best_id = 0;
 };
 
-// MODE 0 any-hit (stops at the first hit), 1 hit count, 2 closest hit.
+// MODE 0 any-hit (stops at the first hit), 1 hit count, 2 closest hit,
+// 3 2-D any-hit (z of rays and triangles is 0; stops at the first hit).
 template <int MODE, bool STATS>
 __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_t end, double ox,
                                       double oy, double oz, double dx, double dy, double dz,
                                       double tmax, LaneStats& st) {
   RayResult res;
-  // t in [0, tmax] is impossible for a negative or NaN tmax (reference.py:844)
-  if (!(tmax >= 0.0) || root == kEndNode) return res;
-  const bool bounded = tmax < __builtin_inf();
+  // 3-D: t in [0, tmax] is impossible for a negative or NaN tmax
+  // (reference.py:844).  2-D takes any tmax: the segment o -> o + d * tmax
+  // is tested as it is (reference.py:783-803).
+  if ((MODE != 3 && !(tmax >= 0.0)) || root == kEndNode) return res;
+  // A segment with a non-finite end (2-D) or an unbounded ray is tested
+  // against every primitive: the box prefilter is switched off (pad = inf).
+  const bool bounded = MODE == 3 ? (fabs(dx * tmax) < __builtin_inf() && fabs(dy * tmax) < __builtin_inf())
+                                  : tmax < __builtin_inf();
   const float o32[3] = {(float)ox, (float)oy, (float)oz};
   float D32[3] = {0.0f, 0.0f, 0.0f};
   float inv[3] = {0.0f, 0.0f, 0.0f};
@@ -497,10 +556,14 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
         v[3] = b.x; v[4] = b.y; v[5] = b.z;
         v[6] = c.x; v[7] = c.y; v[8] = c.z;
       }
-      double th;
-      if (!mt_hit(ox, oy, oz, dx, dy, dz, tmax, v, &th)) continue;
+      double th = 0.0;
+      if (MODE == 3) {
+        if (!hit_2d(ox, oy, dx, dy, tmax, v)) continue;
+      } else if (!mt_hit(ox, oy, oz, dx, dy, dz, tmax, v, &th)) {
+        continue;
+      }
       ++res.hits;
-      if (MODE == 0) {
+      if (MODE == 0 || MODE == 3) {
         done = true;
         break;
       }

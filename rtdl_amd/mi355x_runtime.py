@@ -29,7 +29,7 @@ import numpy as np
 
 from .api import compile_kernel
 from .ir import CompiledKernel
-from .records import Point, Point3D, Polygon, Ray3D, Triangle3D, normalize_records, project_rows
+from .records import Point, Point3D, Polygon, Ray2D, Ray3D, Triangle, Triangle3D, normalize_records, project_rows
 
 # --------------------------------------------------------------------------
 # C record and row layouts (reference: src/rtdsl/hiprt_runtime.py:108-267,
@@ -46,6 +46,12 @@ TRIANGLE3D_DTYPE = np.dtype({"names": ["id", *_TRI_FIELDS], "formats": ["<u4"] +
 _RAY_FIELDS = ["ox", "oy", "oz", "dx", "dy", "dz", "tmax"]
 RAY3D_DTYPE = np.dtype({"names": ["id", *_RAY_FIELDS], "formats": ["<u4"] + ["<f8"] * 7,
                         "offsets": [0] + [4 + 8 * i for i in range(7)], "itemsize": 60})
+_TRI2_FIELDS = ["x0", "y0", "x1", "y1", "x2", "y2"]
+TRIANGLE2D_DTYPE = np.dtype({"names": ["id", *_TRI2_FIELDS], "formats": ["<u4"] + ["<f8"] * 6,
+                             "offsets": [0] + [8 + 8 * i for i in range(6)], "itemsize": 56})
+_RAY2_FIELDS = ["ox", "oy", "dx", "dy", "tmax"]
+RAY2D_DTYPE = np.dtype({"names": ["id", *_RAY2_FIELDS], "formats": ["<u4"] + ["<f8"] * 5,
+                        "offsets": [0] + [4 + 8 * i for i in range(5)], "itemsize": 44})
 POLYGON_REF_DTYPE = np.dtype([("id", "<u4"), ("vertex_offset", "<u4"), ("vertex_count", "<u4")])
 
 FRN_ROW_DTYPE = np.dtype({"names": ["query_id", "neighbor_id", "distance"],
@@ -129,6 +135,7 @@ _SIGNATURES = {
     "rtdl_mi355x_run_knn_rows_3d": ([_VP, _SZ, _VP, _SZ, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_bounded_knn_rows_2d": ([_VP, _SZ, _VP, _SZ, ctypes.c_double, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_bounded_knn_rows_3d": ([_VP, _SZ, _VP, _SZ, ctypes.c_double, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_run_ray_anyhit_2d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_ray_anyhit_3d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_ray_hitcount_3d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_ray_closest_hit_3d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
@@ -138,6 +145,8 @@ _SIGNATURES = {
     "rtdl_mi355x_run_prepared_fixed_radius_neighbors": ([_VP, _VP, _SZ, ctypes.c_double, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_knn_rows": ([_VP, _VP, _SZ, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_bounded_knn_rows": ([_VP, _VP, _SZ, ctypes.c_double, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_prepare_triangles_2d": ([_VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_run_prepared_ray_anyhit_2d": ([_VP, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_prepare_triangles_3d": ([_VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_ray_anyhit_3d": ([_VP, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_ray_hitcount_3d": ([_VP, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
@@ -343,6 +352,46 @@ def pack_rays(payload, *, name: str = "rays") -> np.ndarray:
     return _pack(RAY3D_DTYPE, {k: arr[k] for k in arr.dtype.names})
 
 
+def pack_triangles_2d(payload, *, name: str = "triangles") -> np.ndarray:
+    """Pack 2-D triangles into RtdlTriangle records (56 bytes, natural alignment)."""
+    cols = _columns(payload, ("id", *_TRI2_FIELDS))
+    if cols is not None:
+        return _pack(TRIANGLE2D_DTYPE, cols)
+    records = normalize_records(name, "triangles", payload)
+    if any(not isinstance(r, Triangle) for r in records):
+        raise ValueError(f"input `{name}` mixes 2-D and 3-D triangles")
+    arr = np.array([(r.id, *(getattr(r, f) for f in _TRI2_FIELDS)) for r in records],
+                   dtype=[("id", "<u4")] + [(f, "<f8") for f in _TRI2_FIELDS])
+    return _pack(TRIANGLE2D_DTYPE, {k: arr[k] for k in arr.dtype.names})
+
+
+def pack_rays_2d(payload, *, name: str = "rays") -> np.ndarray:
+    """Pack 2-D rays into packed RtdlRay2D records (44 bytes)."""
+    cols = _columns(payload, ("id", *_RAY2_FIELDS))
+    if cols is not None:
+        return _pack(RAY2D_DTYPE, cols)
+    records = normalize_records(name, "rays", payload)
+    if any(not isinstance(r, Ray2D) for r in records):
+        raise ValueError(f"input `{name}` mixes 2-D and 3-D rays")
+    arr = np.array([(r.id, *(getattr(r, f) for f in _RAY2_FIELDS)) for r in records],
+                   dtype=[("id", "<u4")] + [(f, "<f8") for f in _RAY2_FIELDS])
+    return _pack(RAY2D_DTYPE, {k: arr[k] for k in arr.dtype.names})
+
+
+def _ray_input_dim(payload, geometry: str, name: str) -> int:
+    """2 for Ray2D / Triangle payloads, 3 for Ray3D / Triangle3D ones (or empty)."""
+    if isinstance(payload, np.ndarray) and payload.dtype.names:
+        names = set(payload.dtype.names)
+    elif isinstance(payload, Mapping):
+        names = set(payload)
+    else:
+        records = normalize_records(name, geometry, payload)
+        if not records:
+            return 3
+        return 2 if isinstance(records[0], (Ray2D, Triangle)) else 3
+    return 3 if ("oz" in names or "z0" in names) else 2
+
+
 def pack_polygons(payload, *, name: str = "polygons") -> tuple[np.ndarray, np.ndarray]:
     """Return (RtdlPolygonRef records, interleaved FP64 vertices)."""
     if isinstance(payload, Mapping) and {"refs", "vertices_xy"} <= set(payload):
@@ -422,6 +471,16 @@ def knn_rows_mi355x(query_points, search_points, *, k: int) -> np.ndarray:
 
 
 def ray_triangle_mi355x(rays, triangles, *, mode: str = "any_hit") -> np.ndarray:
+    rdim = _ray_input_dim(rays, "rays", "rays")
+    tdim = _ray_input_dim(triangles, "triangles", "triangles")
+    if rdim == 2 or tdim == 2:
+        if mode != "any_hit":
+            raise NotImplementedError(
+                f"MI355X 2-D rays support only any_hit, not {mode}; No CPU fallback is used."
+            )
+        r2 = pack_rays_2d(rays)
+        t2 = pack_triangles_2d(triangles)
+        return _call_rows("rtdl_mi355x_run_ray_anyhit_2d", ANYHIT_ROW_DTYPE, _ptr(r2), len(r2), _ptr(t2), len(t2))
     r = pack_rays(rays)
     t = pack_triangles(triangles)
     name, dtype = {
@@ -494,8 +553,13 @@ def _validate_kernel(compiled: CompiledKernel) -> tuple[str, tuple[str, str], di
         raise ValueError(f"MI355X {predicate} requires ray probe input followed by triangle build input")
     if left.role != "probe" or right.role != "build":
         raise ValueError(f"MI355X {predicate} requires rays role='probe' and triangles role='build'")
-    if (left.layout.name, right.layout.name) != ("Ray3D", "Triangle3D"):
-        raise _unsupported(predicate, "only Ray3D/Triangle3D layouts are implemented")
+    pair = (left.layout.name, right.layout.name)
+    if pair == ("Ray2D", "Triangle2D") and predicate == "ray_triangle_any_hit":
+        options["layout_pair"] = pair
+        return predicate, (left.name, right.name), options
+    if pair != ("Ray3D", "Triangle3D"):
+        raise _unsupported(predicate, "only Ray3D/Triangle3D layouts (and Ray2D/Triangle2D any-hit) are implemented")
+    options["layout_pair"] = pair
     return predicate, (left.name, right.name), options
 
 
@@ -615,6 +679,10 @@ class PreparedMi355xKernel:
             positive = opts.get("result_mode", "full_matrix") == "positive_hits"
             return _call_rows("rtdl_mi355x_run_prepared_point_in_polygon", PIP_ROW_DTYPE, self.handle,
                               _ptr(pts), len(pts), int(positive))
+        if self.dim == 2:
+            rays2 = pack_rays_2d(payload, name=self.probe_name)
+            return _call_rows("rtdl_mi355x_run_prepared_ray_anyhit_2d", ANYHIT_ROW_DTYPE, self.handle,
+                              _ptr(rays2), len(rays2))
         rays = pack_rays(payload, name=self.probe_name)
         name, dtype = {
             "ray_triangle_any_hit": ("rtdl_mi355x_run_prepared_ray_anyhit_3d", ANYHIT_ROW_DTYPE),
@@ -654,8 +722,13 @@ def prepare_mi355x(kernel_fn_or_compiled, *, result_mode: str = "dict", **inputs
         handle = _call_prepare("rtdl_mi355x_prepare_polygons", _ptr(refs), len(refs), _ptr(verts), len(verts))
         dim = 2
     else:
-        tris = pack_triangles(payload, name=build)
-        handle = _call_prepare("rtdl_mi355x_prepare_triangles_3d", _ptr(tris), len(tris))
+        if options["layout_pair"] == ("Ray2D", "Triangle2D"):
+            tris2 = pack_triangles_2d(payload, name=build)
+            handle = _call_prepare("rtdl_mi355x_prepare_triangles_2d", _ptr(tris2), len(tris2))
+            dim = 2
+        else:
+            tris = pack_triangles(payload, name=build)
+            handle = _call_prepare("rtdl_mi355x_prepare_triangles_3d", _ptr(tris), len(tris))
     return PreparedMi355xKernel(compiled, predicate, probe, build, options, handle, dim, result_mode)
 
 
@@ -678,7 +751,9 @@ __all__ = [
     "pack_points",
     "pack_polygons",
     "pack_rays",
+    "pack_rays_2d",
     "pack_triangles",
+    "pack_triangles_2d",
     "point_in_polygon_mi355x",
     "prepare_mi355x",
     "ray_triangle_mi355x",

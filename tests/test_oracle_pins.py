@@ -1,7 +1,10 @@
 """Pins the oracle before it is trusted as the checker.
 
 * The reference's own authored cases and their hard-coded expectations
-  (tests/goal198, goal204, goal548, goal552, goal632 in the reference).
+  (tests/goal198, goal204, goal548, goal552, goal632, goal633 in the reference).
+* 2-D any-hit against an independent separating-axis decision on random
+  segments (both signs of tmax) and triangles, plus special cases (a segment
+  inside a triangle, an end on an edge, unbounded and negative tmax).
 * Closed forms a plausible mistake would break: inclusive radius boundary,
   truncation to k_max, (distance, id) tie order, grouping by query id, no
   KNN padding, KNN rows as a superset of FRN rows at the k-th distance,
@@ -239,3 +242,124 @@ def test_libm_pow_is_not_the_correctly_rounded_square():
     xs = rng.random(200000).tolist()
     diffs = sum(1 for x in xs if math.pow(x, 2.0) != x * x)
     assert diffs < len(xs) * 0.01
+
+
+# ------------------------------------------------ 2-D any-hit and visibility --
+
+
+def _sat_segment_triangle(seg, tri, margin=1e-6):
+    """Separating-axis decision for a segment and a triangle in the plane:
+    1 intersect, 0 disjoint, None when some axis is within `margin` of touching.
+    Independent of the reference's barycentric / crossing formulation."""
+    (ax, ay), (bx, by) = seg
+    verts = np.array(tri).reshape(3, 2)
+    sv = np.array([[ax, ay], [bx, by]])
+    axes = [np.array([-(by - ay), bx - ax])]
+    for i in range(3):
+        e = verts[(i + 1) % 3] - verts[i]
+        axes.append(np.array([-e[1], e[0]]))
+    verdict = 1
+    for n in axes:
+        n = n / np.linalg.norm(n)
+        tp, sp = verts @ n, sv @ n
+        gap = max(tp.min() - sp.max(), sp.min() - tp.max())
+        if abs(gap) < margin:
+            return None
+        if gap > 0:
+            verdict = 0
+    return verdict
+
+
+def _random_2d_case(rng, n_rays, n_tris):
+    rays = []
+    for i in range(n_rays):
+        ox, oy = rng.random(2)
+        ang = rng.uniform(0, 2 * math.pi)
+        tmax = rng.choice([rng.uniform(0.02, 0.4), -rng.uniform(0.02, 0.4)])
+        rays.append(Ray2D(i, float(ox), float(oy), math.cos(ang), math.sin(ang), float(tmax)))
+    tris = []
+    while len(tris) < n_tris:
+        c = rng.random(2)
+        v = c + rng.normal(0, 0.08, (3, 2))
+        area = abs((v[1, 0] - v[0, 0]) * (v[2, 1] - v[0, 1]) - (v[2, 0] - v[0, 0]) * (v[1, 1] - v[0, 1])) / 2
+        if area > 1e-3:
+            tris.append(Triangle(len(tris) + 100, *map(float, v.reshape(-1))))
+    return tuple(rays), tuple(tris)
+
+
+def test_2d_any_hit_matches_separating_axes():
+    """Random segments (both signs of tmax) against random triangles: every
+    pair that the separating-axis test decides clearly has the same verdict in
+    the Python and C restatements."""
+    rng = np.random.default_rng(633)
+    rays, tris = _random_2d_case(rng, 60, 80)
+    rm = np.array([[r.ox, r.oy, r.dx, r.dy, r.tmax] for r in rays])
+    decided = hits = 0
+    for r, row in zip(rays, rm):
+        seg = ((r.ox, r.oy), (r.ox + r.dx * r.tmax, r.oy + r.dy * r.tmax))
+        for t in tris:
+            want = _sat_segment_triangle(seg, (t.x0, t.y0, t.x1, t.y1, t.x2, t.y2))
+            if want is None:
+                continue
+            decided += 1
+            hits += want
+            got_py = ref.ray_triangle_any_hit_cpu((r,), (t,))[0]["any_hit"]
+            tm = np.array([[t.x0, t.y0, t.x1, t.y1, t.x2, t.y2]])
+            got_c = int(native.rays_2d_any(row[None], tm, brute=True)[0])
+            assert got_py == got_c == want, (r, t)
+    assert decided > 4000 and 50 < hits < decided - 50
+
+
+def test_2d_any_hit_special_cases():
+    tri = Triangle(7, 0.0, 0.0, 4.0, 0.0, 0.0, 4.0)
+    cases_ = [
+        (Ray2D(1, 0.5, 0.5, 1.0, 0.0, 1.0), 1),     # segment wholly inside: no edge crossing
+        (Ray2D(2, 5.0, 5.0, 1.0, 1.0, 3.0), 0),     # outside, pointing away
+        (Ray2D(3, 5.0, 1.0, 1.0, 0.0, -3.0), 1),    # negative tmax reaches back across the hypotenuse
+        (Ray2D(4, 5.0, 1.0, 1.0, 0.0, 3.0), 0),     # the same ray forward misses
+        (Ray2D(5, 1.0, 1.0, 0.0, 0.0, 0.0), 1),     # a point inside
+        (Ray2D(6, -1.0, 1.0, 1.0, 0.0, 0.5), 0),    # stops before the left edge
+        (Ray2D(7, -1.0, 1.0, 1.0, 0.0, 1.0), 1),    # ends on the left edge (inclusive)
+        (Ray2D(8, 1.0, 1.0, 1.0, 0.0, math.inf), 1),   # origin inside, unbounded
+        (Ray2D(9, -1.0, 1.0, 1.0, 0.0, math.inf), 0),  # outside origin, unbounded: no finite crossing
+    ]
+    for ray, want in cases_:
+        assert ref.ray_triangle_any_hit_cpu((ray,), (tri,))[0]["any_hit"] == want, ray
+        rm = np.array([[ray.ox, ray.oy, ray.dx, ray.dy, ray.tmax]])
+        tm = np.array([[tri.x0, tri.y0, tri.x1, tri.y1, tri.x2, tri.y2]])
+        for brute in (True, False):
+            assert int(native.rays_2d_any(rm, tm, brute=brute)[0]) == want, (ray, brute)
+
+
+@pytest.mark.parametrize("brute", [False, True])
+def test_c_oracle_matches_python_oracle_rays_2d(brute):
+    rng = np.random.default_rng(634)
+    rays, tris = _random_2d_case(rng, 300, 400)
+    rm = np.array([[r.ox, r.oy, r.dx, r.dy, r.tmax] for r in rays])
+    tm = np.array([[t.x0, t.y0, t.x1, t.y1, t.x2, t.y2] for t in tris])
+    want = [x["any_hit"] for x in ref.ray_triangle_any_hit_cpu(rays, tris)]
+    assert native.rays_2d_any(rm, tm, brute=brute).tolist() == want
+    assert 0 < sum(want) < len(want)
+
+
+def test_visibility_authored_goal633():
+    o, t = (Point(1, 0.0, 0.0),), (Point(2, 10.0, 0.0),)
+    assert ref.visibility_rows_cpu(o, t, (Triangle(10, 5.0, -1.0, 5.0, 1.0, 6.0, 0.0),)) == \
+        ({"observer_id": 1, "target_id": 2, "visible": 0},)
+    assert ref.visibility_rows_cpu(o, t, (Triangle(11, 12.0, -1.0, 12.0, 1.0, 13.0, 0.0),)) == \
+        ({"observer_id": 1, "target_id": 2, "visible": 1},)
+    obs = (Point(1, 0.0, 0.0), Point(2, 0.0, 2.0))
+    tgt = (Point(10, 10.0, 0.0), Point(11, 10.0, 2.0))
+    assert ref.visibility_rows_cpu(obs, tgt, ()) == tuple(
+        {"observer_id": a, "target_id": b, "visible": 1} for a in (1, 2) for b in (10, 11))
+    o3, t3 = (Point3D(1, 0.0, 0.0, 0.0),), (Point3D(2, 10.0, 0.0, 0.0),)
+    between = (Triangle3D(20, 5.0, -1.0, -1.0, 5.0, 1.0, -1.0, 5.0, 0.0, 1.0),)
+    behind = (Triangle3D(21, 12.0, -1.0, -1.0, 12.0, 1.0, -1.0, 12.0, 0.0, 1.0),)
+    assert ref.visibility_rows_cpu(o3, t3, between)[0]["visible"] == 0
+    assert ref.visibility_rows_cpu(o3, t3, behind)[0]["visible"] == 1
+    with pytest.raises(ValueError, match="one dimensionality"):
+        ref.visibility_rows_cpu((Point(1, 0.0, 0.0),), (Point3D(2, 1.0, 0.0, 0.0),), ())
+    with pytest.raises(ValueError, match="distinct observer and target"):
+        ref.visibility_rows_cpu((Point(1, 0.0, 0.0),), (Point(2, 0.0, 0.0),), ())
+    with pytest.raises(ValueError, match="blocker triangles must match"):
+        ref.visibility_rows_cpu((Point(1, 0.0, 0.0),), (Point(2, 1.0, 0.0),), between)
