@@ -504,16 +504,27 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
   // (reference.py:844).  2-D takes any tmax: the segment o -> o + d * tmax
   // is tested as it is (reference.py:783-803).
   if ((MODE != 3 && !(tmax >= 0.0)) || root == kEndNode) return res;
-  // A segment with a non-finite end (2-D) or an unbounded ray is tested
-  // against every primitive: the box prefilter is switched off (pad = inf).
-  const bool bounded = MODE == 3 ? (fabs(dx * tmax) < __builtin_inf() && fabs(dy * tmax) < __builtin_inf())
-                                  : tmax < __builtin_inf();
+  // The prefilter walks the segment o -> o + d * seg_t.  A 3-D ray with an
+  // infinite tmax can only hit inside the scene's box [-M, M]^3, where
+  // |o + t d| <= M bounds t by (M + |o|) / |d| (max norms); the walk takes
+  // four times that, so the FP64 hits of the reference all lie on it.  A
+  // segment with a non-finite end (2-D, or a zero or NaN direction) is
+  // tested against every primitive: the box prefilter is switched off
+  // (pad = inf).
+  double seg_t = tmax;
+  if (MODE != 3 && !(tmax < __builtin_inf())) {
+    const double dn = fmax(fmax(fabs(dx), fabs(dy)), fabs(dz));
+    const double on = fmax(fmax(fabs(ox), fabs(oy)), fabs(oz));
+    seg_t = 4.0 * (((double)scene_m(s) + on) / dn);
+  }
+  const bool bounded = fabs(dx * seg_t) < __builtin_inf() && fabs(dy * seg_t) < __builtin_inf() &&
+                       fabs(dz * seg_t) < __builtin_inf();
   const float o32[3] = {(float)ox, (float)oy, (float)oz};
   float D32[3] = {0.0f, 0.0f, 0.0f};
   float inv[3] = {0.0f, 0.0f, 0.0f};
   float pad = __builtin_inff();
   if (bounded) {
-    const double D[3] = {dx * tmax, dy * tmax, dz * tmax};
+    const double D[3] = {dx * seg_t, dy * seg_t, dz * seg_t};
     const double omax = fmax(fmax(fabs(ox), fabs(oy)), fabs(oz));
     const double dmax = fmax(fmax(fabs(D[0]), fabs(D[1])), fabs(D[2]));
     pad = __double2float_ru(((double)scene_m(s) + omax + dmax) * 3.814697265625e-06);  // 2^-18
@@ -571,7 +582,7 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
         if (!(res.best_t == res.best_t) || th < res.best_t || (th == res.best_t && ident < res.best_id)) {
           res.best_t = th;
           res.best_id = ident;
-          if (bounded && tmax > 0.0) slim = fminf(1.0f, __double2float_ru(th / tmax));
+          if (bounded && seg_t > 0.0) slim = fminf(1.0f, __double2float_ru(th / seg_t));
         }
       }
     }

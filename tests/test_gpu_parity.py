@@ -266,6 +266,37 @@ def test_rays_random_bit_exact(gpu_ready, fp32, mode):
         assert np.array_equal(got["t"], tt[has])
 
 
+@pytest.mark.parametrize("mode", ["any", "count", "closest"])
+def test_unbounded_rays_bit_exact(gpu_ready, mode):
+    """tmax = inf: the walk bounds the ray by the scene box, the FP64 test
+    still takes every t >= 0.  Origins inside and outside the scene, on a
+    scene scaled and shifted away from the origin."""
+    _say(f"case: unbounded rays mode={mode}")
+    rng = np.random.default_rng(3100)
+    t = cases.triangles(rng, 20000, sigma=0.01, fp32=False)
+    for f in cases.TRI_FIELDS:
+        t[f] = t[f] * 1000.0 + 250.0
+    r = cases.rays(rng, 6000, tmax=np.inf, fp32=False)
+    r["tmax"][:] = np.inf
+    for f, base in (("ox", 250.0), ("oy", 250.0), ("oz", 250.0)):
+        r[f] = r[f] * 1000.0 + base
+    r["ox"][:2000] = -3000.0  # outside the scene, some pointing at it
+    r["dx"][:1000] = np.abs(r["dx"][:1000]) + 0.5
+    want = oracle.rays_3d(mode, cases.ray_matrix(r), cases.tri_matrix(t), t["id"])
+    api_mode = {"any": "any_hit", "count": "hit_count", "closest": "closest_hit"}[mode]
+    got = m.ray_triangle_mi355x(r, t, mode=api_mode)
+    if mode == "any":
+        assert np.array_equal(got["any_hit"], want)
+        assert 500 < int(want.sum()) < len(want) - 500
+    elif mode == "count":
+        assert np.array_equal(got["hit_count"], want)
+    else:
+        tid, tt, has = want
+        assert np.array_equal(got["ray_id"], r["id"][has])
+        assert np.array_equal(got["triangle_id"], tid[has])
+        assert np.array_equal(got["t"], tt[has])
+
+
 def test_ray_edge_cases(gpu_ready):
     _say("case: ray edge cases (axis-aligned, zero tmax, negative tmax, edge hits)")
     tris = {"id": np.array([7, 8], np.uint32)}

@@ -261,8 +261,14 @@ def ray_anyhit_counts(sc: dict, ray, *, instance: int = 0) -> tuple[int, int]:
         return 0, 0
     o32 = (F32(ox), F32(oy), F32(oz))
     d32, inv, pad = [F32(0)] * 3, [F32(0)] * 3, F32(np.inf)
-    if tmax < math.inf:
-        dd = (dx * tmax, dy * tmax, dz * tmax)
+    seg_t = tmax
+    if not tmax < math.inf:
+        # an unbounded ray walks 4 (M + |o|) / |d| of its length (max norms)
+        dn = max(max(abs(dx), abs(dy)), abs(dz))
+        on = max(max(abs(ox), abs(oy)), abs(oz))
+        seg_t = 4.0 * ((float(sc["m"]) + on) / dn) if dn > 0 else math.inf
+    if all(abs(v * seg_t) < math.inf for v in (dx, dy, dz)):
+        dd = (dx * seg_t, dy * seg_t, dz * seg_t)
         omax = max(max(abs(ox), abs(oy)), abs(oz))
         dmax = max(max(abs(dd[0]), abs(dd[1])), abs(dd[2]))
         pad = f32_ru((float(sc["m"]) + omax + dmax) * 3.814697265625e-06)
