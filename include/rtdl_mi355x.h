@@ -185,6 +185,21 @@ int rtdl_mi355x_run_ray_anyhit_2d(const RtdlRay2D* rays, size_t ray_count,
                                   RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
                                   char* error_out, size_t error_size);
 
+/* replaces rtdl_hiprt_run_ray_hitcount_2d (src/native/hiprt/rtdl_hiprt_api.cpp:536-553):
+ * one row per ray, the number of triangles the segment hits */
+int rtdl_mi355x_run_ray_hitcount_2d(const RtdlRay2D* rays, size_t ray_count,
+                                    const RtdlTriangle* triangles, size_t triangle_count,
+                                    RtdlRayHitCountRow** rows_out, size_t* row_count_out,
+                                    char* error_out, size_t error_size);
+
+/* 2-D closest hit as the reference's CPU path defines it (reference.py:248-266,
+ * 775-780; no HIPRT counterpart): every 2-D hit has t = 0.0, so a hit ray's row
+ * carries the lowest hit triangle id and t = 0.0; rays without hits emit no row */
+int rtdl_mi355x_run_ray_closest_hit_2d(const RtdlRay2D* rays, size_t ray_count,
+                                       const RtdlTriangle* triangles, size_t triangle_count,
+                                       RtdlRayClosestHitRow** rows_out, size_t* row_count_out,
+                                       char* error_out, size_t error_size);
+
 /* replaces rtdl_hiprt_run_ray_hitcount_3d (src/native/hiprt/rtdl_hiprt_api.cpp:183-286) */
 int rtdl_mi355x_run_ray_hitcount_3d(const RtdlRay3D* rays, size_t ray_count,
                                     const RtdlTriangle3D* triangles, size_t triangle_count,
@@ -240,6 +255,13 @@ int rtdl_mi355x_prepare_triangles_2d(const RtdlTriangle* triangles, size_t trian
 int rtdl_mi355x_run_prepared_ray_anyhit_2d(void* prepared, const RtdlRay2D* rays, size_t ray_count,
                                            RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
                                            char* error_out, size_t error_size);
+/* prepared forms of rtdl_mi355x_run_ray_hitcount_2d / _closest_hit_2d above */
+int rtdl_mi355x_run_prepared_ray_hitcount_2d(void* prepared, const RtdlRay2D* rays, size_t ray_count,
+                                             RtdlRayHitCountRow** rows_out, size_t* row_count_out,
+                                             char* error_out, size_t error_size);
+int rtdl_mi355x_run_prepared_ray_closest_hit_2d(void* prepared, const RtdlRay2D* rays, size_t ray_count,
+                                                RtdlRayClosestHitRow** rows_out, size_t* row_count_out,
+                                                char* error_out, size_t error_size);
 
 /* replaces rtdl_hiprt_prepare_ray_hitcount_3d / _run_prepared_ray_hitcount_3d
  * (src/native/hiprt/rtdl_hiprt_api.cpp:15-79); the prepared triangle scene serves

@@ -129,6 +129,13 @@ def rays_3d(mode: str, rays: np.ndarray, tris: np.ndarray, tri_ids: np.ndarray, 
 def rays_2d_any(rays: np.ndarray, tris: np.ndarray, *, threads: int | None = None,
                 brute: bool = False) -> np.ndarray:
     """2-D any-hit flags.  rays: (n,5) ox oy dx dy tmax; tris: (m,6) x0 y0 x1 y1 x2 y2."""
+    return rays_2d("any", rays, tris, np.arange(len(tris), dtype=np.uint32), threads=threads, brute=brute)
+
+
+def rays_2d(mode: str, rays: np.ndarray, tris: np.ndarray, tri_ids: np.ndarray, *,
+            threads: int | None = None, brute: bool = False):
+    """2-D rays: (n,5) ox oy dx dy tmax; tris: (m,6).  mode 'any'|'count'|'closest'
+    (closest: the lowest hit id, t = 0.0), returned as rays_3d does."""
     r2 = _f64(rays).reshape(-1, 5)
     t2 = _f64(tris).reshape(-1, 6)
     r = np.zeros((r2.shape[0], 7))
@@ -139,11 +146,14 @@ def rays_2d_any(rays: np.ndarray, tris: np.ndarray, *, threads: int | None = Non
     out_u = np.zeros(n, dtype=np.uint32)
     out_t = np.zeros(n, dtype=np.float64)
     out_h = np.zeros(n, dtype=np.uint8)
-    ids = np.arange(t.shape[0], dtype=np.uint32)
+    ids = _u32(tri_ids)
     library().oracle_rays_3d(
-        3, n, _dp(r), t.shape[0], _dp(t), _up(ids), int(threads or default_threads()), int(bool(brute)),
+        {"any": 3, "count": 4, "closest": 5}[mode], n, _dp(r), t.shape[0], _dp(t), _up(ids),
+        int(threads or default_threads()), int(bool(brute)),
         _up(out_u), _dp(out_t), out_h.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
     )
+    if mode == "closest":
+        return out_u, out_t, out_h.astype(bool)
     return out_u
 
 

@@ -544,7 +544,7 @@ typedef struct {
   const double* rays;  /* nr x 7: ox oy oz dx dy dz tmax */
   const double* tris;  /* nt x 9 */
   const uint32_t* tid; /* nt */
-  int mode;            /* 0 any, 1 count, 2 closest, 3 2-D any */
+  int mode;            /* 0 any, 1 count, 2 closest; 3, 4, 5: the same in 2-D */
   int brute;
   grid_t grid;
   double pad;
@@ -589,7 +589,7 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
     double seg[7];
     int brute = c->brute;
     const double* rg = r;
-    if (c->mode == 3) {
+    if (c->mode >= 3) {
       seg[0] = r[0];
       seg[1] = r[1];
       seg[2] = 0.0;
@@ -609,7 +609,7 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
   do {                                                                                    \
     double t_ = 0.0;                                                                      \
     const double* v_ = c->tris + (size_t)(TRI) * 9;                                       \
-    if (c->mode == 3 ? hits_2d(r, v_) : hit_t_3d(r, v_, &t_)) {                           \
+    if (c->mode >= 3 ? hits_2d(r, v_) : hit_t_3d(r, v_, &t_)) {                           \
       any = 1;                                                                            \
       ++count;                                                                            \
       uint32_t id_ = c->tid[TRI];                                                         \
@@ -666,7 +666,7 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
 #undef VISIT
     if (c->mode == 0 || c->mode == 3)
       c->out_u[i] = (uint32_t)any;
-    else if (c->mode == 1)
+    else if (c->mode == 1 || c->mode == 4)
       c->out_u[i] = count;
     else {
       c->out_u[i] = best_id;
@@ -677,7 +677,8 @@ static void ray_range(void* p, size_t b, size_t e, int worker) {
 }
 
 /* mode 0: out_u = any_hit; 1: out_u = hit_count; 2: closest (out_u id, out_t, out_has);
- * 3: out_u = 2-D any_hit (rays ox oy 0 dx dy 0 tmax, triangles with z = 0). */
+ * 3, 4, 5: the same for 2-D (rays ox oy 0 dx dy 0 tmax, triangles with z = 0;
+ * a 2-D hit has t = 0.0, reference.py:775-780). */
 int oracle_rays_3d(int mode, size_t nr, const double* rays, size_t nt, const double* tris,
                    const uint32_t* tid, int threads, int brute, uint32_t* out_u, double* out_t,
                    uint8_t* out_has) {
@@ -738,7 +739,7 @@ int oracle_rays_3d(int mode, size_t nr, const double* rays, size_t nt, const dou
     grid_build(&c.grid, 3, nt, bmin, bmax, want);
     free(bmin);
     free(bmax);
-    if (mode != 0) {
+    if (mode != 0 && mode != 3) {
       c.stamp = (uint32_t**)calloc((size_t)threads, sizeof(uint32_t*));
       for (int w = 0; w < threads; ++w) c.stamp[w] = (uint32_t*)calloc(nt, sizeof(uint32_t));
     }

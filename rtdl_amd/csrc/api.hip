@@ -209,18 +209,41 @@ int rtdl_mi355x_run_ray_anyhit_3d(const RtdlRay3D* rays, size_t ray_count,
                        row_count_out, error_out, error_size);
 }
 
+static int rays_2d_per_call(const RtdlRay2D* rays, size_t ray_count, const RtdlTriangle* triangles,
+                            size_t triangle_count, int mode, void** rows_out, size_t* row_count_out,
+                            char* error_out, size_t error_size) {
+  static_assert(sizeof(RtdlRay2D) == 44 && sizeof(RtdlTriangle) == 56, "reference record layouts");
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, row_count_out);
+    require((ray_count == 0 || rays != nullptr) && (triangle_count == 0 || triangles != nullptr),
+            "input pointers must not be null when counts are nonzero");
+    per_call([&](Prepared& p) { prepare_triangles_2d(p, triangles, triangle_count); },
+             [&](Prepared& p) { run_rays_2d(p, rays, ray_count, mode, rows_out, row_count_out); });
+  });
+}
+
 int rtdl_mi355x_run_ray_anyhit_2d(const RtdlRay2D* rays, size_t ray_count,
                                   const RtdlTriangle* triangles, size_t triangle_count,
                                   RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
                                   char* error_out, size_t error_size) {
-  static_assert(sizeof(RtdlRay2D) == 44 && sizeof(RtdlTriangle) == 56, "reference record layouts");
-  return guarded(error_out, error_size, [&] {
-    zero_outputs((void**)rows_out, row_count_out);
-    require((ray_count == 0 || rays != nullptr) && (triangle_count == 0 || triangles != nullptr),
-            "input pointers must not be null when counts are nonzero");
-    per_call([&](Prepared& p) { prepare_triangles_2d(p, triangles, triangle_count); },
-             [&](Prepared& p) { run_rays_2d(p, rays, ray_count, (void**)rows_out, row_count_out); });
-  });
+  return rays_2d_per_call(rays, ray_count, triangles, triangle_count, 0, (void**)rows_out, row_count_out,
+                          error_out, error_size);
+}
+
+int rtdl_mi355x_run_ray_hitcount_2d(const RtdlRay2D* rays, size_t ray_count,
+                                    const RtdlTriangle* triangles, size_t triangle_count,
+                                    RtdlRayHitCountRow** rows_out, size_t* row_count_out,
+                                    char* error_out, size_t error_size) {
+  return rays_2d_per_call(rays, ray_count, triangles, triangle_count, 1, (void**)rows_out, row_count_out,
+                          error_out, error_size);
+}
+
+int rtdl_mi355x_run_ray_closest_hit_2d(const RtdlRay2D* rays, size_t ray_count,
+                                       const RtdlTriangle* triangles, size_t triangle_count,
+                                       RtdlRayClosestHitRow** rows_out, size_t* row_count_out,
+                                       char* error_out, size_t error_size) {
+  return rays_2d_per_call(rays, ray_count, triangles, triangle_count, 2, (void**)rows_out, row_count_out,
+                          error_out, error_size);
 }
 
 int rtdl_mi355x_run_ray_hitcount_3d(const RtdlRay3D* rays, size_t ray_count,
@@ -331,13 +354,33 @@ int rtdl_mi355x_prepare_triangles_2d(const RtdlTriangle* triangles, size_t trian
   });
 }
 
+static int rays_2d_prepared(void* prepared, const RtdlRay2D* rays, size_t nr, int mode, void** rows_out,
+                            size_t* n_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, n_out);
+    on_prepared(prepared, [&](Prepared& p) { run_rays_2d(p, rays, nr, mode, rows_out, n_out); });
+  });
+}
+
 int rtdl_mi355x_run_prepared_ray_anyhit_2d(void* prepared, const RtdlRay2D* rays, size_t ray_count,
                                            RtdlRayAnyHitRow** rows_out, size_t* row_count_out,
                                            char* error_out, size_t error_size) {
-  return guarded(error_out, error_size, [&] {
-    zero_outputs((void**)rows_out, row_count_out);
-    on_prepared(prepared, [&](Prepared& p) { run_rays_2d(p, rays, ray_count, (void**)rows_out, row_count_out); });
-  });
+  return rays_2d_prepared(prepared, rays, ray_count, 0, (void**)rows_out, row_count_out, error_out,
+                          error_size);
+}
+
+int rtdl_mi355x_run_prepared_ray_hitcount_2d(void* prepared, const RtdlRay2D* rays, size_t ray_count,
+                                             RtdlRayHitCountRow** rows_out, size_t* row_count_out,
+                                             char* error_out, size_t error_size) {
+  return rays_2d_prepared(prepared, rays, ray_count, 1, (void**)rows_out, row_count_out, error_out,
+                          error_size);
+}
+
+int rtdl_mi355x_run_prepared_ray_closest_hit_2d(void* prepared, const RtdlRay2D* rays, size_t ray_count,
+                                                RtdlRayClosestHitRow** rows_out, size_t* row_count_out,
+                                                char* error_out, size_t error_size) {
+  return rays_2d_prepared(prepared, rays, ray_count, 2, (void**)rows_out, row_count_out, error_out,
+                          error_size);
 }
 
 static int rays_prepared(void* prepared, const RtdlRay3D* rays, size_t nr, int mode,

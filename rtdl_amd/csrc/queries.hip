@@ -112,9 +112,9 @@ __global__ __launch_bounds__(256) void ray_kernel(SceneView s, const double* __r
     const double* r = rays + 7 * (size_t)ri;
     const RayResult res =
         trace_ray<MODE, STATS>(s, s.root, s.end, r[0], r[1], r[2], r[3], r[4], r[5], r[6], st);
-    if (MODE == 0 || MODE == 3) out_u[ri] = res.hits ? 1u : 0u;
-    if (MODE == 1) out_u[ri] = res.hits;
-    if (MODE == 2) {
+    if ((MODE & 3) == 0) out_u[ri] = res.hits ? 1u : 0u;
+    if ((MODE & 3) == 1) out_u[ri] = res.hits;
+    if ((MODE & 3) == 2) {
       out_u[ri] = res.best_id;
       out_t[ri] = res.best_t;
     }
@@ -278,10 +278,15 @@ void launch_rays(const SceneView& s, const double* rays, const uint32_t* order, 
                  int mode, uint32_t* out_u, double* out_t, unsigned long long* stats, int block,
                  hipStream_t stream) {
   if (!nr) return;
-  if (mode == 0) rays_t<0>(s, rays, order, nr, out_u, out_t, stats, block, stream);
-  else if (mode == 1) rays_t<1>(s, rays, order, nr, out_u, out_t, stats, block, stream);
-  else if (mode == 2) rays_t<2>(s, rays, order, nr, out_u, out_t, stats, block, stream);
-  else rays_t<3>(s, rays, order, nr, out_u, out_t, stats, block, stream);
+  switch (mode) {
+    case 0: rays_t<0>(s, rays, order, nr, out_u, out_t, stats, block, stream); break;
+    case 1: rays_t<1>(s, rays, order, nr, out_u, out_t, stats, block, stream); break;
+    case 2: rays_t<2>(s, rays, order, nr, out_u, out_t, stats, block, stream); break;
+    case 4: rays_t<4>(s, rays, order, nr, out_u, out_t, stats, block, stream); break;
+    case 5: rays_t<5>(s, rays, order, nr, out_u, out_t, stats, block, stream); break;
+    case 6: rays_t<6>(s, rays, order, nr, out_u, out_t, stats, block, stream); break;
+    default: throw Error("unknown ray mode " + std::to_string(mode));
+  }
 }
 
 void launch_pip(const SceneView& s, const double* pts, const uint32_t* order, uint32_t np,

@@ -47,7 +47,8 @@ void launch_collect_all(const SceneView& s, const double* q, const uint32_t* que
 // ---- rays ---------------------------------------------------------------
 // rays: 7 FP64 per ray (ox, oy, oz, dx, dy, dz, tmax).  mode 0: any-hit flag,
 // 1: hit count, 2: closest hit (triangle id in out_u, t in out_t, NaN t when
-// the ray hits nothing).
+// the ray hits nothing).  mode + 4: the same for 2-D rays (oz = dz = 0 and
+// triangles with z = 0), whose hits all have t = 0.
 void launch_rays(const SceneView& s, const double* rays, const uint32_t* order, uint32_t nr,
                  int mode, uint32_t* out_u, double* out_t, unsigned long long* stats, int block,
                  hipStream_t stream);

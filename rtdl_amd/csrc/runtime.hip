@@ -524,40 +524,13 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
 
 // -------------------------------------------------------------------- rays --
 
-void run_rays_2d(Prepared& p, const void* raw_rays, size_t nr_in, void** rows_out, size_t* n_out) {
-  require(p.type == SceneType::Triangles2D, "prepared handle is not a 2-D triangle scene");
-  require(nr_in == 0 || raw_rays != nullptr, "ray pointer must not be null when ray count is nonzero");
-  require(nr_in < 0xFFFFFFFFull, "too many rays");
-  PhaseClock& clock = thread_clock();
-  const uint32_t nr = (uint32_t)nr_in;
-  hipStream_t s = p.stream;
-  if (nr == 0) {
-    *rows_out = host_rows(0, 8);
-    *n_out = 0;
-    return;
-  }
-  auto bytes = upload_bytes(raw_rays, (size_t)nr * 44, s);
-  DeviceArray<double> rays(7 * (size_t)nr);
-  DeviceArray<uint32_t> rid(nr);
-  unpack_rays_2d(bytes.get(), nr, rays.get(), rid.get(), s);
-  auto order = traversal_order(rays.get(), 7, nr, p, s);
-  RTDL_HIP(hipStreamSynchronize(s));
-  clock.lap(0);
-  DeviceArray<uint32_t> out_u(nr);
-  launch_rays(p.scene.view(0), rays.get(), order.get(), nr, 3, out_u.get(), nullptr, nullptr, 256, s);
-  RTDL_HIP(hipStreamSynchronize(s));
-  clock.lap(2);
-  DeviceArray<uint8_t> rows((size_t)nr * 8);
-  emit_pair_rows(nr, rid.get(), out_u.get(), rows.get(), s);
-  RTDL_HIP(hipStreamSynchronize(s));
-  clock.lap(3);
-  copy_rows_back(rows.get(), nr, 8, rows_out, n_out, s);
-  clock.lap(4);
-}
-
-void run_rays(Prepared& p, const void* raw_rays, size_t nr_in, int mode, void** rows_out,
-              size_t* n_out) {
-  require(p.type == SceneType::Triangles3D, "prepared handle is not a triangle scene");
+// dim 3: 60-byte RtdlRay3D records against a Triangles3D scene; dim 2:
+// 44-byte RtdlRay2D records against a Triangles2D scene.
+static void run_rays_dim(Prepared& p, const void* raw_rays, size_t nr_in, int mode, int dim,
+                         void** rows_out, size_t* n_out) {
+  require(p.type == (dim == 2 ? SceneType::Triangles2D : SceneType::Triangles3D),
+          dim == 2 ? "prepared handle is not a 2-D triangle scene" : "prepared handle is not a triangle scene");
+  require(mode >= 0 && mode <= 2, "unknown ray mode");
   require(nr_in == 0 || raw_rays != nullptr, "ray pointer must not be null when ray count is nonzero");
   require(nr_in < 0xFFFFFFFFull, "too many rays");
   const size_t row_bytes = mode == 2 ? 16 : 8;
@@ -569,17 +542,20 @@ void run_rays(Prepared& p, const void* raw_rays, size_t nr_in, int mode, void** 
     *n_out = 0;
     return;
   }
-  auto bytes = upload_bytes(raw_rays, (size_t)nr * 60, s);
+  auto bytes = upload_bytes(raw_rays, (size_t)nr * (dim == 2 ? 44 : 60), s);
   DeviceArray<double> rays(7 * (size_t)nr);
   DeviceArray<uint32_t> rid(nr);
-  unpack_rays(bytes.get(), nr, rays.get(), rid.get(), s);
+  if (dim == 2)
+    unpack_rays_2d(bytes.get(), nr, rays.get(), rid.get(), s);
+  else
+    unpack_rays(bytes.get(), nr, rays.get(), rid.get(), s);
   auto order = traversal_order(rays.get(), 7, nr, p, s);
   RTDL_HIP(hipStreamSynchronize(s));
   clock.lap(0);
   DeviceArray<uint32_t> out_u(nr);
   DeviceArray<double> out_t(mode == 2 ? nr : 0);
-  launch_rays(p.scene.view(0), rays.get(), order.get(), nr, mode, out_u.get(), out_t.get(), nullptr,
-              256, s);
+  launch_rays(p.scene.view(0), rays.get(), order.get(), nr, dim == 2 ? mode + 4 : mode, out_u.get(),
+              out_t.get(), nullptr, 256, s);
   RTDL_HIP(hipStreamSynchronize(s));
   clock.lap(2);
   if (mode != 2) {
@@ -601,6 +577,14 @@ void run_rays(Prepared& p, const void* raw_rays, size_t nr_in, int mode, void** 
     copy_rows_back(rows.get(), total, 16, rows_out, n_out, s);
   }
   clock.lap(4);
+}
+
+void run_rays_2d(Prepared& p, const void* raw_rays, size_t nr, int mode, void** rows_out, size_t* n_out) {
+  run_rays_dim(p, raw_rays, nr, mode, 2, rows_out, n_out);
+}
+
+void run_rays(Prepared& p, const void* raw_rays, size_t nr, int mode, void** rows_out, size_t* n_out) {
+  run_rays_dim(p, raw_rays, nr, mode, 3, rows_out, n_out);
 }
 
 // --------------------------------------------------------------------- PIP --

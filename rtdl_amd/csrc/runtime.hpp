@@ -74,8 +74,9 @@ void prepare_polygon_instances(Prepared& p, const uint32_t* voff, const uint32_t
 enum class NeighborMode : int { FixedRadius = 0, Knn = 1, BoundedKnn = 2 };
 void run_neighbors(Prepared& p, const void* raw_queries, size_t nq, NeighborMode mode,
                    double radius, uint32_t k, void** rows_out, size_t* n_out);
-// RtdlRay2D records against a 2-D triangle scene: any-hit rows.
-void run_rays_2d(Prepared& p, const void* raw_rays, size_t nr, void** rows_out, size_t* n_out);
+// RtdlRay2D records against a 2-D triangle scene; mode 0 any-hit, 1 hit
+// count, 2 closest hit (rows as run_rays).
+void run_rays_2d(Prepared& p, const void* raw_rays, size_t nr, int mode, void** rows_out, size_t* n_out);
 void run_rays(Prepared& p, const void* raw_rays, size_t nr, int mode, void** rows_out,
               size_t* n_out);
 void run_pip(Prepared& p, const void* raw_points, size_t np, bool positive_only, void** rows_out,

@@ -493,17 +493,20 @@ struct RayResult {
   uint32_t best_id = 0;
 };
 
-// MODE 0 any-hit (stops at the first hit), 1 hit count, 2 closest hit,
-// 3 2-D any-hit (z of rays and triangles is 0; stops at the first hit).
+// MODE & 3: 0 any-hit (stops at the first hit), 1 hit count, 2 closest hit;
+// MODE & 4: 2-D (z of rays and triangles is 0, the hit test of
+// reference.py:783-803, hit t = 0.0 as reference.py:780 returns).
 template <int MODE, bool STATS>
 __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_t end, double ox,
                                       double oy, double oz, double dx, double dy, double dz,
                                       double tmax, LaneStats& st) {
+  constexpr int KIND = MODE & 3;
+  constexpr bool D2 = (MODE & 4) != 0;
   RayResult res;
   // 3-D: t in [0, tmax] is impossible for a negative or NaN tmax
   // (reference.py:844).  2-D takes any tmax: the segment o -> o + d * tmax
   // is tested as it is (reference.py:783-803).
-  if ((MODE != 3 && !(tmax >= 0.0)) || root == kEndNode) return res;
+  if ((!D2 && !(tmax >= 0.0)) || root == kEndNode) return res;
   // The prefilter walks the segment o -> o + d * seg_t.  A 3-D ray with an
   // infinite tmax can only hit inside the scene's box [-M, M]^3, where
   // |o + t d| <= M bounds t by (M + |o|) / |d| (max norms); the walk takes
@@ -512,7 +515,7 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
   // tested against every primitive: the box prefilter is switched off
   // (pad = inf).
   double seg_t = tmax;
-  if (MODE != 3 && !(tmax < __builtin_inf())) {
+  if (!D2 && !(tmax < __builtin_inf())) {
     const double dn = fmax(fmax(fabs(dx), fabs(dy)), fabs(dz));
     const double on = fmax(fmax(fabs(ox), fabs(oy)), fabs(oz));
     seg_t = 4.0 * (((double)scene_m(s) + on) / dn);
@@ -568,21 +571,22 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
         v[6] = c.x; v[7] = c.y; v[8] = c.z;
       }
       double th = 0.0;
-      if (MODE == 3) {
+      if (D2) {
         if (!hit_2d(ox, oy, dx, dy, tmax, v)) continue;
       } else if (!mt_hit(ox, oy, oz, dx, dy, dz, tmax, v, &th)) {
         continue;
       }
       ++res.hits;
-      if (MODE == 0 || MODE == 3) {
+      if (KIND == 0) {
         done = true;
         break;
       }
-      if (MODE == 2) {
+      if (KIND == 2) {
         if (!(res.best_t == res.best_t) || th < res.best_t || (th == res.best_t && ident < res.best_id)) {
           res.best_t = th;
           res.best_id = ident;
-          if (bounded && seg_t > 0.0) slim = fminf(1.0f, __double2float_ru(th / seg_t));
+          // 2-D hits all report t = 0: the segment cannot be shortened
+          if (!D2 && bounded && seg_t > 0.0) slim = fminf(1.0f, __double2float_ru(th / seg_t));
         }
       }
     }

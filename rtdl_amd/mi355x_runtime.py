@@ -136,6 +136,8 @@ _SIGNATURES = {
     "rtdl_mi355x_run_bounded_knn_rows_2d": ([_VP, _SZ, _VP, _SZ, ctypes.c_double, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_bounded_knn_rows_3d": ([_VP, _SZ, _VP, _SZ, ctypes.c_double, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_ray_anyhit_2d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_run_ray_hitcount_2d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_run_ray_closest_hit_2d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_ray_anyhit_3d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_ray_hitcount_3d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_ray_closest_hit_3d": ([_VP, _SZ, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
@@ -147,6 +149,8 @@ _SIGNATURES = {
     "rtdl_mi355x_run_prepared_bounded_knn_rows": ([_VP, _VP, _SZ, ctypes.c_double, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_prepare_triangles_2d": ([_VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_ray_anyhit_2d": ([_VP, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_run_prepared_ray_hitcount_2d": ([_VP, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_run_prepared_ray_closest_hit_2d": ([_VP, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_prepare_triangles_3d": ([_VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_ray_anyhit_3d": ([_VP, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_ray_hitcount_3d": ([_VP, _VP, _SZ, *_ROWS, *_ERR], ctypes.c_int),
@@ -378,8 +382,9 @@ def pack_rays_2d(payload, *, name: str = "rays") -> np.ndarray:
     return _pack(RAY2D_DTYPE, {k: arr[k] for k in arr.dtype.names})
 
 
-def _ray_input_dim(payload, geometry: str, name: str) -> int:
-    """2 for Ray2D / Triangle payloads, 3 for Ray3D / Triangle3D ones (or empty)."""
+def _ray_input_dim(payload, geometry: str, name: str) -> int | None:
+    """2 for Ray2D / Triangle payloads, 3 for Ray3D / Triangle3D ones, None for
+    an empty record sequence."""
     if isinstance(payload, np.ndarray) and payload.dtype.names:
         names = set(payload.dtype.names)
     elif isinstance(payload, Mapping):
@@ -387,7 +392,7 @@ def _ray_input_dim(payload, geometry: str, name: str) -> int:
     else:
         records = normalize_records(name, geometry, payload)
         if not records:
-            return 3
+            return None
         return 2 if isinstance(records[0], (Ray2D, Triangle)) else 3
     return 3 if ("oz" in names or "z0" in names) else 2
 
@@ -470,25 +475,30 @@ def knn_rows_mi355x(query_points, search_points, *, k: int) -> np.ndarray:
                       _ptr(search), len(search), int(k))
 
 
-def ray_triangle_mi355x(rays, triangles, *, mode: str = "any_hit") -> np.ndarray:
+_RAY_MODES = {
+    "any_hit": ("ray_anyhit", ANYHIT_ROW_DTYPE),
+    "hit_count": ("ray_hitcount", HITCOUNT_ROW_DTYPE),
+    "closest_hit": ("ray_closest_hit", CLOSEST_ROW_DTYPE),
+}
+
+
+def ray_triangle_mi355x(rays, triangles, *, mode: str = "any_hit", dim: int | None = None) -> np.ndarray:
+    """Per-call ray queries; `dim` (2 or 3) is taken from the inputs when not given."""
     rdim = _ray_input_dim(rays, "rays", "rays")
     tdim = _ray_input_dim(triangles, "triangles", "triangles")
-    if rdim == 2 or tdim == 2:
-        if mode != "any_hit":
-            raise NotImplementedError(
-                f"MI355X 2-D rays support only any_hit, not {mode}; No CPU fallback is used."
-            )
+    if mode not in _RAY_MODES:
+        raise ValueError(f"unknown ray mode {mode!r}")
+    fn, dtype = _RAY_MODES[mode]
+    dims = {d for d in (rdim, tdim, dim) if d is not None}
+    if len(dims) > 1:
+        raise ValueError("ray_triangle queries require rays and triangles to both be 2D or both be 3D")
+    if dims == {2}:
         r2 = pack_rays_2d(rays)
         t2 = pack_triangles_2d(triangles)
-        return _call_rows("rtdl_mi355x_run_ray_anyhit_2d", ANYHIT_ROW_DTYPE, _ptr(r2), len(r2), _ptr(t2), len(t2))
+        return _call_rows(f"rtdl_mi355x_run_{fn}_2d", dtype, _ptr(r2), len(r2), _ptr(t2), len(t2))
     r = pack_rays(rays)
     t = pack_triangles(triangles)
-    name, dtype = {
-        "any_hit": ("rtdl_mi355x_run_ray_anyhit_3d", ANYHIT_ROW_DTYPE),
-        "hit_count": ("rtdl_mi355x_run_ray_hitcount_3d", HITCOUNT_ROW_DTYPE),
-        "closest_hit": ("rtdl_mi355x_run_ray_closest_hit_3d", CLOSEST_ROW_DTYPE),
-    }[mode]
-    return _call_rows(name, dtype, _ptr(r), len(r), _ptr(t), len(t))
+    return _call_rows(f"rtdl_mi355x_run_{fn}_3d", dtype, _ptr(r), len(r), _ptr(t), len(t))
 
 
 def point_in_polygon_mi355x(points, polygons, *, positive_only: bool) -> np.ndarray:
@@ -554,11 +564,8 @@ def _validate_kernel(compiled: CompiledKernel) -> tuple[str, tuple[str, str], di
     if left.role != "probe" or right.role != "build":
         raise ValueError(f"MI355X {predicate} requires rays role='probe' and triangles role='build'")
     pair = (left.layout.name, right.layout.name)
-    if pair == ("Ray2D", "Triangle2D") and predicate == "ray_triangle_any_hit":
-        options["layout_pair"] = pair
-        return predicate, (left.name, right.name), options
-    if pair != ("Ray3D", "Triangle3D"):
-        raise _unsupported(predicate, "only Ray3D/Triangle3D layouts (and Ray2D/Triangle2D any-hit) are implemented")
+    if pair not in {("Ray3D", "Triangle3D"), ("Ray2D", "Triangle2D")}:
+        raise _unsupported(predicate, "only matching Ray3D/Triangle3D or Ray2D/Triangle2D layouts are implemented")
     options["layout_pair"] = pair
     return predicate, (left.name, right.name), options
 
@@ -602,7 +609,8 @@ def run_mi355x(kernel_fn_or_compiled, *, result_mode: str = "dict", **inputs):
     else:
         mode = {"ray_triangle_any_hit": "any_hit", "ray_triangle_hit_count": "hit_count",
                 "ray_triangle_closest_hit": "closest_hit"}[predicate]
-        rows = ray_triangle_mi355x(inputs[probe], inputs[build], mode=mode)
+        rows = ray_triangle_mi355x(inputs[probe], inputs[build], mode=mode,
+                                   dim=2 if options["layout_pair"][0] == "Ray2D" else 3)
     return _finish(compiled, rows, result_mode)
 
 
@@ -679,17 +687,10 @@ class PreparedMi355xKernel:
             positive = opts.get("result_mode", "full_matrix") == "positive_hits"
             return _call_rows("rtdl_mi355x_run_prepared_point_in_polygon", PIP_ROW_DTYPE, self.handle,
                               _ptr(pts), len(pts), int(positive))
-        if self.dim == 2:
-            rays2 = pack_rays_2d(payload, name=self.probe_name)
-            return _call_rows("rtdl_mi355x_run_prepared_ray_anyhit_2d", ANYHIT_ROW_DTYPE, self.handle,
-                              _ptr(rays2), len(rays2))
-        rays = pack_rays(payload, name=self.probe_name)
-        name, dtype = {
-            "ray_triangle_any_hit": ("rtdl_mi355x_run_prepared_ray_anyhit_3d", ANYHIT_ROW_DTYPE),
-            "ray_triangle_hit_count": ("rtdl_mi355x_run_prepared_ray_hitcount_3d", HITCOUNT_ROW_DTYPE),
-            "ray_triangle_closest_hit": ("rtdl_mi355x_run_prepared_ray_closest_hit_3d", CLOSEST_ROW_DTYPE),
-        }[p]
-        return _call_rows(name, dtype, self.handle, _ptr(rays), len(rays))
+        fn, dtype = _RAY_MODES[{"ray_triangle_any_hit": "any_hit", "ray_triangle_hit_count": "hit_count",
+                                "ray_triangle_closest_hit": "closest_hit"}[p]]
+        rays = (pack_rays_2d if self.dim == 2 else pack_rays)(payload, name=self.probe_name)
+        return _call_rows(f"rtdl_mi355x_run_prepared_{fn}_{self.dim}d", dtype, self.handle, _ptr(rays), len(rays))
 
     def run(self, **inputs):
         return _finish(self.compiled, self.run_raw(**inputs), self.result_mode)

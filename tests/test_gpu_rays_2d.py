@@ -34,6 +34,24 @@ def anyhit_2d():
     return rt.emit(h, fields=["ray_id", "any_hit"])
 
 
+@rt.kernel(backend="rtdl", precision="float_approx")
+def count_2d():
+    rays = rt.input("rays", rt.Rays, layout=rt.Ray2DLayout, role="probe")
+    tris = rt.input("triangles", rt.Triangles, layout=rt.Triangle2DLayout, role="build")
+    c = rt.traverse(rays, tris, accel="bvh")
+    h = rt.refine(c, predicate=rt.ray_triangle_hit_count(exact=False))
+    return rt.emit(h, fields=["ray_id", "hit_count"])
+
+
+@rt.kernel(backend="rtdl", precision="float_approx")
+def closest_2d():
+    rays = rt.input("rays", rt.Rays, layout=rt.Ray2DLayout, role="probe")
+    tris = rt.input("triangles", rt.Triangles, layout=rt.Triangle2DLayout, role="build")
+    c = rt.traverse(rays, tris, accel="bvh")
+    h = rt.refine(c, predicate=rt.ray_triangle_closest_hit(exact=False))
+    return rt.emit(h, fields=["ray_id", "triangle_id", "t"])
+
+
 def _tris_2d(rng, n: int, *, sigma: float = 0.002, fp32: bool = False) -> dict:
     c = rng.random((n, 1, 2))
     v = (c + rng.normal(0, sigma, (n, 3, 2))).reshape(n, 6)
@@ -70,6 +88,42 @@ def test_authored_2d_any_hit_goal632(gpu_ready):
     got = rt.run_mi355x(anyhit_2d, rays=rays, triangles=tris)
     assert got == ({"ray_id": 1, "any_hit": 1}, {"ray_id": 2, "any_hit": 0})
     assert got == ref.ray_triangle_any_hit_cpu(rays, tris)
+    counts = rt.run_mi355x(count_2d, rays=rays, triangles=tris)
+    assert [r["hit_count"] for r in counts] == [2, 0]
+    assert rt.run_mi355x(closest_2d, rays=rays, triangles=tris) == ({"ray_id": 1, "triangle_id": 10, "t": 0.0},)
+    assert rt.run_mi355x(anyhit_2d, rays=(), triangles=tris) == ()
+
+
+def _check_mode(got: np.ndarray, want, mode: str, ray_ids: np.ndarray) -> None:
+    if mode == "any":
+        assert np.array_equal(got["ray_id"], ray_ids)
+        assert np.array_equal(got["any_hit"], want), int((got["any_hit"] != want).sum())
+    elif mode == "count":
+        assert np.array_equal(got["ray_id"], ray_ids)
+        assert np.array_equal(got["hit_count"], want)
+    else:
+        tid, tt, has = want
+        assert np.array_equal(got["ray_id"], ray_ids[has])
+        assert np.array_equal(got["triangle_id"], tid[has])
+        assert np.array_equal(got["t"], tt[has])
+
+
+_API = {"any": "any_hit", "count": "hit_count", "closest": "closest_hit"}
+
+
+@pytest.mark.parametrize("mode", ["count", "closest"])
+def test_random_2d_count_and_closest_match_the_oracle(gpu_ready, mode):
+    _say(f"case: random 2-D {mode} 30000 rays x 40000 triangles")
+    rng = np.random.default_rng(6325)
+    t = _tris_2d(rng, 40000)
+    r = _rays_2d(rng, 30000)
+    r["tmax"][:4] = np.inf
+    r["tmax"][4:6] = np.nan
+    got = m.ray_triangle_mi355x(r, t, mode=_API[mode])
+    want = oracle.rays_2d(mode, _mat(r, _RN), _mat(t, _TN), t["id"])
+    _check_mode(got, want, mode, r["id"])
+    if mode == "count":
+        assert int(want.max()) > 3
 
 
 @pytest.mark.parametrize("fp32", [False, True])
@@ -117,6 +171,11 @@ def test_prepared_2d_any_hit_serves_several_batches(gpu_ready):
             want = oracle.rays_2d_any(_mat(r, _RN), _mat(t, _TN))
             assert np.array_equal(got["any_hit"], want)
             assert np.array_equal(got, m.ray_triangle_mi355x(r, t))
+    for mode, kernel in (("count", count_2d), ("closest", closest_2d)):
+        with rt.prepare_mi355x(kernel, result_mode="raw", triangles=tris) as prepared:
+            r = _rays_2d(rng, 4000)
+            got = prepared.run(rays=m.pack_rays_2d(r))
+            _check_mode(got, oracle.rays_2d(mode, _mat(r, _RN), _mat(t, _TN), t["id"]), mode, r["id"])
 
 
 def _random_points(rng, n: int, dim: int, first_id: int):

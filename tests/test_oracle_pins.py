@@ -335,11 +335,37 @@ def test_2d_any_hit_special_cases():
 def test_c_oracle_matches_python_oracle_rays_2d(brute):
     rng = np.random.default_rng(634)
     rays, tris = _random_2d_case(rng, 300, 400)
+    perm = rng.permutation(len(tris))
+    tris = tuple(Triangle(int(perm[i]) + 100, t.x0, t.y0, t.x1, t.y1, t.x2, t.y2) for i, t in enumerate(tris))
     rm = np.array([[r.ox, r.oy, r.dx, r.dy, r.tmax] for r in rays])
     tm = np.array([[t.x0, t.y0, t.x1, t.y1, t.x2, t.y2] for t in tris])
     want = [x["any_hit"] for x in ref.ray_triangle_any_hit_cpu(rays, tris)]
     assert native.rays_2d_any(rm, tm, brute=brute).tolist() == want
     assert 0 < sum(want) < len(want)
+    ids = np.array([t.id for t in tris], dtype=np.uint32)
+    counts = [x["hit_count"] for x in ref.ray_triangle_hit_count_cpu(rays, tris)]
+    assert native.rays_2d("count", rm, tm, ids, brute=brute).tolist() == counts
+    assert max(counts) > 1
+    tid, tt, has = native.rays_2d("closest", rm, tm, ids, brute=brute)
+    closest = ref.ray_triangle_closest_hit_cpu(rays, tris)
+    assert [(x["ray_id"], x["triangle_id"], x["t"]) for x in closest] == \
+        [(int(i), int(tid[i]), float(tt[i])) for i in np.flatnonzero(has)]
+
+
+def test_2d_closest_hit_is_the_lowest_hit_id_at_t_zero():
+    """2-D hits carry t = 0.0 (reference.py:775-780), so the (t, id) order of
+    closest-hit reduces to the lowest id among the triangles hit one by one."""
+    rng = np.random.default_rng(635)
+    rays, tris = _random_2d_case(rng, 80, 120)
+    perm = rng.permutation(len(tris))  # ids out of input order
+    tris = tuple(Triangle(int(perm[i]) + 100, t.x0, t.y0, t.x1, t.y1, t.x2, t.y2) for i, t in enumerate(tris))
+    rows = {x["ray_id"]: x for x in ref.ray_triangle_closest_hit_cpu(rays, tris)}
+    for r in rays:
+        hit_ids = [t.id for t in tris if ref.ray_triangle_any_hit_cpu((r,), (t,))[0]["any_hit"]]
+        if hit_ids:
+            assert rows[r.id]["triangle_id"] == min(hit_ids) and rows[r.id]["t"] == 0.0
+        else:
+            assert r.id not in rows
 
 
 def test_visibility_authored_goal633():

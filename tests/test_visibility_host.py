@@ -48,5 +48,6 @@ def test_two_d_records_pack_to_the_reference_layouts():
     assert r.tobytes() == np.array([5], "<u4").tobytes() + np.array([1.0, 2.0, 3.0, 4.0, 0.5]).tobytes()
     t = m.pack_triangles_2d((rt.Triangle(9, 1.0, 2.0, 3.0, 4.0, 5.0, 6.0),))
     assert t.tobytes() == np.array([9, 0], "<u4").tobytes() + np.arange(1.0, 7.0).tobytes()
-    with pytest.raises(NotImplementedError, match="only any_hit"):
-        m.ray_triangle_mi355x((rt.Ray2D(5, 1.0, 2.0, 3.0, 4.0, 0.5),), (), mode="hit_count")
+    with pytest.raises(ValueError, match="both be 2D or both be 3D"):
+        m.ray_triangle_mi355x((rt.Ray2D(5, 1.0, 2.0, 3.0, 4.0, 0.5),),
+                              (rt.Triangle3D(1, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0),), mode="hit_count")
