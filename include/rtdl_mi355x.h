@@ -318,6 +318,7 @@ enum {
  * any number of containing polygons within that budget. */
 #define RTDL_STATUS_ROW_OVERFLOW 1u /* rows beyond row_capacity were dropped */
 #define RTDL_STATUS_REJECTED 4u     /* served request larger than a slot's row region */
+#define RTDL_STATUS_INTERNAL 8u     /* an internal invariant failed: the job's rows are not valid */
 
 typedef struct RtdlBatchJob {
   void* scene;           /* prepared handle */

@@ -30,6 +30,7 @@ JOB_PIP_POSITIVE = 6
 
 STATUS_ROW_OVERFLOW = 1
 STATUS_REJECTED = 4
+STATUS_INTERNAL = 8
 PIP_HIT_CAP = 16  # default PIP row budget per point (average); one point may exceed it
 
 ROW_DTYPES = {

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "common.hpp"
 #include "engine.hpp"
@@ -278,8 +279,14 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
 // by construction.  A query whose buckets 0..b still hold more than
 // kNbrCandMax neighbours after four counts (or that finds the parking space
 // full) is computed by batch_fix_kernel.
+// The count walk also stages the slots it counts, up to kNbrStage of them per
+// query.  When the last count round stays within that, the staged slots
+// are a superset of the rows (all of them for a query with c <= k, those in
+// buckets 0..b otherwise): the emit walk skips the query and nbr_sort_kernel
+// recomputes the FP64 distances of the staged slots, filters and sorts them.
 constexpr uint32_t kNbrBuckets = 32;
 constexpr uint32_t kNbrCandMax = 128;
+constexpr uint32_t kNbrStage = 256;
 constexpr int kSortBlock = 64;
 // histogram counters are 16-bit, two per LDS word; a query that meets more
 // neighbours than one counter holds goes to the exact fix-up
@@ -322,18 +329,19 @@ struct NbrCount {
   uint32_t n_cand;   // more than k: neighbours in buckets 0..bucket
   bool heavy;        // more than k neighbours: keep the k nearest
   bool refine;       // leave the query to the exact fix-up
+  bool staged;       // the last round's `count` slots are in the stage
 };
 
 template <bool STATS>
 __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint32_t* my_hist,
-                                           LaneStats& st) {
+                                           uint32_t* __restrict__ stage, LaneStats& st) {
   auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
   double qx, qy, qz;
   load_query3(j, qi, qx, qy, qz);
   const float qf[3] = {(float)qx, (float)qy, (float)qz};
   const float m = coord_bound(scene_m(j.scene), qx, qy, qz);
   const uint32_t k = j.k;
-  NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false};  // reference.py:357
+  NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false, false};  // reference.py:357
   const uint32_t step = k / 2 > 4u ? k / 2 : 4u;
   for (uint32_t round = 0;; ++round) {
     for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
@@ -350,6 +358,7 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
       // no result needed: a plain LDS add
       __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (count <= kNbrStage) stage[count - 1] = slot;
       if (count < next_check) return;
       next_check = count + step;
       uint32_t cum = 0;
@@ -368,6 +377,7 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
     r.heavy = shrunk || count > k;
     r.bucket = top;
     r.n_cand = 0;
+    r.staged = count <= kNbrStage;
     if (!r.heavy) return r;
     if (count > kNbrCountMax) {
       r.refine = true;
@@ -398,7 +408,8 @@ template <bool STATS>
 __global__ __launch_bounds__(256) void nbr_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
     uint32_t* item_counter, NbrPlan* __restrict__ plans, unsigned long long* cand_count,
-    uint64_t cand_cap, uint4* amb, uint32_t* amb_count, unsigned long long* stats) {
+    uint64_t cand_cap, uint32_t* __restrict__ stage, uint4* amb, uint32_t* amb_count,
+    unsigned long long* stats) {
   // buckets 2w and 2w + 1 of thread t in the low and high half of hist[w * 256 + t]
   __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
   const uint32_t lane = threadIdx.x & 63;
@@ -414,10 +425,10 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
     const bool active = lane < w.count;
     const uint32_t k = j.k;
     uint32_t qi = 0;
-    NbrCount r{0.0, 0.0, 0u, 0u, 0u, false, false};
+    NbrCount r{0.0, 0.0, 0u, 0u, 0u, false, false, false};
     if (active) {
       qi = job_query(j, w.first + lane);
-      r = nbr_count_query<STATS>(j, qi, my_hist, st);
+      r = nbr_count_query<STATS>(j, qi, my_hist, stage + ((size_t)it * 64 + lane) * kNbrStage, st);
     }
     uint32_t n_rows = r.heavy ? k : r.count;
     bool refine = r.refine;
@@ -427,7 +438,8 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
       n_rows = 0;
       refine = false;
     }
-    const uint32_t want = (n_rows && r.heavy && !refine) ? r.n_cand : 0u;
+    const bool staged = n_rows && !refine && r.staged;
+    const uint32_t want = (n_rows && r.heavy && !refine && !staged) ? r.n_cand : 0u;
     uint32_t total;
     const uint32_t prefix = wave_exclusive_sum(want, &total);
     const uint64_t cand_at = lane0_add64(cand_count, total) + prefix;
@@ -438,7 +450,7 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
     }
     if (active)
       plans[(size_t)it * 64 + lane] = NbrPlan{row_at,   cand_at, r.radius, r.radius_sq, refine ? 0u : n_rows,
-                                              r.heavy ? r.n_cand : 0u, r.bucket, qi};
+                                              r.heavy ? r.n_cand : 0u, r.bucket, qi, staged ? r.count : 0u, 0u};
   }
   if (STATS) flush_lane_stats(stats, st);
 }
@@ -487,8 +499,9 @@ __global__ __launch_bounds__(256) void nbr_emit_kernel(const DeviceJob* __restri
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
     const NbrPlan pl = lane < w.count ? plans[(size_t)it * 64 + lane] : NbrPlan{};
-    // every lane returns to the item fetch together; lanes without rows idle
-    if (pl.n_rows) nbr_emit_query(j, pl, cands, st);
+    // every lane returns to the item fetch together; lanes without rows (or
+    // with staged rows) idle
+    if (pl.n_rows && !pl.staged) nbr_emit_query(j, pl, cands, st);
   }
 }
 
@@ -546,7 +559,10 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
                                                              const WorkItem* __restrict__ items,
                                                              uint32_t n_items, uint32_t* item_counter,
                                                              const NbrPlan* __restrict__ plans,
-                                                             const NbrCand* __restrict__ cands) {
+                                                             const NbrCand* __restrict__ cands,
+                                                             const uint32_t* __restrict__ stage) {
+  __shared__ uint64_t pack_d[kNbrCandMax];
+  __shared__ uint32_t pack_i[kNbrCandMax];
   const uint32_t lane = threadIdx.x;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -557,11 +573,53 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
     const DeviceJob& j = jobs[w.job];
     for (uint32_t q = 0; q < w.count; ++q) {  // one query at a time, all lanes
       const NbrPlan pl = plans[(size_t)it * 64 + q];
-      if (pl.n_rows == 0 || (pl.n_cand == 0 && pl.n_rows == 1)) continue;
+      if (pl.n_rows == 0 || (pl.n_cand == 0 && pl.n_rows == 1 && !pl.staged)) continue;
       uint64_t d0 = ~0ull, d1 = ~0ull;
       uint32_t i0 = ~0u, i1 = ~0u;
       uint32_t n;
-      if (pl.n_cand == 0) {  // rows written in walk order, sorted in place
+      if (pl.staged) {
+        // the count walk's slots: recompute the FP64 distances, keep buckets
+        // 0..b (all of them for a query with at most k neighbours) and pack
+        // the kept ones, at most kNbrCandMax, into LDS
+        double qx, qy, qz;
+        load_query3(j, pl.qi, qx, qy, qz);
+        const double scale = (double)kNbrBuckets / pl.radius;
+        const uint32_t* sq = stage + ((size_t)it * 64 + q) * kNbrStage;
+        uint32_t kept = 0;
+        for (uint32_t e0 = 0; e0 < pl.staged; e0 += 64) {
+          const uint32_t e = e0 + lane;
+          bool keep = false;
+          uint64_t d = 0;
+          uint32_t id = 0;
+          if (e < pl.staged) {
+            const uint32_t slot = sq[e];
+            const float4 p = j.scene.prims[slot];
+            const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
+            keep = pl.n_cand == 0 || nbr_bucket(dist, scale) <= pl.bucket;
+            d = __builtin_bit_cast(uint64_t, dist);
+            id = __float_as_uint(p.w);
+          }
+          const unsigned long long mask = __ballot(keep);
+          const uint32_t at = kept + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+          if (keep && at < kNbrCandMax) {
+            pack_d[at] = d;
+            pack_i[at] = id;
+          }
+          kept += (uint32_t)__popcll(mask);
+        }
+        __syncthreads();
+        if (kept > kNbrCandMax && lane == 0) atomicOr(&j.counters->status, kStatusInternal);
+        n = kept < kNbrCandMax ? kept : kNbrCandMax;
+        if (lane < n) {
+          d0 = pack_d[lane];
+          i0 = pack_i[lane];
+        }
+        if (lane + 64 < n) {
+          d1 = pack_d[lane + 64];
+          i1 = pack_i[lane + 64];
+        }
+        __syncthreads();
+      } else if (pl.n_cand == 0) {  // rows written in walk order, sorted in place
         n = pl.n_rows;
         if (lane < n) {
           double dist;
@@ -720,6 +778,26 @@ __device__ inline uint32_t spread10b(uint32_t v) {
   v = (v | (v << 4)) & 0x030C30C3u;
   v = (v | (v << 2)) & 0x09249249u;
   return v;
+}
+
+// Item locality keys: family, scene index, then the Morton code of the
+// item's first query (keys holds the sorted query keys, job << 32 | code).
+__global__ void item_keys_kernel(const WorkItem* __restrict__ items, const uint64_t* __restrict__ starts,
+                                 const uint64_t* __restrict__ keys, size_t n, int scene_bits,
+                                 uint64_t* __restrict__ item_keys, uint32_t* __restrict__ item_vals) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const WorkItem w = items[i];
+  const uint64_t code = keys[starts[w.job] + w.first] & 0x3FFFFFFFull;
+  const uint64_t fam = w.group >> 24, scene = w.group & 0xFFFFFFu;
+  item_keys[i] = (fam << (30 + scene_bits)) | (scene << 30) | code;
+  item_vals[i] = (uint32_t)i;
+}
+
+__global__ void gather_items_kernel(const WorkItem* __restrict__ items, const uint32_t* __restrict__ idx,
+                                    size_t n, WorkItem* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = items[idx[i]];
 }
 
 __global__ void batch_keys_kernel(const DeviceJob* __restrict__ jobs,
@@ -881,6 +959,9 @@ void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
     if (host_items_) RTDL_HIP(hipHostFree(host_items_));
     RTDL_HIP(hipHostMalloc((void**)&host_items_, max_items * sizeof(WorkItem), hipHostMallocDefault));
     items_.resize(max_items);
+    items_sorted_.resize(max_items);
+    item_keys_.resize(max_items);
+    item_vals_.resize(max_items);
     nbr_plan_.resize(max_items * 64);
     cap_items_ = max_items;
   }
@@ -894,6 +975,21 @@ void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
   }
 }
 
+// Parking space and stage of the fixed-radius jobs of a batch.
+void BatchEngine::reserve_nbr(const std::vector<HostJob>& jobs) {
+  uint64_t nbr_queries = 0, nbr_items = 0;
+  for (const HostJob& h : jobs)
+    if (h.kind == kJobFixedRadius) {
+      nbr_queries += h.nq;
+      nbr_items += (h.nq + 63) / 64;
+    }
+  if (!nbr_queries) return;
+  const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
+  if (nbr_cand_.size() < want) nbr_cand_.resize(want);
+  const size_t stage = (size_t)nbr_items * 64 * kNbrStage;
+  if (nbr_stage_.size() < stage) nbr_stage_.resize(stage);
+}
+
 void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   const size_t n = jobs.size();
   uint64_t total = 0;
@@ -902,6 +998,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   reserve(n, total);
   // order jobs' items by predicate family; each family gets one launch
   std::vector<std::vector<WorkItem>> fam_items(kLaunchFamilies);
+  std::map<std::pair<const void*, uint32_t>, uint32_t> scene_index;
   uint64_t start = 0;
   for (size_t i = 0; i < n; ++i) {
     const HostJob& h = jobs[i];
@@ -925,19 +1022,18 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     d.qstride = h.qstride;
     d.order_base = (uint32_t)start;
     host_starts_[i] = start;
+    // a scene is one instance's tree: jobs on it share its nodes
+    const auto sk = std::make_pair((const void*)h.scene.nodes, h.scene.root);
+    auto found = scene_index.find(sk);
+    if (found == scene_index.end()) found = scene_index.emplace(sk, (uint32_t)scene_index.size()).first;
+    const uint32_t group = ((uint32_t)fam << 24) | found->second;
     for (uint64_t f = 0; f < h.nq; f += 64)
       fam_items[fam].push_back(
-          WorkItem{(uint32_t)i, (uint32_t)f, (uint32_t)std::min<uint64_t>(64, h.nq - f), 0});
+          WorkItem{(uint32_t)i, (uint32_t)f, (uint32_t)std::min<uint64_t>(64, h.nq - f), group});
     start += h.nq;
   }
   host_starts_[n] = start;
-  uint64_t nbr_queries = 0;
-  for (const HostJob& h : jobs)
-    if (h.kind == kJobFixedRadius) nbr_queries += h.nq;
-  if (nbr_queries) {
-    const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
-    if (nbr_cand_.size() < want) nbr_cand_.resize(want);
-  }
+  reserve_nbr(jobs);
   size_t item_off[kLaunchFamilies + 1] = {0};
   for (int f = 0; f < kLaunchFamilies; ++f) {
     std::memcpy(host_items_ + item_off[f], fam_items[f].data(), fam_items[f].size() * sizeof(WorkItem));
@@ -968,6 +1064,26 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   } else if (morton_sort && total == 1) {
     fill_u32(order_.get(), 1, 0u, s);
   }
+  // Items of one family run in (scene, Morton code of their first query)
+  // order across jobs: a queue's contiguous share (one XCD's) then covers one
+  // region of a scene at a time, whose nodes and primitives stay in that
+  // XCD's L2, instead of every job's whole scene.
+  const size_t n_items_all = item_off[kLaunchFamilies];
+  const WorkItem* items_run = items_.get();
+  if (morton_sort && total > 1 && n_items_all > 1) {
+    int scene_bits = 0;
+    while ((1ull << scene_bits) < scene_index.size()) ++scene_bits;
+    require(scene_bits <= 24, "batch spans more than 2^24 scenes");
+    item_keys_kernel<<<blocks_for(n_items_all, 256), 256, 0, s>>>(items_.get(), starts_.get(), keys_.get(),
+                                                                  n_items_all, scene_bits, item_keys_.get(),
+                                                                  item_vals_.get());
+    RTDL_LAUNCH_CHECK();
+    radix_sort_pairs_async(item_keys_.get(), item_vals_.get(), n_items_all, 30 + scene_bits + 4, sort_ws_, s);
+    gather_items_kernel<<<blocks_for(n_items_all, 256), 256, 0, s>>>(items_.get(), item_vals_.get(),
+                                                                     n_items_all, items_sorted_.get());
+    RTDL_LAUNCH_CHECK();
+    items_run = items_sorted_.get();
+  }
   const unsigned max_grid = (unsigned)cus_ * current_tuning().blocks_per_cu;
   auto grid_for = [&](size_t items) {
     return (unsigned)std::max<size_t>(1, std::min<size_t>((items + 3) / 4, max_grid));
@@ -981,7 +1097,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   for (int f = 0; f < kLaunchFamilies; ++f) {
     const uint32_t n_items = (uint32_t)(item_off[f + 1] - item_off[f]);
     if (!n_items) continue;
-    const WorkItem* it = items_.get() + item_off[f];
+    const WorkItem* it = items_run + item_off[f];
     uint32_t* ctr = item_counters_.get() + f * kXcds;
     const unsigned grid = grid_for(n_items);
     if (timing_on_) {
@@ -1003,21 +1119,22 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
         RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamNbr) {
+      require(nbr_stage_.size() >= (size_t)n_items * 64 * kNbrStage, "fixed-radius stage not reserved");
       if (stats_on_)
         nbr_count_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
                                                      nbr_cand_count_.get(), nbr_cand_.size(),
-                                                     amb_.get(), amb_count_.get(), st);
+                                                     nbr_stage_.get(), amb_.get(), amb_count_.get(), st);
       else
         nbr_count_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
                                                       nbr_cand_count_.get(), nbr_cand_.size(),
-                                                      amb_.get(), amb_count_.get(), st);
+                                                      nbr_stage_.get(), amb_.get(), amb_count_.get(), st);
       RTDL_LAUNCH_CHECK();
       nbr_emit_kernel<<<grid, 256, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrEmit * kXcds,
                                            nbr_plan_.get(), nbr_cand_.get());
       RTDL_LAUNCH_CHECK();
       const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 32));
       nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrSort * kXcds,
-                                                 nbr_plan_.get(), nbr_cand_.get());
+                                                 nbr_plan_.get(), nbr_cand_.get(), nbr_stage_.get());
       RTDL_LAUNCH_CHECK();
       if (f == last_nbr) {
         batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
@@ -1052,17 +1169,11 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
 
 hipGraphExec_t BatchEngine::capture(const std::vector<HostJob>& jobs, bool morton_sort) {
   // every allocation enqueue would make happens here, outside the capture
-  uint64_t total = 0, nbr_queries = 0;
-  for (const HostJob& h : jobs) {
-    total += h.nq;
-    if (h.kind == kJobFixedRadius) nbr_queries += h.nq;
-  }
+  uint64_t total = 0;
+  for (const HostJob& h : jobs) total += h.nq;
   require(total < 0xFFFFFFFFull, "batch exceeds 2^32 queries");
   reserve(jobs.size(), total);
-  if (nbr_queries) {
-    const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
-    if (nbr_cand_.size() < want) nbr_cand_.resize(want);
-  }
+  reserve_nbr(jobs);
   enable_timing(false);
   enable_stats(false);
   RTDL_HIP(hipStreamSynchronize(stream_));

@@ -61,6 +61,7 @@ constexpr uint32_t kPipHitCap = 16;   // containing polygons held in registers p
 constexpr int kPointBuckets = 7;    // KNN list sizes 1, 2, 4, ..., 64
 constexpr int kLaunchFamilies = kPointBuckets + 5;  // + fixed radius, ray any / count / closest, PIP
 constexpr uint32_t kStatusRowOverflow = 1u;
+constexpr uint32_t kStatusInternal = 8u;
 
 struct JobCounters {
   uint32_t rows;
@@ -96,6 +97,8 @@ struct NbrPlan {
   uint32_t n_cand;   // parked candidates; 0 when the rows are written directly
   uint32_t bucket;   // last distance bucket parked
   uint32_t qi;       // job-local query index
+  uint32_t staged;   // slots the count walk staged (rows come from them), or 0
+  uint32_t pad;
 };
 
 // A parked neighbour candidate.
@@ -109,7 +112,7 @@ struct WorkItem {
   uint32_t job;
   uint32_t first;  // first position in the job
   uint32_t count;  // <= 64
-  uint32_t pad;
+  uint32_t group;  // launch family << 24 | scene index (items sort by group, then place)
 };
 
 struct HostJob {
@@ -134,6 +137,8 @@ class BatchEngine {
   BatchEngine& operator=(const BatchEngine&) = delete;
 
   void reserve(size_t max_jobs, size_t max_queries);
+  // Sizes the fixed-radius parking space and stage for a batch's jobs.
+  void reserve_nbr(const std::vector<HostJob>& jobs);
   // Enqueues the whole batch; returns immediately.
   void enqueue(const std::vector<HostJob>& jobs, bool morton_sort);
   // Records the batch into an executable hipGraph instead of running it:
@@ -166,6 +171,9 @@ class BatchEngine {
   JobCounters* host_counters_ = nullptr; // pinned
   DeviceArray<DeviceJob> jobs_;
   DeviceArray<WorkItem> items_;
+  DeviceArray<WorkItem> items_sorted_;   // items_ in (family, scene, Morton) order
+  DeviceArray<uint64_t> item_keys_;
+  DeviceArray<uint32_t> item_vals_;
   DeviceArray<uint64_t> starts_;
   DeviceArray<JobCounters> counters_;
   DeviceArray<uint32_t> item_counters_;  // eight per launch (one per XCD)
@@ -176,6 +184,7 @@ class BatchEngine {
   DeviceArray<uint32_t> amb_count_;
   DeviceArray<NbrPlan> nbr_plan_;        // cap_items_ * 64
   DeviceArray<NbrCand> nbr_cand_;        // parked candidates of a batch
+  DeviceArray<uint32_t> nbr_stage_;      // kNbrStage slots per fixed-radius query
   DeviceArray<unsigned long long> nbr_cand_count_;
   DeviceArray<unsigned long long> stats_;
   SortWorkspace sort_ws_;
