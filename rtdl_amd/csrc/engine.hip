@@ -347,6 +347,7 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
     for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
     const double scale = (double)kNbrBuckets / r.radius;
     uint32_t count = 0, top = kNbrBuckets - 1, next_check = k;
+    uint4 sbuf = make_uint4(0u, 0u, 0u, 0u);  // staged slots go out four at a time
     bool shrunk = false;
     float thr2 = frn_threshold2(r.radius, m);
     auto visit = [&](uint32_t slot, const float4& p, float) {
@@ -358,7 +359,14 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
       // no result needed: a plain LDS add
       __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (count <= kNbrStage) stage[count - 1] = slot;
+      if (count <= kNbrStage) {
+        const uint32_t e = (count - 1) & 3u;
+        sbuf.x = e == 0 ? slot : sbuf.x;
+        sbuf.y = e == 1 ? slot : sbuf.y;
+        sbuf.z = e == 2 ? slot : sbuf.z;
+        sbuf.w = e == 3 ? slot : sbuf.w;
+        if (e == 3) reinterpret_cast<uint4*>(stage)[(count - 1) >> 2] = sbuf;
+      }
       if (count < next_check) return;
       next_check = count + step;
       uint32_t cum = 0;
@@ -373,6 +381,8 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
       }
     };
     walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit);
+    // the last, partial group of four (entries past `count` are never read)
+    if ((count & 3u) && count <= kNbrStage) reinterpret_cast<uint4*>(stage)[(count - 1) >> 2] = sbuf;
     r.count = count;
     r.heavy = shrunk || count > k;
     r.bucket = top;
@@ -505,6 +515,17 @@ __global__ __launch_bounds__(256) void nbr_emit_kernel(const DeviceJob* __restri
   }
 }
 
+// Lane q's value (q wave-uniform).
+__device__ inline uint64_t readlane64(uint64_t x, uint32_t q) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, q);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), q);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline double readlane_f64(double x, uint32_t q) {
+  return __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, x), q));
+}
+
 __device__ inline uint64_t shfl_xor64(uint64_t x, uint32_t m) {
   const uint32_t lo = __shfl_xor((uint32_t)x, (int)m, 64);
   const uint32_t hi = __shfl_xor((uint32_t)(x >> 32), (int)m, 64);
@@ -571,8 +592,22 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
     if (it == kItemRetry) continue;
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
+    // each lane loads one query's plan and coordinates; the loop below takes
+    // query q's from lane q (readlane) instead of 64 dependent loads
+    const NbrPlan mine = lane < w.count ? plans[(size_t)it * 64 + lane] : NbrPlan{};
+    double mqx = 0.0, mqy = 0.0, mqz = 0.0;
+    if (mine.staged) load_query3(j, mine.qi, mqx, mqy, mqz);
     for (uint32_t q = 0; q < w.count; ++q) {  // one query at a time, all lanes
-      const NbrPlan pl = plans[(size_t)it * 64 + q];
+      NbrPlan pl;
+      pl.row_at = readlane64(mine.row_at, q);
+      pl.cand_at = readlane64(mine.cand_at, q);
+      pl.radius = readlane_f64(mine.radius, q);
+      pl.radius_sq = readlane_f64(mine.radius_sq, q);
+      pl.n_rows = __builtin_amdgcn_readlane(mine.n_rows, q);
+      pl.n_cand = __builtin_amdgcn_readlane(mine.n_cand, q);
+      pl.bucket = __builtin_amdgcn_readlane(mine.bucket, q);
+      pl.qi = __builtin_amdgcn_readlane(mine.qi, q);
+      pl.staged = __builtin_amdgcn_readlane(mine.staged, q);
       if (pl.n_rows == 0 || (pl.n_cand == 0 && pl.n_rows == 1 && !pl.staged)) continue;
       uint64_t d0 = ~0ull, d1 = ~0ull;
       uint32_t i0 = ~0u, i1 = ~0u;
@@ -581,8 +616,7 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
         // the count walk's slots: recompute the FP64 distances, keep buckets
         // 0..b (all of them for a query with at most k neighbours) and pack
         // the kept ones, at most kNbrCandMax, into LDS
-        double qx, qy, qz;
-        load_query3(j, pl.qi, qx, qy, qz);
+        const double qx = readlane_f64(mqx, q), qy = readlane_f64(mqy, q), qz = readlane_f64(mqz, q);
         const double scale = (double)kNbrBuckets / pl.radius;
         const uint32_t* sq = stage + ((size_t)it * 64 + q) * kNbrStage;
         uint32_t kept = 0;
