@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
         for (int i = 0; i < K; ++i) {
           if (i >= K - (int)k && list.e[i] != ~0ull) {
             const uint32_t slot = (uint32_t)list.e[i];
-            const float4 p = j.scene.prims[slot];
+            const float4 p = load_prim(j.scene.prims, slot);
             const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
             write_neighbor_row(j, at0 + r, qi, __float_as_uint(p.w), dist, r + 1);
             ++r;
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
           uint32_t id = 0;
           if (e < pl.staged) {
             const uint32_t slot = sq[e];
-            const float4 p = j.scene.prims[slot];
+            const float4 p = load_prim(j.scene.prims, slot);
             const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
             keep = pl.n_cand == 0 || nbr_bucket(dist, scale) <= pl.bucket;
             d = __builtin_bit_cast(uint64_t, dist);
@@ -851,7 +851,7 @@ __global__ void batch_keys_kernel(const DeviceJob* __restrict__ jobs,
   const float c[3] = {q[0], q[1], j.qstride >= 3 ? q[2] : 0.0f};
   uint32_t code = 0;
   if (j.scene.root != kEndNode) {
-    const BvhNode& r = j.scene.nodes[j.scene.root];
+    const BvhNode r = load_node(j.scene.nodes, j.scene.root);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float e = r.hi[a] - r.lo[a];

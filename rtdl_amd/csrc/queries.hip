@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void collect_packed_kernel(
     for (int i = 0; i < K; ++i) {
       if (i >= K - (int)k && list.e[i] != ~0ull) {
         const uint32_t slot = (uint32_t)list.e[i];
-        const float4 p = s.prims[slot];
+        const float4 p = load_prim(s.prims, slot);
         const size_t at = (size_t)qi * k + (uint32_t)(i - (K - (int)k));
         list_d[at] = sqrt(point_d2(s, slot, p, qx, qy, qz));
         list_id[at] = __float_as_uint(p.w);

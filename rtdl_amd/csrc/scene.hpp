@@ -37,6 +37,45 @@ struct SceneView {
   const float* m_dev;  // when set, M is read from device memory instead
 };
 
+// Loads from scene arrays through the global address space.  The arrays
+// always live in device memory; a pointer read from a job descriptor would
+// otherwise be accessed with flat loads, which also count against the
+// LDS / scalar wait counter.  A node is fetched as two 16-byte loads.
+typedef unsigned int GlobalU32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline GlobalU32x4 load_raw16(const void* p) {
+  return *(const __attribute__((address_space(1))) GlobalU32x4*)p;
+}
+
+__device__ inline BvhNode load_node(const BvhNode* nodes, uint32_t i) {
+  const GlobalU32x4 a = load_raw16(nodes + i);
+  const GlobalU32x4 b = load_raw16(reinterpret_cast<const uint8_t*>(nodes + i) + 16);
+  BvhNode n;
+  n.lo[0] = __uint_as_float(a[0]);
+  n.lo[1] = __uint_as_float(a[1]);
+  n.lo[2] = __uint_as_float(a[2]);
+  n.link = a[3];
+  n.hi[0] = __uint_as_float(b[0]);
+  n.hi[1] = __uint_as_float(b[1]);
+  n.hi[2] = __uint_as_float(b[2]);
+  n.escape = b[3];
+  return n;
+}
+
+__device__ inline float4 load_prim(const float4* prims, size_t i) {
+  const GlobalU32x4 v = load_raw16(prims + i);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
+__device__ inline uint4 load_u4(const uint4* a, size_t i) {
+  const GlobalU32x4 v = load_raw16(a + i);
+  return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+__device__ inline double load_f64(const double* a, size_t i) {
+  return *(const __attribute__((address_space(1))) double*)(a + i);
+}
+
 // A set of instances built into one LBVH: instance i owns primitives
 // [offsets[i], offsets[i+1]) of the input (contiguous) and gets its own
 // subtree; `roots[i]` / `ends[i]` delimit its traversal.

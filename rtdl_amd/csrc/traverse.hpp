@@ -40,9 +40,9 @@ __device__ inline double point_d2(const SceneView& s, uint32_t slot, const float
                                   double qy, double qz) {
   double px, py, pz;
   if (s.exact) {
-    px = s.exact[3 * (size_t)slot];
-    py = s.exact[3 * (size_t)slot + 1];
-    pz = s.exact[3 * (size_t)slot + 2];
+    px = load_f64(s.exact, 3 * (size_t)slot);
+    py = load_f64(s.exact, 3 * (size_t)slot + 1);
+    pz = load_f64(s.exact, 3 * (size_t)slot + 2);
   } else {
     px = p.x;
     py = p.y;
@@ -62,7 +62,7 @@ __device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t e
   // a preorder walk visits each node at most once; the guard only bounds the
   // loop should the tree ever be malformed
   for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
-    const BvhNode nd = s.nodes[node];
+    const BvhNode nd = load_node(s.nodes, node);
     if (STATS) ++st.nodes;
     if (node == skip) {  // a subtree the caller has already walked
       node = nd.escape;
@@ -81,7 +81,7 @@ __device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t e
     const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
     for (uint32_t j = 0; j < cnt; ++j) {
       const uint32_t slot = first + j;
-      const float4 p = s.prims[slot];
+      const float4 p = load_prim(s.prims, slot);
       if (STATS) ++st.prims;
       const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
       const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
@@ -107,13 +107,13 @@ __device__ inline void walk_near_first(const SceneView& s, uint32_t root, uint32
   uint32_t up[kUp + 1];  // up[i]: i steps above the leaf; the root repeats above it
 #pragma unroll
   for (int i = 0; i <= kUp; ++i) up[i] = root;
-  BvhNode nd = s.nodes[root];
+  BvhNode nd = load_node(s.nodes, root);
   if (STATS) ++st.nodes;
   for (uint32_t guard = s.n_nodes + 1; !node_is_leaf(nd.link) && guard; --guard) {
     const uint32_t l = nd.link;
-    const BvhNode ln = s.nodes[l];
+    const BvhNode ln = load_node(s.nodes, l);
     const uint32_t r = ln.escape;
-    const BvhNode rn = s.nodes[r];
+    const BvhNode rn = load_node(s.nodes, r);
     if (STATS) st.nodes += 2;
     const bool right = box_dist2(qf, rn.lo, rn.hi) < box_dist2(qf, ln.lo, ln.hi);
 #pragma unroll
@@ -131,7 +131,7 @@ __device__ inline void walk_near_first(const SceneView& s, uint32_t root, uint32
         if (i == ring) from = up[i];
     }
     if (from == done) continue;
-    const uint32_t stop = from == root ? end : s.nodes[from].escape;
+    const uint32_t stop = from == root ? end : load_node(s.nodes, from).escape;
     walk_points<STATS>(s, from, stop, qf, thr2, st, visit, done);
     if (from == root) break;
     done = from;
@@ -170,7 +170,7 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
   if (root == kEndNode || root == end) return true;
   bool complete = true;
   uint32_t depth = 0;
-  BvhNode nd = s.nodes[root];
+  BvhNode nd = load_node(s.nodes, root);
   if (STATS) ++st.nodes;
   if (!(box_dist2(qf, nd.lo, nd.hi) <= thr2)) return true;
   // every node is pushed at most once; the guard only bounds the loop should
@@ -179,9 +179,9 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
     bool leaf = true;
     while (!node_is_leaf(nd.link)) {
       const uint32_t l = nd.link;
-      const BvhNode ln = s.nodes[l];
+      const BvhNode ln = load_node(s.nodes, l);
       const uint32_t r = ln.escape;  // the right child follows the left subtree
-      const BvhNode rn = s.nodes[r];
+      const BvhNode rn = load_node(s.nodes, r);
       if (STATS) st.nodes += 2;
       const float dl = box_dist2(qf, ln.lo, ln.hi), dr = box_dist2(qf, rn.lo, rn.hi);
       const bool right_first = dr < dl;
@@ -202,7 +202,7 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
       const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
       for (uint32_t j = 0; j < cnt; ++j) {
         const uint32_t slot = first + j;
-        const float4 p = s.prims[slot];
+        const float4 p = load_prim(s.prims, slot);
         if (STATS) ++st.prims;
         const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
         const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
@@ -211,7 +211,7 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
     }
     bool found = false;
     while (depth) {
-      nd = s.nodes[stack[stride * --depth]];
+      nd = load_node(s.nodes, stack[stride * --depth]);
       if (STATS) ++st.nodes;
       if (box_dist2(qf, nd.lo, nd.hi) <= thr2) {
         found = true;
@@ -541,7 +541,7 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
   uint32_t node = root;
   bool done = false;
   for (uint32_t guard = s.n_nodes + 1; node != end && !done && guard; --guard) {
-    const BvhNode nd = s.nodes[node];
+    const BvhNode nd = load_node(s.nodes, node);
     if (STATS) ++st.nodes;
     const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
     const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
@@ -558,14 +558,14 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
       const uint32_t slot = first + j;
       if (STATS) ++st.prims;
       double v[9];
-      const float4 a = s.prims[3 * (size_t)slot];
+      const float4 a = load_prim(s.prims, 3 * (size_t)slot);
       const uint32_t ident = __float_as_uint(a.w);
       if (s.exact) {
 #pragma unroll
-        for (int e = 0; e < 9; ++e) v[e] = s.exact[9 * (size_t)slot + e];
+        for (int e = 0; e < 9; ++e) v[e] = load_f64(s.exact, 9 * (size_t)slot + e);
       } else {
-        const float4 b = s.prims[3 * (size_t)slot + 1];
-        const float4 c = s.prims[3 * (size_t)slot + 2];
+        const float4 b = load_prim(s.prims, 3 * (size_t)slot + 1);
+        const float4 c = load_prim(s.prims, 3 * (size_t)slot + 2);
         v[0] = a.x; v[1] = a.y; v[2] = a.z;
         v[3] = b.x; v[4] = b.y; v[5] = b.z;
         v[6] = c.x; v[7] = c.y; v[8] = c.z;
@@ -617,8 +617,9 @@ __device__ inline bool point_on_segment(double px, double py, double ax, double 
 }
 
 // reference.py:717-742
-__device__ inline bool point_in_polygon(double x, double y, const double* v, uint32_t n,
+__device__ inline bool point_in_polygon(double x, double y, const double* vertices, uint32_t n,
                                         double eps_sq) {
+  const auto* v = (const __attribute__((address_space(1))) double*)vertices;  // device memory
   for (uint32_t i = 0; i < n; ++i) {
     const uint32_t j = (i + 1 == n) ? 0 : i + 1;
     if (point_on_segment(x, y, v[2 * i], v[2 * i + 1], v[2 * j], v[2 * j + 1], eps_sq)) return true;
@@ -648,7 +649,7 @@ __device__ inline void walk_pip(const SceneView& s, uint32_t root, uint32_t end,
   const float pad = m * 9.5367431640625e-07f + 1.0e-11f;
   uint32_t node = root;
   for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
-    const BvhNode nd = s.nodes[node];
+    const BvhNode nd = load_node(s.nodes, node);
     if (STATS) ++st.nodes;
     const bool in = pf0 >= nd.lo[0] - pad && pf0 <= nd.hi[0] + pad && pf1 >= nd.lo[1] - pad &&
                     pf1 <= nd.hi[1] + pad;
@@ -665,10 +666,12 @@ __device__ inline void walk_pip(const SceneView& s, uint32_t root, uint32_t end,
       const uint32_t slot = first + j;
       if (STATS) ++st.prims;
       if (positive_only) {  // reference.py:122-125 exact bbox prefilter
-        const double* bb = s.poly_box + 4 * (size_t)slot;
-        if (x < bb[0] || x > bb[2] || y < bb[1] || y > bb[3]) continue;
+        const size_t b = 4 * (size_t)slot;
+        if (x < load_f64(s.poly_box, b) || x > load_f64(s.poly_box, b + 2) || y < load_f64(s.poly_box, b + 1) ||
+            y > load_f64(s.poly_box, b + 3))
+          continue;
       }
-      const uint4 ref = s.poly[slot];
+      const uint4 ref = load_u4(s.poly, slot);
       if (point_in_polygon(x, y, s.poly_verts + 2 * (size_t)ref.x, ref.y, eps_sq)) hit(slot);
     }
     node = nd.escape;
