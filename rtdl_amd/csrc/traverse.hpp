@@ -607,6 +607,11 @@ __device__ inline bool point_on_segment(double px, double py, double ax, double 
   const double length_sq = ex * ex + ey * ey;
   if (length_sq <= eps_sq) return fabs(px - ax) <= eps && fabs(py - ay) <= eps;
   const double cross = (px - ax) * (by - ay) - (py - ay) * (bx - ax);
+  // cross^2 > 4 eps^2 length_sq (FP64, no underflow since length_sq >
+  // eps_sq) implies |cross| > 2 eps sqrt(length_sq) (1 - 2^-50), beyond any
+  // rounding of the exact test below: the usual far-from-the-edge case
+  // skips the square root with the same answer
+  if (cross * cross > 4.0 * eps_sq * length_sq) return false;
   const double length = sqrt(length_sq);
   if (fabs(cross) > eps * length) return false;
   const double dot = (px - ax) * (bx - ax) + (py - ay) * (by - ay);
@@ -620,20 +625,23 @@ __device__ inline bool point_on_segment(double px, double py, double ax, double 
 __device__ inline bool point_in_polygon(double x, double y, const double* vertices, uint32_t n,
                                         double eps_sq) {
   const auto* v = (const __attribute__((address_space(1))) double*)vertices;  // device memory
-  for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t j = (i + 1 == n) ? 0 : i + 1;
-    if (point_on_segment(x, y, v[2 * i], v[2 * i + 1], v[2 * j], v[2 * j + 1], eps_sq)) return true;
-  }
+  // One pass over the edges a = v[k] -> b = v[k + 1]: any edge holding the
+  // point answers true (the reference's first loop), otherwise the
+  // even-odd parity of the crossings decides (its second loop, whose pair
+  // (i, j = i - 1) is (b, a) here; the parity does not depend on the order).
   bool inside = false;
-  uint32_t j = n - 1;
-  for (uint32_t i = 0; i < n; ++i) {
-    const double xi = v[2 * i], yi = v[2 * i + 1], xj = v[2 * j], yj = v[2 * j + 1];
-    if ((yi > y) != (yj > y)) {
-      double den = yj - yi;
+  double ax = v[0], ay = v[1];
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t kb = (k + 1 == n) ? 0 : k + 1;
+    const double bx = v[2 * kb], by = v[2 * kb + 1];
+    if (point_on_segment(x, y, ax, ay, bx, by, eps_sq)) return true;
+    if ((by > y) != (ay > y)) {
+      double den = ay - by;
       if (den == 0.0) den = 1.0e-20;
-      if (x <= (xj - xi) * (y - yi) / den + xi) inside = !inside;
+      if (x <= (ax - bx) * (y - by) / den + bx) inside = !inside;
     }
-    j = i;
+    ax = bx;
+    ay = by;
   }
   return inside;
 }

@@ -358,11 +358,21 @@ def test_pip_random_star_polygons(gpu_ready, positive_only):
     polys = cases.star_polygons(rng, 1000 if positive_only else 200, rmin=0.01, rmax=0.05)
     n = 20000 if positive_only else 3000
     pts = cases.points(rng, n, 2)
-    # put some probes exactly on vertices and edge midpoints
+    # put some probes exactly on vertices, on edge midpoints, and just off
+    # edges inside and outside the 1e-12 on-edge tolerance
     v = polys["vertices_xy"].reshape(-1, 2)
     pts["x"][:100] = v[:100, 0]
     pts["y"][:100] = v[:100, 1]
     refs = polys["refs"]
+    a = v[refs["vertex_offset"][:100]]
+    b = v[refs["vertex_offset"][:100] + 1]
+    mid = (a + b) / 2
+    nrm = np.stack([a[:, 1] - b[:, 1], b[:, 0] - a[:, 0]], 1)
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    for j, off in enumerate((0.0, 5e-13, -5e-13, 3e-12, -3e-12)):
+        sl = slice(100 + 100 * j, 200 + 100 * j)
+        pts["x"][sl] = mid[:, 0] + off * nrm[:, 0]
+        pts["y"][sl] = mid[:, 1] + off * nrm[:, 1]
     pairs = oracle.pip_pairs(pts["x"], pts["y"], refs["vertex_offset"], refs["vertex_count"],
                              polys["vertices_xy"], positive_only=positive_only)
     got = m.point_in_polygon_mi355x(pts, polys, positive_only=positive_only)
