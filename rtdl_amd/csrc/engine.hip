@@ -335,11 +335,12 @@ struct NbrCount {
 template <bool STATS>
 __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint32_t* my_hist,
                                            uint32_t* __restrict__ stage, LaneStats& st) {
+  const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
   auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
   double qx, qy, qz;
   load_query3(j, qi, qx, qy, qz);
   const float qf[3] = {(float)qx, (float)qy, (float)qz};
-  const float m = coord_bound(scene_m(j.scene), qx, qy, qz);
+  const float m = coord_bound(scene_m(sc), qx, qy, qz);
   const uint32_t k = j.k;
   NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false, false};  // reference.py:357
   const uint32_t step = k / 2 > 4u ? k / 2 : 4u;
@@ -351,7 +352,7 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
     bool shrunk = false;
     float thr2 = frn_threshold2(r.radius, m);
     auto visit = [&](uint32_t slot, const float4& p, float) {
-      const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
+      const double d2 = point_d2(sc, slot, p, qx, qy, qz);
       if (d2 > r.radius_sq) return;  // reference.py:361 inclusive radius
       const uint32_t b = nbr_bucket(sqrt(d2), scale);
       if (b > top) return;  // beyond the k nearest
@@ -380,7 +381,7 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
         }
       }
     };
-    walk_points<STATS>(j.scene, j.scene.root, j.scene.end, qf, thr2, st, visit);
+    walk_points<STATS>(sc, sc.root, sc.end, qf, thr2, st, visit);
     // the last, partial group of four (entries past `count` are never read)
     if ((count & 3u) && count <= kNbrStage) reinterpret_cast<uint4*>(stage)[(count - 1) >> 2] = sbuf;
     r.count = count;
@@ -469,18 +470,19 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
 // ball; the walk meets the same candidates in the same FP64 arithmetic as the
 // count.
 __device__ inline void nbr_emit_query(const DeviceJob& j, const NbrPlan& pl, NbrCand* cands, LaneStats& st) {
+  const SceneView sc = j.scene;
   double qx, qy, qz;
   load_query3(j, pl.qi, qx, qy, qz);
   const float qf[3] = {(float)qx, (float)qy, (float)qz};
   const double scale = (double)kNbrBuckets / pl.radius;
   const bool parked = pl.n_cand != 0;
   const double bound = parked ? fmin(pl.radius, nbr_bucket_edge(pl.bucket, scale)) : pl.radius;
-  float thr2 = frn_threshold2(bound, coord_bound(scene_m(j.scene), qx, qy, qz));
+  float thr2 = frn_threshold2(bound, coord_bound(scene_m(sc), qx, qy, qz));
   uint32_t r = 0;
   NbrCand* park = cands + pl.cand_at;
-  walk_points<false>(j.scene, j.scene.root, j.scene.end, qf, thr2, st,
+  walk_points<false>(sc, sc.root, sc.end, qf, thr2, st,
                      [&](uint32_t slot, const float4& p, float) {
-                       const double d2 = point_d2(j.scene, slot, p, qx, qy, qz);
+                       const double d2 = point_d2(sc, slot, p, qx, qy, qz);
                        if (d2 > pl.radius_sq) return;
                        const double dist = sqrt(d2);
                        const uint32_t id = __float_as_uint(p.w);

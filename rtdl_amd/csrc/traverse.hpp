@@ -64,30 +64,24 @@ __device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t e
   for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
     const BvhNode nd = load_node(s.nodes, node);
     if (STATS) ++st.nodes;
-    if (node == skip) {  // a subtree the caller has already walked
-      node = nd.escape;
-      continue;
-    }
     const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
     const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
-    if (!(box_dist2(qf, lo, hi) <= thr2)) {
-      node = nd.escape;
-      continue;
+    // `skip`: a subtree the caller has already walked
+    const bool in = (node != skip) & (box_dist2(qf, lo, hi) <= thr2);
+    const bool leaf = node_is_leaf(nd.link);
+    if (in && leaf) {  // the only branch of the step: the leaf's primitives
+      const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const uint32_t slot = first + j;
+        const float4 p = load_prim(s.prims, slot);
+        if (STATS) ++st.prims;
+        const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
+        const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+        if (d2f <= thr2) visit(slot, p, d2f);
+      }
     }
-    if (!node_is_leaf(nd.link)) {
-      node = nd.link;
-      continue;
-    }
-    const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const uint32_t slot = first + j;
-      const float4 p = load_prim(s.prims, slot);
-      if (STATS) ++st.prims;
-      const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
-      const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
-      if (d2f <= thr2) visit(slot, p, d2f);
-    }
-    node = nd.escape;
+    // into an overlapped internal node's subtree, else past the node
+    node = (in && !leaf) ? nd.link : nd.escape;
   }
 }
 
