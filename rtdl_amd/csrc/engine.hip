@@ -173,8 +173,9 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
       qx = qp[0];
       qy = qp[1];
       qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
-      ambiguous = collect_packed<K, STATS>(j.scene, j.scene.root, j.scene.end, qx, qy, qz, j.radius,
-                                           k, list, st, stack + threadIdx.x, 256);
+      const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
+      ambiguous = collect_packed<K, STATS>(sc, sc.root, sc.end, qx, qy, qz, j.radius, k, list, st,
+                                           stack + threadIdx.x, 256);
 #pragma unroll
       for (int i = 0; i < K; ++i)
         if (i >= K - (int)k && list.e[i] != ~0ull) ++c;
@@ -702,8 +703,8 @@ __global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __rest
     if (active) {
       qi = job_query(j, w.first + lane);
       const float* r = j.queries + (size_t)qi * j.qstride;
-      res = trace_ray<MODE, STATS>(j.scene, j.scene.root, j.scene.end, r[0], r[1], r[2], r[3],
-                                   r[4], r[5], r[6], st);
+      const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
+      res = trace_ray<MODE, STATS>(sc, sc.root, sc.end, r[0], r[1], r[2], r[3], r[4], r[5], r[6], st);
     }
     if (MODE != 2) {
       if (active)
@@ -751,10 +752,11 @@ __global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restr
     if (active) {
       qi = job_query(j, w.first + lane);
       const float* p = j.queries + (size_t)qi * j.qstride;
-      walk_pip<STATS>(j.scene, j.scene.root, j.scene.end, p[0], p[1], true, eps_sq, st,
+      const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
+      walk_pip<STATS>(sc, sc.root, sc.end, p[0], p[1], true, eps_sq, st,
                       [&](uint32_t slot) {
                         if (found < kPipHitCap) {
-                          hit[kPipHitCap - 1] = ((uint64_t)j.scene.slot_to_input[slot] << 32) | slot;
+                          hit[kPipHitCap - 1] = ((uint64_t)sc.slot_to_input[slot] << 32) | slot;
 #pragma unroll
                           for (int i = (int)kPipHitCap - 1; i > 0; --i) {
                             const uint64_t a = hit[i - 1], b = hit[i];

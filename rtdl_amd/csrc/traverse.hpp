@@ -539,12 +539,12 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
     if (STATS) ++st.nodes;
     const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
     const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
-    if (!segment_box(o32, D32, inv, lo, hi, pad, slim)) {
-      node = nd.escape;
-      continue;
-    }
-    if (!node_is_leaf(nd.link)) {
-      node = nd.link;
+    const bool in = segment_box(o32, D32, inv, lo, hi, pad, slim);
+    const bool leaf = node_is_leaf(nd.link);
+    // into an overlapped internal node's subtree, else past the node
+    const uint32_t next = (in && !leaf) ? nd.link : nd.escape;
+    if (!(in && leaf)) {
+      node = next;
       continue;
     }
     const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
@@ -653,14 +653,13 @@ __device__ inline void walk_pip(const SceneView& s, uint32_t root, uint32_t end,
   for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
     const BvhNode nd = load_node(s.nodes, node);
     if (STATS) ++st.nodes;
-    const bool in = pf0 >= nd.lo[0] - pad && pf0 <= nd.hi[0] + pad && pf1 >= nd.lo[1] - pad &&
-                    pf1 <= nd.hi[1] + pad;
-    if (!in) {
-      node = nd.escape;
-      continue;
-    }
-    if (!node_is_leaf(nd.link)) {
-      node = nd.link;
+    const bool in = (pf0 >= nd.lo[0] - pad) & (pf0 <= nd.hi[0] + pad) & (pf1 >= nd.lo[1] - pad) &
+                    (pf1 <= nd.hi[1] + pad);
+    const bool leaf = node_is_leaf(nd.link);
+    // into an overlapped internal node's subtree, else past the node
+    const uint32_t next = (in && !leaf) ? nd.link : nd.escape;
+    if (!(in && leaf)) {
+      node = next;
       continue;
     }
     const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
