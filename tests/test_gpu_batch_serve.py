@@ -381,6 +381,46 @@ def test_batch_fixed_radius_lattice_ties_and_duplicates(gpu_ready):
         _check_nn(rows, want, False)
 
 
+def test_batch_fixed_radius_points_on_bucket_edges_and_radius(gpu_ready):
+    _say("case: batched fixed radius, neighbours on distance-bucket edges and on the radius, +-1 ulp")
+    rng = np.random.default_rng(93)
+    radius = 0.25  # bucket edges j * radius / 32 = j / 128 are FP32 values
+    q0 = np.array([0.5, 0.5, 0.5])
+    pts = []
+    for j in range(1, 33):
+        e = j / 128.0
+        for sign in (-1.0, 1.0):
+            for axis in range(3):
+                for off in (np.float32(e), np.nextafter(np.float32(e), np.float32(0)),
+                            np.nextafter(np.float32(e), np.float32(1))):
+                    p = q0.copy()
+                    p[axis] = np.float32(q0[axis] + sign * np.float64(off))
+                    pts.append(p)
+    # diagonal points: irrational distances close to the edges
+    for j in range(1, 33):
+        d = j / 128.0 / np.sqrt(3.0)
+        pts.append(np.float32(q0 + d))
+        pts.append(np.float32(q0 - d))
+    pts = np.concatenate([np.array(pts), rng.random((2000, 3))]).astype(np.float32).astype(np.float64)
+    pts = pts[rng.permutation(len(pts))]
+    sdict = {"id": np.arange(len(pts), dtype=np.uint32), "x": pts[:, 0], "y": pts[:, 1], "z": pts[:, 2]}
+    scene = _prepared_points(pts)
+    qg = np.concatenate([q0[None, :], q0[None, :] + np.array([[1 / 128.0, 0, 0]]), rng.random((300, 3))])
+    qg = qg.astype(np.float32).astype(np.float64)
+    qd = {"id": np.arange(len(qg), dtype=np.uint32), "x": qg[:, 0], "y": qg[:, 1], "z": qg[:, 2]}
+    jobs, wants = [], []
+    for k, r in ((64, radius), (8, radius), (1, radius), (64, 1.0 / 128.0)):
+        jobs.append(B.BatchJob(scene, B.JOB_FIXED_RADIUS, _dev(qg), radius=r, k=k))
+        wants.append(oracle.neighbors("frn", qd, sdict, radius=r, k=k))
+    _say("  run_batch: 4 fixed-radius jobs on bucket edges")
+    results, _ = B.run_batch(jobs)
+    for job, res, want in zip(jobs, results, wants):
+        assert res["status"] == 0
+        rows = B.job_rows(job, res)
+        _check_frn_order(rows)
+        _check_nn(rows, want, False)
+
+
 def test_batch_knn_small_scene_and_duplicates(gpu_ready):
     _say("case: batched KNN with k above the scene size and with duplicate points")
     rng = np.random.default_rng(93)

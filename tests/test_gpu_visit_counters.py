@@ -60,3 +60,30 @@ def test_ray_anyhit_visit_counters_equal_a_host_replay(gpu_ready):
         nodes += n
         prims += p
     assert (nodes, prims) == visits, ((nodes, prims), visits)
+
+
+def test_fixed_radius_visit_counters_equal_a_host_replay(gpu_ready):
+    _say("case: fixed-radius count walk visit counters vs host replay")
+    rng = np.random.default_rng(98)
+    centres = rng.random((64, 3))
+    pts = cases.points(rng, 60000, 3, kind="mixture", centres=centres)
+    s3 = np.stack([pts["x"], pts["y"], pts["z"]], 1).astype(np.float32)
+    # 300 copies of one point: queries next to it need more than one count round
+    s3 = np.concatenate([s3, np.repeat(s3[:1], 300, axis=0)])
+    q = cases.points(rng, 1100, 3, kind="mixture", centres=centres)
+    q3 = np.stack([q["x"], q["y"], q["z"]], 1).astype(np.float32)
+    q3 = np.concatenate([q3, s3[:1] + rng.normal(0.0, 1e-4, (20, 3)).astype(np.float32)])
+    scene = B.prepare_point_instances(DeviceBuffer.from_numpy(s3), [0, len(s3)])
+    sc = W.export_scene(scene.handle)
+    assert sc["exact"] is None  # FP32 device input: the FP32 copy is exact
+    for radius, k in ((0.05, 64), (0.01, 16), (0.02, 1)):
+        _say(f"  radius {radius}, k_max {k}")
+        job = B.BatchJob(scene, B.JOB_FIXED_RADIUS, DeviceBuffer.from_numpy(q3), radius=radius, k=k)
+        res, _, visits = B.run_batch_profiled([job], count_visits=True)
+        assert res[0]["status"] == 0
+        nodes = prims = 0
+        for row in q3.astype(np.float64):
+            n, p = W.frn_counts(sc, row, radius, k)
+            nodes += n
+            prims += p
+        assert (nodes, prims) == visits, (radius, k, (nodes, prims), visits)

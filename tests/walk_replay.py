@@ -3,8 +3,9 @@
 It restates, step by step and with the same FP32 / FP64 arithmetic, the
 device walks whose in-kernel counters bench.py turns into algorithmic bytes:
 the KNN nearest-child-first walk with its packed k-list
-(rtdl_amd/csrc/traverse.hpp: walk_nearest, collect_packed) and the ray
-any-hit walk (trace_ray, mode 0).  A replay returns the nodes fetched and the
+(rtdl_amd/csrc/traverse.hpp: walk_nearest, collect_packed), the ray
+any-hit walk (trace_ray, mode 0) and the fixed-radius count walk with its
+distance histogram (rtdl_amd/csrc/engine.hip: nbr_count_query).  A replay returns the nodes fetched and the
 primitives tested, which must equal the kernels' counters.
 """
 from __future__ import annotations
@@ -207,6 +208,101 @@ def knn_counts(sc: dict, q, k: int, *, instance: int = 0, stack_depth: int = 32)
                 leaf_tests(link, True)
             node = int(nodes[node]["escape"])
     return state["nodes"], state["prims"]
+
+
+# ----------------------------------------------------------- fixed radius --
+
+NBR_BUCKETS = 32
+NBR_CAND_MAX = 128
+NBR_COUNT_MAX = 0xFFFF
+
+
+def _nbr_bucket(dist: float, scale: float) -> int:
+    x = dist * scale
+    if x < NBR_BUCKETS:
+        return int(x) if x >= 0.0 else 0
+    return NBR_BUCKETS - 1
+
+
+def _nbr_bucket_edge(b: int, scale: float) -> float:
+    return float(b + 1) / scale * 1.0000000000009094947
+
+
+def frn_counts(sc: dict, q, radius: float, k: int, *, instance: int = 0) -> tuple[int, int]:
+    """Nodes fetched and primitives tested by the batched fixed-radius count
+    walk of query q (FP64 coordinates of FP32 values), over all its count
+    rounds: a rope preorder walk whose threshold shrinks to the edge of the
+    distance bucket holding the k-th neighbour, checked every max(k/2, 4)
+    neighbours, recounted over the finer buckets of that ball while the
+    buckets up to it hold more than 128 neighbours (at most four rounds)."""
+    nodes, prims, exact = sc["nodes"], sc["prims"], sc["exact"]
+    root, end = int(sc["roots"][instance]), int(sc["ends"][instance])
+    qx, qy, qz = (float(v) for v in q)
+    qf = (F32(qx), F32(qy), F32(qz))
+    m = coord_bound(sc["m"], qx, qy, qz)
+    rad, rad_sq = radius, radius * radius
+    step = k // 2 if k // 2 > 4 else 4
+    n_nodes = n_prims = 0
+    for rnd in range(4):
+        hist = [0] * NBR_BUCKETS
+        scale = float(NBR_BUCKETS) / rad
+        st = {"count": 0, "top": NBR_BUCKETS - 1, "next": k, "shrunk": False,
+              "thr2": frn_threshold2(rad, m)}
+
+        def visit(slot: int) -> None:
+            if exact is not None:
+                px, py, pz = exact[3 * slot], exact[3 * slot + 1], exact[3 * slot + 2]
+            else:
+                px, py, pz = (float(v) for v in prims[slot, :3])
+            dx, dy, dz = px - qx, py - qy, pz - qz
+            d2 = dx * dx + dy * dy + dz * dz
+            if d2 > rad_sq:
+                return
+            b = _nbr_bucket(math.sqrt(d2), scale)
+            if b > st["top"]:
+                return
+            st["count"] += 1
+            hist[b] += 1
+            if st["count"] < st["next"]:
+                return
+            st["next"] = st["count"] + step
+            cum = 0
+            for c in range(st["top"]):
+                cum += hist[c]
+                if cum >= k:
+                    st["top"] = c
+                    st["shrunk"] = True
+                    st["thr2"] = frn_threshold2(min(rad, _nbr_bucket_edge(c, scale)), m)
+                    break
+
+        node = root
+        while root != end and node != end:
+            n_nodes += 1
+            nd = nodes[node]
+            link = int(nd["link"])
+            inside = box_dist2(qf, nd["lo"], nd["hi"]) <= st["thr2"]
+            if inside and link & LEAF:
+                for slot in leaf_range(link):
+                    n_prims += 1
+                    p = prims[slot]
+                    fx, fy, fz = F32(p[0] - qf[0]), F32(p[1] - qf[1]), F32(p[2] - qf[2])
+                    if fma32(fz, fz, fma32(fy, fy, F32(fx * fx))) <= st["thr2"]:
+                        visit(slot)
+            node = link if (inside and not link & LEAF) else int(nd["escape"])
+        count, top = st["count"], st["top"]
+        if not (st["shrunk"] or count > k) or count > NBR_COUNT_MAX:
+            break
+        bucket, n_cand, cum = top, 0, 0
+        for c in range(top + 1):
+            cum += hist[c]
+            if cum >= k:
+                bucket, n_cand = c, cum
+                break
+        if n_cand <= NBR_CAND_MAX:
+            break
+        rad = min(rad, _nbr_bucket_edge(bucket, scale))
+        rad_sq = min(rad_sq, rad * rad)
+    return n_nodes, n_prims
 
 
 # ------------------------------------------------------------------- rays --
