@@ -345,6 +345,69 @@ def plain_per_call_leg(rtm, scenes, stream, offsets, host, budget_s: float):
             "queries_per_s": done_q / spent if spent else 0.0}
 
 
+def per_call_configs_leg(rtm):
+    """The plain per-call backend at the two single-GPU per-call shapes of the
+    reference's benchmark list: 3-D ray any-hit, 2,000,000 triangles x
+    1,000,000 rays (tmax 0.5), and KNN k = 16, 4,000,000 x 4,000,000 3-D
+    Gaussian-mixture points; FP32 values, seeds 20261203 + 1 and + 2.  Each
+    runs once untimed, then once timed: host records in, LBVH built, walked
+    and rows copied back to the host, with the library's phase timings."""
+    out = {}
+    rng = np.random.default_rng(20261203 + 1)
+    c = rng.random((2_000_000, 3))
+    v = (c[:, None, :] + rng.normal(0.0, 0.005, (2_000_000, 3, 3))).reshape(-1, 9)
+    tris = np.zeros(2_000_000, dtype=rtm.TRIANGLE3D_DTYPE)
+    tris["id"] = np.arange(2_000_000)
+    for j, f in enumerate(["x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2"]):
+        tris[f] = v[:, j].astype(np.float32)
+    o = rng.random((1_000_000, 3))
+    d = rng.normal(size=(1_000_000, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = np.zeros(1_000_000, dtype=rtm.RAY3D_DTYPE)
+    rays["id"] = np.arange(1_000_000)
+    for j, f in enumerate(["ox", "oy", "oz"]):
+        rays[f] = o[:, j].astype(np.float32)
+    for j, f in enumerate(["dx", "dy", "dz"]):
+        rays[f] = d[:, j].astype(np.float32)
+    rays["tmax"] = 0.5
+    p = rtm._ptr
+    for rep in range(2):
+        t0 = time.perf_counter()
+        rows = rtm._call_rows("rtdl_mi355x_run_ray_anyhit_3d", rtm.ANYHIT_ROW_DTYPE, p(rays), len(rays),
+                              p(tris), len(tris))
+        wall = time.perf_counter() - t0
+    ph = rtm.last_phase_timings()
+    out["ray_anyhit_2M_triangles_1M_rays"] = {
+        "wall_s": wall, "rays_per_s": len(rays) / wall, "hits": int(rows["any_hit"].sum()),
+        "phases_ms": {k: v * 1e3 for k, v in ph.items()},
+        "device_rays_per_s": len(rays) / ph["traverse"] if ph.get("traverse") else None}
+    log(f"per-call ray any-hit 2M x 1M: {wall*1e3:.1f} ms wall, phases {out['ray_anyhit_2M_triangles_1M_rays']['phases_ms']}")
+    del rows, tris, rays, v, c, o, d
+
+    rng = np.random.default_rng(20261203 + 2)
+    centres = rng.random((64, 3))
+    pts = []
+    for _ in range(2):
+        xyz = (centres[rng.integers(0, 64, 4_000_000)] + rng.normal(0.0, 0.02, (4_000_000, 3))).astype(np.float32)
+        rec = np.zeros(4_000_000, dtype=rtm.POINT3D_DTYPE)
+        rec["id"] = np.arange(4_000_000)
+        rec["x"], rec["y"], rec["z"] = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+        pts.append(rec)
+    search, query = pts
+    for rep in range(2):
+        t0 = time.perf_counter()
+        rows = rtm._call_rows("rtdl_mi355x_run_knn_rows_3d", rtm.KNN_ROW_DTYPE, p(query), len(query),
+                              p(search), len(search), 16)
+        wall = time.perf_counter() - t0
+    ph = rtm.last_phase_timings()
+    out["knn16_4M_x_4M_mixture"] = {
+        "wall_s": wall, "queries_per_s": len(query) / wall, "rows": int(len(rows)),
+        "phases_ms": {k: v * 1e3 for k, v in ph.items()},
+        "device_queries_per_s": len(query) / ph["traverse"] if ph.get("traverse") else None}
+    log(f"per-call KNN k=16 4M x 4M: {wall*1e3:.1f} ms wall, phases {out['knn16_4M_x_4M_mixture']['phases_ms']}")
+    return out
+
+
 def per_request_resident_leg(B, scenes, stream, offsets, arenas, budget_s: float):
     """Resident scenes, one synchronous single-job batch per request."""
     kind_map = {0: B.JOB_FIXED_RADIUS, 1: B.JOB_KNN, 2: B.JOB_RAY_ANYHIT, 3: B.JOB_PIP_POSITIVE}
@@ -588,6 +651,7 @@ def main() -> int:
         baselines["cpu_python_reference"] = python_baseline_leg()
         log(f"python reference: {baselines['cpu_python_reference']}")
         baselines["embree"] = embree_status()
+        baselines["per_call_configs"] = per_call_configs_leg(rtm)
 
     plain = baselines.get("plain_per_call", {}).get("queries_per_s")
     line = {

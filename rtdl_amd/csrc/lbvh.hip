@@ -14,8 +14,11 @@
 //                the highest node whose range starts at leaf b + 1 (found by
 //                an atomicMax over range lengths), so traversal needs no
 //                stack;
-//   6. refit:    bottom-up box unions, one climbing thread per leaf, the
-//                second thread to reach a node continues.
+//   6. refit:    bottom-up box unions in rounds, one launch per tree level:
+//                a node is refit in the launch after both children are
+//                done, so no thread waits on another and no fence is needed
+//                (an agent-scope fence writes back the L2 of the thread's
+//                XCD, which made a climbing-thread refit the slowest step).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -332,11 +335,10 @@ __global__ void links_kernel(uint32_t m, const uint32_t* left, const uint32_t* r
   nodes[k].escape = escape_after(range_hi[k], m, highest);
 }
 
-__global__ void refit_kernel(uint32_t m, const uint32_t* leaf_first, const uint32_t* leaf_count,
-                             const float4* sorted_lo, const float4* sorted_hi,
-                             const unsigned long long* highest, const uint32_t* parent,
-                             const uint32_t* left, const uint32_t* right, uint32_t* visits,
-                             BvhNode* nodes) {
+// Leaf records: box of the leaf's primitives, first slot and count, rope.
+__global__ void leaf_nodes_kernel(uint32_t m, const uint32_t* leaf_first, const uint32_t* leaf_count,
+                                  const float4* sorted_lo, const float4* sorted_hi,
+                                  const unsigned long long* highest, BvhNode* nodes) {
   const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= m) return;
   const uint32_t first = leaf_first[l], count = leaf_count[l];
@@ -351,7 +353,6 @@ __global__ void refit_kernel(uint32_t m, const uint32_t* leaf_first, const uint3
     hi[1] = fmaxf(hi[1], b.y);
     hi[2] = fmaxf(hi[2], b.z);
   }
-  uint32_t node = (m - 1) + l;
   BvhNode out;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -360,21 +361,32 @@ __global__ void refit_kernel(uint32_t m, const uint32_t* leaf_first, const uint3
   }
   out.link = kLeafFlag | ((count - 1) << 28) | first;
   out.escape = escape_after(l, m, highest);
-  nodes[node] = out;
-  while (node != 0) {
-    const uint32_t p = parent[node];
-    __threadfence();
-    if (atomicAdd(&visits[p], 1u) == 0) return;  // sibling not finished yet
-    __threadfence();
-    const BvhNode a = nodes[left[p]];
-    const BvhNode b = nodes[right[p]];
+  nodes[(m - 1) + l] = out;
+}
+
+constexpr uint32_t kNotRefit = 0xFFFFFFFFu;
+
+// One round of the bottom-up refit: internal node k takes the union of its
+// children's boxes once both were finished in earlier launches (leaves from
+// the start, internal children with done[] < round), and records the round.
+// A node finishes in the round equal to its height, so the rounds need no
+// atomics or fences: every box a round reads was written by an earlier
+// launch on the same stream.
+__global__ void refit_round_kernel(uint32_t n_internal, uint32_t round, const uint32_t* left,
+                                   const uint32_t* right, uint32_t* done, BvhNode* nodes) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_internal || done[k] != kNotRefit) return;
+  const uint32_t a = left[k], b = right[k];
+  const bool ready_a = a >= n_internal || done[a] < round;
+  const bool ready_b = b >= n_internal || done[b] < round;
+  if (!(ready_a && ready_b)) return;
+  const BvhNode na = nodes[a], nb = nodes[b];
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      nodes[p].lo[ax] = fminf(a.lo[ax], b.lo[ax]);
-      nodes[p].hi[ax] = fmaxf(a.hi[ax], b.hi[ax]);
-    }
-    node = p;
+  for (int ax = 0; ax < 3; ++ax) {
+    nodes[k].lo[ax] = fminf(na.lo[ax], nb.lo[ax]);
+    nodes[k].hi[ax] = fmaxf(na.hi[ax], nb.hi[ax]);
   }
+  done[k] = round;
 }
 
 // Root of each instance: the node whose leaf range is exactly the instance's.
@@ -580,10 +592,10 @@ void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* c
   const size_t n_internal = m - 1;
   DeviceArray<uint32_t> left(n_internal ? n_internal : 1), right(n_internal ? n_internal : 1);
   DeviceArray<uint32_t> parent(n_nodes), range_lo(n_internal ? n_internal : 1),
-      range_hi(n_internal ? n_internal : 1), visits(n_internal ? n_internal : 1);
+      range_hi(n_internal ? n_internal : 1), done(n_internal ? n_internal : 1);
   DeviceArray<unsigned long long> highest(m);
   RTDL_HIP(hipMemsetAsync(highest.get(), 0, m * sizeof(unsigned long long), stream));
-  RTDL_HIP(hipMemsetAsync(visits.get(), 0, visits.bytes(), stream));
+  RTDL_HIP(hipMemsetAsync(done.get(), 0xFF, done.bytes(), stream));
   RTDL_HIP(hipMemsetAsync(parent.get(), 0xFF, parent.bytes(), stream));
   if (n_internal) {
     karras_kernel<<<blocks_for(n_internal, 256), 256, 0, stream>>>(
@@ -594,11 +606,28 @@ void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* c
         (uint32_t)m, left.get(), range_hi.get(), highest.get(), scene.nodes.get());
     RTDL_LAUNCH_CHECK();
   }
-  // 5./6. leaf records, ropes and bottom-up refit
-  refit_kernel<<<blocks_for(m, 256), 256, 0, stream>>>(
-      (uint32_t)m, leaf_first.get(), leaf_count.get(), sorted_lo.get(), sorted_hi.get(),
-      highest.get(), parent.get(), left.get(), right.get(), visits.get(), scene.nodes.get());
+  // 5. leaf records and ropes
+  leaf_nodes_kernel<<<blocks_for(m, 256), 256, 0, stream>>>(
+      (uint32_t)m, leaf_first.get(), leaf_count.get(), sorted_lo.get(), sorted_hi.get(), highest.get(),
+      scene.nodes.get());
   RTDL_LAUNCH_CHECK();
+  // 6. bottom-up refit, one launch per tree level, in groups of 16 launches
+  // until the root (internal node 0) is done; a Karras tree over 64-bit keys
+  // with index tie-breaks is at most about 100 levels deep
+  if (n_internal) {
+    uint32_t root_done = kNotRefit;
+    uint32_t round = 1;
+    while (root_done == kNotRefit) {
+      require(round < 4096, "LBVH refit did not reach the root (malformed hierarchy)");
+      for (int r = 0; r < 16; ++r, ++round) {
+        refit_round_kernel<<<blocks_for(n_internal, 256), 256, 0, stream>>>(
+            (uint32_t)n_internal, round, left.get(), right.get(), done.get(), scene.nodes.get());
+        RTDL_LAUNCH_CHECK();
+      }
+      RTDL_HIP(hipMemcpyAsync(&root_done, done.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+      RTDL_HIP(hipStreamSynchronize(stream));
+    }
+  }
   roots_kernel<<<blocks_for(n_inst, 64), 64, 0, stream>>>(
       n_inst, (uint32_t)m, d_leaf_offsets.get(), parent.get(), range_lo.get(), range_hi.get(),
       scene.nodes.get(), scene.roots.get(), scene.ends.get());

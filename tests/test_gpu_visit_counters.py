@@ -87,3 +87,45 @@ def test_fixed_radius_visit_counters_equal_a_host_replay(gpu_ready):
             nodes += n
             prims += p
         assert (nodes, prims) == visits, (radius, k, (nodes, prims), visits)
+
+
+def _check_boxes(sc, leaf_points):
+    nodes = sc["nodes"]
+    lo = nodes["lo"].astype(np.float32)
+    hi = nodes["hi"].astype(np.float32)
+    link = nodes["link"].astype(np.int64)
+    leaf = (link & W.LEAF) != 0
+    internal = np.flatnonzero(~leaf)
+    left = link[internal]
+    right = nodes["escape"][left].astype(np.int64)  # the right child follows the left subtree
+    assert np.array_equal(lo[internal], np.minimum(lo[left], lo[right]))
+    assert np.array_equal(hi[internal], np.maximum(hi[left], hi[right]))
+    first = link[leaf] & W.START_MASK
+    count = ((link[leaf] >> 28) & 7) + 1
+    assert count.sum() == len(leaf_points)
+    llo, lhi = lo[leaf], hi[leaf]
+    for j in range(int(count.max())):
+        ok = j < count
+        p = leaf_points[first[ok] + j]
+        assert np.all(llo[ok] <= p) and np.all(lhi[ok] >= p)
+    return int(len(internal))
+
+
+def test_lbvh_node_boxes_are_the_unions_of_their_children(gpu_ready):
+    _say("case: LBVH refit: internal boxes equal the union of the children, leaves hold their primitives")
+    rng = np.random.default_rng(99)
+    centres = rng.random((64, 3))
+    for n, kind in ((100000, "uniform"), (250001, "mixture"), (1, "uniform"), (5, "uniform")):
+        pts = cases.points(rng, n, 3, kind=kind, centres=centres)
+        s3 = np.stack([pts["x"], pts["y"], pts["z"]], 1).astype(np.float32)
+        s3[: n // 3] = s3[0]  # duplicate keys: index tie-breaks deepen the tree
+        scene = B.prepare_point_instances(DeviceBuffer.from_numpy(s3), [0, n])
+        sc = W.export_scene(scene.handle)
+        _check_boxes(sc, sc["prims"][:, :3])
+    t = cases.triangles(rng, 50000, sigma=0.01)
+    scene = B.prepare_triangle_instances(DeviceBuffer.from_numpy(cases.tri_matrix(t).astype(np.float32)),
+                                         [0, len(t["id"])])
+    sc = W.export_scene(scene.handle)
+    v = sc["prims"].reshape(-1, 3, 4)[:, :, :3]
+    _check_boxes(sc, v.min(axis=1))
+    _check_boxes(sc, v.max(axis=1))
