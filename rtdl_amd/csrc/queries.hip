@@ -48,15 +48,30 @@ template <int K, bool COUNT_ALL>
 __global__ __launch_bounds__(256) void collect_exact_kernel(
     SceneView s, const double* __restrict__ q, const uint32_t* __restrict__ order, uint32_t nq,
     double radius, uint32_t k, double* __restrict__ list_d, uint32_t* __restrict__ list_id,
-    uint32_t* __restrict__ count, uint32_t* __restrict__ total) {
+    uint32_t* __restrict__ count, uint32_t* __restrict__ total, bool seeded) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   LaneStats st;
   const uint32_t qi = order ? order[t] : t;
   const double qx = q[3 * (size_t)qi], qy = q[3 * (size_t)qi + 1], qz = q[3 * (size_t)qi + 2];
+  // Seeded: the row holds the FP64 distances of k distinct points found by
+  // the packed walk, so the k nearest (and every point tied with the k-th)
+  // lie within the largest of them; the factor 1 + 2^-40 keeps each of those
+  // points inside the squared radius after rounding.
+  double rq = radius;
+  if (seeded && count[qi] == k) {
+    double dmax = 0.0;
+    bool finite = true;
+    for (uint32_t i = 0; i < k; ++i) {
+      const double d = list_d[(size_t)qi * k + i];
+      finite = finite && d < __builtin_inf();
+      dmax = d > dmax ? d : dmax;
+    }
+    if (finite) rq = fmin(radius, dmax * 1.0000000000009094947);
+  }
   ExactTopK<K> list;
   uint32_t inside = 0;
-  collect_exact<K, COUNT_ALL, false>(s, s.root, s.end, qx, qy, qz, radius, k, list, inside, st);
+  collect_exact<K, COUNT_ALL, false>(s, s.root, s.end, qx, qy, qz, rq, k, list, inside, st);
   uint32_t c = 0;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
@@ -201,9 +216,9 @@ void packed_t(const SceneView& s, const double* q, const uint32_t* order, uint32
 template <int K, bool COUNT_ALL>
 void exact_t(const SceneView& s, const double* q, const uint32_t* order, uint32_t nq, double radius,
              uint32_t k, double* list_d, uint32_t* list_id, uint32_t* count, uint32_t* total,
-             hipStream_t stream) {
+             bool seeded, hipStream_t stream) {
   collect_exact_kernel<K, COUNT_ALL><<<blocks_for(nq, 64), 64, 0, stream>>>(
-      s, q, order, nq, radius, k, list_d, list_id, count, total);
+      s, q, order, nq, radius, k, list_d, list_id, count, total, seeded);
   RTDL_LAUNCH_CHECK();
 }
 
@@ -245,23 +260,23 @@ void launch_collect_packed(const SceneView& s, const double* q, const uint32_t* 
 
 void launch_collect_exact(const SceneView& s, const double* q, const uint32_t* order, uint32_t nq,
                           double radius, uint32_t k, bool count_all, double* list_d,
-                          uint32_t* list_id, uint32_t* count, uint32_t* total,
+                          uint32_t* list_id, uint32_t* count, uint32_t* total, bool seeded,
                           hipStream_t stream) {
   if (!nq) return;
   require(k >= 1 && k <= kMaxListK, "collect k must be in [1, 64]");
   if (count_all) {
     require(k == kMaxListK, "count_all collection needs k == 64");
-    exact_t<64, true>(s, q, order, nq, radius, k, list_d, list_id, count, total, stream);
+    exact_t<64, true>(s, q, order, nq, radius, k, list_d, list_id, count, total, seeded, stream);
     return;
   }
   switch (list_capacity(k)) {
-    case 1: exact_t<1, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, stream); break;
-    case 2: exact_t<2, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, stream); break;
-    case 4: exact_t<4, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, stream); break;
-    case 8: exact_t<8, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, stream); break;
-    case 16: exact_t<16, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, stream); break;
-    case 32: exact_t<32, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, stream); break;
-    default: exact_t<64, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, stream); break;
+    case 1: exact_t<1, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, seeded, stream); break;
+    case 2: exact_t<2, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, seeded, stream); break;
+    case 4: exact_t<4, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, seeded, stream); break;
+    case 8: exact_t<8, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, seeded, stream); break;
+    case 16: exact_t<16, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, seeded, stream); break;
+    case 32: exact_t<32, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, seeded, stream); break;
+    default: exact_t<64, false>(s, q, order, nq, radius, k, list_d, list_id, count, total, seeded, stream); break;
   }
 }
 

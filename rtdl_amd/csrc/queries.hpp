@@ -33,9 +33,12 @@ void launch_collect_packed(const SceneView& s, const double* q, const uint32_t* 
                            unsigned long long* stats, int block, hipStream_t stream);
 // Exact FP64 selection; with count_all (k must be 64) the radius never
 // shrinks and total[query] receives the number of candidates in the radius.
+// Seeded: a query whose row already holds k FP64 distances (from the packed
+// kernel) searches only the ball of the largest of them.
 void launch_collect_exact(const SceneView& s, const double* q, const uint32_t* order, uint32_t nq,
                           double radius, uint32_t k, bool count_all, double* list_d,
-                          uint32_t* list_id, uint32_t* count, uint32_t* total, hipStream_t stream);
+                          uint32_t* list_id, uint32_t* count, uint32_t* total, bool seeded,
+                          hipStream_t stream);
 
 // Writes every candidate inside the radius for the queries listed in
 // `queries` (n entries) into [offset[i], offset[i + 1]) of out_d / out_id,

@@ -6,6 +6,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -202,16 +204,89 @@ void* host_rows(size_t count, size_t row_bytes) {
   return rows;
 }
 
+namespace {
+
+// Pinned staging for large row copies.  A device-to-host copy into pageable
+// memory runs at a few GB/s; rows go instead through two pinned 32 MiB
+// buffers, each chunk's device copy overlapping the host copy of the one
+// before, with the host copies split over threads.  One caller at a time
+// uses the buffers; a concurrent caller (or another device) copies directly.
+constexpr size_t kStageChunk = 32u << 20;
+constexpr size_t kStageMin = 64u << 20;
+constexpr unsigned kStageThreads = 8;
+
+struct RowStage {
+  std::mutex mu;
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int device = -1;
+};
+
+RowStage& row_stage() {
+  static RowStage* stage = new RowStage();  // lives until the process ends
+  return *stage;
+}
+
+void parallel_memcpy(uint8_t* dst, const uint8_t* src, size_t bytes) {
+  const size_t part = (bytes + kStageThreads - 1) / kStageThreads;
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < kStageThreads && t * part < bytes; ++t) {
+    const size_t a = t * part, n = std::min(part, bytes - a);
+    pool.emplace_back([=] { std::memcpy(dst + a, src + a, n); });
+  }
+  std::memcpy(dst, src, std::min(part, bytes));
+  for (auto& th : pool) th.join();
+}
+
+// Returns false (nothing copied) when the staging buffers are busy or belong
+// to another device.
+bool staged_copy(uint8_t* dst, const uint8_t* src, size_t bytes, hipStream_t stream) {
+  RowStage& st = row_stage();
+  std::unique_lock<std::mutex> lock(st.mu, std::try_to_lock);
+  if (!lock.owns_lock()) return false;
+  int dev = 0;
+  RTDL_HIP(hipGetDevice(&dev));
+  if (st.device < 0) {
+    for (int i = 0; i < 2; ++i) {
+      RTDL_HIP(hipHostMalloc(&st.buf[i], kStageChunk, hipHostMallocDefault));
+      RTDL_HIP(hipEventCreateWithFlags(&st.ev[i], hipEventDisableTiming));
+    }
+    st.device = dev;
+  }
+  if (st.device != dev) return false;
+  const size_t n_chunks = (bytes + kStageChunk - 1) / kStageChunk;
+  auto issue = [&](size_t c) {
+    const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
+    RTDL_HIP(hipMemcpyAsync(st.buf[c & 1], src + a, n, hipMemcpyDeviceToHost, stream));
+    RTDL_HIP(hipEventRecord(st.ev[c & 1], stream));
+  };
+  issue(0);
+  for (size_t c = 0; c < n_chunks; ++c) {
+    // the buffer of chunk c + 1 held chunk c - 1, already copied out below
+    if (c + 1 < n_chunks) issue(c + 1);
+    RTDL_HIP(hipEventSynchronize(st.ev[c & 1]));
+    const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
+    parallel_memcpy(dst + a, static_cast<const uint8_t*>(st.buf[c & 1]), n);
+  }
+  return true;
+}
+
+}  // namespace
+
 void copy_rows_back(const void* device_rows, size_t count, size_t row_bytes, void** rows_out,
                     size_t* n_out, hipStream_t stream) {
   void* rows = host_rows(count, row_bytes);
   if (count) {
-    hipError_t err =
-        hipMemcpyAsync(rows, device_rows, count * row_bytes, hipMemcpyDeviceToHost, stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(stream);
-    if (err != hipSuccess) {
+    const size_t bytes = count * row_bytes;
+    try {
+      if (bytes < kStageMin ||
+          !staged_copy(static_cast<uint8_t*>(rows), static_cast<const uint8_t*>(device_rows), bytes, stream)) {
+        RTDL_HIP(hipMemcpyAsync(rows, device_rows, bytes, hipMemcpyDeviceToHost, stream));
+        RTDL_HIP(hipStreamSynchronize(stream));
+      }
+    } catch (const Error& e) {
       std::free(rows);
-      throw Error(std::string("row copy-back failed: ") + hipGetErrorString(err));
+      throw Error(std::string("row copy-back failed: ") + e.what());
     }
   }
   *rows_out = rows;
@@ -439,7 +514,7 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
     // queries whose FP32 selection keys tie at the boundary are recomputed
     // with the exact FP64 list (same slots, same counts)
     launch_collect_exact(view, q.get(), amb_list.get(), n_amb, search_radius, k, false,
-                         cand_d.get(), cand_id.get(), count.get(), nullptr, s);
+                         cand_d.get(), cand_id.get(), count.get(), nullptr, /*seeded=*/true, s);
   } else {
     // Keep the 64 nearest per query and count everything inside the radius;
     // queries with more than 64 candidates are collected in full and sorted.
@@ -448,7 +523,7 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
     DeviceArray<double> list_d((size_t)nq * kMaxListK);
     DeviceArray<uint32_t> list_id((size_t)nq * kMaxListK);
     launch_collect_exact(view, q.get(), order.get(), nq, search_radius, kMaxListK, true,
-                         list_d.get(), list_id.get(), count.get(), total.get(), s);
+                         list_d.get(), list_id.get(), count.get(), total.get(), /*seeded=*/false, s);
     std::vector<uint32_t> host_total(nq);
     RTDL_HIP(hipMemcpyAsync(host_total.data(), total.get(), nq * sizeof(uint32_t),
                             hipMemcpyDeviceToHost, s));

@@ -23,6 +23,7 @@ import ctypes.util
 import functools
 import math
 import os
+import weakref
 from collections.abc import Mapping
 
 import numpy as np
@@ -215,14 +216,15 @@ def _call_rows(fn_name: str, row_dtype: np.dtype, *args) -> np.ndarray:
     if status != 0:
         detail = error.value.decode("utf-8", errors="replace")
         raise RuntimeError(f"{fn_name} failed with status {status}: {detail}")
-    try:
-        n = count.value
-        if n == 0:
-            return np.zeros(0, dtype=row_dtype)
-        buf = (ctypes.c_char * (n * row_dtype.itemsize)).from_address(rows_ptr.value)
-        return np.frombuffer(buf, dtype=row_dtype).copy()
-    finally:
+    n = count.value
+    if n == 0:
         lib.rtdl_mi355x_free_rows(rows_ptr)
+        return np.zeros(0, dtype=row_dtype)
+    # Zero-copy: the array views the library's row buffer, which is freed
+    # with rtdl_mi355x_free_rows once the array and every view of it are gone.
+    buf = (ctypes.c_char * (n * row_dtype.itemsize)).from_address(rows_ptr.value)
+    weakref.finalize(buf, lib.rtdl_mi355x_free_rows, ctypes.c_void_p(rows_ptr.value))
+    return np.frombuffer(buf, dtype=row_dtype)
 
 
 def _call_prepare(fn_name: str, *args) -> ctypes.c_void_p:
