@@ -539,6 +539,8 @@ def main() -> int:
                     help="only build the scenes and streams of a full run and measure this kind's "
                          "roofline batch (the batch the bench line's roofline is quoted on), for "
                          "profiling that batch alone")
+    ap.add_argument("--per-call-only", action="store_true",
+                    help="run only the per-call 2M x 1M ray any-hit and 4M x 4M KNN k=16 calls")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -564,6 +566,9 @@ def main() -> int:
 
     info = rtm.mi355x_context_probe()
     log(f"device {info}")
+    if args.per_call_only:
+        print(json.dumps({"per_call_configs": per_call_configs_leg(rtm)}), flush=True)
+        return 0
     rng = np.random.default_rng(SEED)
     scenes, centres = make_scenes(B, rng, args.max_prims)
     scene_slot = {s["index"]: s["index"] for s in scenes}
