@@ -511,6 +511,79 @@ def python_baseline_leg():
             "sample": f"fixed-radius 4096 x 4096 2-D points, r = 0.02: {len(rows)} rows in {dt:.1f} s"}
 
 
+def python_subsample_leg():
+    """The pure-Python FP64 restatement of the reference, single-threaded, on
+    fixed-seed subsamples of the other benchmark shapes.  Its per-query
+    work is a scan over every primitive, so each full-size figure is a
+    linear extrapolation in the primitive count (labelled as such): exact
+    for KNN up to its sort's log factor, an upper bound for any-hit (a
+    ray stops at its first hit in input order)."""
+    from oracle import cpu_python_reference as ref
+    from rtdl_amd.records import Point, Point3D, Polygon, Ray3D, Triangle3D
+
+    out = {}
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        res = fn()
+        return res, time.perf_counter() - t0
+
+    # ray any-hit, 2M triangles x 1M rays: 16 rays x 20,000 triangles
+    rng = np.random.default_rng(20261203 + 1)
+    c = rng.random((20000, 3))
+    v = (c[:, None, :] + rng.normal(0.0, 0.005, (20000, 3, 3))).reshape(-1, 9).astype(np.float32).astype(float)
+    tris = [Triangle3D(i, *map(float, row)) for i, row in enumerate(v)]
+    o = rng.random((16, 3))
+    d = rng.normal(size=(16, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = [Ray3D(i, *map(float, np.float32(o[i])), *map(float, np.float32(d[i])), 0.5) for i in range(16)]
+    rows, dt = timed(lambda: ref.ray_triangle_any_hit_cpu(rays, tris))
+    per_ray = dt / len(rays)
+    out["ray_anyhit_2M_triangles_1M_rays"] = {
+        "sample": f"16 rays x 20,000 triangles: {sum(r['any_hit'] for r in rows)} hits in {dt:.2f} s",
+        "rays_per_s_at_sample": 1.0 / per_ray,
+        "extrapolated_full_s_upper_bound": per_ray * (2_000_000 / 20000) * 1_000_000}
+    # KNN k = 16, 4M x 4M mixture points: 8 queries x 50,000 points
+    rng = np.random.default_rng(20261203 + 2)
+    centres = rng.random((64, 3))
+    sp = (centres[rng.integers(0, 64, 50000)] + rng.normal(0.0, 0.02, (50000, 3))).astype(np.float32).astype(float)
+    qp = (centres[rng.integers(0, 64, 8)] + rng.normal(0.0, 0.02, (8, 3))).astype(np.float32).astype(float)
+    search = [Point3D(i, *row) for i, row in enumerate(sp.tolist())]
+    queries = [Point3D(i, *row) for i, row in enumerate(qp.tolist())]
+    _, dt = timed(lambda: ref.knn_rows_cpu(queries, search, k=16))
+    per_q = dt / len(queries)
+    out["knn16_4M_x_4M_mixture"] = {
+        "sample": f"8 queries x 50,000 points in {dt:.2f} s",
+        "queries_per_s_at_sample": 1.0 / per_q,
+        "extrapolated_full_s": per_q * (4_000_000 / 50000) * 4_000_000}
+    # PIP positive hits against one instance's 1,000 star polygons: 400 points
+    rng = np.random.default_rng(20261203 + 3)
+    polys = []
+    for i in range(1000):
+        nv = int(rng.integers(8, 33))
+        ang = np.sort(rng.random(nv) * 2.0 * np.pi)
+        rad = rng.uniform(0.005, 0.02, nv)
+        cx, cy = rng.random(2)
+        xy = np.stack([cx + rad * np.cos(ang), cy + rad * np.sin(ang)], 1).astype(np.float32).astype(float)
+        polys.append(Polygon(i, tuple(map(tuple, xy.tolist()))))
+    pts = [Point(i, float(x), float(y)) for i, (x, y) in enumerate(rng.random((400, 2)).astype(np.float32).astype(float))]
+    rows, dt = timed(lambda: ref.pip_cpu(pts, polys, result_mode="positive_hits"))
+    out["pip_positive_1000_polygons"] = {
+        "sample": f"400 points x 1,000 star polygons: {len(rows)} hits in {dt:.2f} s",
+        "points_per_s": len(pts) / dt}
+    # fixed radius r = 0.01 on a 1e5-point uniform scene (the smallest served scene): 16 queries
+    rng = np.random.default_rng(20261203 + 4)
+    sp = rng.random((100000, 3)).astype(np.float32).astype(float)
+    search = [Point3D(i, *row) for i, row in enumerate(sp.tolist())]
+    queries = [Point3D(i, *row) for i, row in enumerate(rng.random((16, 3)).astype(np.float32).astype(float).tolist())]
+    rows, dt = timed(lambda: ref.fixed_radius_neighbors_cpu(queries, search, radius=0.01, k_max=64))
+    out["fixed_radius_r0.01_1e5_points"] = {
+        "sample": f"16 queries x 100,000 points: {len(rows)} rows in {dt:.2f} s",
+        "queries_per_s": len(queries) / dt,
+        "extrapolated_queries_per_s_at_1e7_points": len(queries) / dt / 100.0}
+    return out
+
+
 def embree_status() -> str:
     import ctypes.util
 
@@ -655,6 +728,8 @@ def main() -> int:
         log(f"cpu baseline: {cpu}")
         baselines["cpu_python_reference"] = python_baseline_leg()
         log(f"python reference: {baselines['cpu_python_reference']}")
+        baselines["cpu_python_reference_subsamples"] = python_subsample_leg()
+        log(f"python reference subsamples: {baselines['cpu_python_reference_subsamples']}")
         baselines["embree"] = embree_status()
         baselines["per_call_configs"] = per_call_configs_leg(rtm)
 
