@@ -174,10 +174,104 @@ namespace {
 
 constexpr uint32_t kSortQueriesAbove = 2048;
 
+// Pinned staging for large host <-> device copies.  A copy from or into
+// pageable memory runs at a few tens of GB/s at best; large inputs and row
+// sets go instead through two pinned 32 MiB buffers, each chunk's DMA
+// overlapping the host copy of the next (or previous) chunk, with the host
+// copies split over threads.  One caller at a time uses the buffers; a
+// concurrent caller (or another device) copies directly.
+constexpr size_t kStageChunk = 32u << 20;
+constexpr size_t kStageMin = 64u << 20;        // row copies back to the host
+constexpr size_t kUploadStageMin = 16u << 20;  // inputs to the device
+constexpr unsigned kStageThreads = 8;
+
+struct RowStage {
+  std::mutex mu;
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int device = -1;
+};
+
+RowStage& row_stage() {
+  static RowStage* stage = new RowStage();  // lives until the process ends
+  return *stage;
+}
+
+// Allocates the buffers on first use; false when they belong to another device.
+bool stage_ready(RowStage& st) {
+  int dev = 0;
+  RTDL_HIP(hipGetDevice(&dev));
+  if (st.device < 0) {
+    for (int i = 0; i < 2; ++i) {
+      RTDL_HIP(hipHostMalloc(&st.buf[i], kStageChunk, hipHostMallocDefault));
+      RTDL_HIP(hipEventCreateWithFlags(&st.ev[i], hipEventDisableTiming));
+    }
+    st.device = dev;
+  }
+  return st.device == dev;
+}
+
+void parallel_memcpy(uint8_t* dst, const uint8_t* src, size_t bytes) {
+  const size_t part = (bytes + kStageThreads - 1) / kStageThreads;
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < kStageThreads && t * part < bytes; ++t) {
+    const size_t a = t * part, n = std::min(part, bytes - a);
+    pool.emplace_back([=] { std::memcpy(dst + a, src + a, n); });
+  }
+  std::memcpy(dst, src, std::min(part, bytes));
+  for (auto& th : pool) th.join();
+}
+
+// Returns false (nothing copied) when the staging buffers are busy or belong
+// to another device.
+bool staged_copy(uint8_t* dst, const uint8_t* src, size_t bytes, hipStream_t stream) {
+  RowStage& st = row_stage();
+  std::unique_lock<std::mutex> lock(st.mu, std::try_to_lock);
+  if (!lock.owns_lock() || !stage_ready(st)) return false;
+  const size_t n_chunks = (bytes + kStageChunk - 1) / kStageChunk;
+  auto issue = [&](size_t c) {
+    const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
+    RTDL_HIP(hipMemcpyAsync(st.buf[c & 1], src + a, n, hipMemcpyDeviceToHost, stream));
+    RTDL_HIP(hipEventRecord(st.ev[c & 1], stream));
+  };
+  issue(0);
+  for (size_t c = 0; c < n_chunks; ++c) {
+    // the buffer of chunk c + 1 held chunk c - 1, already copied out below
+    if (c + 1 < n_chunks) issue(c + 1);
+    RTDL_HIP(hipEventSynchronize(st.ev[c & 1]));
+    const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
+    parallel_memcpy(dst + a, static_cast<const uint8_t*>(st.buf[c & 1]), n);
+  }
+  return true;
+}
+
+// Host-to-device counterpart of staged_copy; returns once every chunk has
+// reached the device (the buffers are free for the next caller).
+bool staged_upload(uint8_t* dst, const uint8_t* src, size_t bytes, hipStream_t stream) {
+  RowStage& st = row_stage();
+  std::unique_lock<std::mutex> lock(st.mu, std::try_to_lock);
+  if (!lock.owns_lock() || !stage_ready(st)) return false;
+  const size_t n_chunks = (bytes + kStageChunk - 1) / kStageChunk;
+  for (size_t c = 0; c < n_chunks; ++c) {
+    const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
+    // the buffer's previous chunk (c - 2) must have left it
+    if (c >= 2) RTDL_HIP(hipEventSynchronize(st.ev[c & 1]));
+    parallel_memcpy(static_cast<uint8_t*>(st.buf[c & 1]), src + a, n);
+    RTDL_HIP(hipMemcpyAsync(dst + a, st.buf[c & 1], n, hipMemcpyHostToDevice, stream));
+    RTDL_HIP(hipEventRecord(st.ev[c & 1], stream));
+  }
+  for (size_t c = n_chunks > 2 ? n_chunks - 2 : 0; c < n_chunks; ++c)
+    RTDL_HIP(hipEventSynchronize(st.ev[c & 1]));
+  return true;
+}
+
 template <class T>
 DeviceArray<T> upload(const T* host, size_t n, hipStream_t stream) {
   DeviceArray<T> d(n);
-  if (n) RTDL_HIP(hipMemcpyAsync(d.get(), host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+  const size_t bytes = n * sizeof(T);
+  if (n && (bytes < kUploadStageMin || !staged_upload(reinterpret_cast<uint8_t*>(d.get()),
+                                                reinterpret_cast<const uint8_t*>(host), bytes, stream)))
+    RTDL_HIP(hipMemcpyAsync(d.get(), host, bytes, hipMemcpyHostToDevice, stream));
   return d;
 }
 
@@ -203,75 +297,6 @@ void* host_rows(size_t count, size_t row_bytes) {
   require(rows != nullptr, "host row allocation failed");
   return rows;
 }
-
-namespace {
-
-// Pinned staging for large row copies.  A device-to-host copy into pageable
-// memory runs at a few GB/s; rows go instead through two pinned 32 MiB
-// buffers, each chunk's device copy overlapping the host copy of the one
-// before, with the host copies split over threads.  One caller at a time
-// uses the buffers; a concurrent caller (or another device) copies directly.
-constexpr size_t kStageChunk = 32u << 20;
-constexpr size_t kStageMin = 64u << 20;
-constexpr unsigned kStageThreads = 8;
-
-struct RowStage {
-  std::mutex mu;
-  void* buf[2] = {nullptr, nullptr};
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  int device = -1;
-};
-
-RowStage& row_stage() {
-  static RowStage* stage = new RowStage();  // lives until the process ends
-  return *stage;
-}
-
-void parallel_memcpy(uint8_t* dst, const uint8_t* src, size_t bytes) {
-  const size_t part = (bytes + kStageThreads - 1) / kStageThreads;
-  std::vector<std::thread> pool;
-  for (unsigned t = 1; t < kStageThreads && t * part < bytes; ++t) {
-    const size_t a = t * part, n = std::min(part, bytes - a);
-    pool.emplace_back([=] { std::memcpy(dst + a, src + a, n); });
-  }
-  std::memcpy(dst, src, std::min(part, bytes));
-  for (auto& th : pool) th.join();
-}
-
-// Returns false (nothing copied) when the staging buffers are busy or belong
-// to another device.
-bool staged_copy(uint8_t* dst, const uint8_t* src, size_t bytes, hipStream_t stream) {
-  RowStage& st = row_stage();
-  std::unique_lock<std::mutex> lock(st.mu, std::try_to_lock);
-  if (!lock.owns_lock()) return false;
-  int dev = 0;
-  RTDL_HIP(hipGetDevice(&dev));
-  if (st.device < 0) {
-    for (int i = 0; i < 2; ++i) {
-      RTDL_HIP(hipHostMalloc(&st.buf[i], kStageChunk, hipHostMallocDefault));
-      RTDL_HIP(hipEventCreateWithFlags(&st.ev[i], hipEventDisableTiming));
-    }
-    st.device = dev;
-  }
-  if (st.device != dev) return false;
-  const size_t n_chunks = (bytes + kStageChunk - 1) / kStageChunk;
-  auto issue = [&](size_t c) {
-    const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
-    RTDL_HIP(hipMemcpyAsync(st.buf[c & 1], src + a, n, hipMemcpyDeviceToHost, stream));
-    RTDL_HIP(hipEventRecord(st.ev[c & 1], stream));
-  };
-  issue(0);
-  for (size_t c = 0; c < n_chunks; ++c) {
-    // the buffer of chunk c + 1 held chunk c - 1, already copied out below
-    if (c + 1 < n_chunks) issue(c + 1);
-    RTDL_HIP(hipEventSynchronize(st.ev[c & 1]));
-    const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
-    parallel_memcpy(dst + a, static_cast<const uint8_t*>(st.buf[c & 1]), n);
-  }
-  return true;
-}
-
-}  // namespace
 
 void copy_rows_back(const void* device_rows, size_t count, size_t row_bytes, void** rows_out,
                     size_t* n_out, hipStream_t stream) {
