@@ -5,10 +5,13 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
 #include <map>
 #include <mutex>
 #include <thread>
 #include <vector>
+
+#include <unistd.h>
 
 #include "common.hpp"
 #include "io.hpp"
@@ -185,11 +188,71 @@ constexpr size_t kStageMin = 64u << 20;        // row copies back to the host
 constexpr size_t kUploadStageMin = 16u << 20;  // inputs to the device
 constexpr unsigned kStageThreads = 8;
 
+// Host copy workers of the staging path: kStageThreads - 1 threads, each
+// copying its share of a chunk while the caller copies the first share.
+// Used by one caller at a time (under the stage lock); started on first use
+// and again in a forked child, which inherits no threads.
+struct CopyPool {
+  std::mutex m;
+  std::condition_variable go, done;
+  uint64_t generation = 0;
+  unsigned pending = 0;
+  uint8_t* dst = nullptr;
+  const uint8_t* src = nullptr;
+  size_t bytes = 0, part = 0;
+  pid_t pid = getpid();
+
+  CopyPool() {
+    for (unsigned t = 1; t < kStageThreads; ++t) std::thread([this, t] { run(t); }).detach();
+  }
+
+  void run(unsigned t) {
+    uint64_t seen = 0;
+    for (;;) {
+      uint8_t* d;
+      const uint8_t* s;
+      size_t a, n;
+      {
+        std::unique_lock<std::mutex> lk(m);
+        go.wait(lk, [&] { return generation != seen; });
+        seen = generation;
+        a = t * part;
+        n = a < bytes ? std::min(part, bytes - a) : 0;
+        d = dst;
+        s = src;
+      }
+      if (n) std::memcpy(d + a, s + a, n);
+      std::lock_guard<std::mutex> lk(m);
+      if (--pending == 0) done.notify_one();
+    }
+  }
+
+  // Returns once every share is copied; the next call starts only after
+  // each worker has finished this one, so no worker misses a generation.
+  void copy(uint8_t* d, const uint8_t* s, size_t b) {
+    const size_t p = (b + kStageThreads - 1) / kStageThreads;
+    {
+      std::lock_guard<std::mutex> lk(m);
+      dst = d;
+      src = s;
+      bytes = b;
+      part = p;
+      pending = kStageThreads - 1;
+      ++generation;
+    }
+    go.notify_all();
+    std::memcpy(d, s, std::min(p, b));
+    std::unique_lock<std::mutex> lk(m);
+    done.wait(lk, [&] { return pending == 0; });
+  }
+};
+
 struct RowStage {
   std::mutex mu;
   void* buf[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
   int device = -1;
+  CopyPool* pool = nullptr;  // like the stage, never freed
 };
 
 RowStage& row_stage() {
@@ -211,15 +274,10 @@ bool stage_ready(RowStage& st) {
   return st.device == dev;
 }
 
-void parallel_memcpy(uint8_t* dst, const uint8_t* src, size_t bytes) {
-  const size_t part = (bytes + kStageThreads - 1) / kStageThreads;
-  std::vector<std::thread> pool;
-  for (unsigned t = 1; t < kStageThreads && t * part < bytes; ++t) {
-    const size_t a = t * part, n = std::min(part, bytes - a);
-    pool.emplace_back([=] { std::memcpy(dst + a, src + a, n); });
-  }
-  std::memcpy(dst, src, std::min(part, bytes));
-  for (auto& th : pool) th.join();
+// Caller holds st.mu.
+void parallel_memcpy(RowStage& st, uint8_t* dst, const uint8_t* src, size_t bytes) {
+  if (!st.pool || st.pool->pid != getpid()) st.pool = new CopyPool();
+  st.pool->copy(dst, src, bytes);
 }
 
 // Returns false (nothing copied) when the staging buffers are busy or belong
@@ -240,7 +298,7 @@ bool staged_copy(uint8_t* dst, const uint8_t* src, size_t bytes, hipStream_t str
     if (c + 1 < n_chunks) issue(c + 1);
     RTDL_HIP(hipEventSynchronize(st.ev[c & 1]));
     const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
-    parallel_memcpy(dst + a, static_cast<const uint8_t*>(st.buf[c & 1]), n);
+    parallel_memcpy(st, dst + a, static_cast<const uint8_t*>(st.buf[c & 1]), n);
   }
   return true;
 }
@@ -256,7 +314,7 @@ bool staged_upload(uint8_t* dst, const uint8_t* src, size_t bytes, hipStream_t s
     const size_t a = c * kStageChunk, n = std::min(kStageChunk, bytes - a);
     // the buffer's previous chunk (c - 2) must have left it
     if (c >= 2) RTDL_HIP(hipEventSynchronize(st.ev[c & 1]));
-    parallel_memcpy(static_cast<uint8_t*>(st.buf[c & 1]), src + a, n);
+    parallel_memcpy(st, static_cast<uint8_t*>(st.buf[c & 1]), src + a, n);
     RTDL_HIP(hipMemcpyAsync(dst + a, st.buf[c & 1], n, hipMemcpyHostToDevice, stream));
     RTDL_HIP(hipEventRecord(st.ev[c & 1], stream));
   }
