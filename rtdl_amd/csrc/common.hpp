@@ -99,7 +99,7 @@ constexpr uint32_t kMaxLeafStart = (1u << 28) - 1;
 // Process-wide launch and build choices (set by the autotuner or the caller;
 // every value gives the same rows).
 struct Tuning {
-  uint32_t point_leaf = 4;      // primitives per LBVH leaf, point scenes
+  uint32_t point_leaf = 2;      // primitives per LBVH leaf, point scenes
   uint32_t triangle_leaf = 2;   // triangle scenes
   uint32_t polygon_leaf = 1;    // polygon scenes
   uint32_t blocks_per_cu = 8;   // persistent batch launches: 256-thread workgroups per CU
