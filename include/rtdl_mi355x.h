@@ -384,7 +384,7 @@ int rtdl_mi355x_scene_export(void* prepared, RtdlSceneExport* info, void* nodes,
  * graphs captured earlier keep the values they were made with. */
 typedef struct RtdlTuning {
   uint32_t point_leaf;     /* primitives per LBVH leaf in point scenes, 1..8 (default 2) */
-  uint32_t triangle_leaf;  /* triangle scenes, 1..8 (default 2) */
+  uint32_t triangle_leaf;  /* triangle scenes, 1..8 (default 1) */
   uint32_t polygon_leaf;   /* polygon scenes, 1..8 (default 1) */
   uint32_t blocks_per_cu;  /* persistent batch launches: 256-thread workgroups per CU, 1..64 (default 8) */
 } RtdlTuning;

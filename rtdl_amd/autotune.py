@@ -26,7 +26,7 @@ from . import mi355x_runtime as rt_native
 @dataclass(frozen=True)
 class Tuning:
     point_leaf: int = 2
-    triangle_leaf: int = 2
+    triangle_leaf: int = 1
     polygon_leaf: int = 1
     blocks_per_cu: int = 8
 

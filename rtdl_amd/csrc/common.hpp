@@ -100,7 +100,7 @@ constexpr uint32_t kMaxLeafStart = (1u << 28) - 1;
 // every value gives the same rows).
 struct Tuning {
   uint32_t point_leaf = 2;      // primitives per LBVH leaf, point scenes
-  uint32_t triangle_leaf = 2;   // triangle scenes
+  uint32_t triangle_leaf = 1;   // triangle scenes
   uint32_t polygon_leaf = 1;    // polygon scenes
   uint32_t blocks_per_cu = 8;   // persistent batch launches: 256-thread workgroups per CU
 };

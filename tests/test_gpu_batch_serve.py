@@ -549,7 +549,7 @@ def test_autotune_picks_the_fastest_and_every_candidate_gives_the_same_rows(gpu_
     original = T.get_tuning()
     try:
         out = T.autotune_batch(build, jobs, repeats=2,
-                               knobs={"blocks_per_cu": (4, 16), "point_leaf": (4, 8), "triangle_leaf": (1, 4)})
+                               knobs={"blocks_per_cu": (4, 16), "point_leaf": (4, 8), "triangle_leaf": (2, 4)})
         table = out["table"]
         assert len(table) == 7
         fastest = min(table, key=lambda r: r["seconds"])
