@@ -105,13 +105,21 @@ struct ItemQueue {
   }
 };
 
+__device__ inline uint64_t lane0_add64(unsigned long long* counter, uint32_t value) {
+  const unsigned long long mine = (threadIdx.x & 63) == 0 ? value : 0ull;
+  auto* g = (__attribute__((address_space(1))) unsigned long long*)counter;
+  const unsigned long long old = __hip_atomic_fetch_add(g, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)old);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(old >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Reserves `c` rows per lane in the job's row buffer; returns the lane's
 // first row.
 __device__ inline uint64_t wave_reserve(JobCounters* counters, uint32_t c) {
   uint32_t total;
   const uint32_t prefix = wave_exclusive_sum(c, &total);
-  const uint32_t base = lane0_add(&counters->rows, total);
-  return (uint64_t)base + prefix;
+  return lane0_add64(&counters->rows, total) + prefix;
 }
 
 __device__ inline uint32_t job_query(const DeviceJob& j, uint32_t pos) {
@@ -303,15 +311,6 @@ __device__ inline uint32_t nbr_bucket(double dist, double scale) {
 // so dist < (b + 1) / scale; the factor covers the rounding of the quotient.
 __device__ inline double nbr_bucket_edge(uint32_t b, double scale) {
   return (double)(b + 1) / scale * 1.0000000000009094947;  // 1 + 2^-40
-}
-
-__device__ inline uint64_t lane0_add64(unsigned long long* counter, uint32_t value) {
-  const unsigned long long mine = (threadIdx.x & 63) == 0 ? value : 0ull;
-  auto* g = (__attribute__((address_space(1))) unsigned long long*)counter;
-  const unsigned long long old = __hip_atomic_fetch_add(g, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)old);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(old >> 32));
-  return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ inline void load_query3(const DeviceJob& j, uint32_t qi, double& qx, double& qy, double& qz) {
@@ -709,7 +708,7 @@ __global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __rest
     if (MODE != 2) {
       if (active)
         static_cast<PairRowRec*>(j.rows)[qi] = PairRowRec{qi, MODE == 0 ? (res.hits ? 1u : 0u) : res.hits};
-      (void)lane0_add(&j.counters->rows, w.count);
+      (void)lane0_add64(&j.counters->rows, w.count);
     } else {
       const uint32_t c = (active && res.best_t == res.best_t) ? 1u : 0u;
       const uint64_t at = wave_reserve(j.counters, c);
@@ -1044,6 +1043,8 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     require(h.nq == 0 || h.queries != nullptr, "job queries must not be null");
     require(h.nq == 0 || h.rows != nullptr, "job rows must not be null");
     if (h.kind <= kJobBoundedKnn) require(h.k >= 1 && h.k <= 64, "batched neighbour jobs need 1 <= k <= 64");
+    require(h.row_capacity <= kMaxJobRows && job_row_capacity(h.kind, h.nq, h.k, 0) <= kMaxJobRows,
+            "a batch job holds at most 2^32 - 1 rows (split it into smaller jobs)");
     const int fam = family_of(h.kind, h.k);
     DeviceJob& d = host_jobs_[i];
     std::memset(&d, 0, sizeof(d));

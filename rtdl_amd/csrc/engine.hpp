@@ -63,12 +63,17 @@ constexpr int kLaunchFamilies = kPointBuckets + 5;  // + fixed radius, ray any /
 constexpr uint32_t kStatusRowOverflow = 1u;
 constexpr uint32_t kStatusInternal = 8u;
 
+// Per-job counters.  `rows` counts reserved rows in 64 bits, so reservations
+// past a job's capacity (which set kStatusRowOverflow and write nothing)
+// never wrap round to offsets inside it; capacities stay below 2^32, so every
+// row actually written has a 32-bit offset.
 struct JobCounters {
-  uint32_t rows;
+  unsigned long long rows;
   uint32_t status;
   uint32_t ambiguous;
-  uint32_t pad;
 };
+static_assert(sizeof(JobCounters) == 16, "JobCounters is cleared as four words");
+constexpr uint64_t kMaxJobRows = 0xFFFFFFFFull;
 
 struct DeviceJob {
   SceneView scene;        // root / end delimit the job's instance subtree

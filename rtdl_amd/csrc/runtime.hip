@@ -555,6 +555,14 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
   require(nq_in < 0xFFFFFFFFull, "too many queries");
   require(k >= 1, "k must be positive");
   require(!(radius < 0.0), "radius must be non-negative");
+  // Without a finite radius every scene point is a candidate of every query
+  // beyond the 64-entry list, so an unbounded search keeps k <= 64 (as the
+  // reference's HIPRT backend does, src/rtdsl/hiprt_runtime.py:1307-1308).
+  const bool unbounded = mode == NeighborMode::Knn || !(radius < __builtin_inf());
+  require(!unbounded || k <= kMaxListK,
+          mode == NeighborMode::FixedRadius
+              ? "fixed_radius_neighbors with an infinite radius supports k_max <= 64"
+              : "knn_rows supports k <= 64 (bounded_knn_rows with a finite radius takes larger k_max)");
   const bool rank_rows = mode != NeighborMode::FixedRadius;
   const size_t row_bytes = rank_rows ? 24 : 16;
   PhaseClock& clock = thread_clock();
@@ -580,12 +588,21 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
   DeviceArray<double> cand_d;
   DeviceArray<uint32_t> cand_id;
   DeviceArray<uint64_t> src_start;  // only for the count-then-collect path
+  // Temporaries that kernels or copies still read after the branch that
+  // fills them: they live until the stream is synchronised below, since a
+  // block returned to the pool can be handed to another thread at once.
+  DeviceArray<uint32_t> amb_list, amb_count;
+  DeviceArray<double> list_d;
+  DeviceArray<uint32_t> list_id;
+  std::vector<uint64_t> host_start;
+  std::vector<uint32_t> host_count;
   uint32_t stride = k;
   const bool list_path = k <= kMaxListK;
   if (list_path) {
     cand_d.resize((size_t)nq * k);
     cand_id.resize((size_t)nq * k);
-    DeviceArray<uint32_t> amb_list(nq), amb_count(1);
+    amb_list.resize(nq);
+    amb_count.resize(1);
     RTDL_HIP(hipMemsetAsync(amb_count.get(), 0, sizeof(uint32_t), s));
     launch_collect_packed(view, q.get(), order.get(), nq, search_radius, k, cand_d.get(),
                           cand_id.get(), count.get(), amb_list.get(), amb_count.get(), nullptr,
@@ -602,9 +619,9 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
     // Keep the 64 nearest per query and count everything inside the radius;
     // queries with more than 64 candidates are collected in full and sorted.
     stride = kMaxListK;
-    DeviceArray<uint32_t> total(nq);
-    DeviceArray<double> list_d((size_t)nq * kMaxListK);
-    DeviceArray<uint32_t> list_id((size_t)nq * kMaxListK);
+    DeviceArray<uint32_t> total(nq);  // read back (synchronised) before the branch ends
+    list_d.resize((size_t)nq * kMaxListK);
+    list_id.resize((size_t)nq * kMaxListK);
     launch_collect_exact(view, q.get(), order.get(), nq, search_radius, kMaxListK, true,
                          list_d.get(), list_id.get(), count.get(), total.get(), /*seeded=*/false, s);
     std::vector<uint32_t> host_total(nq);
@@ -628,8 +645,8 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
                             hipMemcpyDeviceToDevice, s));
     RTDL_HIP(hipMemcpyAsync(cand_id.get(), list_id.get(), light_n * sizeof(uint32_t),
                             hipMemcpyDeviceToDevice, s));
-    std::vector<uint64_t> host_start(nq);
-    std::vector<uint32_t> host_count(nq);
+    host_start.resize(nq);
+    host_count.resize(nq);
     for (uint32_t i = 0; i < nq; ++i) {
       host_start[i] = (uint64_t)i * kMaxListK;
       host_count[i] = std::min(host_total[i], k);

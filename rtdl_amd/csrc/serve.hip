@@ -38,7 +38,7 @@ struct Slot {
   bool busy = false;
   std::vector<size_t> requests;  // request indices of the batch in flight
   std::vector<std::pair<size_t, uint8_t*>> captures;  // (request, pinned copy)
-  std::vector<uint64_t> capture_bytes;
+  std::vector<uint64_t> capture_rows;                  // row capacity of each copy
 };
 
 uint32_t stride_of(uint32_t kind) {
@@ -145,7 +145,10 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
       }
       for (size_t ci = 0; ci < s.captures.size(); ++ci) {
         const size_t r = s.captures[ci].first;
-        const size_t bytes = results[r].rows * job_row_bytes(requests[r].kind);
+        // an overflowing request reserved more rows than its buffer holds;
+        // only the buffer was copied
+        const size_t bytes = std::min<uint64_t>(results[r].rows, s.capture_rows[ci]) *
+                             job_row_bytes(requests[r].kind);
         void* out = std::malloc(bytes ? bytes : 1);
         require(out != nullptr, "host allocation for captured rows failed");
         std::memcpy(out, s.captures[ci].second, bytes);
@@ -157,6 +160,7 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
       slot_bytes[si] = 0;
       s.requests.clear();
       s.captures.clear();
+      s.capture_rows.clear();
       s.busy = false;
       progress = true;
     }
@@ -210,6 +214,7 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
         if (bytes)
           RTDL_HIP(hipMemcpyAsync(pinned, jobs[j].rows, bytes, hipMemcpyDeviceToHost, s.stream));
         s.captures.emplace_back(r, pinned);
+        s.capture_rows.push_back(jobs[j].row_capacity);
       }
       RTDL_HIP(hipEventRecord(s.done, s.stream));
       const double td = now();

@@ -281,12 +281,15 @@ __device__ inline void collect_exact(const SceneView& s, uint32_t root, uint32_t
 
 // Packed list: each entry is (FP32 bits of fl32(d2) << 32) | slot, so the
 // whole (key, slot) order is one 64-bit compare and an entry costs two
-// registers.  fl32 is monotone, so a strictly smaller key means a strictly
-// smaller FP64 distance (one FP32 ulp of d2 is far above FP64 sqrt
-// resolution); only equal keys can be ordered differently from the exact
-// (distance, id) order.  The list therefore reports `ambiguous` when a key
-// equal to the final k-th key was dropped or when two kept keys are equal;
-// such queries are recomputed with the exact list.
+// registers.  fl32 is monotone, so a smaller key never means a larger FP64
+// distance, but keys do not decide ties: two squared distances on either side
+// of an FP32 rounding point (adjacent doubles, say) get keys one ulp apart
+// and can still have the same FP64 square root, which the reference orders by
+// id.  Keys two or more ulps apart are a full FP32 ulp (relative 2^-24) apart
+// in d2, so their square roots differ by far more than an FP64 ulp.  The list
+// therefore reports `ambiguous` when a dropped key lies within one ulp of the
+// final k-th key or when two kept keys lie within one ulp of each other; such
+// queries are recomputed with the exact (distance, id) list.
 template <int K>
 struct PackedTopK {
   uint64_t e[K];
@@ -337,7 +340,8 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
       const uint64_t out = list.insert(v);
       if (out != ~0ull) drop_min = min(drop_min, key_bits(out));
       if (list.full()) {
-        // every candidate whose key could equal the k-th key stays inside
+        // every candidate whose key lies within one ulp of the k-th key
+        // stays inside (the factor covers 2^-23 of d2 and its rounding)
         const double kth = (double)__uint_as_float(key_bits(list.e[K - 1]));
         thr2 = frn_threshold2(fmin(radius, sqrt(kth) * 1.0000002384185791015625), m);
       }
@@ -360,10 +364,12 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
   } else {
     walk_neighbors<STATS>(s, root, end, qf, radius, thr2, st, visit);
   }
-  bool ambiguous = list.full() && drop_min == key_bits(list.e[K - 1]);
+  // dropped keys are never below the final k-th key (it only shrinks), and
+  // kept keys ascend, so each test below is a difference of at most one ulp
+  bool ambiguous = list.full() && drop_min - key_bits(list.e[K - 1]) <= 1u;
 #pragma unroll
   for (int i = 1; i < K; ++i) {
-    if (i > K - (int)k && list.e[i] != ~0ull && key_bits(list.e[i]) == key_bits(list.e[i - 1]))
+    if (i > K - (int)k && list.e[i] != ~0ull && key_bits(list.e[i]) - key_bits(list.e[i - 1]) <= 1u)
       ambiguous = true;
   }
   return ambiguous;

@@ -467,7 +467,22 @@ def fixed_radius_neighbors_mi355x(query_points, search_points, *, radius: float,
                       float(radius), int(k_max))
 
 
+# The KNN walk keeps each query's k nearest in registers; like the
+# reference's HIPRT backend (src/rtdsl/hiprt_runtime.py:1307-1308) it takes
+# k <= 64.  Fixed-radius and bounded KNN take larger k_max: their radius
+# bounds the candidates gathered beyond 64.
+KNN_MAX_K = 64
+
+
+def _check_knn_k(k: int) -> None:
+    if int(k) <= 0:
+        raise ValueError("MI355X knn_rows k must be positive")
+    if int(k) > KNN_MAX_K:
+        raise ValueError(f"MI355X knn_rows currently supports k <= {KNN_MAX_K}")
+
+
 def knn_rows_mi355x(query_points, search_points, *, k: int) -> np.ndarray:
+    _check_knn_k(k)
     queries, qdim = pack_points(query_points, name="query_points")
     search, sdim = pack_points(search_points, name="search_points")
     if len(queries) and len(search) and qdim != sdim:
@@ -549,6 +564,8 @@ def _validate_kernel(compiled: CompiledKernel) -> tuple[str, tuple[str, str], di
         if pair not in {("Point2D", "Point2D"), ("Point3D", "Point3D")}:
             raise _unsupported(predicate, "only matching Point2DLayout or Point3DLayout inputs are implemented")
         options["layout_pair"] = pair
+        if predicate == "knn_rows":
+            _check_knn_k(options["k"])
         return predicate, (left.name, right.name), options
     if predicate == "point_in_polygon":
         if left.geometry.name != "points" or right.geometry.name != "polygons":
@@ -682,6 +699,7 @@ class PreparedMi355xKernel:
                 return _call_rows("rtdl_mi355x_run_prepared_bounded_knn_rows", KNN_ROW_DTYPE,
                                   self.handle, _ptr(queries), len(queries), float(opts["radius"]),
                                   int(opts["k_max"]))
+            _check_knn_k(opts["k"])
             return _call_rows("rtdl_mi355x_run_prepared_knn_rows", KNN_ROW_DTYPE, self.handle,
                               _ptr(queries), len(queries), int(opts["k"]))
         if p == "point_in_polygon":

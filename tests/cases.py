@@ -89,3 +89,46 @@ def tri_matrix(t: dict) -> np.ndarray:
 
 def subset(cols: dict, idx) -> dict:
     return {k: v[idx] for k, v in cols.items()}
+
+
+def fl32_tie_pairs(count: int = 1, start: float = 0.01) -> list[tuple[float, float, float, float, float]]:
+    """FP32-exact points A = (x, ya) and B = (x, yb) around a query at the
+    origin whose FP64 squared distances a < b (reference order: x*x + y*y)
+    round to FP32 keys one ulp apart, while sqrt(a) == sqrt(b): the two are
+    tied in the reference's (distance, id) order although their FP32 keys are
+    not.  Returns `count` tuples (x, ya, yb, a, b), searched deterministically
+    upwards from `start` (a few dozen FP32 steps per pair)."""
+    import math
+
+    def as_f32(v: float) -> float:
+        return float(np.float32(v))
+
+    out = []
+    x = np.float32(start)
+    while len(out) < count:
+        x = np.nextafter(x, np.float32(1.0))
+        xx = float(x) * float(x)  # exact: 24-bit by 24-bit
+        lo = np.float32(xx)
+        if float(lo) > xx:
+            lo = np.nextafter(lo, np.float32(0.0))
+        mid = (float(lo) + float(np.nextafter(lo, np.float32(1.0)))) / 2.0  # FP32 rounding point
+        gap = mid - xx
+        if not 0.0 < gap < 1e-13:  # y*y must bridge the gap at FP64 resolution
+            continue
+        y = np.float32(math.sqrt(gap))
+        ys = [y]
+        lo_y = hi_y = y
+        for _ in range(40):
+            lo_y = np.nextafter(lo_y, np.float32(0.0))
+            hi_y = np.nextafter(hi_y, np.float32(1.0))
+            ys += [lo_y, hi_y]
+        d2 = {}
+        for yv in ys:
+            d2.setdefault(float(x) * float(x) + float(yv) * float(yv), float(yv))
+        keys = sorted(d2)
+        pair = next(((a, b) for a in keys for b in keys
+                     if a < b and as_f32(a) != as_f32(b) and math.sqrt(a) == math.sqrt(b)), None)
+        if pair:
+            a, b = pair
+            out.append((float(x), d2[a], d2[b], a, b))
+    return out

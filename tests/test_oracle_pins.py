@@ -105,6 +105,28 @@ def test_pip_authored_squares_goal552():
     assert [(r["point_id"], r["polygon_id"]) for r in pos] == [(1, 10), (2, 10), (3, 20)]
 
 
+def test_bounded_knn_authored_rows_goal263():
+    """The reference's authored bounded-KNN rows
+    (tests/goal263_v0_5_bounded_knn_rows_surface_test.py:27-56): inclusive
+    radius, ranks from 1, points beyond the radius dropped even below k_max."""
+    q = (Point3D(10, 0.0, 0.0, 0.0),)
+    s = (Point3D(1, 0.0, 0.0, 0.0), Point3D(2, 0.0, 0.0, 0.4), Point3D(3, 0.0, 0.0, 0.8))
+    want = ({"query_id": 10, "neighbor_id": 1, "distance": 0.0, "neighbor_rank": 1},
+            {"query_id": 10, "neighbor_id": 2, "distance": 0.4, "neighbor_rank": 2})
+    assert ref.bounded_knn_rows_cpu(q, s, radius=0.5, k_max=3) == want
+    cols = lambda pts: {"id": np.array([p.id for p in pts], np.uint32), "x": np.array([p.x for p in pts]),
+                        "y": np.array([p.y for p in pts]), "z": np.array([p.z for p in pts])}
+    got = native.neighbors("bounded_knn", cols(q), cols(s), radius=0.5, k=3)
+    assert list(zip(got["qid"].tolist(), got["nid"].tolist(), got["dist"].tolist(), got["rank"].tolist())) == \
+        [(10, 1, 0.0, 1), (10, 2, 0.4, 2)]
+    s2 = (Point3D(1, 0.0, 0.0, 0.0), Point3D(2, 0.0, 0.0, 0.25), Point3D(3, 0.0, 0.0, 2.0))
+    rows = ref.bounded_knn_rows_cpu(q, s2, radius=1.0, k_max=2)
+    assert [r["neighbor_id"] for r in rows] == [1, 2]
+    assert [r["neighbor_rank"] for r in rows] == [1, 2]
+    got = native.neighbors("bounded_knn", cols(q), cols(s2), radius=1.0, k=2)
+    assert got["nid"].tolist() == [1, 2] and got["rank"].tolist() == [1, 2]
+
+
 # ------------------------------------------------------------ closed forms --
 
 
@@ -128,6 +150,91 @@ def test_knn_contains_frn_rows_at_kth_distance():
         frn = ref.fixed_radius_neighbors_cpu((query,), pts, radius=kth, k_max=7)
         assert {r["neighbor_id"] for r in frn} <= {r["neighbor_id"] for r in mine}
         assert [r["neighbor_rank"] for r in mine] == list(range(1, 8))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bounded_knn_is_frn_with_ranks(dim):
+    """bounded_knn_rows(r, k) (reference.py:406-434) selects exactly the rows
+    of fixed_radius_neighbors(r, k) (pinned above by the authored cases) and
+    numbers them 1..n within each query; duplicates and points on the radius
+    make the tie order and the inclusive bound decide rows."""
+    rng = np.random.default_rng(70 + dim)
+    q = cases.points(rng, 120, dim, fp32=False, ids=rng.permutation(120))
+    s = cases.points(rng, 300, dim, kind="mixture", fp32=False, ids=rng.permutation(300))
+    for key in ("x", "y", "z")[:dim]:
+        s[key][:30] = s[key][30:60]          # exact duplicates: equal distances, other ids
+        s[key][60:70] = q[key][:10]          # neighbours at distance 0
+    s["x"][70:80] = q["x"][10:20] + 0.125    # exactly on the radius below
+    for key in ("y", "z")[:dim - 1]:
+        s[key][70:80] = q[key][10:20]
+    qr, sr = _as_records(q, dim), _as_records(s, dim)
+    for radius, k in ((0.125, 4), (0.05, 1), (0.3, 40), (0.0, 3)):
+        frn = ref.fixed_radius_neighbors_cpu(qr, sr, radius=radius, k_max=k)
+        bkn = ref.bounded_knn_rows_cpu(qr, sr, radius=radius, k_max=k)
+        assert [(r["query_id"], r["neighbor_id"], r["distance"]) for r in bkn] == \
+            [(r["query_id"], r["neighbor_id"], r["distance"]) for r in frn]
+        rank, last = [], None
+        for r in frn:
+            rank.append(rank[-1] + 1 if rank and r["query_id"] == last else 1)
+            last = r["query_id"]
+        assert [r["neighbor_rank"] for r in bkn] == rank
+        got = native.neighbors("bounded_knn", q, s, radius=radius, k=k)
+        assert got["rank"].tolist() == rank
+        assert got["nid"].tolist() == [r["neighbor_id"] for r in frn]
+
+
+def test_closest_hit_3d_exact_t_and_lower_id_tie():
+    """reference.py:248-266 keeps the smallest t and, at equal t, the lower
+    triangle id.  An axis ray (0.25, 0.25, 0) + t (0, 0, 1) crosses the
+    triangle (0,0,c) (1,0,c) (0,1,c) at exactly t = c (det = -1, u = v = 0.25,
+    every product exact), so t is known in closed form.  The nearest planes
+    come after farther ones in input order and a coplanar duplicate with a
+    lower id follows its twin, so neither "first hit" nor "last hit" passes."""
+    def tri(ident, c):
+        return Triangle3D(ident, 0.0, 0.0, c, 1.0, 0.0, c, 0.0, 1.0, c)
+
+    tris = (tri(7, 2.0), tri(9, 3.0), tri(5, 1.0), tri(2, 1.0), tri(11, 1.0), tri(1, 5.0))
+    rays = (Ray3D(40, 0.25, 0.25, 0.0, 0.0, 0.0, 1.0, 4.0),   # hits c = 1, 1, 1, 2, 3
+            Ray3D(41, 0.25, 0.25, 0.0, 0.0, 0.0, 1.0, 1.0),   # t = tmax = 1 is a hit
+            Ray3D(42, 0.25, 0.25, 0.0, 0.0, 0.0, 1.0, 0.5),   # no hit: no row
+            Ray3D(43, 0.25, 0.25, 2.5, 0.0, 0.0, 1.0, 4.0))   # starts past c = 1, 2; hits 3 and 5
+    want = ({"ray_id": 40, "triangle_id": 2, "t": 1.0}, {"ray_id": 41, "triangle_id": 2, "t": 1.0},
+            {"ray_id": 43, "triangle_id": 9, "t": 0.5})
+    assert ref.ray_triangle_closest_hit_cpu(rays, tris) == want
+    assert [r["hit_count"] for r in ref.ray_triangle_hit_count_cpu(rays, tris)] == [5, 3, 0, 2]
+    rm = np.array([[r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tmax] for r in rays])
+    tm = np.array([[t.x0, t.y0, t.z0, t.x1, t.y1, t.z1, t.x2, t.y2, t.z2] for t in tris])
+    ids = np.array([t.id for t in tris], np.uint32)
+    tid, tt, has = native.rays_3d("closest", rm, tm, ids)
+    assert has.tolist() == [True, True, False, True]
+    assert tid[has].tolist() == [2, 2, 9] and tt[has].tolist() == [1.0, 1.0, 0.5]
+    assert native.rays_3d("count", rm, tm, ids).tolist() == [5, 3, 0, 2]
+
+
+def test_fl32_key_tie_pair_is_ordered_by_id():
+    """Two FP32-exact points whose FP64 squared distances round to FP32 keys
+    one ulp apart share one FP64 square root: the reference orders them by id
+    (reference.py:362-366, 388-389), not by squared distance.  The pair is the
+    case a selection by FP32 keys gets wrong unless it treats keys within one
+    ulp as tied."""
+    (x, ya, yb, a, b), = cases.fl32_tie_pairs(1)
+    assert a < b and math.sqrt(a) == math.sqrt(b)
+    ka, kb = (int(np.float32(v).view(np.uint32)) for v in (a, b))
+    assert kb - ka == 1
+    assert x * x + ya * ya == a and x * x + yb * yb == b  # FP64 in the reference order
+    for dim in (2, 3):
+        pad = (0.0,) * (dim - 2)
+        mk = Point3D if dim == 3 else Point
+        q = (mk(0, 0.0, 0.0, *pad),)
+        s = (mk(9, x, ya, *pad), mk(4, x, yb, *pad), mk(6, 0.5, 0.5, *pad))
+        rows = ref.knn_rows_cpu(q, s, k=2)
+        assert [r["neighbor_id"] for r in rows] == [4, 9]  # the farther d2 first: lower id
+        assert ref.fixed_radius_neighbors_cpu(q, s, radius=0.02, k_max=1)[0]["neighbor_id"] == 4
+        cols = {"id": np.array([9, 4, 6], np.uint32), "x": np.array([x, x, 0.5]), "y": np.array([ya, yb, 0.5])}
+        qc = {"id": np.array([0], np.uint32), "x": np.zeros(1), "y": np.zeros(1)}
+        if dim == 3:
+            cols["z"], qc["z"] = np.zeros(3), np.zeros(1)
+        assert native.neighbors("knn", qc, cols, k=2)["nid"].tolist() == [4, 9]
 
 
 def test_moller_trumbore_special_cases():
