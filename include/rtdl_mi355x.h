@@ -348,6 +348,24 @@ int rtdl_mi355x_run_batch(const RtdlBatchJob* jobs, size_t job_count, uint32_t m
                           RtdlBatchJobResult* results, double* seconds_out, char* error_out,
                           size_t error_size);
 
+/* rtdl_mi355x_run_batch for callers that cannot allocate (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It has no
+ * counterpart in the reference.  Same jobs, rows and results as
+ * rtdl_mi355x_run_batch, but the call allocates nothing: no device, pinned
+ * or host heap memory, no stream and no event.  Every scratch buffer of the
+ * batch (job descriptors, work items, counters, sort keys, fixed-radius
+ * plans, stage and parking space) lives in a static 64 MiB device array that
+ * each GPU holds until the process ends; the caller passes the inputs and the
+ * row buffers as for rtdl_mi355x_run_batch.  Limits: at most 128 jobs and
+ * 16,384 queries per call (larger batches are rejected with an error).  Runs
+ * on the null stream and blocks until the batch is done; seconds_out
+ * (optional) receives the call's wall time.  Calls are serialised in a
+ * process: the array holds one batch at a time, and each call clears the
+ * counters it needs. */
+int rtdl_mi355x_run_batch_noalloc(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                                  RtdlBatchJobResult* results, double* seconds_out, char* error_out,
+                                  size_t error_size);
+
 /* rtdl_mi355x_run_batch with measurement: family_seconds (5 entries:
  * neighbour, ray any-hit, ray hit-count, ray closest-hit, PIP) receives each
  * family's kernel time from events on the launch stream; visits (2 entries,

@@ -245,6 +245,30 @@ def run_batch(jobs: list[BatchJob], *, morton_sort: bool = True):
     return out, seconds.value
 
 
+NOALLOC_MAX_JOBS = 128
+NOALLOC_MAX_QUERIES = 16384
+
+
+def run_batch_noalloc(jobs: list[BatchJob], *, morton_sort: bool = True):
+    """run_batch through the allocation-free entry point: the library
+    allocates nothing for the batch (its scratch is a static device array).
+    Row buffers must already exist (pass `rows`, or run the jobs through
+    run_batch once, which allocates them).  At most NOALLOC_MAX_JOBS jobs and
+    NOALLOC_MAX_QUERIES queries.  Returns (per-job result dicts, wall seconds)."""
+    if any(j.rows is None for j in jobs):
+        raise ValueError("run_batch_noalloc needs the jobs' row buffers allocated beforehand")
+    arr, res = _pack_jobs(jobs)
+    seconds = ctypes.c_double()
+    err = ctypes.create_string_buffer(4096)
+    _sync_foreign()
+    st = _lib().rtdl_mi355x_run_batch_noalloc(ctypes.addressof(arr), len(jobs), int(bool(morton_sort)),
+                                              ctypes.addressof(res), ctypes.byref(seconds), err, 4096)
+    _check(st, err, "rtdl_mi355x_run_batch_noalloc")
+    out = [{"rows": int(res[i].rows), "status": int(res[i].status), "ambiguous": int(res[i].ambiguous)}
+           for i in range(len(jobs))]
+    return out, seconds.value
+
+
 class BatchGraph:
     """A batch recorded once into a hipGraph and replayed by `launch()`.
 

@@ -2,6 +2,7 @@
 // include/rtdl_mi355x.h).  Each wraps a runtime pipeline in the status /
 // error-buffer convention of the reference's native backends
 // (reference: src/native/hiprt/rtdl_hiprt_core.cpp:3323-3335).
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -11,6 +12,7 @@
 #include "common.hpp"
 #include "engine.hpp"
 #include "io.hpp"
+#include "noalloc.hpp"
 #include "runtime.hpp"
 #include "serve.hpp"
 
@@ -503,12 +505,9 @@ int rtdl_mi355x_run_batch_profiled(const RtdlBatchJob* jobs, size_t job_count,
   });
 }
 
-// Validated engine jobs of a batch (the handles and pointers are the caller's).
-static std::vector<HostJob> host_jobs_from(const RtdlBatchJob* jobs, size_t job_count) {
-  require(job_count == 0 || jobs != nullptr, "jobs must not be null when job_count is nonzero");
-  std::vector<HostJob> host(job_count);
-  for (size_t i = 0; i < job_count; ++i) {
-    const RtdlBatchJob& b = jobs[i];
+// A validated engine job (the handles and pointers are the caller's).
+static void host_job_from(const RtdlBatchJob& b, HostJob& h) {
+  {
     Prepared* p = prepared_from(b.scene);
     require(b.instance < p->scene.n_instances || (p->scene.n_instances == 0 && b.instance == 0),
             "job instance out of range");
@@ -523,7 +522,7 @@ static std::vector<HostJob> host_jobs_from(const RtdlBatchJob* jobs, size_t job_
     require(b.kind != kJobPipPositive || b.query_stride == 2, "PIP jobs need query_stride 2");
     require(!point_kind || b.query_stride == 2 || b.query_stride == 3,
             "neighbour jobs need query_stride 2 or 3");
-    HostJob& h = host[i];
+    h = HostJob{};
     h.scene = p->scene.view(b.instance);
     h.kind = b.kind;
     h.nq = b.query_count;
@@ -536,7 +535,43 @@ static std::vector<HostJob> host_jobs_from(const RtdlBatchJob* jobs, size_t job_
     require(h.row_capacity >= job_row_capacity(b.kind, b.query_count, b.k, 0) || b.kind == kJobPipPositive,
             "row_capacity is below the rows the job can emit");
   }
+}
+
+static std::vector<HostJob> host_jobs_from(const RtdlBatchJob* jobs, size_t job_count) {
+  require(job_count == 0 || jobs != nullptr, "jobs must not be null when job_count is nonzero");
+  std::vector<HostJob> host(job_count);
+  for (size_t i = 0; i < job_count; ++i) host_job_from(jobs[i], host[i]);
   return host;
+}
+
+int rtdl_mi355x_run_batch_noalloc(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                                  RtdlBatchJobResult* results, double* seconds_out, char* error_out,
+                                  size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(job_count == 0 || (jobs != nullptr && results != nullptr),
+            "jobs and results must not be null when job_count is nonzero");
+    if (job_count > kNoAllocMaxJobs)
+      throw Error("rtdl_mi355x_run_batch_noalloc takes at most " + std::to_string(kNoAllocMaxJobs) + " jobs");
+    uint64_t total = 0;
+    for (size_t i = 0; i < job_count; ++i) total += jobs[i].query_count;
+    if (total > kNoAllocMaxQueries)
+      throw Error("rtdl_mi355x_run_batch_noalloc takes at most " + std::to_string(kNoAllocMaxQueries) +
+                  " queries per batch");
+    std::lock_guard<std::mutex> lock(noalloc_mutex());
+    const NoAllocContext ctx = noalloc_context();
+    for (size_t i = 0; i < job_count; ++i) host_job_from(jobs[i], ctx.jobs[i]);
+    const auto t0 = std::chrono::steady_clock::now();
+    ctx.engine->enqueue(ctx.jobs, job_count, morton_sort != 0);
+    RTDL_HIP(hipStreamSynchronize(ctx.engine->stream()));
+    if (seconds_out)
+      *seconds_out = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const JobCounters* c = ctx.engine->counters();
+    for (size_t i = 0; i < job_count; ++i) {
+      results[i].rows = c[i].rows;
+      results[i].status = c[i].status;
+      results[i].ambiguous = c[i].ambiguous;
+    }
+  });
 }
 
 int rtdl_mi355x_scene_export(void* prepared, RtdlSceneExport* info, void* nodes, void* prims,

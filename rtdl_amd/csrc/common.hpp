@@ -29,6 +29,10 @@ struct Error : std::runtime_error {
 
 #define RTDL_LAUNCH_CHECK() RTDL_HIP(hipGetLastError())
 
+// A literal message is only turned into a string when the check fails.
+inline void require(bool ok, const char* message) {
+  if (!ok) throw Error(message);
+}
 inline void require(bool ok, const std::string& message) {
   if (!ok) throw Error(message);
 }
@@ -49,7 +53,7 @@ class DeviceArray {
   ~DeviceArray() { reset(); }
   DeviceArray(const DeviceArray&) = delete;
   DeviceArray& operator=(const DeviceArray&) = delete;
-  DeviceArray(DeviceArray&& o) noexcept : ptr_(o.ptr_), n_(o.n_) {
+  DeviceArray(DeviceArray&& o) noexcept : ptr_(o.ptr_), n_(o.n_), owned_(o.owned_) {
     o.ptr_ = nullptr;
     o.n_ = 0;
   }
@@ -58,20 +62,30 @@ class DeviceArray {
       reset();
       ptr_ = o.ptr_;
       n_ = o.n_;
+      owned_ = o.owned_;
       o.ptr_ = nullptr;
       o.n_ = 0;
     }
     return *this;
   }
   void resize(size_t n) {
+    require(owned_ || !ptr_, "a borrowed device array cannot be resized");
     reset();
     if (n) ptr_ = static_cast<T*>(pool_alloc(n * sizeof(T)));
     n_ = n;
   }
+  // Views n elements of memory the array does not own (never freed here).
+  void borrow(T* p, size_t n) {
+    reset();
+    ptr_ = p;
+    n_ = n;
+    owned_ = false;
+  }
   void reset() {
-    if (ptr_) pool_free(ptr_);
+    if (ptr_ && owned_) pool_free(ptr_);
     ptr_ = nullptr;
     n_ = 0;
+    owned_ = true;
   }
   T* get() const { return ptr_; }
   size_t size() const { return n_; }
@@ -80,6 +94,28 @@ class DeviceArray {
  private:
   T* ptr_ = nullptr;
   size_t n_ = 0;
+  bool owned_ = true;
+};
+
+// Carves 256-byte-aligned arrays out of one block in order; with a null base
+// it only measures (the bytes a set of arrays needs).
+struct Carver {
+  uint8_t* base = nullptr;
+  size_t capacity = 0;
+  size_t used = 0;
+  template <class T>
+  T* take(size_t n) {
+    used = (used + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+    used += n * sizeof(T);
+    require(!base || used <= capacity, "carved arrays exceed their block");
+    return p;
+  }
+  template <class T>
+  void take(DeviceArray<T>& a, size_t n) {
+    T* p = take<T>(n);
+    if (base) a.borrow(p, n);
+  }
 };
 
 // ------------------------------------------------------- BVH node format --

@@ -11,7 +11,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <map>
+#include <functional>
+#include <string>
 
 #include "common.hpp"
 #include "engine.hpp"
@@ -952,6 +953,59 @@ BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
   }
 }
 
+// Fixed layout, in the order the growing engine sizes the same buffers.
+void BatchEngine::carve(Carver& dev, Carver& host, size_t max_jobs, size_t max_queries) {
+  max_jobs = std::max<size_t>(max_jobs, 1);
+  max_queries = std::max<size_t>(max_queries, 1);
+  const size_t max_items = max_queries / 64 + max_jobs;
+  dev.take(stats_, 2);
+  dev.take(item_counters_, kItemCounterSets * kXcds);
+  dev.take(amb_count_, 2);
+  dev.take(nbr_cand_count_, 1);
+  dev.take(jobs_, max_jobs);
+  dev.take(starts_, max_jobs + 1);
+  dev.take(counters_, max_jobs);
+  dev.take(items_, max_items);
+  dev.take(items_sorted_, max_items);
+  dev.take(item_keys_, max_items);
+  dev.take(item_vals_, max_items);
+  dev.take(nbr_plan_, max_items * 64);
+  dev.take(keys_, max_queries);
+  dev.take(order_, max_queries);
+  dev.take(amb_, max_queries);
+  dev.take(pip_over_, max_queries);
+  sort_ws_.carve(dev, std::max(max_queries, max_items));
+  dev.take(nbr_cand_, std::max<size_t>(max_queries * kNbrCandPerQuery, 1u << 16));
+  dev.take(nbr_stage_, max_items * 64 * kNbrStage);
+  host_jobs_ = host.take<DeviceJob>(max_jobs);
+  host_starts_ = host.take<uint64_t>(max_jobs + 1);
+  host_counters_ = host.take<JobCounters>(max_jobs);
+  host_items_ = host.take<WorkItem>(max_items);
+  host_scene_order_ = host.take<uint32_t>(max_jobs);
+  host_scene_idx_ = host.take<uint32_t>(max_jobs);
+  cap_jobs_ = max_jobs;
+  cap_queries_ = max_queries;
+  cap_items_ = max_items;
+}
+
+void BatchEngine::fixed_bytes(size_t max_jobs, size_t max_queries, size_t* device_bytes, size_t* host_bytes) {
+  BatchEngine probe{MeasureTag{}};
+  Carver dev, host;
+  probe.carve(dev, host, max_jobs, max_queries);
+  *device_bytes = dev.used;
+  *host_bytes = host.used;
+}
+
+BatchEngine::BatchEngine(hipStream_t stream, const EngineStorage& st) : stream_(stream), fixed_(true) {
+  require(st.device != nullptr && st.host != nullptr, "engine storage must not be null");
+  int dev = 0;
+  RTDL_HIP(hipGetDevice(&dev));
+  RTDL_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
+  Carver d{static_cast<uint8_t*>(st.device), st.device_bytes};
+  Carver h{static_cast<uint8_t*>(st.host), st.host_bytes};
+  carve(d, h, st.max_jobs, st.max_queries);
+}
+
 void BatchEngine::family_seconds(double out[5]) const {
   for (int g = 0; g < 5; ++g) out[g] = 0.0;
   for (int f = 0; f < kLaunchFamilies; ++f) {
@@ -970,6 +1024,7 @@ BatchEngine::~BatchEngine() {
     if (fam_stop_[f]) (void)hipEventDestroy(fam_stop_[f]);
   }
   if (done_) (void)hipEventDestroy(done_);
+  if (fixed_) return;  // host arrays belong to the caller's storage
   if (host_jobs_) (void)hipHostFree(host_jobs_);
   if (host_items_) (void)hipHostFree(host_items_);
   if (host_starts_) (void)hipHostFree(host_starts_);
@@ -980,7 +1035,16 @@ void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
   max_jobs = std::max<size_t>(max_jobs, 1);
   max_queries = std::max<size_t>(max_queries, 1);
   const size_t max_items = max_queries / 64 + max_jobs;
+  if (fixed_) {
+    if (max_jobs > cap_jobs_ || max_queries > cap_queries_)
+      throw Error("batch exceeds the fixed engine's limits of " + std::to_string(cap_jobs_) + " jobs and " +
+                  std::to_string(cap_queries_) + " queries");
+    return;
+  }
   if (max_jobs > cap_jobs_) {
+    host_scene_storage_.resize(2 * max_jobs);
+    host_scene_order_ = host_scene_storage_.data();
+    host_scene_idx_ = host_scene_storage_.data() + max_jobs;
     if (host_jobs_) RTDL_HIP(hipHostFree(host_jobs_));
     if (host_starts_) RTDL_HIP(hipHostFree(host_starts_));
     if (host_counters_) RTDL_HIP(hipHostFree(host_counters_));
@@ -1013,29 +1077,47 @@ void BatchEngine::reserve(size_t max_jobs, size_t max_queries) {
 }
 
 // Parking space and stage of the fixed-radius jobs of a batch.
-void BatchEngine::reserve_nbr(const std::vector<HostJob>& jobs) {
+void BatchEngine::reserve_nbr(const HostJob* jobs, size_t n) {
   uint64_t nbr_queries = 0, nbr_items = 0;
-  for (const HostJob& h : jobs)
-    if (h.kind == kJobFixedRadius) {
-      nbr_queries += h.nq;
-      nbr_items += (h.nq + 63) / 64;
+  for (size_t i = 0; i < n; ++i)
+    if (jobs[i].kind == kJobFixedRadius) {
+      nbr_queries += jobs[i].nq;
+      nbr_items += (jobs[i].nq + 63) / 64;
     }
   if (!nbr_queries) return;
   const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
-  if (nbr_cand_.size() < want) nbr_cand_.resize(want);
   const size_t stage = (size_t)nbr_items * 64 * kNbrStage;
+  if (fixed_) {  // carved for max_queries / max_items, which reserve() checked
+    require(nbr_cand_.size() >= want && nbr_stage_.size() >= stage, "fixed engine: fixed-radius space");
+    return;
+  }
+  if (nbr_cand_.size() < want) nbr_cand_.resize(want);
   if (nbr_stage_.size() < stage) nbr_stage_.resize(stage);
 }
 
-void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
-  const size_t n = jobs.size();
+void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
+  require(n == 0 || jobs != nullptr, "jobs must not be null");
   uint64_t total = 0;
-  for (const HostJob& j : jobs) total += j.nq;
+  for (size_t i = 0; i < n; ++i) total += jobs[i].nq;
   require(total < 0xFFFFFFFFull, "batch exceeds 2^32 queries");
   reserve(n, total);
+  // A scene is one instance's tree: jobs on it share its nodes.  Jobs sorted
+  // by (node array, root) get one index per distinct scene (in place, so a
+  // batch allocates nothing on the host).
+  auto scene_less = [&](uint32_t a, uint32_t b) {
+    const SceneView &x = jobs[a].scene, &y = jobs[b].scene;
+    return x.nodes != y.nodes ? std::less<const void*>()(x.nodes, y.nodes) : x.root < y.root;
+  };
+  for (size_t i = 0; i < n; ++i) host_scene_order_[i] = (uint32_t)i;
+  std::sort(host_scene_order_, host_scene_order_ + n, scene_less);
+  uint32_t n_scenes = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t j = host_scene_order_[i];
+    if (i == 0 || scene_less(host_scene_order_[i - 1], j)) ++n_scenes;
+    host_scene_idx_[j] = n_scenes - 1;
+  }
   // order jobs' items by predicate family; each family gets one launch
-  std::vector<std::vector<WorkItem>> fam_items(kLaunchFamilies);
-  std::map<std::pair<const void*, uint32_t>, uint32_t> scene_index;
+  size_t item_off[kLaunchFamilies + 1] = {0};
   uint64_t start = 0;
   for (size_t i = 0; i < n; ++i) {
     const HostJob& h = jobs[i];
@@ -1061,22 +1143,23 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
     d.qstride = h.qstride;
     d.order_base = (uint32_t)start;
     host_starts_[i] = start;
-    // a scene is one instance's tree: jobs on it share its nodes
-    const auto sk = std::make_pair((const void*)h.scene.nodes, h.scene.root);
-    auto found = scene_index.find(sk);
-    if (found == scene_index.end()) found = scene_index.emplace(sk, (uint32_t)scene_index.size()).first;
-    const uint32_t group = ((uint32_t)fam << 24) | found->second;
-    for (uint64_t f = 0; f < h.nq; f += 64)
-      fam_items[fam].push_back(
-          WorkItem{(uint32_t)i, (uint32_t)f, (uint32_t)std::min<uint64_t>(64, h.nq - f), group});
+    item_off[fam + 1] += (h.nq + 63) / 64;
     start += h.nq;
   }
   host_starts_[n] = start;
-  reserve_nbr(jobs);
-  size_t item_off[kLaunchFamilies + 1] = {0};
-  for (int f = 0; f < kLaunchFamilies; ++f) {
-    std::memcpy(host_items_ + item_off[f], fam_items[f].data(), fam_items[f].size() * sizeof(WorkItem));
-    item_off[f + 1] = item_off[f] + fam_items[f].size();
+  reserve_nbr(jobs, n);
+  for (int f = 0; f < kLaunchFamilies; ++f) item_off[f + 1] += item_off[f];
+  {
+    size_t cursor[kLaunchFamilies];
+    for (int f = 0; f < kLaunchFamilies; ++f) cursor[f] = item_off[f];
+    for (size_t i = 0; i < n; ++i) {
+      const HostJob& h = jobs[i];
+      const int fam = family_of(h.kind, h.k);
+      const uint32_t group = ((uint32_t)fam << 24) | host_scene_idx_[i];
+      for (uint64_t f = 0; f < h.nq; f += 64)
+        host_items_[cursor[fam]++] =
+            WorkItem{(uint32_t)i, (uint32_t)f, (uint32_t)std::min<uint64_t>(64, h.nq - f), group};
+    }
   }
   hipStream_t s = stream_;
   if (n) {
@@ -1111,7 +1194,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   const WorkItem* items_run = items_.get();
   if (morton_sort && total > 1 && n_items_all > 1) {
     int scene_bits = 0;
-    while ((1ull << scene_bits) < scene_index.size()) ++scene_bits;
+    while ((1ull << scene_bits) < n_scenes) ++scene_bits;
     require(scene_bits <= 24, "batch spans more than 2^24 scenes");
     item_keys_kernel<<<blocks_for(n_items_all, 256), 256, 0, s>>>(items_.get(), starts_.get(), keys_.get(),
                                                                   n_items_all, scene_bits, item_keys_.get(),
@@ -1203,7 +1286,7 @@ void BatchEngine::enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
   if (n)
     RTDL_HIP(hipMemcpyAsync(host_counters_, counters_.get(), n * sizeof(JobCounters),
                             hipMemcpyDeviceToHost, s));
-  if (!capturing_) RTDL_HIP(hipEventRecord(done_, s));
+  if (!capturing_ && done_) RTDL_HIP(hipEventRecord(done_, s));
 }
 
 hipGraphExec_t BatchEngine::capture(const std::vector<HostJob>& jobs, bool morton_sort) {
@@ -1212,7 +1295,7 @@ hipGraphExec_t BatchEngine::capture(const std::vector<HostJob>& jobs, bool morto
   for (const HostJob& h : jobs) total += h.nq;
   require(total < 0xFFFFFFFFull, "batch exceeds 2^32 queries");
   reserve(jobs.size(), total);
-  reserve_nbr(jobs);
+  reserve_nbr(jobs.data(), jobs.size());
   enable_timing(false);
   enable_stats(false);
   RTDL_HIP(hipStreamSynchronize(stream_));

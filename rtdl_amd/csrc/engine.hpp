@@ -132,20 +132,43 @@ struct HostJob {
   uint64_t row_capacity = 0;
 };
 
+// Memory an engine works in when it must allocate nothing: every device
+// array is carved out of `device`, every host staging array out of `host`
+// (plain memory: copies from it are staged by the runtime), sized for at most
+// `max_jobs` jobs and `max_queries` queries per batch (see fixed_bytes).
+struct EngineStorage {
+  void* device = nullptr;
+  size_t device_bytes = 0;
+  void* host = nullptr;
+  size_t host_bytes = 0;
+  size_t max_jobs = 0;
+  size_t max_queries = 0;
+};
+
 // One engine drives one batch at a time on its stream; reuse it only after
 // the previous batch has completed (done_event / stream sync).
 class BatchEngine {
  public:
+  // Grows its device buffers (caching pool) and pinned host staging on demand.
   explicit BatchEngine(hipStream_t stream);
+  // Allocation-free engine over caller-provided storage: it creates no
+  // events, never grows, and rejects batches beyond the storage's limits.
+  BatchEngine(hipStream_t stream, const EngineStorage& storage);
   ~BatchEngine();
   BatchEngine(const BatchEngine&) = delete;
   BatchEngine& operator=(const BatchEngine&) = delete;
 
+  // Bytes of device and host storage a fixed engine needs for these limits.
+  static void fixed_bytes(size_t max_jobs, size_t max_queries, size_t* device_bytes, size_t* host_bytes);
+
   void reserve(size_t max_jobs, size_t max_queries);
   // Sizes the fixed-radius parking space and stage for a batch's jobs.
-  void reserve_nbr(const std::vector<HostJob>& jobs);
+  void reserve_nbr(const HostJob* jobs, size_t n);
   // Enqueues the whole batch; returns immediately.
-  void enqueue(const std::vector<HostJob>& jobs, bool morton_sort);
+  void enqueue(const HostJob* jobs, size_t n, bool morton_sort);
+  void enqueue(const std::vector<HostJob>& jobs, bool morton_sort) {
+    enqueue(jobs.data(), jobs.size(), morton_sort);
+  }
   // Records the batch into an executable hipGraph instead of running it:
   // buffers are sized first, then the stream is captured while the batch is
   // enqueued, so every launch of the graph replays the whole batch (job
@@ -153,6 +176,7 @@ class BatchEngine {
   // call.  Timing and visit counting are off in a captured batch.  The
   // engine then belongs to the graph: it must not enqueue other batches.
   hipGraphExec_t capture(const std::vector<HostJob>& jobs, bool morton_sort);
+  bool fixed() const { return fixed_; }
   // Valid once the batch has completed.
   const JobCounters* counters() const { return host_counters_; }
   hipStream_t stream() const { return stream_; }
@@ -167,13 +191,21 @@ class BatchEngine {
   void family_seconds(double out[5]) const;
 
  private:
+  struct MeasureTag {};
+  explicit BatchEngine(MeasureTag) : stream_(nullptr), fixed_(true) {}
+  // Lays every buffer out in `dev` / `host` (measures when their bases are null).
+  void carve(Carver& dev, Carver& host, size_t max_jobs, size_t max_queries);
   hipStream_t stream_;
   hipEvent_t done_ = nullptr;
   size_t cap_jobs_ = 0, cap_queries_ = 0, cap_items_ = 0;
-  DeviceJob* host_jobs_ = nullptr;       // pinned
-  WorkItem* host_items_ = nullptr;       // pinned
-  uint64_t* host_starts_ = nullptr;      // pinned
-  JobCounters* host_counters_ = nullptr; // pinned
+  DeviceJob* host_jobs_ = nullptr;       // pinned (fixed engine: caller storage)
+  WorkItem* host_items_ = nullptr;       // pinned (fixed engine: caller storage)
+  uint64_t* host_starts_ = nullptr;      // pinned (fixed engine: caller storage)
+  JobCounters* host_counters_ = nullptr; // pinned (fixed engine: caller storage)
+  uint32_t* host_scene_order_ = nullptr; // jobs by scene key (fixed engine: caller storage)
+  uint32_t* host_scene_idx_ = nullptr;   // scene index of each job
+  std::vector<uint32_t> host_scene_storage_;  // the two arrays above (growing engine)
+  bool fixed_ = false;
   DeviceArray<DeviceJob> jobs_;
   DeviceArray<WorkItem> items_;
   DeviceArray<WorkItem> items_sorted_;   // items_ in (family, scene, Morton) order

@@ -194,7 +194,7 @@ void scan_async_level(const Tin* in, uint64_t* out, size_t n, ScanWorkspace& ws,
     RTDL_LAUNCH_CHECK();
     return;
   }
-  require(level < ws.tiles.size(), "scan workspace too small");
+  require(level < (size_t)ws.levels, "scan workspace too small");
   const size_t tiles = (n + kScanTile - 1) / kScanTile;
   require(tiles <= ws.tiles[level], "scan workspace too small");
   scan_tile_kernel<Tin><<<(unsigned)tiles, kScanBlock, 0, stream>>>(in, out, n, ws.sums[level]);
@@ -259,28 +259,41 @@ void exclusive_scan(const uint32_t* in, uint64_t* out, size_t n, hipStream_t str
   scan_impl<uint32_t>(in, out, n, stream);
 }
 
-void ScanWorkspace::reserve(size_t max_n) {
-  if (max_n <= capacity_ && !tiles.empty()) return;
-  tiles.clear();
+size_t ScanWorkspace::layout(size_t max_n) {
+  levels = 0;
   size_t n = max_n ? max_n : 1;
   size_t total = 0;
   for (;;) {
     const size_t t = (n + kScanTile - 1) / kScanTile;
-    tiles.push_back(t);
+    require(levels < kMaxLevels, "scan workspace: too many levels");
+    tiles[levels++] = t;
     total += 2 * t + 1;
     if (t <= 1) break;
     n = t;
   }
-  storage_.resize(total);
-  sums.clear();
-  offs.clear();
-  uint64_t* p = storage_.get();
-  for (size_t t : tiles) {
-    sums.push_back(p);
-    p += t;
-    offs.push_back(p);
-    p += t + 1;
+  return total;
+}
+
+void ScanWorkspace::point(uint64_t* p) {
+  for (int l = 0; l < levels; ++l) {
+    sums[l] = p;
+    p += tiles[l];
+    offs[l] = p;
+    p += tiles[l] + 1;
   }
+}
+
+void ScanWorkspace::reserve(size_t max_n) {
+  if (max_n <= capacity_ && levels) return;
+  storage_.resize(layout(max_n));
+  point(storage_.get());
+  capacity_ = max_n;
+}
+
+void ScanWorkspace::carve(Carver& c, size_t max_n) {
+  const size_t words = layout(max_n);
+  c.take(storage_, words);
+  if (c.base) point(storage_.get());
   capacity_ = max_n;
 }
 
@@ -299,6 +312,17 @@ void SortWorkspace::reserve(size_t max_n) {
   hist.resize(256 * tiles);
   offs.resize(256 * tiles + 1);
   scan.reserve(256 * tiles);
+  capacity_ = max_n;
+}
+
+void SortWorkspace::carve(Carver& c, size_t max_n) {
+  const size_t n = max_n ? max_n : 1;
+  const size_t tiles = (n + kSortTile - 1) / kSortTile;
+  c.take(keys_tmp, n);
+  c.take(vals_tmp, n);
+  c.take(hist, 256 * tiles);
+  c.take(offs, 256 * tiles + 1);
+  scan.carve(c, 256 * tiles);
   capacity_ = max_n;
 }
 

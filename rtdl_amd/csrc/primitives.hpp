@@ -33,13 +33,20 @@ bool is_nondecreasing(const uint32_t* keys, size_t n, hipStream_t stream);
 // Device memory for exclusive_scan_async over up to `max_n` inputs.
 class ScanWorkspace {
  public:
+  static constexpr int kMaxLevels = 8;  // 2^64 inputs at 256 per tile and level
   void reserve(size_t max_n);
+  // Lays the workspace out in `c` (borrowed storage; see Carver).
+  void carve(Carver& c, size_t max_n);
   size_t capacity() const { return capacity_; }
   // per level: tile sums and their exclusive prefix
-  std::vector<uint64_t*> sums, offs;
-  std::vector<size_t> tiles;
+  uint64_t* sums[kMaxLevels] = {};
+  uint64_t* offs[kMaxLevels] = {};
+  size_t tiles[kMaxLevels] = {};
+  int levels = 0;
 
  private:
+  size_t layout(size_t max_n);  // fills tiles / levels, returns the words needed
+  void point(uint64_t* p);      // sets sums / offs into storage at p
   size_t capacity_ = 0;
   DeviceArray<uint64_t> storage_;
 };
@@ -51,6 +58,7 @@ void exclusive_scan_async(const uint32_t* in, uint64_t* out, size_t n, ScanWorks
 class SortWorkspace {
  public:
   void reserve(size_t max_n);
+  void carve(Carver& c, size_t max_n);  // borrowed storage laid out in `c`
   size_t capacity() const { return capacity_; }
   DeviceArray<uint64_t> keys_tmp;
   DeviceArray<uint32_t> vals_tmp;

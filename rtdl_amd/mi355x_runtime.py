@@ -162,6 +162,8 @@ _SIGNATURES = {
     "rtdl_mi355x_selftest_fp64": ([_VP, _VP, _SZ, _VP, _VP, *_ERR], ctypes.c_int),
     # batched / served execution (structs passed by address; see rtdl_amd.batch)
     "rtdl_mi355x_run_batch": ([_VP, _SZ, ctypes.c_uint32, _VP, ctypes.POINTER(ctypes.c_double), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_run_batch_noalloc": ([_VP, _SZ, ctypes.c_uint32, _VP, ctypes.POINTER(ctypes.c_double), *_ERR],
+                                      ctypes.c_int),
     "rtdl_mi355x_run_batch_profiled": ([_VP, _SZ, ctypes.c_uint32, _VP, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_uint64), *_ERR], ctypes.c_int),
     "rtdl_mi355x_prepare_point_instances": ([_VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
