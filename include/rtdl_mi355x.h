@@ -294,6 +294,43 @@ int rtdl_mi355x_run_prepared_point_in_polygon(void* prepared, const RtdlPoint* p
 /* replaces rtdl_hiprt_destroy_prepared_* (src/native/hiprt/rtdl_hiprt_api.cpp:48-50,456-458) */
 void rtdl_mi355x_destroy_prepared(void* prepared);
 
+/* ---- reference-shaped prepared handles ----
+ * The HIPRT backend's prepared calls with their exact parameter lists, so a
+ * binding written for them needs only the prefix changed.  They share the
+ * prepared scenes above (a handle from one prepare call may also be passed
+ * to the generic run/destroy calls of its scene type). */
+
+/* replaces rtdl_hiprt_prepare_fixed_radius_neighbors_3d (src/native/hiprt/rtdl_hiprt_api.cpp:439-454):
+ * the radius is bound at prepare time; radius < 0 (or NaN) and an empty
+ * search set are errors, as in rtdl_hiprt_core.cpp:1048-1063. */
+int rtdl_mi355x_prepare_fixed_radius_neighbors_3d(const RtdlPoint3D* search_points, size_t search_count,
+                                                  double radius, void** prepared_out, char* error_out,
+                                                  size_t error_size);
+/* replaces rtdl_hiprt_run_prepared_fixed_radius_neighbors_3d (src/native/hiprt/rtdl_hiprt_api.cpp:460-486).
+ * k_max = 0 is an error; unlike HIPRT (k_max <= 64) any k_max is taken: the
+ * bound radius limits the candidates gathered beyond 64. */
+int rtdl_mi355x_run_prepared_fixed_radius_neighbors_3d(void* prepared, const RtdlPoint3D* queries,
+                                                       size_t query_count, uint32_t k_max,
+                                                       RtdlFixedRadiusNeighborRow** rows_out,
+                                                       size_t* row_count_out, char* error_out,
+                                                       size_t error_size);
+/* replaces rtdl_hiprt_destroy_prepared_fixed_radius_neighbors_3d (src/native/hiprt/rtdl_hiprt_api.cpp:456-458) */
+void rtdl_mi355x_destroy_prepared_fixed_radius_neighbors_3d(void* prepared);
+
+/* replaces rtdl_hiprt_prepare_ray_hitcount_3d (src/native/hiprt/rtdl_hiprt_api.cpp:15-46); run with
+ * rtdl_mi355x_run_prepared_ray_hitcount_3d above (same parameters as :52-79) */
+int rtdl_mi355x_prepare_ray_hitcount_3d(const RtdlTriangle3D* triangles, size_t triangle_count,
+                                        void** prepared_out, char* error_out, size_t error_size);
+/* replaces rtdl_hiprt_destroy_prepared_ray_hitcount_3d (src/native/hiprt/rtdl_hiprt_api.cpp:48-50) */
+void rtdl_mi355x_destroy_prepared_ray_hitcount_3d(void* prepared);
+
+/* replaces rtdl_hiprt_prepare_ray_anyhit_2d (src/native/hiprt/rtdl_hiprt_api.cpp:81-151); run with
+ * rtdl_mi355x_run_prepared_ray_anyhit_2d above (same parameters as :157-181) */
+int rtdl_mi355x_prepare_ray_anyhit_2d(const RtdlTriangle* triangles, size_t triangle_count,
+                                      void** prepared_out, char* error_out, size_t error_size);
+/* replaces rtdl_hiprt_destroy_prepared_ray_anyhit_2d (src/native/hiprt/rtdl_hiprt_api.cpp:153-155) */
+void rtdl_mi355x_destroy_prepared_ray_anyhit_2d(void* prepared);
+
 /* ---- batched and served execution (no counterpart in the reference) ----
  *
  * These entry points take device-resident inputs (FP32 query arrays whose

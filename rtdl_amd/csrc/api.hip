@@ -288,6 +288,8 @@ int rtdl_mi355x_run_point_in_polygon(const RtdlPoint* points, size_t point_count
 int rtdl_mi355x_prepare_points_2d(const RtdlPoint* search_points, size_t search_count,
                                   void** prepared_out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
+    require(search_count == 0 || search_points != nullptr,
+            "search point pointer must not be null when count is nonzero");
     make_prepared(prepared_out, [&](Prepared& p) { prepare_points(p, search_points, search_count, 2); });
   });
 }
@@ -295,6 +297,8 @@ int rtdl_mi355x_prepare_points_2d(const RtdlPoint* search_points, size_t search_
 int rtdl_mi355x_prepare_points_3d(const RtdlPoint3D* search_points, size_t search_count,
                                   void** prepared_out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
+    require(search_count == 0 || search_points != nullptr,
+            "search point pointer must not be null when count is nonzero");
     make_prepared(prepared_out, [&](Prepared& p) { prepare_points(p, search_points, search_count, 3); });
   });
 }
@@ -345,6 +349,8 @@ int rtdl_mi355x_run_prepared_bounded_knn_rows(void* prepared, const void* querie
 int rtdl_mi355x_prepare_triangles_3d(const RtdlTriangle3D* triangles, size_t triangle_count,
                                      void** prepared_out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
+    require(triangle_count == 0 || triangles != nullptr,
+            "triangle pointer must not be null when count is nonzero");
     make_prepared(prepared_out, [&](Prepared& p) { prepare_triangles(p, triangles, triangle_count); });
   });
 }
@@ -352,6 +358,8 @@ int rtdl_mi355x_prepare_triangles_3d(const RtdlTriangle3D* triangles, size_t tri
 int rtdl_mi355x_prepare_triangles_2d(const RtdlTriangle* triangles, size_t triangle_count,
                                      void** prepared_out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
+    require(triangle_count == 0 || triangles != nullptr,
+            "triangle pointer must not be null when count is nonzero");
     make_prepared(prepared_out, [&](Prepared& p) { prepare_triangles_2d(p, triangles, triangle_count); });
   });
 }
@@ -421,6 +429,10 @@ int rtdl_mi355x_prepare_polygons(const RtdlPolygonRef* polygons, size_t polygon_
                                  const double* vertices_xy, size_t vertex_xy_count,
                                  void** prepared_out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
+    require(polygon_count == 0 || polygons != nullptr,
+            "polygon pointer must not be null when count is nonzero");
+    require(vertex_xy_count == 0 || vertices_xy != nullptr,
+            "vertex pointer must not be null when vertex count is nonzero");
     make_prepared(prepared_out, [&](Prepared& p) {
       prepare_polygons(p, polygons, polygon_count, vertices_xy, vertex_xy_count);
     });
@@ -445,6 +457,65 @@ void rtdl_mi355x_destroy_prepared(void* prepared) {
   if (p->magic != 0x52544D58u) return;
   delete p;
 }
+
+// ------------------------------------- reference-shaped prepared handles --
+//
+// The HIPRT backend's prepared calls, with their parameter lists, over the
+// same prepared scenes (src/native/hiprt/rtdl_hiprt_api.cpp:15-79, 81-181,
+// 439-486).
+
+int rtdl_mi355x_prepare_fixed_radius_neighbors_3d(const RtdlPoint3D* search_points, size_t search_count,
+                                                  double radius, void** prepared_out, char* error_out,
+                                                  size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(prepared_out != nullptr, "prepared_out must not be null");
+    *prepared_out = nullptr;
+    // rtdl_hiprt_core.cpp:1048-1063
+    require(!(radius < 0.0) && radius == radius, "fixed_radius_neighbors radius must be non-negative");
+    require(search_points != nullptr || search_count == 0,
+            "search point pointer must not be null when count is nonzero");
+    require(search_count > 0, "prepared fixed_radius_neighbors_3d requires at least one search point");
+    make_prepared(prepared_out, [&](Prepared& p) {
+      prepare_points(p, search_points, search_count, 3);
+      p.radius = radius;
+    });
+  });
+}
+
+int rtdl_mi355x_run_prepared_fixed_radius_neighbors_3d(void* prepared, const RtdlPoint3D* queries,
+                                                       size_t query_count, uint32_t k_max,
+                                                       RtdlFixedRadiusNeighborRow** rows_out,
+                                                       size_t* row_count_out, char* error_out,
+                                                       size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    zero_outputs(rows_out, row_count_out);
+    require(prepared != nullptr, "prepared fixed-radius-neighbors handle must not be null");
+    require(k_max > 0, "fixed_radius_neighbors k_max must be positive");
+    on_prepared(prepared, [&](Prepared& p) {
+      require(p.type == SceneType::Points3D, "prepared handle is not a 3-D fixed-radius scene");
+      run_neighbors(p, queries, query_count, NeighborMode::FixedRadius, p.radius, k_max, (void**)rows_out,
+                    row_count_out);
+    });
+  });
+}
+
+void rtdl_mi355x_destroy_prepared_fixed_radius_neighbors_3d(void* prepared) {
+  rtdl_mi355x_destroy_prepared(prepared);
+}
+
+int rtdl_mi355x_prepare_ray_hitcount_3d(const RtdlTriangle3D* triangles, size_t triangle_count,
+                                        void** prepared_out, char* error_out, size_t error_size) {
+  return rtdl_mi355x_prepare_triangles_3d(triangles, triangle_count, prepared_out, error_out, error_size);
+}
+
+void rtdl_mi355x_destroy_prepared_ray_hitcount_3d(void* prepared) { rtdl_mi355x_destroy_prepared(prepared); }
+
+int rtdl_mi355x_prepare_ray_anyhit_2d(const RtdlTriangle* triangles, size_t triangle_count,
+                                      void** prepared_out, char* error_out, size_t error_size) {
+  return rtdl_mi355x_prepare_triangles_2d(triangles, triangle_count, prepared_out, error_out, error_size);
+}
+
+void rtdl_mi355x_destroy_prepared_ray_anyhit_2d(void* prepared) { rtdl_mi355x_destroy_prepared(prepared); }
 
 // ------------------------------------------------------ batched / served --
 

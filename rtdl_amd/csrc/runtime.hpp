@@ -45,6 +45,7 @@ struct Prepared {
   DeviceArray<uint4> refs_input;  // polygons: refs in input order
   size_t n_polygons = 0;
   uint32_t last_ambiguous = 0;  // queries of the last call recomputed exactly
+  double radius = 0.0;          // fixed radius bound at prepare time (reference-shaped FRN handle)
   bool owns_stream = true;
   // own_stream = false borrows the calling thread's stream (per-call path).
   explicit Prepared(bool own_stream = true);

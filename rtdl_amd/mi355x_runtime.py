@@ -159,6 +159,16 @@ _SIGNATURES = {
     "rtdl_mi355x_prepare_polygons": ([_VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_point_in_polygon": ([_VP, _VP, _SZ, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_destroy_prepared": ([_VP], None),
+    # reference-shaped prepared handles (the HIPRT parameter lists)
+    "rtdl_mi355x_prepare_fixed_radius_neighbors_3d": ([_VP, _SZ, ctypes.c_double, ctypes.POINTER(_VP), *_ERR],
+                                                      ctypes.c_int),
+    "rtdl_mi355x_run_prepared_fixed_radius_neighbors_3d": ([_VP, _VP, _SZ, ctypes.c_uint32, *_ROWS, *_ERR],
+                                                           ctypes.c_int),
+    "rtdl_mi355x_destroy_prepared_fixed_radius_neighbors_3d": ([_VP], None),
+    "rtdl_mi355x_prepare_ray_hitcount_3d": ([_VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_destroy_prepared_ray_hitcount_3d": ([_VP], None),
+    "rtdl_mi355x_prepare_ray_anyhit_2d": ([_VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_destroy_prepared_ray_anyhit_2d": ([_VP], None),
     "rtdl_mi355x_selftest_fp64": ([_VP, _VP, _SZ, _VP, _VP, *_ERR], ctypes.c_int),
     # batched / served execution (structs passed by address; see rtdl_amd.batch)
     "rtdl_mi355x_run_batch": ([_VP, _SZ, ctypes.c_uint32, _VP, ctypes.POINTER(ctypes.c_double), *_ERR], ctypes.c_int),

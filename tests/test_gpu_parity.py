@@ -479,3 +479,38 @@ def test_concurrent_threads_on_two_prepared_scenes(gpu_ready):
         th.join(timeout=240)
     assert not any(th.is_alive() for th in threads)
     assert not errors, errors
+
+
+def test_reference_shaped_prepared_fixed_radius_and_hitcount(gpu_ready):
+    """The HIPRT-shaped prepared calls: fixed radius with the radius bound at
+    prepare time and k_max per run (src/native/hiprt/rtdl_hiprt_api.cpp:439-486),
+    and 3-D hit count (:15-79), each against the oracle over several runs."""
+    _say("case: reference-shaped prepared fixed radius and 3-D hit count")
+    rng = np.random.default_rng(4390)
+    s = cases.points(rng, 20000, 3, kind="mixture", fp32=False)
+    arr, _ = m.pack_points(s)
+    lib = m._library()
+    handle = m._call_prepare("rtdl_mi355x_prepare_fixed_radius_neighbors_3d", m._ptr(arr), len(arr), 0.03)
+    try:
+        for k_max in (1, 16, 64, 150):
+            q = cases.points(rng, 4000, 3, kind="mixture", fp32=False)
+            qa, _ = m.pack_points(q)
+            got = m._call_rows("rtdl_mi355x_run_prepared_fixed_radius_neighbors_3d", m.FRN_ROW_DTYPE, handle,
+                               m._ptr(qa), len(qa), k_max)
+            _assert_nn_equal(got, oracle.neighbors("frn", q, s, radius=0.03, k=k_max), ranked=False)
+    finally:
+        lib.rtdl_mi355x_destroy_prepared_fixed_radius_neighbors_3d(handle)
+    t = cases.triangles(rng, 30000, sigma=0.01, fp32=False)
+    tris = m.pack_triangles(t)
+    handle = m._call_prepare("rtdl_mi355x_prepare_ray_hitcount_3d", m._ptr(tris), len(tris))
+    try:
+        for n in (2000, 7000):
+            r = cases.rays(rng, n, tmax=0.4, fp32=False)
+            rays = m.pack_rays(r)
+            got = m._call_rows("rtdl_mi355x_run_prepared_ray_hitcount_3d", m.HITCOUNT_ROW_DTYPE, handle,
+                               m._ptr(rays), len(rays))
+            want = oracle.rays_3d("count", cases.ray_matrix(r), cases.tri_matrix(t), t["id"])
+            assert np.array_equal(got["ray_id"], r["id"])
+            assert np.array_equal(got["hit_count"], want)
+    finally:
+        lib.rtdl_mi355x_destroy_prepared_ray_hitcount_3d(handle)

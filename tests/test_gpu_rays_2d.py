@@ -243,3 +243,24 @@ def test_visibility_pair_rows_match_the_oracle(gpu_ready, dim):
     want = tuple({"observer_id": a.id, "target_id": b.id, "visible": 1 - h["any_hit"]}
                  for (a, b), h in zip(pairs, hits))
     assert got == want
+
+
+def test_reference_shaped_prepared_any_hit_2d(gpu_ready):
+    """rtdl_mi355x_prepare_ray_anyhit_2d / run_prepared_ray_anyhit_2d /
+    destroy_prepared_ray_anyhit_2d with the HIPRT parameter lists
+    (src/native/hiprt/rtdl_hiprt_api.cpp:81-181), against the oracle."""
+    _say("case: reference-shaped prepared 2-D any-hit")
+    rng = np.random.default_rng(6331)
+    t = _tris_2d(rng, 15000)
+    tris = m.pack_triangles_2d(t)
+    handle = m._call_prepare("rtdl_mi355x_prepare_ray_anyhit_2d", m._ptr(tris), len(tris))
+    try:
+        for b in range(2):
+            r = _rays_2d(rng, 3000 + 2000 * b)
+            rays = m.pack_rays_2d(r)
+            got = m._call_rows("rtdl_mi355x_run_prepared_ray_anyhit_2d", m.ANYHIT_ROW_DTYPE, handle,
+                               m._ptr(rays), len(rays))
+            assert np.array_equal(got["ray_id"], r["id"])
+            assert np.array_equal(got["any_hit"], oracle.rays_2d_any(_mat(r, _RN), _mat(t, _TN)))
+    finally:
+        m._library().rtdl_mi355x_destroy_prepared_ray_anyhit_2d(handle)

@@ -72,3 +72,27 @@ def test_missing_library_fails_loudly(monkeypatch):
             m._library()
     finally:
         m._library.cache_clear()
+
+
+def test_reference_shaped_prepared_calls_validate_before_the_device():
+    """The HIPRT-shaped prepared calls take the HIPRT parameter lists
+    (src/native/hiprt/rtdl_hiprt_api.cpp:15-79, 81-181, 439-486) and reject
+    what rtdl_hiprt_core.cpp:1048-1063 rejects before any device work."""
+    lib = m._library()
+    handle = ctypes.c_void_p(7)
+    err = ctypes.create_string_buffer(512)
+    st = lib.rtdl_mi355x_prepare_fixed_radius_neighbors_3d(None, 3, 0.5, ctypes.byref(handle), err, 512)
+    assert st != 0 and handle.value is None
+    st = lib.rtdl_mi355x_prepare_fixed_radius_neighbors_3d(None, 0, -1.0, ctypes.byref(handle), err, 512)
+    assert st != 0 and b"non-negative" in err.value
+    st = lib.rtdl_mi355x_prepare_fixed_radius_neighbors_3d(None, 0, 0.5, ctypes.byref(handle), err, 512)
+    assert st != 0 and b"at least one search point" in err.value
+    status, rows, n, msg = _call("rtdl_mi355x_run_prepared_fixed_radius_neighbors_3d", None, None, 0, 4)
+    assert status != 0 and rows is None and n == 0 and "handle" in msg
+    st = lib.rtdl_mi355x_prepare_ray_hitcount_3d(None, 2, ctypes.byref(handle), err, 512)
+    assert st != 0 and b"must not be null" in err.value
+    st = lib.rtdl_mi355x_prepare_ray_anyhit_2d(None, 2, ctypes.byref(handle), err, 512)
+    assert st != 0 and b"must not be null" in err.value
+    for name in ("rtdl_mi355x_destroy_prepared_fixed_radius_neighbors_3d",
+                 "rtdl_mi355x_destroy_prepared_ray_hitcount_3d", "rtdl_mi355x_destroy_prepared_ray_anyhit_2d"):
+        getattr(lib, name)(None)  # a null handle is ignored, as delete nullptr is
