@@ -131,6 +131,17 @@ void rtdl_mi355x_free_rows(void* rows);
  * (cf. src/rtdsl/optix_runtime.py:4880-4902).  Returns the count written. */
 int rtdl_mi355x_get_last_phase_timings(double* seconds_out, size_t capacity);
 
+/* Traversal measurement of this thread's per-call and prepared calls (no
+ * counterpart in the reference; for the roofline of the per-call kernels).
+ * get_last_walk_stats gives the last call's traversal kernel time (HIP
+ * events on the call's stream: the neighbour walk with its exact re-run for
+ * k <= 64, the ray walk, the PIP counting walk) and, while counting is on,
+ * visits[0] node records fetched and visits[1] primitive records tested
+ * (counting kernels are slower: time and count in separate calls). */
+int rtdl_mi355x_set_walk_counting(uint32_t on, char* error_out, size_t error_size);
+int rtdl_mi355x_get_last_walk_stats(double* kernel_seconds, uint64_t* visits, char* error_out,
+                                    size_t error_size);
+
 /* ---- plain per-call entry points ---- */
 
 /* replaces rtdl_hiprt_run_fixed_radius_neighbors_2d (src/native/hiprt/rtdl_hiprt_api.cpp:488-515) */
@@ -353,8 +364,8 @@ enum {
 /* status bits of a job / request.  A PIP job's default row budget is 16 rows
  * per point on average (row_capacity = 16 * query_count); one point may have
  * any number of containing polygons within that budget. */
-#define RTDL_STATUS_ROW_OVERFLOW 1u /* rows beyond row_capacity were dropped */
-#define RTDL_STATUS_REJECTED 4u     /* served request larger than a slot's row region */
+#define RTDL_STATUS_ROW_OVERFLOW 1u /* the job's rows did not fit row_capacity: rows are incomplete */
+#define RTDL_STATUS_REJECTED 4u     /* served request whose rows can exceed the serving row arena */
 #define RTDL_STATUS_INTERNAL 8u     /* an internal invariant failed: the job's rows are not valid */
 
 typedef struct RtdlBatchJob {
@@ -368,7 +379,10 @@ typedef struct RtdlBatchJob {
   uint32_t k;            /* k (KNN) or k_max (fixed radius, bounded KNN), 1..64 */
   uint32_t pad;
   void* rows;            /* device row buffer */
-  uint64_t row_capacity; /* rows the buffer holds */
+  uint64_t row_capacity; /* rows the buffer holds, < 2^32: at least query_count for any-hit and
+                            hit-count jobs; other kinds may hold fewer than the most they can
+                            emit (k per query, ...) and set RTDL_STATUS_ROW_OVERFLOW, with
+                            incomplete rows, if they run out */
 } RtdlBatchJob;
 
 typedef struct RtdlBatchJobResult {
@@ -498,7 +512,7 @@ typedef struct RtdlServeConfig {
   uint32_t k;                   /* KNN requests, 1..64 */
   uint32_t slots;               /* batches in flight, each on its own stream */
   uint32_t morton_sort;         /* order each batch's queries by (request, Morton code) */
-  uint64_t memory_budget_bytes; /* device bytes for in-flight row buffers, split over slots */
+  uint64_t memory_budget_bytes; /* device bytes for the row buffers of all in-flight batches */
   uint64_t max_batch_queries;
   const float* point_queries;   /* device arena, 3 FP32 per query (neighbour requests) */
   const float* ray_queries;     /* device arena, 7 FP32 per ray */
@@ -522,14 +536,16 @@ typedef struct RtdlServeStats {
   uint64_t batches;
   uint64_t queries;
   uint64_t rows;
-  uint64_t peak_inflight_bytes;
-  uint64_t region_bytes;
+  uint64_t peak_inflight_bytes; /* largest sum of in-flight row buffers (<= region_bytes) */
+  uint64_t region_bytes;        /* the row arena: min(budget, slots x the largest admissible batch) */
 } RtdlServeStats;
 
 /* Replays a request stream against resident scenes with continuous
  * batching: up to `slots` batches in flight, each admitted from the
  * requests that have arrived, in order, while their row buffers fit the
- * slot's share of the memory budget.  Latency = done_s - arrival_s. */
+ * largest free range of one row arena bounded by the memory budget (the
+ * first request that does not fit waits for completions to free memory; one
+ * that can never fit is rejected).  Latency = done_s - arrival_s. */
 int rtdl_mi355x_serve_replay(void* const* scenes, size_t scene_count,
                              const RtdlServeRequest* requests, size_t request_count,
                              const RtdlServeConfig* config, RtdlServeResult* results,

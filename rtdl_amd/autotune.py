@@ -94,7 +94,7 @@ def _canonical_rows(job: B.BatchJob, result: dict) -> dict[str, np.ndarray]:
 
 def autotune_batch(build_scenes: Callable[[], object], make_jobs: Callable[[object], list],
                    *, knobs: dict[str, tuple[int, ...]] | None = None, repeats: int = 3,
-                   morton_sort: bool = True, check_rows: bool = True,
+                   morton_sort: bool = True, check_rows: bool = True, check_jobs: list[int] | None = None,
                    cache_path: str | None = None, key: str | None = None) -> dict:
     """Tunes the library for one workload and leaves the best tuning set.
 
@@ -103,8 +103,10 @@ def autotune_batch(build_scenes: Callable[[], object], make_jobs: Callable[[obje
     as the median of `repeats` runs after a warm run. With `check_rows`,
     every candidate's rows must equal the first candidate's, field by field
     after the canonical order, or RuntimeError is raised and the original
-    tuning restored. With `cache_path` and `key`, a cached choice is applied
-    without timing, and a new choice is stored."""
+    tuning restored; `check_jobs` limits the comparison to those job indices
+    (for batches whose full row set is too large to compare on the host).
+    With `cache_path` and `key`, a cached choice is applied without timing,
+    and a new choice is stored."""
     if cache_path and key and os.path.exists(cache_path):
         with open(cache_path) as f:
             cache = json.load(f)
@@ -124,7 +126,8 @@ def autotune_batch(build_scenes: Callable[[], object], make_jobs: Callable[[obje
         if any(r["status"] for r in res):
             raise RuntimeError(f"tuning {t}: a job reported status {[r['status'] for r in res]}")
         if check_rows:
-            rows = [_canonical_rows(j, r) for j, r in zip(jobs, res)]
+            picked = range(len(jobs)) if check_jobs is None else check_jobs
+            rows = [_canonical_rows(jobs[i], res[i]) for i in picked]
             if reference is None:
                 reference = rows
             else:

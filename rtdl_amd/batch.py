@@ -349,7 +349,7 @@ def job_rows(job: BatchJob, result: dict) -> np.ndarray:
     """Copies a job's rows to the host as a numpy structured array."""
     dtype = ROW_DTYPES[job.kind]
     n = min(int(result["rows"]), int(job.row_capacity))
-    if isinstance(job.rows, DeviceBuffer):
+    if hasattr(job.rows, "download"):  # DeviceBuffer or DeviceView
         raw = job.rows.download(n * dtype.itemsize)
     else:
         raw = job.rows[: n * dtype.itemsize].cpu().numpy()

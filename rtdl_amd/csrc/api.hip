@@ -103,6 +103,23 @@ int rtdl_mi355x_context_probe(char* device_name, size_t device_name_size, int* d
 
 void rtdl_mi355x_free_rows(void* rows) { std::free(rows); }
 
+int rtdl_mi355x_set_walk_counting(uint32_t on, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] { thread_walk_stats().counting = on != 0; });
+}
+
+int rtdl_mi355x_get_last_walk_stats(double* kernel_seconds, uint64_t* visits, char* error_out,
+                                    size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    const WalkStats& w = thread_walk_stats();
+    require(w.valid, "no traversal has been measured on this thread");
+    if (kernel_seconds) *kernel_seconds = w.kernel_s;
+    if (visits) {
+      visits[0] = w.nodes;
+      visits[1] = w.prims;
+    }
+  });
+}
+
 int rtdl_mi355x_get_last_phase_timings(double* seconds_out, size_t capacity) {
   if (!seconds_out) return 0;
   const auto& s = thread_clock().seconds;
@@ -603,8 +620,11 @@ static void host_job_from(const RtdlBatchJob& b, HostJob& h) {
     h.k = b.k;
     h.rows = b.rows;
     h.row_capacity = b.row_capacity;
-    require(h.row_capacity >= job_row_capacity(b.kind, b.query_count, b.k, 0) || b.kind == kJobPipPositive,
-            "row_capacity is below the rows the job can emit");
+    // any-hit and hit-count rows are written at their ray's index; every
+    // other kind reserves rows and reports RTDL_STATUS_ROW_OVERFLOW past the
+    // capacity, so a buffer may be sized by the rows expected
+    const bool indexed = b.kind == kJobRayAnyHit || b.kind == kJobRayHitCount;
+    require(!indexed || h.row_capacity >= b.query_count, "any-hit and hit-count jobs need one row per ray");
   }
 }
 

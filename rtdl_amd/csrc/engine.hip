@@ -289,14 +289,15 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
 // by construction.  A query whose buckets 0..b still hold more than
 // kNbrCandMax neighbours after four counts (or that finds the parking space
 // full) is computed by batch_fix_kernel.
-// The count walk also stages the slots it counts, up to kNbrStage of them per
-// query.  When the last count round stays within that, the staged slots
+// The count walk also stages the slots it counts, up to `stage_slots` of them
+// per query (a multiple of 4, at most kNbrStageMax, chosen per batch by its
+// memory).  When the last count round stays within that, the staged slots
 // are a superset of the rows (all of them for a query with c <= k, those in
 // buckets 0..b otherwise): the emit walk skips the query and nbr_sort_kernel
 // recomputes the FP64 distances of the staged slots, filters and sorts them.
 constexpr uint32_t kNbrBuckets = 32;
 constexpr uint32_t kNbrCandMax = 128;
-constexpr uint32_t kNbrStage = 256;
+constexpr uint32_t kNbrStageMax = 256;
 constexpr int kSortBlock = 64;
 // histogram counters are 16-bit, two per LDS word; a query that meets more
 // neighbours than one counter holds goes to the exact fix-up
@@ -335,7 +336,8 @@ struct NbrCount {
 
 template <bool STATS>
 __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint32_t* my_hist,
-                                           uint32_t* __restrict__ stage, LaneStats& st) {
+                                           uint32_t* __restrict__ stage, uint32_t stage_slots,
+                                           LaneStats& st) {
   const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
   auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
   double qx, qy, qz;
@@ -361,7 +363,7 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
       // no result needed: a plain LDS add
       __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (count <= kNbrStage) {
+      if (count <= stage_slots) {
         const uint32_t e = (count - 1) & 3u;
         sbuf.x = e == 0 ? slot : sbuf.x;
         sbuf.y = e == 1 ? slot : sbuf.y;
@@ -384,12 +386,12 @@ __device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint
     };
     walk_points<STATS>(sc, sc.root, sc.end, qf, thr2, st, visit);
     // the last, partial group of four (entries past `count` are never read)
-    if ((count & 3u) && count <= kNbrStage) reinterpret_cast<uint4*>(stage)[(count - 1) >> 2] = sbuf;
+    if ((count & 3u) && count <= stage_slots) reinterpret_cast<uint4*>(stage)[(count - 1) >> 2] = sbuf;
     r.count = count;
     r.heavy = shrunk || count > k;
     r.bucket = top;
     r.n_cand = 0;
-    r.staged = count <= kNbrStage;
+    r.staged = count <= stage_slots;
     if (!r.heavy) return r;
     if (count > kNbrCountMax) {
       r.refine = true;
@@ -420,7 +422,7 @@ template <bool STATS>
 __global__ __launch_bounds__(256) void nbr_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
     uint32_t* item_counter, NbrPlan* __restrict__ plans, unsigned long long* cand_count,
-    uint64_t cand_cap, uint32_t* __restrict__ stage, uint4* amb, uint32_t* amb_count,
+    uint64_t cand_cap, uint32_t* __restrict__ stage, uint32_t stage_slots, uint4* amb, uint32_t* amb_count,
     unsigned long long* stats) {
   // buckets 2w and 2w + 1 of thread t in the low and high half of hist[w * 256 + t]
   __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
@@ -440,7 +442,8 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
     NbrCount r{0.0, 0.0, 0u, 0u, 0u, false, false, false};
     if (active) {
       qi = job_query(j, w.first + lane);
-      r = nbr_count_query<STATS>(j, qi, my_hist, stage + ((size_t)it * 64 + lane) * kNbrStage, st);
+      r = nbr_count_query<STATS>(j, qi, my_hist, stage + ((size_t)it * 64 + lane) * stage_slots,
+                                 stage_slots, st);
     }
     uint32_t n_rows = r.heavy ? k : r.count;
     bool refine = r.refine;
@@ -584,7 +587,8 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
                                                              uint32_t n_items, uint32_t* item_counter,
                                                              const NbrPlan* __restrict__ plans,
                                                              const NbrCand* __restrict__ cands,
-                                                             const uint32_t* __restrict__ stage) {
+                                                             const uint32_t* __restrict__ stage,
+                                                             uint32_t stage_slots) {
   __shared__ uint64_t pack_d[kNbrCandMax];
   __shared__ uint32_t pack_i[kNbrCandMax];
   const uint32_t lane = threadIdx.x;
@@ -621,7 +625,7 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
         // the kept ones, at most kNbrCandMax, into LDS
         const double qx = readlane_f64(mqx, q), qy = readlane_f64(mqy, q), qz = readlane_f64(mqz, q);
         const double scale = (double)kNbrBuckets / pl.radius;
-        const uint32_t* sq = stage + ((size_t)it * 64 + q) * kNbrStage;
+        const uint32_t* sq = stage + ((size_t)it * 64 + q) * stage_slots;
         uint32_t kept = 0;
         for (uint32_t e0 = 0; e0 < pl.staged; e0 += 64) {
           const uint32_t e = e0 + lane;
@@ -895,6 +899,16 @@ constexpr int kQueueNbrEmit = kLaunchFamilies;
 constexpr int kQueueNbrSort = kLaunchFamilies + 1;
 // parking space for fixed-radius candidates, per fixed-radius query of a batch
 constexpr uint64_t kNbrCandPerQuery = 80;
+// Fixed-radius scratch of one batch: the stage (stage_slots x 4 B per query)
+// and the parking space (kNbrCandPerQuery x 16 B per query) shrink, in that
+// order of preference, for batches whose full-size scratch would exceed these
+// budgets.  A smaller stage sends more queries through the emit walk; a
+// smaller parking space sends more queries to the exact fix-up.  Rows are the
+// same either way.
+constexpr uint64_t kNbrStageBudget = uint64_t(24) << 30;
+constexpr uint64_t kNbrParkBudget = uint64_t(16) << 30;
+constexpr uint32_t kNbrStageMin = 32;
+constexpr uint64_t kNbrCandPerQueryMin = 16;
 
 int family_of(uint32_t kind, uint32_t k) {
   switch (kind) {
@@ -976,7 +990,7 @@ void BatchEngine::carve(Carver& dev, Carver& host, size_t max_jobs, size_t max_q
   dev.take(pip_over_, max_queries);
   sort_ws_.carve(dev, std::max(max_queries, max_items));
   dev.take(nbr_cand_, std::max<size_t>(max_queries * kNbrCandPerQuery, 1u << 16));
-  dev.take(nbr_stage_, max_items * 64 * kNbrStage);
+  dev.take(nbr_stage_, max_items * 64 * kNbrStageMax);
   host_jobs_ = host.take<DeviceJob>(max_jobs);
   host_starts_ = host.take<uint64_t>(max_jobs + 1);
   host_counters_ = host.take<JobCounters>(max_jobs);
@@ -1085,12 +1099,21 @@ void BatchEngine::reserve_nbr(const HostJob* jobs, size_t n) {
       nbr_items += (jobs[i].nq + 63) / 64;
     }
   if (!nbr_queries) return;
-  const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
-  const size_t stage = (size_t)nbr_items * 64 * kNbrStage;
-  if (fixed_) {  // carved for max_queries / max_items, which reserve() checked
-    require(nbr_cand_.size() >= want && nbr_stage_.size() >= stage, "fixed engine: fixed-radius space");
+  if (fixed_) {  // carved for max_queries / max_items at the full stage, which reserve() checked
+    stage_slots_ = kNbrStageMax;
+    require(nbr_cand_.size() >= nbr_queries * kNbrCandPerQuery &&
+                nbr_stage_.size() >= (size_t)nbr_items * 64 * kNbrStageMax,
+            "fixed engine: fixed-radius space");
     return;
   }
+  uint32_t slots = kNbrStageMax;
+  while (slots > kNbrStageMin && nbr_items * 64 * slots * sizeof(uint32_t) > kNbrStageBudget) slots >>= 1;
+  stage_slots_ = slots;
+  uint64_t per_query = kNbrCandPerQuery;
+  while (per_query > kNbrCandPerQueryMin && nbr_queries * per_query * sizeof(NbrCand) > kNbrParkBudget)
+    per_query = std::max<uint64_t>(per_query / 2, kNbrCandPerQueryMin);
+  const size_t want = (size_t)std::max<uint64_t>(nbr_queries * per_query, 1u << 16);
+  const size_t stage = (size_t)nbr_items * 64 * slots;
   if (nbr_cand_.size() < want) nbr_cand_.resize(want);
   if (nbr_stage_.size() < stage) nbr_stage_.resize(stage);
 }
@@ -1241,22 +1264,24 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
         RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamNbr) {
-      require(nbr_stage_.size() >= (size_t)n_items * 64 * kNbrStage, "fixed-radius stage not reserved");
+      require(nbr_stage_.size() >= (size_t)n_items * 64 * stage_slots_, "fixed-radius stage not reserved");
       if (stats_on_)
         nbr_count_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
                                                      nbr_cand_count_.get(), nbr_cand_.size(),
-                                                     nbr_stage_.get(), amb_.get(), amb_count_.get(), st);
+                                                     nbr_stage_.get(), stage_slots_, amb_.get(),
+                                                     amb_count_.get(), st);
       else
         nbr_count_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
                                                       nbr_cand_count_.get(), nbr_cand_.size(),
-                                                      nbr_stage_.get(), amb_.get(), amb_count_.get(), st);
+                                                      nbr_stage_.get(), stage_slots_, amb_.get(),
+                                                      amb_count_.get(), st);
       RTDL_LAUNCH_CHECK();
       nbr_emit_kernel<<<grid, 256, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrEmit * kXcds,
                                            nbr_plan_.get(), nbr_cand_.get());
       RTDL_LAUNCH_CHECK();
       const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 32));
       nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrSort * kXcds,
-                                                 nbr_plan_.get(), nbr_cand_.get(), nbr_stage_.get());
+                                                 nbr_plan_.get(), nbr_cand_.get(), nbr_stage_.get(), stage_slots_);
       RTDL_LAUNCH_CHECK();
       if (f == last_nbr) {
         batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());

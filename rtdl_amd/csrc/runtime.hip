@@ -111,6 +111,59 @@ thread_local ThreadStream t_stream;
 thread_local PhaseClock t_clock;
 }  // namespace
 
+namespace {
+// Events and counters behind WalkStats, per thread.
+struct WalkProbe {
+  WalkStats stats;
+  hipEvent_t start = nullptr, stop = nullptr;
+  DeviceArray<unsigned long long> counters;
+  ~WalkProbe() {
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+  }
+};
+thread_local WalkProbe t_probe;
+
+// Marks the start of a call's traversal kernels; returns the counters the
+// kernels add to while counting is on (null otherwise).
+unsigned long long* walk_begin(hipStream_t s) {
+  WalkProbe& w = t_probe;
+  w.stats.valid = false;
+  if (!w.start) {
+    RTDL_HIP(hipEventCreate(&w.start));
+    RTDL_HIP(hipEventCreate(&w.stop));
+  }
+  unsigned long long* ctr = nullptr;
+  if (w.stats.counting) {
+    if (!w.counters.size()) w.counters.resize(2);
+    fill_u32(reinterpret_cast<uint32_t*>(w.counters.get()), 4, 0u, s);
+    ctr = w.counters.get();
+  }
+  RTDL_HIP(hipEventRecord(w.start, s));
+  return ctr;
+}
+
+void walk_end(hipStream_t s) { RTDL_HIP(hipEventRecord(t_probe.stop, s)); }
+
+// After the stream has been synchronised past walk_end.
+void walk_collect() {
+  WalkProbe& w = t_probe;
+  float ms = 0.0f;
+  RTDL_HIP(hipEventElapsedTime(&ms, w.start, w.stop));
+  w.stats.kernel_s = ms * 1e-3;
+  w.stats.nodes = w.stats.prims = 0;
+  if (w.stats.counting) {
+    unsigned long long v[2] = {0, 0};
+    RTDL_HIP(hipMemcpy(v, w.counters.get(), sizeof(v), hipMemcpyDeviceToHost));
+    w.stats.nodes = v[0];
+    w.stats.prims = v[1];
+  }
+  w.stats.valid = true;
+}
+}  // namespace
+
+WalkStats& thread_walk_stats() { return t_probe.stats; }
+
 hipStream_t thread_stream() {
   if (!t_stream.s) RTDL_HIP(hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking));
   return t_stream.s;
@@ -604,8 +657,9 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
     amb_list.resize(nq);
     amb_count.resize(1);
     RTDL_HIP(hipMemsetAsync(amb_count.get(), 0, sizeof(uint32_t), s));
+    unsigned long long* ctr = walk_begin(s);
     launch_collect_packed(view, q.get(), order.get(), nq, search_radius, k, cand_d.get(),
-                          cand_id.get(), count.get(), amb_list.get(), amb_count.get(), nullptr,
+                          cand_id.get(), count.get(), amb_list.get(), amb_count.get(), ctr,
                           256, s);
     uint32_t n_amb = 0;
     RTDL_HIP(hipMemcpyAsync(&n_amb, amb_count.get(), sizeof(n_amb), hipMemcpyDeviceToHost, s));
@@ -615,6 +669,9 @@ void run_neighbors(Prepared& p, const void* raw_queries, size_t nq_in, NeighborM
     // with the exact FP64 list (same slots, same counts)
     launch_collect_exact(view, q.get(), amb_list.get(), n_amb, search_radius, k, false,
                          cand_d.get(), cand_id.get(), count.get(), nullptr, /*seeded=*/true, s);
+    walk_end(s);
+    RTDL_HIP(hipStreamSynchronize(s));
+    walk_collect();
   } else {
     // Keep the 64 nearest per query and count everything inside the radius;
     // queries with more than 64 candidates are collected in full and sorted.
@@ -729,9 +786,12 @@ static void run_rays_dim(Prepared& p, const void* raw_rays, size_t nr_in, int mo
   clock.lap(0);
   DeviceArray<uint32_t> out_u(nr);
   DeviceArray<double> out_t(mode == 2 ? nr : 0);
+  unsigned long long* ctr = walk_begin(s);
   launch_rays(p.scene.view(0), rays.get(), order.get(), nr, dim == 2 ? mode + 4 : mode, out_u.get(),
-              out_t.get(), nullptr, 256, s);
+              out_t.get(), ctr, 256, s);
+  walk_end(s);
   RTDL_HIP(hipStreamSynchronize(s));
+  walk_collect();
   clock.lap(2);
   if (mode != 2) {
     DeviceArray<uint8_t> rows((size_t)nr * 8);
@@ -792,10 +852,14 @@ void run_pip(Prepared& p, const void* raw_points, size_t np_in, bool positive_on
   DeviceArray<uint32_t> count(np);
   DeviceArray<uint64_t> offsets((size_t)np + 1);
   const SceneView view = p.scene.view(0);
+  // the measured walk is the counting pass; the writing pass repeats it
+  unsigned long long* ctr = walk_begin(s);
   launch_pip(view, xy.get(), order.get(), np, positive_only, eps_sq, count.get(), nullptr, nullptr,
-             nullptr, 256, s);
+             ctr, 256, s);
+  walk_end(s);
   exclusive_scan(count.get(), offsets.get(), np, s);
   const uint64_t total = read_u64(offsets.get() + np, s);
+  walk_collect();
   DeviceArray<uint32_t> poly_idx(total ? total : 1);
   launch_pip(view, xy.get(), order.get(), np, positive_only, eps_sq, count.get(), offsets.get(),
              poly_idx.get(), nullptr, 256, s);

@@ -30,6 +30,19 @@ struct PhaseClock {
 };
 PhaseClock& thread_clock();
 
+// Traversal measurement of the per-call and prepared calls on this thread:
+// the time of the last call's traversal kernels (HIP events on the call's
+// stream) and, while counting is on, the node records fetched and primitive
+// records tested (counting kernels, slower).  Neighbour calls with k <= 64,
+// ray calls and PIP calls record it.
+struct WalkStats {
+  bool counting = false;
+  bool valid = false;
+  double kernel_s = 0.0;
+  unsigned long long nodes = 0, prims = 0;
+};
+WalkStats& thread_walk_stats();
+
 enum class SceneType : int { Points2D = 0, Points3D = 1, Triangles3D = 2, Polygons = 3, Triangles2D = 4 };
 
 // A prepared (device-resident) scene.  Calls on one handle are serialised by

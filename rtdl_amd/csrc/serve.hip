@@ -1,16 +1,21 @@
-// Request loop with continuous batching.
+// Request loop with continuous batching and memory-budget admission.
 //
 // Requests (a query set of one predicate against one resident scene) arrive
-// at their replay times.  The loop keeps `slots` batches in flight, each on
-// its own stream with its own BatchEngine and its own slice of the row
-// budget.  Whenever a slot is free, every request that has arrived and not
-// yet been dispatched is admitted into a new batch, in arrival order, while
-// its row buffer fits the slot's slice of the memory budget and the batch
-// stays under `max_batch_queries`; the batch is enqueued and the loop goes
-// back to polling.  Batches on different slots overlap on the device, so
-// requests that arrive while others run join the next batch as soon as a
-// slot frees up.  A request's latency runs from its arrival time to the
-// moment the loop observes its batch's completion event.
+// at their replay times.  The loop keeps up to `slots` batches in flight,
+// each on its own stream with its own BatchEngine.  Row buffers of every
+// in-flight batch come from one device arena of min(budget, slots x the
+// largest admissible batch) bytes: a batch occupies one contiguous range of
+// it, placed first-fit in the free space left by the batches still in
+// flight, so the rows in flight never exceed the budget.  Whenever a slot is
+// free, the requests that have arrived and are not yet dispatched are
+// admitted in arrival order while their row buffers fit the largest free
+// range and the batch stays under `max_batch_queries`; the first request
+// that does not fit waits (head of line) until completions free space.  A
+// request whose rows can exceed the whole arena is rejected.  Batches on
+// different slots overlap on the device, so requests that arrive while
+// others run join the next batch as soon as a slot and memory free up.  A
+// request's latency runs from its arrival time to the moment the loop
+// observes its batch's completion event.
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
@@ -33,12 +38,10 @@ constexpr uint32_t kStatusRejected = 4u;
 struct Slot {
   hipStream_t stream = nullptr;
   std::unique_ptr<BatchEngine> engine;
-  DeviceArray<uint8_t> region;
   hipEvent_t done = nullptr;
   bool busy = false;
-  std::vector<size_t> requests;  // request indices of the batch in flight
-  std::vector<std::pair<size_t, uint8_t*>> captures;  // (request, pinned copy)
-  std::vector<uint64_t> capture_rows;                  // row capacity of each copy
+  uint64_t range_at = 0, range_bytes = 0;  // the batch's rows in the arena
+  std::vector<size_t> requests;           // request indices of the batch in flight
 };
 
 uint32_t stride_of(uint32_t kind) {
@@ -51,13 +54,33 @@ uint32_t stride_of(uint32_t kind) {
   }
 }
 
+// Largest free range of the arena given the ranges of the busy slots.
+void largest_gap(const std::vector<Slot>& slots, uint64_t arena, uint64_t* at, uint64_t* bytes) {
+  std::pair<uint64_t, uint64_t> used[16];
+  size_t n = 0;
+  for (const Slot& s : slots)
+    if (s.busy && s.range_bytes) used[n++] = {s.range_at, s.range_bytes};
+  std::sort(used, used + n);
+  uint64_t cursor = 0;
+  *at = 0;
+  *bytes = 0;
+  for (size_t i = 0; i <= n; ++i) {
+    const uint64_t end = i < n ? used[i].first : arena;
+    if (end - cursor > *bytes) {
+      *at = cursor;
+      *bytes = end - cursor;
+    }
+    if (i < n) cursor = used[i].first + used[i].second;
+  }
+}
+
 }  // namespace
 
 void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* requests,
                   size_t n_requests, const RtdlServeConfig& cfg, RtdlServeResult* results,
                   RtdlServeStats* stats) {
   require(cfg.slots >= 1 && cfg.slots <= 16, "slots must be in [1, 16]");
-  require(cfg.memory_budget_bytes >= cfg.slots * 4096ull, "memory budget too small");
+  require(cfg.memory_budget_bytes >= 4096ull, "memory budget too small");
   require(cfg.max_batch_queries >= 1, "max_batch_queries must be positive");
   std::vector<SceneView> views(n_scenes);
   std::vector<SceneType> types(n_scenes);
@@ -85,30 +108,61 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
   require(cfg.k >= 1 && cfg.k <= 64 && cfg.k_max >= 1 && cfg.k_max <= 64,
           "served k and k_max must be in [1, 64]");
 
+  auto row_capacity = [&](const RtdlServeRequest& q) -> uint64_t {
+    return job_row_capacity(q.kind, q.query_count, q.kind == kJobKnn ? cfg.k : cfg.k_max, kPipHitCap);
+  };
+  auto row_need = [&](const RtdlServeRequest& q) -> uint64_t {
+    return (row_capacity(q) * job_row_bytes(q.kind) + 255) & ~uint64_t(255);
+  };
+
   const size_t n_slots = cfg.slots;
-  // Each slot owns an equal share of the budget; physically it only needs
-  // what its largest admissible batch can emit.
+  // The arena never exceeds the budget; physically it only needs what
+  // `slots` batches of the largest admissible size can emit.
   const uint64_t per_query = std::max<uint64_t>(
       {(uint64_t)cfg.k_max * sizeof(FrnRowRec), (uint64_t)cfg.k * sizeof(KnnRowRec),
        (uint64_t)kPipHitCap * sizeof(PipRowRec), sizeof(ClosestRowRec)});
   const uint64_t batch_cap = (cfg.max_batch_queries + 65536) * per_query + (uint64_t(64) << 20);
-  const uint64_t region_bytes =
-      std::min<uint64_t>(cfg.memory_budget_bytes / n_slots, batch_cap) & ~uint64_t(255);
+  const uint64_t arena_bytes =
+      std::min<uint64_t>(cfg.memory_budget_bytes, n_slots * batch_cap) & ~uint64_t(255);
+  DeviceArray<uint8_t> arena(arena_bytes);
+
+  // Captured rows go to one pinned pool sized before the loop, so the loop
+  // itself only enqueues copies; host copies are made after it.
+  std::vector<uint64_t> capture_at(n_requests, 0);
+  uint64_t capture_total = 0;
+  for (size_t r = 0; r < n_requests; ++r) {
+    if (!requests[r].capture) continue;
+    capture_at[r] = capture_total;
+    capture_total += row_capacity(requests[r]) * job_row_bytes(requests[r].kind);
+  }
+  uint8_t* capture_pool = nullptr;
+  if (capture_total) RTDL_HIP(hipHostMalloc((void**)&capture_pool, capture_total, hipHostMallocDefault));
+  struct PinnedPool {
+    uint8_t* p;
+    ~PinnedPool() {
+      if (p) (void)hipHostFree(p);
+    }
+  } pool_guard{capture_pool};
+
   std::vector<Slot> slots(n_slots);
+  struct SlotGuard {
+    std::vector<Slot>& s;
+    ~SlotGuard() {
+      for (Slot& x : s) {
+        if (x.stream) (void)hipStreamSynchronize(x.stream);
+        x.engine.reset();
+        if (x.done) (void)hipEventDestroy(x.done);
+        if (x.stream) (void)hipStreamDestroy(x.stream);
+      }
+    }
+  } slot_guard{slots};
   for (Slot& s : slots) {
     RTDL_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     RTDL_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     s.engine = std::make_unique<BatchEngine>(s.stream);
     s.engine->reserve(4096, cfg.max_batch_queries + 65536);
-    s.region.resize(region_bytes);
   }
   RTDL_HIP(hipDeviceSynchronize());
-
-  auto row_need = [&](const RtdlServeRequest& q) -> uint64_t {
-    const uint32_t k = q.kind == kJobKnn ? cfg.k : cfg.k_max;
-    const uint64_t rows = job_row_capacity(q.kind, q.query_count, k, kPipHitCap);
-    return (rows * job_row_bytes(q.kind) + 255) & ~uint64_t(255);
-  };
 
   using clock = std::chrono::steady_clock;
   const auto t0 = clock::now();
@@ -120,7 +174,6 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
   std::vector<HostJob> jobs;
   std::vector<uint64_t> need_of(n_requests);
   for (size_t r = 0; r < n_requests; ++r) need_of[r] = row_need(requests[r]);
-  std::vector<uint64_t> slot_bytes(n_slots, 0);
 
   while (completed < n_requests) {
     bool progress = false;
@@ -128,8 +181,7 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
     while (next < n_requests && (!cfg.realtime || requests[next].arrival_s <= t)) pending.push_back(next++);
 
     // completions
-    for (size_t si = 0; si < n_slots; ++si) {
-      Slot& s = slots[si];
+    for (Slot& s : slots) {
       if (!s.busy) continue;
       const hipError_t q = hipEventQuery(s.done);
       if (q == hipErrorNotReady) continue;
@@ -143,78 +195,62 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
         res.status |= c[j].status;
         total_rows += c[j].rows;
       }
-      for (size_t ci = 0; ci < s.captures.size(); ++ci) {
-        const size_t r = s.captures[ci].first;
-        // an overflowing request reserved more rows than its buffer holds;
-        // only the buffer was copied
-        const size_t bytes = std::min<uint64_t>(results[r].rows, s.capture_rows[ci]) *
-                             job_row_bytes(requests[r].kind);
-        void* out = std::malloc(bytes ? bytes : 1);
-        require(out != nullptr, "host allocation for captured rows failed");
-        std::memcpy(out, s.captures[ci].second, bytes);
-        RTDL_HIP(hipHostFree(s.captures[ci].second));
-        results[r].captured_rows = out;
-      }
       completed += s.requests.size();
-      inflight_bytes -= slot_bytes[si];
-      slot_bytes[si] = 0;
+      inflight_bytes -= s.range_bytes;
+      s.range_bytes = 0;
       s.requests.clear();
-      s.captures.clear();
-      s.capture_rows.clear();
       s.busy = false;
       progress = true;
     }
 
     // dispatch
-    for (size_t si = 0; si < n_slots && !pending.empty(); ++si) {
-      Slot& s = slots[si];
+    for (Slot& s : slots) {
+      if (pending.empty()) break;
       if (s.busy) continue;
+      uint64_t gap_at = 0, gap = 0;
+      largest_gap(slots, arena_bytes, &gap_at, &gap);
       jobs.clear();
       uint64_t used = 0, queries = 0;
       while (!pending.empty()) {
         const size_t r = pending.front();
         const RtdlServeRequest& q = requests[r];
         const uint64_t need = need_of[r];
-        if (need > region_bytes) {  // can never be admitted
+        if (need > arena_bytes) {  // can never be admitted
           results[r].status |= kStatusRejected;
           results[r].dispatch_s = results[r].done_s = now();
           pending.pop_front();
           ++completed;
           continue;
         }
-        if (!jobs.empty() && (used + need > region_bytes || queries + q.query_count > cfg.max_batch_queries))
-          break;
+        if (used + need > gap || (!jobs.empty() && queries + q.query_count > cfg.max_batch_queries)) break;
         HostJob h;
         h.scene = views[q.scene];
         h.kind = q.kind;
         h.nq = q.query_count;
         h.qstride = stride_of(q.kind);
-        const float* arena = q.kind <= kJobBoundedKnn ? cfg.point_queries
-                             : q.kind == kJobPipPositive ? cfg.pip_queries
-                                                         : cfg.ray_queries;
-        h.queries = arena + q.query_offset * h.qstride;
+        const float* qa = q.kind <= kJobBoundedKnn ? cfg.point_queries
+                          : q.kind == kJobPipPositive ? cfg.pip_queries
+                                                      : cfg.ray_queries;
+        h.queries = qa + q.query_offset * h.qstride;
         h.radius = cfg.radius;
         h.k = q.kind == kJobKnn ? cfg.k : cfg.k_max;
-        h.rows = s.region.get() + used;
-        h.row_capacity = job_row_capacity(q.kind, q.query_count, h.k, kPipHitCap);
+        h.rows = arena.get() + gap_at + used;
+        h.row_capacity = row_capacity(q);
         jobs.push_back(h);
         s.requests.push_back(r);
         used += need;
         queries += q.query_count;
         pending.pop_front();
       }
-      if (jobs.empty()) continue;
+      if (jobs.empty()) break;  // the head of the line waits for memory
       s.engine->enqueue(jobs, cfg.morton_sort != 0);
       for (size_t j = 0; j < jobs.size(); ++j) {
         const size_t r = s.requests[j];
         if (!requests[r].capture) continue;
         const uint64_t bytes = jobs[j].row_capacity * job_row_bytes(requests[r].kind);
-        uint8_t* pinned = nullptr;
-        RTDL_HIP(hipHostMalloc((void**)&pinned, bytes ? bytes : 1, hipHostMallocDefault));
         if (bytes)
-          RTDL_HIP(hipMemcpyAsync(pinned, jobs[j].rows, bytes, hipMemcpyDeviceToHost, s.stream));
-        s.captures.emplace_back(r, pinned);
-        s.capture_rows.push_back(jobs[j].row_capacity);
+          RTDL_HIP(hipMemcpyAsync(capture_pool + capture_at[r], jobs[j].rows, bytes, hipMemcpyDeviceToHost,
+                                  s.stream));
       }
       RTDL_HIP(hipEventRecord(s.done, s.stream));
       const double td = now();
@@ -225,7 +261,8 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
       }
       ++batch_id;
       ++batches;
-      slot_bytes[si] = used;
+      s.range_at = gap_at;
+      s.range_bytes = used;
       inflight_bytes += used;
       peak_bytes = std::max(peak_bytes, inflight_bytes);
       s.busy = true;
@@ -244,11 +281,17 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
     }
   }
   const double wall = now();
-  for (Slot& s : slots) {
-    (void)hipStreamSynchronize(s.stream);
-    s.engine.reset();
-    (void)hipEventDestroy(s.done);
-    (void)hipStreamDestroy(s.stream);
+  for (Slot& s : slots) RTDL_HIP(hipStreamSynchronize(s.stream));
+  for (size_t r = 0; r < n_requests; ++r) {
+    if (!requests[r].capture || (results[r].status & kStatusRejected)) continue;
+    // an overflowing request reserved more rows than its buffer holds; only
+    // the buffer was copied
+    const uint64_t rows = std::min<uint64_t>(results[r].rows, row_capacity(requests[r]));
+    const size_t bytes = rows * job_row_bytes(requests[r].kind);
+    void* out = std::malloc(bytes ? bytes : 1);
+    require(out != nullptr, "host allocation for captured rows failed");
+    if (bytes) std::memcpy(out, capture_pool + capture_at[r], bytes);
+    results[r].captured_rows = out;
   }
   if (stats) {
     stats->wall_s = wall;
@@ -256,7 +299,7 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
     stats->queries = total_q;
     stats->rows = total_rows;
     stats->peak_inflight_bytes = peak_bytes;
-    stats->region_bytes = region_bytes;
+    stats->region_bytes = arena_bytes;
   }
 }
 

@@ -133,6 +133,30 @@ class DeviceView:
     def numel(self) -> int:
         return int(np.prod(self.shape, dtype=np.int64))
 
+    @property
+    def nbytes(self) -> int:
+        return self.numel() * self.dtype.itemsize
+
+    def download(self, nbytes: int | None = None) -> np.ndarray:
+        """Copies the view's first `nbytes` (default all) to a uint8 host array."""
+        n = self.nbytes if nbytes is None else int(nbytes)
+        if n < 0 or n > self.nbytes:
+            raise ValueError(f"download of {n} bytes from a {self.nbytes}-byte view")
+        out = np.empty(n, dtype=np.uint8)
+        if n:
+            err = ctypes.create_string_buffer(1024)
+            _check(rt_native._library().rtdl_mi355x_copy_to_host(out.ctypes.data, self._ptr, n, err, 1024),
+                   err, "rtdl_mi355x_copy_to_host")
+        return out
+
+    def __getitem__(self, rows) -> "DeviceView":
+        if not isinstance(rows, slice) or rows.step not in (None, 1):
+            raise TypeError("device views support contiguous row slices only")
+        start, stop, _ = rows.indices(self.shape[0])
+        stop = max(start, stop)
+        row_bytes = self.nbytes // max(1, self.shape[0])
+        return DeviceView(self.base, self._ptr + start * row_bytes, (stop - start, *self.shape[1:]))
+
 
 def to_device(array, dtype=None) -> DeviceBuffer:
     return DeviceBuffer.from_numpy(array, dtype=dtype)

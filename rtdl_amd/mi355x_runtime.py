@@ -159,6 +159,9 @@ _SIGNATURES = {
     "rtdl_mi355x_prepare_polygons": ([_VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_run_prepared_point_in_polygon": ([_VP, _VP, _SZ, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_destroy_prepared": ([_VP], None),
+    "rtdl_mi355x_set_walk_counting": ([ctypes.c_uint32, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_get_last_walk_stats": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64), *_ERR],
+                                        ctypes.c_int),
     # reference-shaped prepared handles (the HIPRT parameter lists)
     "rtdl_mi355x_prepare_fixed_radius_neighbors_3d": ([_VP, _SZ, ctypes.c_double, ctypes.POINTER(_VP), *_ERR],
                                                       ctypes.c_int),
@@ -248,6 +251,24 @@ def _call_prepare(fn_name: str, *args) -> ctypes.c_void_p:
         detail = error.value.decode("utf-8", errors="replace")
         raise RuntimeError(f"{fn_name} failed with status {status}: {detail}")
     return handle
+
+
+def set_walk_counting(on: bool) -> None:
+    """Count node / primitive visits in this thread's per-call traversals."""
+    err = ctypes.create_string_buffer(512)
+    if _library().rtdl_mi355x_set_walk_counting(int(bool(on)), err, 512) != 0:
+        raise RuntimeError(err.value.decode())
+
+
+def last_walk_stats() -> dict:
+    """Traversal kernel seconds of this thread's last per-call traversal and,
+    when counting was on, its node and primitive visits."""
+    t = ctypes.c_double()
+    v = (ctypes.c_uint64 * 2)()
+    err = ctypes.create_string_buffer(512)
+    if _library().rtdl_mi355x_get_last_walk_stats(ctypes.byref(t), v, err, 512) != 0:
+        raise RuntimeError(err.value.decode())
+    return {"kernel_s": t.value, "node_visits": int(v[0]), "prim_tests": int(v[1])}
 
 
 def last_phase_timings() -> dict[str, float]:

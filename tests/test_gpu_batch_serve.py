@@ -559,3 +559,36 @@ def test_autotune_picks_the_fastest_and_every_candidate_gives_the_same_rows(gpu_
     finally:
         T.set_tuning(original)
     assert T.get_tuning() == original
+
+
+def test_serve_admission_waits_for_memory_under_a_small_budget(gpu_ready):
+    """The device budget binds: 40 fixed-radius requests of 1,000-1,600
+    queries (1.0-1.6 MB of rows each) arrive at once against a 4 MB budget
+    and 4 slots.  Rows in flight never exceed the budget, requests wait for
+    memory instead of being rejected, and every request's rows equal the
+    oracle's."""
+    _say("case: serve admission bound by a 4 MB budget")
+    rng = np.random.default_rng(82)
+    sdict, s3 = _points3(rng, 30000)
+    scene = _prepared_points(s3)
+    sizes = rng.integers(1000, 1601, 40)
+    pq = rng.random((int(sizes.sum()), 3)).astype(np.float32)
+    reqs = np.zeros(40, dtype=B.SERVE_REQUEST_DTYPE)
+    off = 0
+    for i in range(40):
+        reqs[i] = (B.JOB_FIXED_RADIUS, 0, off, sizes[i], 1, 0.0)
+        off += sizes[i]
+    budget = 4 << 20
+    # k_max 64: 64 rows x 16 B = 1 KB per query
+    res, stats, captured = B.serve_replay([scene], reqs, radius=0.03, k_max=64, k=1, point_queries=_dev(pq),
+                                          slots=4, memory_budget_bytes=budget)
+    assert np.all(res["status"] == 0)
+    assert stats["region_bytes"] <= budget
+    assert stats["peak_inflight_bytes"] <= budget
+    assert stats["peak_inflight_bytes"] > budget // 2  # the budget was used, not a smaller cap
+    assert stats["batches"] >= 40 * 1.0e6 // budget    # admission waited for completions
+    for i in range(40):
+        o, n = int(reqs["query_offset"][i]), int(reqs["query_count"][i])
+        q = pq[o:o + n].astype(np.float64)
+        qd = {"id": np.arange(n, dtype=np.uint32), "x": q[:, 0], "y": q[:, 1], "z": q[:, 2]}
+        _check_nn(captured[i], oracle.neighbors("frn", qd, sdict, radius=0.03, k=64), False)

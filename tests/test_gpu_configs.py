@@ -7,17 +7,20 @@
 - Config [2]: KNN k = 16, 4,000,000 x 4,000,000 3-D Gaussian-mixture points at
   full size; 20,000 sampled queries checked against the C oracle over all
   search points.
-- Config [3]: the batched fixed-radius + PIP mix with 256 instances (the
-  configuration has 2,048 of up to 2.6e5 points; the instance count and the
-  size range are scaled down so the oracle finishes in seconds), 2-D and 3-D
-  instances, radius log-uniform in [0.005, 0.05], every 4th instance PIP
-  against 1,000 star polygons; every job checked.
+- Config [3]: the batched fixed-radius + PIP mix at full size (2,048
+  instances of 1e3-2.6e5 points, 2-D and 3-D, radius log-uniform in
+  [0.005, 0.05], every 4th instance PIP against 1,000 star polygons),
+  autotuned and replayed from a hipGraph; sampled queries of every job
+  checked against the C oracle over the whole instance.
+- Config [4]: the served stream at its scene scale (16 resident scenes of
+  1e5-1e7 primitives) and request sizes (1-65,536 queries): half a second of
+  Poisson arrivals at 20,000 requests/s; sampled queries of every 25th
+  request checked against the C oracle over the whole scene, and a few
+  captured requests replayed alone with the same rows as in the mix.
 
 Every comparison is bit-exact: ids, flags and FP64 distances.
 """
 from __future__ import annotations
-
-import math
 
 import numpy as np
 import pytest
@@ -26,15 +29,21 @@ from oracle import cpu_python_reference as ref
 from oracle import native as oracle
 from rtdl_amd import batch as B
 from rtdl_amd import mi355x_runtime as m
-from rtdl_amd.device import DeviceBuffer
 from rtdl_amd.records import Point
 from tests import cases
 
 pytestmark = pytest.mark.gpu
 
 
+_T0 = [0.0]
+
+
 def _say(msg: str) -> None:
-    print(msg, flush=True)
+    import time
+
+    if msg.startswith("case:"):
+        _T0[0] = time.perf_counter()
+    print(f"[{time.perf_counter() - _T0[0]:7.1f}s] {msg}", flush=True)
 
 
 def test_config0_fixed_radius_matches_the_python_reference_at_full_size(gpu_ready):
@@ -85,70 +94,95 @@ def test_config2_knn16_full_size_sampled_against_the_oracle(gpu_ready):
     assert np.array_equal(rows["neighbor_rank"], want["rank"])
 
 
-def test_config3_batched_fixed_radius_and_pip_instances(gpu_ready):
-    _say("case: config [3] batched fixed radius + PIP, 256 instances")
+def test_config3_batched_fixed_radius_and_pip_at_full_size(gpu_ready):
+    """Config [3] at its stated workload: 2,048 instances of 1e3-2.6e5 points
+    (2-D / 3-D 50/50, r log-uniform in [0.005, 0.05], k_max 64), every 4th a
+    PIP instance against 1,000 star polygons, in one batch.  The batch is
+    autotuned (launch size and point leaf size; every candidate's rows
+    compared on 12 sampled jobs), then recorded into a hipGraph and replayed; every
+    replay's row counts and sampled rows must equal the eager run's, and 8
+    sampled queries of every job are checked against the C oracle over the
+    whole instance."""
+    import bench
+    from rtdl_amd import autotune as AT
+
+    _say("case: config [3] 2,048 instances at full size")
     rng = np.random.default_rng(20261206)
-    n_inst = 256
-    sizes = np.round(np.exp(rng.uniform(math.log(1e3), math.log(2.6e4), n_inst))).astype(np.int64)
-    dims = rng.integers(2, 4, n_inst)
-    radii = np.exp(rng.uniform(math.log(0.005), math.log(0.05), n_inst))
-    frn = [i for i in range(n_inst) if i % 4 != 3]
-    pip = [i for i in range(n_inst) if i % 4 == 3]
-    # point instances: build and query draws of the same size, z = 0 for 2-D
-    coords, offs, inst = [], [0], {}
-    for i in frn:
-        p = rng.random((sizes[i], 3)).astype(np.float32)
-        if dims[i] == 2:
-            p[:, 2] = 0
-        inst[i] = p
-        coords.append(p)
-        offs.append(offs[-1] + len(p))
-    pscene = B.prepare_point_instances(DeviceBuffer.from_numpy(np.concatenate(coords)), offs)
-    polys = {i: cases.star_polygons(rng, 1000) for i in pip}
-    voff, vcnt, verts, poffs, base = [], [], [], [0], 0
-    for i in pip:
-        refs = polys[i]["refs"]
-        voff.append(refs["vertex_offset"] + base)
-        vcnt.append(refs["vertex_count"])
-        verts.append(polys[i]["vertices_xy"].reshape(-1, 2))
-        base += len(verts[-1])
-        poffs.append(poffs[-1] + len(refs))
-    gscene = B.prepare_polygon_instances(DeviceBuffer.from_numpy(np.concatenate(voff), dtype=np.int32),
-                                         DeviceBuffer.from_numpy(np.concatenate(vcnt), dtype=np.int32),
-                                         DeviceBuffer.from_numpy(np.concatenate(verts), dtype=np.float32),
-                                         poffs)
-    jobs, wants = [], []
-    for slot, i in enumerate(frn):
-        d = int(dims[i])
-        q = rng.random((sizes[i], 3)).astype(np.float32)
-        jobs.append(B.BatchJob(pscene, B.JOB_FIXED_RADIUS, DeviceBuffer.from_numpy(q[:, :d]),
-                               instance=slot, radius=float(radii[i]), k=64))
-        keys = "xyz"[:d]
-        qd = {"id": np.arange(len(q), dtype=np.uint32), **{c: q[:, j].astype(np.float64) for j, c in enumerate(keys)}}
-        sd = {"id": np.arange(len(inst[i]), dtype=np.uint32),
-              **{c: inst[i][:, j].astype(np.float64) for j, c in enumerate(keys)}}
-        wants.append(("frn", oracle.neighbors("frn", qd, sd, radius=float(radii[i]), k=64)))
-    for slot, i in enumerate(pip):
-        pts = cases.points(rng, int(sizes[i]), 2)
-        jobs.append(B.BatchJob(gscene, B.JOB_PIP_POSITIVE,
-                               DeviceBuffer.from_numpy(np.stack([pts["x"], pts["y"]], 1), dtype=np.float32),
-                               instance=slot))
-        refs = polys[i]["refs"]
-        wants.append(("pip", oracle.pip_pairs(pts["x"], pts["y"], refs["vertex_offset"], refs["vertex_count"],
-                                             polys[i]["vertices_xy"], positive_only=True)))
-    _say(f"  run_batch: {len(jobs)} jobs, {int(sizes.sum())} queries")
-    results, _ = B.run_batch(jobs)
-    for job, res, (kind, want) in zip(jobs, results, wants):
-        assert res["status"] == 0
-        rows = B.job_rows(job, res)
-        if kind == "frn":
-            order = np.lexsort((rows["neighbor_id"], rows["query_id"]))
-            rows = rows[order]
-            w = want[np.lexsort((want["nid"], want["qid"]))]
-            assert np.array_equal(rows["query_id"], w["qid"])
-            assert np.array_equal(rows["neighbor_id"], w["nid"])
-            assert np.array_equal(rows["distance"], w["dist"])
-        else:
-            order = np.lexsort((rows["polygon_id"], rows["point_id"]))
-            assert np.array_equal(rows["point_id"][order], want[:, 0])
-            assert np.array_equal(rows["polygon_id"][order], want[:, 1])
+    ws = bench.config3_workload(rng)
+    c3 = bench.Config3(B, ws)
+    n_jobs = len(ws["sizes"])
+    _say(f"  {int(ws['point_offsets'][-1])} fixed-radius + {int(ws['probe_offsets'][-1])} PIP queries")
+    original = AT.get_tuning()
+    try:
+        check = sorted(rng.choice(n_jobs, 12, replace=False).tolist())
+        _say("  autotune_batch")
+        tuned = AT.autotune_batch(c3.build_scenes, c3.jobs, repeats=1, check_jobs=check,
+                                  knobs={"blocks_per_cu": (4, 8), "point_leaf": (2, 4)})
+        assert len(tuned["table"]) >= 3
+        jobs = c3.jobs()
+        _say("  eager run_batch")
+        eager, _ = B.run_batch(jobs)
+        assert all(r["status"] == 0 for r in eager)
+        sample_rows = {i: bench._rows_by_query(B.job_rows(jobs[i], eager[i]),
+                                               "query_id" if i < len(ws["frn_ids"]) else "point_id",
+                                               np.arange(0))[0] for i in check[:4]}
+        with B.BatchGraph(jobs) as graph:
+            for rep in range(3):
+                _say(f"  graph replay {rep}")
+                res, _ = graph.launch()
+                assert [r["rows"] for r in res] == [r["rows"] for r in eager]
+                assert all(r["status"] == 0 for r in res)
+                for i, want in sample_rows.items():
+                    key = "query_id" if i < len(ws["frn_ids"]) else "point_id"
+                    got = bench._rows_by_query(B.job_rows(jobs[i], res[i]), key, np.arange(0))[0]
+                    for name in want.dtype.names:
+                        assert np.array_equal(got[name], want[name]), (i, name)
+        _say("  oracle check, 8 queries of every job")
+        rep = bench.config3_check(B, ws, jobs, eager, rng, jobs_checked=n_jobs, per_job=8)
+        assert rep["jobs_checked"] == n_jobs
+        assert rep["mismatched_queries"] == 0, rep
+        assert rep["rows_checked"] > 8 * n_jobs
+    finally:
+        AT.set_tuning(original)
+
+
+def test_config4_served_stream_at_scene_scale(gpu_ready):
+    import bench
+
+    _say("case: config [4] served stream, 16 scenes of 1e5-1e7 primitives, 0.5 s of requests")
+    rng = np.random.default_rng(20261207)
+    scenes, centres = bench.make_scenes(B, rng, 1e7)
+    assert max(s["n"] for s in scenes) == 10_000_000 and min(s["n"] for s in scenes) == 100_000
+    handles = [s["handle"] for s in scenes]
+    pools_host, pools_dev = bench.make_pools(rng, centres)
+    stream = bench.make_stream(rng, 0.5, 20000.0, scenes)
+    assert stream["size"].max() > 30000  # requests up to the 65,536-query range
+    offsets = bench.assign_windows(rng, stream, scenes)
+    reqs = bench.requests_array(B, stream, offsets, {s["index"]: s["index"] for s in scenes}, capture_every=25)
+    kw = dict(radius=bench.RADIUS, k_max=bench.K_MAX, k=bench.K_KNN, slots=4,
+              point_queries=pools_dev["point"], ray_queries=pools_dev["ray"], pip_queries=pools_dev["pip"])
+    _say(f"  serve_replay: {len(reqs)} requests, {int(stream['size'].sum())} queries")
+    results, stats, captured = B.serve_replay(handles, reqs, **kw)
+    assert np.all(results["status"] == 0)
+    assert stats["queries"] == int(stream["size"].sum())
+    assert len(captured) == len(range(0, len(reqs), 25))
+    check_rng = np.random.default_rng(7)
+    checks = [bench.sample_capture(check_rng, stream, offsets, i, rows) for i, rows in captured.items()]
+    _say(f"  oracle check of {len(checks)} captured requests")
+    rep = bench.verify_captures(checks, scenes, pools_host)
+    assert rep["mismatched_queries"] == 0, rep
+    assert rep["requests_checked"] == len(checks) and rep["served_rows_checked"] > 0
+    kinds_seen = {int(stream["kind"][i]) for i in captured}
+    assert kinds_seen == {0, 1, 2, 3}
+    for kind in range(4):
+        i = next(i for i in sorted(captured) if int(stream["kind"][i]) == kind)
+        alone = reqs[i:i + 1].copy()
+        alone["arrival_s"] = 0.0
+        _say(f"  request {i} (kind {kind}, {int(alone['query_count'][0])} queries) alone")
+        res1, _, cap1 = B.serve_replay(handles, alone, **kw)
+        assert res1["rows"][0] == results["rows"][i]
+        field = bench._QID_FIELD[kind]
+        a = bench._rows_by_query(captured[i], field, np.arange(0))[0]
+        b = bench._rows_by_query(cap1[0], field, np.arange(0))[0]
+        for name in a.dtype.names:
+            assert np.array_equal(a[name], b[name]), (kind, name)
