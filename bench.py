@@ -11,12 +11,13 @@ scenes are star polygons (8-32 vertices) whose radii U[0.005, 0.02] are
 scaled by sqrt(1000 / N) so that a point lies in about as many polygons as
 with 1,000 polygons.  Admission is bounded by a 200 GB device-memory budget.
 
-The 60 s stream is replayed in `--steps` consecutive chunks of 60 / steps
-seconds (one step = one chunk), after `--warmup` untimed chunks.  Query sets
-are windows of seeded per-distribution query pools resident in HBM, so the
-whole stream runs with bounded host and device memory.  `value` is the number
-of queries the served backend completed divided by the summed wall time of
-the chunks' replays (inputs resident in HBM before each clock starts).
+The 60 s stream is replayed in one continuous serving loop and timed as
+`--steps` steps of 60 / steps seconds, after `--warmup` untimed steps of
+stream.  Query sets are windows of seeded per-distribution query pools
+resident in HBM, so the whole stream runs with bounded host and device
+memory.  `value` is the number of queries the served backend completed
+divided by the wall time of the replay (inputs resident in HBM before the
+clock starts).
 Beside it: p50/p99 latency (arrival to rows ready in device memory), the
 rows of every 1,000th request checked against the C FP64 oracle on a sample
 of their queries after the timed region, the same requests replayed
@@ -591,43 +592,31 @@ def python_subsample_leg():
 # ------------------------------------------------------ served stream --
 
 
-def replay_chunks(B, handles, scenes, pools_dev, rng_stream, rng_win, chunks: int, chunk_s: float, rate: float,
-                  serve_kw: dict, capture_every: int = 0, rng_check=None, keep_streams: bool = False):
-    """Replays `chunks` consecutive chunks of the request stream (each from its
-    own t = 0, each chunk's query windows drawn from the pools) and returns
-    the totals, the latencies, the sampled captured rows and the streams."""
-    out = {"served_q": 0, "wall_s": 0.0, "batches": 0, "rows": 0, "rejected": 0, "requests": 0,
-           "queries": 0, "peak_inflight_bytes": 0, "region_bytes": 0}
-    lat, lat_kind, checks, streams = [], [], [], []
-    for c in range(chunks):
-        st = make_stream(rng_stream, chunk_s, rate, scenes)
-        offs = assign_windows(rng_win, st, scenes)
-        reqs = requests_array(B, st, offs, {s["index"]: s["index"] for s in scenes}, capture_every)
-        results, stats, captured = B.serve_replay(handles, reqs, point_queries=pools_dev["point"],
-                                                  ray_queries=pools_dev["ray"], pip_queries=pools_dev["pip"],
-                                                  **serve_kw)
-        ok = results["status"] == 0
-        out["served_q"] += int(st["size"][ok].sum())
-        out["wall_s"] += stats["wall_s"]
-        out["batches"] += stats["batches"]
-        out["rows"] += stats["rows"]
-        out["rejected"] += int((~ok).sum())
-        out["requests"] += len(reqs)
-        out["queries"] += int(st["size"].sum())
-        out["peak_inflight_bytes"] = max(out["peak_inflight_bytes"], stats["peak_inflight_bytes"])
-        out["region_bytes"] = stats["region_bytes"]
-        lat.append(((results["done_s"] - results["arrival_s"]) * 1e3)[ok])
-        lat_kind.append(st["kind"][ok])
-        if rng_check is not None:
-            for i, rows in captured.items():
-                checks.append(sample_capture(rng_check, st, offs, i, rows))
-        if keep_streams:
-            streams.append((st, offs))
-        log(f"chunk {c + 1}/{chunks}: {len(reqs)} requests, {int(st['size'].sum())} queries, "
-            f"wall {stats['wall_s']:.3f}s, {stats['batches']} batches, rejected {int((~ok).sum())}")
-    out["latency_ms"] = np.concatenate(lat) if lat else np.zeros(0)
-    out["latency_kind"] = np.concatenate(lat_kind) if lat_kind else np.zeros(0, np.int64)
-    return out, checks, streams
+def replay_stream(B, handles, scenes, pools_dev, rng_stream, rng_win, seconds: float, rate: float,
+                  serve_kw: dict, capture_every: int = 0, rng_check=None):
+    """Replays `seconds` of the request stream in one continuous serving loop
+    (query windows drawn from the resident pools) and returns the totals, the
+    latencies, the sampled captured rows and the stream itself."""
+    st = make_stream(rng_stream, seconds, rate, scenes)
+    offs = assign_windows(rng_win, st, scenes)
+    reqs = requests_array(B, st, offs, {s["index"]: s["index"] for s in scenes}, capture_every)
+    log(f"replay: {len(reqs)} requests, {int(st['size'].sum())} queries over {seconds:.1f} s")
+    results, stats, captured = B.serve_replay(handles, reqs, point_queries=pools_dev["point"],
+                                              ray_queries=pools_dev["ray"], pip_queries=pools_dev["pip"],
+                                              **serve_kw)
+    ok = results["status"] == 0
+    out = {"served_q": int(st["size"][ok].sum()), "wall_s": stats["wall_s"], "batches": stats["batches"],
+           "rows": stats["rows"], "rejected": int((~ok).sum()), "requests": len(reqs),
+           "queries": int(st["size"].sum()), "peak_inflight_bytes": stats["peak_inflight_bytes"],
+           "region_bytes": stats["region_bytes"],
+           "latency_ms": ((results["done_s"] - results["arrival_s"]) * 1e3)[ok], "latency_kind": st["kind"][ok],
+           "arrival_s": st["arrival"][ok]}
+    checks = []
+    if rng_check is not None:
+        for i, rows in captured.items():
+            checks.append(sample_capture(rng_check, st, offs, i, rows))
+    log(f"replay done: wall {stats['wall_s']:.3f}s, {stats['batches']} batches, rejected {out['rejected']}")
+    return out, checks, (st, offs, reqs)
 
 
 _QID_FIELD = {0: "query_id", 1: "query_id", 2: "ray_id", 3: "point_id"}
@@ -1048,15 +1037,15 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20,
-                    help="chunks the 60 s request stream is replayed in (one step = 60 / steps seconds)")
-    ap.add_argument("--warmup", type=int, default=2, help="untimed chunks of the same length before them")
+                    help="steps the 60 s request stream is timed as (one step = 60 / steps seconds of it)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps of stream replayed before it")
     ap.add_argument("--rate", type=float, default=20000.0)
     ap.add_argument("--max-prims", type=float, default=1e7)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-batch-queries", type=int, default=4 << 20)
     ap.add_argument("--saturated-steps", type=int, default=4,
-                    help="timed chunks replayed again with every request arrived at once (capacity)")
+                    help="timed steps whose requests are replayed again, all arrived at once (capacity)")
     ap.add_argument("--percall-seconds", type=float, default=8.0)
     ap.add_argument("--resident-seconds", type=float, default=4.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -1124,17 +1113,15 @@ def main() -> int:
                           **roof[args.roofline_only]}), flush=True)
         return 0
 
-    chunk_s = STREAM_SECONDS / args.steps
-    rng_stream = np.random.default_rng(SEED + 2)
-    rng_win = np.random.default_rng(SEED + 3)
+    step_s = STREAM_SECONDS / args.steps
     if args.warmup:
-        log(f"warmup: {args.warmup} chunks of {chunk_s:.2f} s")
-        replay_chunks(B, handles, scenes, pools_dev, np.random.default_rng(SEED + 5),
-                      np.random.default_rng(SEED + 6), args.warmup, chunk_s, args.rate, serve_kw)
-    log(f"timed: {args.steps} chunks of {chunk_s:.2f} s ({STREAM_SECONDS:.0f} s of stream)")
-    served, checks, streams = replay_chunks(B, handles, scenes, pools_dev, rng_stream, rng_win, args.steps,
-                                            chunk_s, args.rate, serve_kw, CAPTURE_EVERY,
-                                            np.random.default_rng(SEED + 4), keep_streams=True)
+        log(f"warmup: {args.warmup} steps ({args.warmup * step_s:.1f} s of stream)")
+        replay_stream(B, handles, scenes, pools_dev, np.random.default_rng(SEED + 5),
+                      np.random.default_rng(SEED + 6), args.warmup * step_s, args.rate, serve_kw)
+    log(f"timed: {args.steps} steps of {step_s:.2f} s ({STREAM_SECONDS:.0f} s of stream, one serving loop)")
+    served, checks, (stream, offsets, reqs) = replay_stream(
+        B, handles, scenes, pools_dev, np.random.default_rng(SEED + 2), np.random.default_rng(SEED + 3),
+        STREAM_SECONDS, args.rate, serve_kw, CAPTURE_EVERY, np.random.default_rng(SEED + 4))
     value = served["served_q"] / served["wall_s"]
     lat = served["latency_ms"]
     log(f"served: {served['served_q']} queries in {served['wall_s']:.3f}s ({value/1e6:.2f} Mq/s), "
@@ -1146,31 +1133,31 @@ def main() -> int:
         if sel.any():
             per_kind[KINDS[k]] = {"requests": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
                                   "p99_ms": float(np.percentile(lat[sel], 99))}
+    per_step_p99 = [float(np.percentile(lat[(served["arrival_s"] >= i * step_s) & (served["arrival_s"] < (i + 1) * step_s)], 99))
+                    for i in range(args.steps)]
     t0 = time.perf_counter()
     verified = verify_captures(checks, scenes, pools_host)
     verified["seconds"] = time.perf_counter() - t0
     log(f"served rows checked against the oracle: {verified}")
 
-    # capacity: the first chunks' requests again, all arrived at time 0
-    sat_q, sat_wall = 0, 0.0
-    for st, offs in streams[:args.saturated_steps]:
-        reqs = requests_array(B, st, offs, {sc["index"]: sc["index"] for sc in scenes})
-        reqs["arrival_s"] = 0.0
-        res, sst, _ = B.serve_replay(handles, reqs, point_queries=pools_dev["point"], ray_queries=pools_dev["ray"],
-                                     pip_queries=pools_dev["pip"], **serve_kw)
-        sat_q += int(st["size"][res["status"] == 0].sum())
-        sat_wall += sst["wall_s"]
-    capacity = sat_q / sat_wall if sat_wall else 0.0
-    log(f"saturated replay of {min(args.saturated_steps, len(streams))} chunks: {capacity/1e6:.2f} Mq/s")
+    # capacity: the requests of the first steps again, all arrived at time 0
+    n_sat = int(np.searchsorted(stream["arrival"], args.saturated_steps * step_s))
+    sat = reqs[:n_sat].copy()
+    sat["capture"] = 0
+    sat["arrival_s"] = 0.0
+    res, sst, _ = B.serve_replay(handles, sat, point_queries=pools_dev["point"], ray_queries=pools_dev["ray"],
+                                 pip_queries=pools_dev["pip"], **serve_kw)
+    capacity = int(stream["size"][:n_sat][res["status"] == 0].sum()) / sst["wall_s"]
+    log(f"saturated replay of {args.saturated_steps} steps' requests: {capacity/1e6:.2f} Mq/s")
 
     roof = roofline_leg(B, scenes, roof_stream, roof_offsets, pools_dev, args.roofline_queries)
     measured = {k: v for k, v in roof.items() if "kernel_s" in v}
     # the kernel family that takes the largest share of the stream's device time
-    stream_q = {k: sum(int(st["size"][st["kind"] == KINDS.index(k)].sum()) for st, _ in streams) for k in measured}
+    stream_q = {k: int(stream["size"][stream["kind"] == KINDS.index(k)].sum()) for k in measured}
     dominant = max(measured, key=lambda k: measured[k]["kernel_s"] / max(1, measured[k]["queries"]) * stream_q[k])
     d = measured[dominant]
 
-    first_stream, first_offsets = streams[0]
+    first_stream, first_offsets = stream, offsets
     baselines = {}
     cpu = None
     if not args.skip_baselines:
@@ -1213,7 +1200,7 @@ def main() -> int:
                         "[1, 65536], 16 resident scenes of 1e5-1e7 FP32 primitives (uniform / 64-Gaussian "
                         "mixture), 200 GB admission budget",
             "stream_seconds": STREAM_SECONDS,
-            "chunk_seconds": chunk_s,
+            "step_seconds": step_s,
             "requests": served["requests"],
             "queries": served["queries"],
             "slots": args.slots,
@@ -1226,7 +1213,7 @@ def main() -> int:
         "rows_per_s": served["rows"] / served["wall_s"],
         "served": {"wall_s": served["wall_s"], "batches": served["batches"], "rows": served["rows"],
                    "rejected": served["rejected"], "peak_inflight_bytes": served["peak_inflight_bytes"],
-                   "per_kind": per_kind},
+                   "per_kind": per_kind, "p99_ms_per_step": per_step_p99},
         "served_rows_checked": verified["served_rows_checked"],
         "mismatches": verified["mismatched_queries"],
         "served_check": verified,

@@ -899,16 +899,11 @@ constexpr int kQueueNbrEmit = kLaunchFamilies;
 constexpr int kQueueNbrSort = kLaunchFamilies + 1;
 // parking space for fixed-radius candidates, per fixed-radius query of a batch
 constexpr uint64_t kNbrCandPerQuery = 80;
-// Fixed-radius scratch of one batch: the stage (stage_slots x 4 B per query)
-// and the parking space (kNbrCandPerQuery x 16 B per query) shrink, in that
-// order of preference, for batches whose full-size scratch would exceed these
-// budgets.  A smaller stage sends more queries through the emit walk; a
-// smaller parking space sends more queries to the exact fix-up.  Rows are the
-// same either way.
-constexpr uint64_t kNbrStageBudget = uint64_t(24) << 30;
-constexpr uint64_t kNbrParkBudget = uint64_t(16) << 30;
-constexpr uint32_t kNbrStageMin = 32;
-constexpr uint64_t kNbrCandPerQueryMin = 16;
+// A fixed-radius family of more work items than this runs as consecutive
+// sub-launches (count, emit, sort) of at most this many items, in stream
+// order, reusing one stage (1 KiB per query) and one parking space (1.25 KiB
+// per query): the scratch is bounded by 4M queries however large the batch.
+constexpr uint32_t kNbrChunkItems = 65536;
 
 int family_of(uint32_t kind, uint32_t k) {
   switch (kind) {
@@ -1099,21 +1094,15 @@ void BatchEngine::reserve_nbr(const HostJob* jobs, size_t n) {
       nbr_items += (jobs[i].nq + 63) / 64;
     }
   if (!nbr_queries) return;
-  if (fixed_) {  // carved for max_queries / max_items at the full stage, which reserve() checked
-    stage_slots_ = kNbrStageMax;
-    require(nbr_cand_.size() >= nbr_queries * kNbrCandPerQuery &&
-                nbr_stage_.size() >= (size_t)nbr_items * 64 * kNbrStageMax,
-            "fixed engine: fixed-radius space");
+  // one sub-launch's worth (see kNbrChunkItems)
+  nbr_items = std::min<uint64_t>(nbr_items, kNbrChunkItems);
+  nbr_queries = std::min<uint64_t>(nbr_queries, (uint64_t)kNbrChunkItems * 64);
+  const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
+  const size_t stage = (size_t)nbr_items * 64 * kNbrStageMax;
+  if (fixed_) {  // carved for max_queries / max_items, which reserve() checked
+    require(nbr_cand_.size() >= want && nbr_stage_.size() >= stage, "fixed engine: fixed-radius space");
     return;
   }
-  uint32_t slots = kNbrStageMax;
-  while (slots > kNbrStageMin && nbr_items * 64 * slots * sizeof(uint32_t) > kNbrStageBudget) slots >>= 1;
-  stage_slots_ = slots;
-  uint64_t per_query = kNbrCandPerQuery;
-  while (per_query > kNbrCandPerQueryMin && nbr_queries * per_query * sizeof(NbrCand) > kNbrParkBudget)
-    per_query = std::max<uint64_t>(per_query / 2, kNbrCandPerQueryMin);
-  const size_t want = (size_t)std::max<uint64_t>(nbr_queries * per_query, 1u << 16);
-  const size_t stage = (size_t)nbr_items * 64 * slots;
   if (nbr_cand_.size() < want) nbr_cand_.resize(want);
   if (nbr_stage_.size() < stage) nbr_stage_.resize(stage);
 }
@@ -1264,25 +1253,37 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
         RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamNbr) {
-      require(nbr_stage_.size() >= (size_t)n_items * 64 * stage_slots_, "fixed-radius stage not reserved");
-      if (stats_on_)
-        nbr_count_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
-                                                     nbr_cand_count_.get(), nbr_cand_.size(),
-                                                     nbr_stage_.get(), stage_slots_, amb_.get(),
-                                                     amb_count_.get(), st);
-      else
-        nbr_count_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, nbr_plan_.get(),
-                                                      nbr_cand_count_.get(), nbr_cand_.size(),
-                                                      nbr_stage_.get(), stage_slots_, amb_.get(),
-                                                      amb_count_.get(), st);
-      RTDL_LAUNCH_CHECK();
-      nbr_emit_kernel<<<grid, 256, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrEmit * kXcds,
+      // sub-launches of at most kNbrChunkItems items; the first one's queue
+      // counters and parking cursor were cleared above, later ones clear
+      // theirs (fill kernels: graph-safe) after the previous sort
+      for (uint32_t c0 = 0; c0 < n_items; c0 += kNbrChunkItems) {
+        const uint32_t m = std::min<uint32_t>(kNbrChunkItems, n_items - c0);
+        const WorkItem* ic = it + c0;
+        const unsigned gm = grid_for(m);
+        require(nbr_stage_.size() >= (size_t)m * 64 * kNbrStageMax, "fixed-radius stage not reserved");
+        if (c0) {
+          fill_u32(ctr, kXcds, 0u, s);
+          fill_u32(item_counters_.get() + kQueueNbrEmit * kXcds, 2 * kXcds, 0u, s);
+          fill_u32(reinterpret_cast<uint32_t*>(nbr_cand_count_.get()), 2, 0u, s);
+        }
+        if (stats_on_)
+          nbr_count_kernel<true><<<gm, 256, 0, s>>>(dj, ic, m, ctr, nbr_plan_.get(), nbr_cand_count_.get(),
+                                                    nbr_cand_.size(), nbr_stage_.get(), kNbrStageMax,
+                                                    amb_.get(), amb_count_.get(), st);
+        else
+          nbr_count_kernel<false><<<gm, 256, 0, s>>>(dj, ic, m, ctr, nbr_plan_.get(), nbr_cand_count_.get(),
+                                                     nbr_cand_.size(), nbr_stage_.get(), kNbrStageMax,
+                                                     amb_.get(), amb_count_.get(), st);
+        RTDL_LAUNCH_CHECK();
+        nbr_emit_kernel<<<gm, 256, 0, s>>>(dj, ic, m, item_counters_.get() + kQueueNbrEmit * kXcds,
                                            nbr_plan_.get(), nbr_cand_.get());
-      RTDL_LAUNCH_CHECK();
-      const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(n_items, (size_t)cus_ * 32));
-      nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, it, n_items, item_counters_.get() + kQueueNbrSort * kXcds,
-                                                 nbr_plan_.get(), nbr_cand_.get(), nbr_stage_.get(), stage_slots_);
-      RTDL_LAUNCH_CHECK();
+        RTDL_LAUNCH_CHECK();
+        const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(m, (size_t)cus_ * 32));
+        nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, ic, m, item_counters_.get() + kQueueNbrSort * kXcds,
+                                                   nbr_plan_.get(), nbr_cand_.get(), nbr_stage_.get(),
+                                                   kNbrStageMax);
+        RTDL_LAUNCH_CHECK();
+      }
       if (f == last_nbr) {
         batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
         RTDL_LAUNCH_CHECK();

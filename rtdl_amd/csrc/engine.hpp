@@ -221,8 +221,7 @@ class BatchEngine {
   DeviceArray<uint32_t> amb_count_;
   DeviceArray<NbrPlan> nbr_plan_;        // cap_items_ * 64
   DeviceArray<NbrCand> nbr_cand_;        // parked candidates of a batch
-  DeviceArray<uint32_t> nbr_stage_;      // stage_slots_ slots per fixed-radius query
-  uint32_t stage_slots_ = 256;           // chosen per batch by reserve_nbr
+  DeviceArray<uint32_t> nbr_stage_;      // kNbrStageMax slots per fixed-radius query of a sub-launch
   DeviceArray<unsigned long long> nbr_cand_count_;
   DeviceArray<unsigned long long> stats_;
   SortWorkspace sort_ws_;
