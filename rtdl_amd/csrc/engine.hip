@@ -334,100 +334,52 @@ struct NbrCount {
   bool staged;       // the last round's `count` slots are in the stage
 };
 
-template <bool STATS>
-__device__ inline NbrCount nbr_count_query(const DeviceJob& j, uint32_t qi, uint32_t* my_hist,
-                                           uint32_t* __restrict__ stage, uint32_t stage_slots,
-                                           LaneStats& st) {
-  const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
-  auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
-  double qx, qy, qz;
-  load_query3(j, qi, qx, qy, qz);
-  const float qf[3] = {(float)qx, (float)qy, (float)qz};
-  const float m = coord_bound(scene_m(sc), qx, qy, qz);
-  const uint32_t k = j.k;
-  NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false, false};  // reference.py:357
-  const uint32_t step = k / 2 > 4u ? k / 2 : 4u;
-  for (uint32_t round = 0;; ++round) {
-    for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
-    const double scale = (double)kNbrBuckets / r.radius;
-    uint32_t count = 0, top = kNbrBuckets - 1, next_check = k;
-    uint4 sbuf = make_uint4(0u, 0u, 0u, 0u);  // staged slots go out four at a time
-    bool shrunk = false;
-    float thr2 = frn_threshold2(r.radius, m);
-    auto visit = [&](uint32_t slot, const float4& p, float) {
-      const double d2 = point_d2(sc, slot, p, qx, qy, qz);
-      if (d2 > r.radius_sq) return;  // reference.py:361 inclusive radius
-      const uint32_t b = nbr_bucket(sqrt(d2), scale);
-      if (b > top) return;  // beyond the k nearest
-      ++count;
-      // no result needed: a plain LDS add
-      __hip_atomic_fetch_add(&my_hist[256 * (b >> 1)], 1u << (16 * (b & 1)), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (count <= stage_slots) {
-        const uint32_t e = (count - 1) & 3u;
-        sbuf.x = e == 0 ? slot : sbuf.x;
-        sbuf.y = e == 1 ? slot : sbuf.y;
-        sbuf.z = e == 2 ? slot : sbuf.z;
-        sbuf.w = e == 3 ? slot : sbuf.w;
-        if (e == 3) reinterpret_cast<uint4*>(stage)[(count - 1) >> 2] = sbuf;
-      }
-      if (count < next_check) return;
-      next_check = count + step;
-      uint32_t cum = 0;
-      for (uint32_t c = 0; c < top; ++c) {
-        cum += bucket_count(c);
-        if (cum >= k) {  // the k nearest lie in buckets 0..c: walk no farther
-          top = c;
-          shrunk = true;
-          thr2 = frn_threshold2(fmin(r.radius, nbr_bucket_edge(c, scale)), m);
-          break;
-        }
-      }
-    };
-    walk_points<STATS>(sc, sc.root, sc.end, qf, thr2, st, visit);
-    // the last, partial group of four (entries past `count` are never read)
-    if ((count & 3u) && count <= stage_slots) reinterpret_cast<uint4*>(stage)[(count - 1) >> 2] = sbuf;
-    r.count = count;
-    r.heavy = shrunk || count > k;
-    r.bucket = top;
-    r.n_cand = 0;
-    r.staged = count <= stage_slots;
-    if (!r.heavy) return r;
-    if (count > kNbrCountMax) {
-      r.refine = true;
-      return r;
-    }
-    uint32_t cum = 0;
-    for (uint32_t c = 0; c <= top; ++c) {
-      cum += bucket_count(c);
-      if (cum >= k) {
-        r.bucket = c;
-        r.n_cand = cum;
-        break;
-      }
-    }
-    if (r.n_cand <= kNbrCandMax) return r;
-    if (round == 3) {
-      r.refine = true;
-      return r;
-    }
-    // too coarse: count again with kNbrBuckets slices of bucket b's ball,
-    // which holds the k nearest
-    r.radius = fmin(r.radius, nbr_bucket_edge(r.bucket, scale));
-    r.radius_sq = fmin(r.radius_sq, r.radius * r.radius);
-  }
+// The count walk with the FP64 visit taken out of the lanes' walks.  Each
+// lane walks its query's ball (one node step per lane per iteration) and only
+// prefilters leaf points in FP32; the points that pass go into a per-wavefront
+// queue in LDS as (owner lane, slot).  Whenever 64 are queued, or no lane is
+// still walking, the wavefront drains 64 of them at once with every lane
+// active: FP64 squared distance in the reference's order, the inclusive
+// radius test, the distance bucket, the owner's histogram (LDS atomics) and
+// stage (an LDS atomic cursor gives the position).  A divergent walk thus no
+// longer runs the FP64 visit a few lanes at a time.  After each drain every
+// owner whose count reached its checkpoint shrinks its ball to the k-th
+// bucket's edge, as the walk itself did before.  Counting rounds are
+// wavefront-synchronous: lanes that must recount over finer buckets do so
+// together in the next round.  Every decision still rests on the FP64
+// distance of each point, and the set of points counted in buckets up to the
+// final cutoff does not depend on the order they are drained in, so the rows
+// are exact as before.
+// Drains run after each point index of the leaves being prefiltered, so the
+// queue never holds more than 63 + 64 entries.
+constexpr uint32_t kNbrQueueCap = 128;
+
+__device__ inline double shfl_f64(double x, uint32_t src) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __shfl((uint32_t)b, (int)src, 64);
+  const uint32_t hi = __shfl((uint32_t)(b >> 32), (int)src, 64);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+__device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
 template <bool STATS>
-__global__ __launch_bounds__(256) void nbr_count_kernel(
+__global__ __launch_bounds__(256, 6) void nbr_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
     uint32_t* item_counter, NbrPlan* __restrict__ plans, unsigned long long* cand_count,
     uint64_t cand_cap, uint32_t* __restrict__ stage, uint32_t stage_slots, uint4* amb, uint32_t* amb_count,
     unsigned long long* stats) {
   // buckets 2w and 2w + 1 of thread t in the low and high half of hist[w * 256 + t]
   __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
+  __shared__ uint32_t q_slot[4 * kNbrQueueCap];
+  __shared__ uint8_t q_owner[4 * kNbrQueueCap];
+  __shared__ uint32_t l_count[256];  // neighbours counted this round, per thread
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wbase = threadIdx.x & ~63u;
   uint32_t* my_hist = hist + threadIdx.x;
+  uint32_t* qs = q_slot + (threadIdx.x >> 6) * kNbrQueueCap;
+  uint8_t* qo = q_owner + (threadIdx.x >> 6) * kNbrQueueCap;
+  auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -436,14 +388,156 @@ __global__ __launch_bounds__(256) void nbr_count_kernel(
     if (it == kItemRetry) continue;
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
+    const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
     const bool active = lane < w.count;
     const uint32_t k = j.k;
+    const uint32_t step = k / 2 > 4u ? k / 2 : 4u;
     uint32_t qi = 0;
-    NbrCount r{0.0, 0.0, 0u, 0u, 0u, false, false, false};
+    double qx = 0.0, qy = 0.0, qz = 0.0;
     if (active) {
       qi = job_query(j, w.first + lane);
-      r = nbr_count_query<STATS>(j, qi, my_hist, stage + ((size_t)it * 64 + lane) * stage_slots,
-                                 stage_slots, st);
+      load_query3(j, qi, qx, qy, qz);
+    }
+    const float qf[3] = {(float)qx, (float)qy, (float)qz};
+    const float m = coord_bound(scene_m(sc), qx, qy, qz);
+    uint32_t* const stage_it = stage + (size_t)it * 64 * stage_slots;
+    NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false, false};  // reference.py:357
+    bool going = active;  // counts in the current round
+    for (uint32_t round = 0; round < 4 && __ballot(going); ++round) {
+      const double scale = (double)kNbrBuckets / r.radius;
+      float thr2 = frn_threshold2(r.radius, m);
+      uint32_t top = kNbrBuckets - 1, next_check = k;
+      bool shrunk = false;
+      if (going) {
+        for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
+        l_count[threadIdx.x] = 0u;
+      }
+      uint32_t node = sc.root;
+      bool walking = going && sc.root != kEndNode;
+      uint32_t guard = sc.n_nodes + 1;  // a preorder walk visits each node at most once
+      uint32_t head = 0, size = 0;      // the wavefront's queue (uniform)
+      for (;;) {
+        uint32_t first = 0, cnt = 0;
+        if (walking) {
+          const BvhNode nd = load_node(sc.nodes, node);
+          if (STATS) ++st.nodes;
+          const bool in = box_dist2(qf, nd.lo, nd.hi) <= thr2;
+          const bool leaf = node_is_leaf(nd.link);
+          if (in && leaf) {
+            first = leaf_first(nd.link);
+            cnt = leaf_count(nd.link);
+          }
+          node = (in && !leaf) ? nd.link : nd.escape;
+          walking = node != sc.end && --guard != 0;
+        }
+        // One drain: the first min(64, size) queued points, one per lane, then
+        // the owners' checkpoints (shrink to the k-th bucket's edge once k
+        // neighbours are counted).
+        auto drain = [&]() {
+          const uint32_t n = size < 64 ? size : 64;
+          wave_fence();
+          const bool has = lane < n;
+          const uint32_t e = (head + (has ? lane : 0u)) & (kNbrQueueCap - 1);
+          const uint32_t slot = qs[e];
+          const uint32_t o = has ? qo[e] : lane;
+          // the owner's query and ball (every lane takes part in the shuffles)
+          const double ox = shfl_f64(qx, o), oy = shfl_f64(qy, o), oz = shfl_f64(qz, o);
+          const double orsq = shfl_f64(r.radius_sq, o), oscale = shfl_f64(scale, o);
+          const uint32_t otop = __shfl(top, (int)o, 64);
+          if (has) {
+            const float4 p = load_prim(sc.prims, slot);
+            const double d2 = point_d2(sc, slot, p, ox, oy, oz);
+            if (d2 <= orsq) {  // reference.py:361 inclusive radius
+              const uint32_t b = nbr_bucket(sqrt(d2), oscale);
+              if (b <= otop) {  // not beyond the owner's k nearest
+                __hip_atomic_fetch_add(&hist[256 * (b >> 1) + wbase + o], 1u << (16 * (b & 1)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t at = __hip_atomic_fetch_add(&l_count[wbase + o], 1u, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (at < stage_slots) stage_it[(size_t)o * stage_slots + at] = slot;
+              }
+            }
+          }
+          head = (head + n) & (kNbrQueueCap - 1);
+          size -= n;
+          wave_fence();
+          if (going) {
+            const uint32_t count = l_count[threadIdx.x];
+            if (count >= next_check) {
+              next_check = count + step;
+              uint32_t cum = 0;
+              for (uint32_t c = 0; c < top; ++c) {
+                cum += bucket_count(c);
+                if (cum >= k) {  // the k nearest lie in buckets 0..c: walk no farther
+                  top = c;
+                  shrunk = true;
+                  thr2 = frn_threshold2(fmin(r.radius, nbr_bucket_edge(c, scale)), m);
+                  break;
+                }
+              }
+            }
+          }
+        };
+        // FP32 prefilter of the leaf points; survivors are queued in (point
+        // index within the leaf, lane) order, with drains between indices
+        for (uint32_t e = 0;; ++e) {
+          const bool act = e < cnt;
+          if (!__ballot(act)) break;
+          bool push = false;
+          const uint32_t slot = first + e;
+          if (act) {
+            const float4 p = load_prim(sc.prims, slot);
+            if (STATS) ++st.prims;
+            const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
+            push = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)) <= thr2;
+          }
+          const uint64_t pm = __ballot(push);
+          if (push) {
+            const uint32_t pos = (head + size + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))) & (kNbrQueueCap - 1);
+            qs[pos] = slot;
+            qo[pos] = (uint8_t)lane;
+          }
+          size += (uint32_t)__popcll(pm);
+          while (size >= 64) drain();
+        }
+        const bool any_walk = __ballot(walking) != 0;
+        while (!any_walk && size != 0) drain();
+        if (!any_walk && size == 0) break;
+      }
+      wave_fence();
+      if (going) {
+        const uint32_t count = l_count[threadIdx.x];
+        going = false;
+        r.count = count;
+        r.heavy = shrunk || count > k;
+        r.bucket = top;
+        r.n_cand = 0;
+        r.staged = count <= stage_slots;
+        if (r.heavy && count > kNbrCountMax) {
+          r.refine = true;
+        } else if (r.heavy) {
+          uint32_t cum = 0;
+          for (uint32_t c = 0; c <= top; ++c) {
+            cum += bucket_count(c);
+            if (cum >= k) {
+              r.bucket = c;
+              r.n_cand = cum;
+              break;
+            }
+          }
+          if (r.n_cand > kNbrCandMax) {
+            if (round == 3) {
+              r.refine = true;
+            } else {
+              // too coarse: count again with kNbrBuckets slices of bucket b's
+              // ball, which holds the k nearest
+              r.radius = fmin(r.radius, nbr_bucket_edge(r.bucket, scale));
+              r.radius_sq = fmin(r.radius_sq, r.radius * r.radius);
+              going = true;
+            }
+          }
+        }
+      }
     }
     uint32_t n_rows = r.heavy ? k : r.count;
     bool refine = r.refine;

@@ -79,11 +79,14 @@ def test_fixed_radius_visit_counters_equal_a_host_replay(gpu_ready):
     for radius, k in ((0.05, 64), (0.01, 16), (0.02, 1)):
         _say(f"  radius {radius}, k_max {k}")
         job = B.BatchJob(scene, B.JOB_FIXED_RADIUS, DeviceBuffer.from_numpy(q3), radius=radius, k=k)
-        res, _, visits = B.run_batch_profiled([job], count_visits=True)
+        # without the Morton reordering a work item is 64 consecutive queries,
+        # one per lane, walked in lock step: the replay runs per item
+        res, _, visits = B.run_batch_profiled([job], morton_sort=False, count_visits=True)
         assert res[0]["status"] == 0
         nodes = prims = 0
-        for row in q3.astype(np.float64):
-            n, p = W.frn_counts(sc, row, radius, k)
+        q64 = q3.astype(np.float64)
+        for first in range(0, len(q64), 64):
+            n, p = W.frn_counts(sc, q64[first:first + 64], radius, k)
             nodes += n
             prims += p
         assert (nodes, prims) == visits, (radius, k, (nodes, prims), visits)

@@ -228,80 +228,130 @@ def _nbr_bucket_edge(b: int, scale: float) -> float:
     return float(b + 1) / scale * 1.0000000000009094947
 
 
-def frn_counts(sc: dict, q, radius: float, k: int, *, instance: int = 0) -> tuple[int, int]:
+def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -> tuple[int, int]:
     """Nodes fetched and primitives tested by the batched fixed-radius count
-    walk of query q (FP64 coordinates of FP32 values), over all its count
-    rounds: a rope preorder walk whose threshold shrinks to the edge of the
-    distance bucket holding the k-th neighbour, checked every max(k/2, 4)
-    neighbours, recounted over the finer buckets of that ball while the
-    buckets up to it hold more than 128 neighbours (at most four rounds)."""
+    walk of one work item: up to 64 queries (FP64 coordinates of FP32
+    values), one per lane of a wavefront, over all count rounds.
+
+    The wavefront runs in lock step.  Each iteration every walking lane takes
+    one node step of its rope preorder walk; the points of a leaf that pass
+    the FP32 prefilter are queued (point index within the leaf first, then
+    lane), and after each point index, while 64 or more are queued (and, once
+    no lane walks, while any are), a batch of up to 64 is taken in queue order: each point inside the
+    owner's FP64 radius whose distance bucket does not lie beyond the owner's
+    last counted bucket is counted; then every lane whose count reached its
+    checkpoint (every max(k/2, 4)) shrinks its threshold to the edge of the
+    bucket holding its k-th neighbour.  Rounds are wavefront-synchronous: a
+    lane whose buckets up to the k-th still hold more than 128 neighbours
+    counts again over the finer buckets of that ball (at most four rounds)."""
     nodes, prims, exact = sc["nodes"], sc["prims"], sc["exact"]
     root, end = int(sc["roots"][instance]), int(sc["ends"][instance])
-    qx, qy, qz = (float(v) for v in q)
-    qf = (F32(qx), F32(qy), F32(qz))
-    m = coord_bound(sc["m"], qx, qy, qz)
-    rad, rad_sq = radius, radius * radius
+    lanes = len(queries)
+    q = [tuple(float(v) for v in row) for row in queries]
+    qf = [(F32(x), F32(y), F32(z)) for x, y, z in q]
+    m = [coord_bound(sc["m"], *row) for row in q]
+    rad = [radius] * lanes
+    rad_sq = [radius * radius] * lanes
     step = k // 2 if k // 2 > 4 else 4
+    going = [True] * lanes
     n_nodes = n_prims = 0
+
+    def d2_of(lane: int, slot: int) -> float:
+        if exact is not None:
+            px, py, pz = exact[3 * slot], exact[3 * slot + 1], exact[3 * slot + 2]
+        else:
+            px, py, pz = (float(v) for v in prims[slot, :3])
+        dx, dy, dz = px - q[lane][0], py - q[lane][1], pz - q[lane][2]
+        return dx * dx + dy * dy + dz * dz
+
     for rnd in range(4):
-        hist = [0] * NBR_BUCKETS
-        scale = float(NBR_BUCKETS) / rad
-        st = {"count": 0, "top": NBR_BUCKETS - 1, "next": k, "shrunk": False,
-              "thr2": frn_threshold2(rad, m)}
+        if not any(going):
+            break
+        scale = [float(NBR_BUCKETS) / rad[i] for i in range(lanes)]
+        thr2 = [frn_threshold2(rad[i], m[i]) for i in range(lanes)]
+        top = [NBR_BUCKETS - 1] * lanes
+        nxt = [k] * lanes
+        shrunk = [False] * lanes
+        hist = [[0] * NBR_BUCKETS for _ in range(lanes)]
+        count = [0] * lanes
+        node = [root] * lanes
+        walking = [going[i] and root != end for i in range(lanes)]
+        queue: list[tuple[int, int]] = []
+        while True:
+            leaf = [(0, 0)] * lanes
+            for i in range(lanes):
+                if not walking[i]:
+                    continue
+                n_nodes += 1
+                nd = nodes[node[i]]
+                link = int(nd["link"])
+                inside = box_dist2(qf[i], nd["lo"], nd["hi"]) <= thr2[i]
+                if inside and link & LEAF:
+                    leaf[i] = (link & START_MASK, ((link >> 28) & 7) + 1)
+                node[i] = link if (inside and not link & LEAF) else int(nd["escape"])
+                walking[i] = node[i] != end
+            def drain() -> None:
+                nonlocal queue
+                batch, queue = queue[:64], queue[64:]
+                tops = list(top)
+                for o, slot in batch:
+                    d2 = d2_of(o, slot)
+                    if d2 > rad_sq[o]:
+                        continue
+                    b = _nbr_bucket(math.sqrt(d2), scale[o])
+                    if b <= tops[o]:
+                        hist[o][b] += 1
+                        count[o] += 1
+                for i in range(lanes):
+                    if not going[i] or count[i] < nxt[i]:
+                        continue
+                    nxt[i] = count[i] + step
+                    cum = 0
+                    for c in range(top[i]):
+                        cum += hist[i][c]
+                        if cum >= k:
+                            top[i] = c
+                            shrunk[i] = True
+                            thr2[i] = frn_threshold2(min(rad[i], _nbr_bucket_edge(c, scale[i])), m[i])
+                            break
 
-        def visit(slot: int) -> None:
-            if exact is not None:
-                px, py, pz = exact[3 * slot], exact[3 * slot + 1], exact[3 * slot + 2]
-            else:
-                px, py, pz = (float(v) for v in prims[slot, :3])
-            dx, dy, dz = px - qx, py - qy, pz - qz
-            d2 = dx * dx + dy * dy + dz * dz
-            if d2 > rad_sq:
-                return
-            b = _nbr_bucket(math.sqrt(d2), scale)
-            if b > st["top"]:
-                return
-            st["count"] += 1
-            hist[b] += 1
-            if st["count"] < st["next"]:
-                return
-            st["next"] = st["count"] + step
-            cum = 0
-            for c in range(st["top"]):
-                cum += hist[c]
-                if cum >= k:
-                    st["top"] = c
-                    st["shrunk"] = True
-                    st["thr2"] = frn_threshold2(min(rad, _nbr_bucket_edge(c, scale)), m)
+            for e in range(8):
+                if not any(e < leaf[i][1] for i in range(lanes)):
                     break
-
-        node = root
-        while root != end and node != end:
-            n_nodes += 1
-            nd = nodes[node]
-            link = int(nd["link"])
-            inside = box_dist2(qf, nd["lo"], nd["hi"]) <= st["thr2"]
-            if inside and link & LEAF:
-                for slot in leaf_range(link):
+                for i in range(lanes):
+                    first, cnt = leaf[i]
+                    if e >= cnt:
+                        continue
                     n_prims += 1
+                    slot = first + e
                     p = prims[slot]
-                    fx, fy, fz = F32(p[0] - qf[0]), F32(p[1] - qf[1]), F32(p[2] - qf[2])
-                    if fma32(fz, fz, fma32(fy, fy, F32(fx * fx))) <= st["thr2"]:
-                        visit(slot)
-            node = link if (inside and not link & LEAF) else int(nd["escape"])
-        count, top = st["count"], st["top"]
-        if not (st["shrunk"] or count > k) or count > NBR_COUNT_MAX:
-            break
-        bucket, n_cand, cum = top, 0, 0
-        for c in range(top + 1):
-            cum += hist[c]
-            if cum >= k:
-                bucket, n_cand = c, cum
+                    fx, fy, fz = F32(p[0] - qf[i][0]), F32(p[1] - qf[i][1]), F32(p[2] - qf[i][2])
+                    if fma32(fz, fz, fma32(fy, fy, F32(fx * fx))) <= thr2[i]:
+                        queue.append((i, slot))
+                while len(queue) >= 64:
+                    drain()
+            any_walk = any(walking)
+            while not any_walk and queue:
+                drain()
+            if not any_walk and not queue:
                 break
-        if n_cand <= NBR_CAND_MAX:
-            break
-        rad = min(rad, _nbr_bucket_edge(bucket, scale))
-        rad_sq = min(rad_sq, rad * rad)
+        for i in range(lanes):
+            if not going[i]:
+                continue
+            going[i] = False
+            heavy = shrunk[i] or count[i] > k
+            if not heavy or count[i] > NBR_COUNT_MAX:
+                continue
+            bucket, n_cand, cum = top[i], 0, 0
+            for c in range(top[i] + 1):
+                cum += hist[i][c]
+                if cum >= k:
+                    bucket, n_cand = c, cum
+                    break
+            if n_cand > NBR_CAND_MAX and rnd < 3:
+                rad[i] = min(rad[i], _nbr_bucket_edge(bucket, scale[i]))
+                rad_sq[i] = min(rad_sq[i], rad[i] * rad[i])
+                going[i] = True
     return n_nodes, n_prims
 
 
