@@ -139,6 +139,15 @@ int rtdl_mi355x_get_last_phase_timings(double* seconds_out, size_t capacity);
  * visits[0] node records fetched and visits[1] primitive records tested
  * (counting kernels are slower: time and count in separate calls). */
 int rtdl_mi355x_set_walk_counting(uint32_t on, char* error_out, size_t error_size);
+
+/* Arms a host-forced fault for testing failure handling (no counterpart in
+ * the reference; the RTDL_MI355X_FAULT environment variable arms one at load):
+ * "alloc", "launch" or "timeout", optionally ":N" for the N-th occurrence
+ * (default 1) from now: the N-th device allocation fails, the N-th kernel
+ * launch check fails, or the N-th served batch to complete is treated as
+ * past its deadline.  Each fires once.  "" disarms.  No kernel faults: the
+ * failure is raised on the host. */
+int rtdl_mi355x_set_fault(const char* spec, char* error_out, size_t error_size);
 int rtdl_mi355x_get_last_walk_stats(double* kernel_seconds, uint64_t* visits, char* error_out,
                                     size_t error_size);
 
@@ -367,6 +376,9 @@ enum {
 #define RTDL_STATUS_ROW_OVERFLOW 1u /* the job's rows did not fit row_capacity: rows are incomplete */
 #define RTDL_STATUS_REJECTED 4u     /* served request whose rows can exceed the serving row arena */
 #define RTDL_STATUS_INTERNAL 8u     /* an internal invariant failed: the job's rows are not valid */
+#define RTDL_STATUS_FAILED 16u      /* served request whose batch failed, and which failed again alone */
+#define RTDL_STATUS_TIMEOUT 32u     /* served request not done by its deadline (its rows may be late) */
+#define RTDL_STATUS_INVALID 64u     /* served request naming a missing scene, a scene of another type or a null arena */
 
 typedef struct RtdlBatchJob {
   void* scene;           /* prepared handle */
@@ -518,7 +530,8 @@ typedef struct RtdlServeConfig {
   const float* ray_queries;     /* device arena, 7 FP32 per ray */
   const float* pip_queries;     /* device arena, 2 FP32 per point */
   uint32_t realtime;            /* 1: honour arrival times; 0: everything arrives at 0 */
-  uint32_t pad;
+  uint32_t timeout_ms;          /* > 0: requests whose batch is not done this long after dispatch
+                                   are marked RTDL_STATUS_TIMEOUT (the loop still waits for it) */
 } RtdlServeConfig;
 
 typedef struct RtdlServeResult {
@@ -538,6 +551,8 @@ typedef struct RtdlServeStats {
   uint64_t rows;
   uint64_t peak_inflight_bytes; /* largest sum of in-flight row buffers (<= region_bytes) */
   uint64_t region_bytes;        /* the row arena: min(budget, slots x the largest admissible batch) */
+  uint64_t retried;             /* requests re-run alone after their batch failed */
+  uint64_t failed;              /* requests that failed alone as well (RTDL_STATUS_FAILED) */
 } RtdlServeStats;
 
 /* Replays a request stream against resident scenes with continuous
@@ -545,7 +560,11 @@ typedef struct RtdlServeStats {
  * requests that have arrived, in order, while their row buffers fit the
  * largest free range of one row arena bounded by the memory budget (the
  * first request that does not fit waits for completions to free memory; one
- * that can never fit is rejected).  Latency = done_s - arrival_s. */
+ * that can never fit is rejected).  Latency = done_s - arrival_s.
+ * Failures stay with the requests they belong to: an invalid request is
+ * marked RTDL_STATUS_INVALID and skipped; a batch whose launch fails is
+ * re-run one request at a time, and only a request that fails alone too is
+ * marked RTDL_STATUS_FAILED; the other requests are served normally. */
 int rtdl_mi355x_serve_replay(void* const* scenes, size_t scene_count,
                              const RtdlServeRequest* requests, size_t request_count,
                              const RtdlServeConfig* config, RtdlServeResult* results,

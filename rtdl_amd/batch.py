@@ -31,6 +31,9 @@ JOB_PIP_POSITIVE = 6
 STATUS_ROW_OVERFLOW = 1
 STATUS_REJECTED = 4
 STATUS_INTERNAL = 8
+STATUS_FAILED = 16
+STATUS_TIMEOUT = 32
+STATUS_INVALID = 64
 PIP_HIT_CAP = 16  # default PIP row budget per point (average); one point may exceed it
 
 ROW_DTYPES = {
@@ -65,7 +68,7 @@ class _ServeConfig(ctypes.Structure):
         ("slots", ctypes.c_uint32), ("morton_sort", ctypes.c_uint32),
         ("memory_budget_bytes", ctypes.c_uint64), ("max_batch_queries", ctypes.c_uint64),
         ("point_queries", ctypes.c_void_p), ("ray_queries", ctypes.c_void_p),
-        ("pip_queries", ctypes.c_void_p), ("realtime", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+        ("pip_queries", ctypes.c_void_p), ("realtime", ctypes.c_uint32), ("timeout_ms", ctypes.c_uint32),
     ]
 
 
@@ -73,7 +76,7 @@ class _ServeStats(ctypes.Structure):
     _fields_ = [
         ("wall_s", ctypes.c_double), ("batches", ctypes.c_uint64), ("queries", ctypes.c_uint64),
         ("rows", ctypes.c_uint64), ("peak_inflight_bytes", ctypes.c_uint64),
-        ("region_bytes", ctypes.c_uint64),
+        ("region_bytes", ctypes.c_uint64), ("retried", ctypes.c_uint64), ("failed", ctypes.c_uint64),
     ]
 
 
@@ -359,7 +362,7 @@ def job_rows(job: BatchJob, result: dict) -> np.ndarray:
 def serve_replay(scenes, requests: np.ndarray, *, radius: float, k_max: int, k: int,
                  point_queries=None, ray_queries=None, pip_queries=None, slots: int = 4,
                  morton_sort: bool = True, memory_budget_bytes: int = 200 << 30,
-                 max_batch_queries: int = 4 << 20, realtime: bool = True):
+                 max_batch_queries: int = 4 << 20, realtime: bool = True, timeout_ms: int = 0):
     """Replays `requests` (SERVE_REQUEST_DTYPE) against `scenes`.
 
     Returns (results as SERVE_RESULT_DTYPE, stats dict, captured rows by request index)."""
@@ -377,6 +380,7 @@ def serve_replay(scenes, requests: np.ndarray, *, radius: float, k_max: int, k: 
     cfg.ray_queries = _ptr(ray_queries)
     cfg.pip_queries = _ptr(pip_queries)
     cfg.realtime = int(bool(realtime))
+    cfg.timeout_ms = int(timeout_ms)
     results = np.zeros(len(reqs), dtype=SERVE_RESULT_DTYPE)
     stats = _ServeStats()
     err = ctypes.create_string_buffer(4096)

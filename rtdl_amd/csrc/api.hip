@@ -103,6 +103,10 @@ int rtdl_mi355x_context_probe(char* device_name, size_t device_name_size, int* d
 
 void rtdl_mi355x_free_rows(void* rows) { std::free(rows); }
 
+int rtdl_mi355x_set_fault(const char* spec, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] { set_fault(spec); });
+}
+
 int rtdl_mi355x_set_walk_counting(uint32_t on, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] { thread_walk_stats().counting = on != 0; });
 }
@@ -849,6 +853,9 @@ int rtdl_mi355x_device_alloc(size_t bytes, void** ptr_out, char* error_out, size
   return guarded(error_out, error_size, [&] {
     require(ptr_out != nullptr, "ptr_out must not be null");
     *ptr_out = nullptr;
+    if (fault_fires(FaultKind::Alloc))
+      throw Error("injected fault (RTDL_MI355X_FAULT=alloc): device allocation of " + std::to_string(bytes) +
+                  " bytes failed");
     void* p = nullptr;
     RTDL_HIP(hipMalloc(&p, bytes ? bytes : 1));
     *ptr_out = p;

@@ -27,7 +27,21 @@ struct Error : std::runtime_error {
     }                                                                               \
   } while (0)
 
-#define RTDL_LAUNCH_CHECK() RTDL_HIP(hipGetLastError())
+// Host-forced faults for testing failure handling (RTDL_MI355X_FAULT, or
+// rtdl_mi355x_set_fault): "alloc", "launch" or "timeout", optionally ":N" for
+// the N-th occurrence (default 1) counted from the moment the fault is armed.
+// fault_fires(kind) is true exactly once, at that occurrence; the failure
+// is raised on the host, so no kernel ever faults.
+enum class FaultKind : int { None = 0, Alloc = 1, Launch = 2, Timeout = 3 };
+bool fault_fires(FaultKind kind);
+void set_fault(const char* spec);  // "" disarms; throws on an unknown spec
+
+#define RTDL_LAUNCH_CHECK()                                                               \
+  do {                                                                                    \
+    if (::rtdl::fault_fires(::rtdl::FaultKind::Launch))                                  \
+      throw ::rtdl::Error("injected fault (RTDL_MI355X_FAULT=launch): kernel launch failed"); \
+    RTDL_HIP(hipGetLastError());                                                          \
+  } while (0)
 
 // A literal message is only turned into a string when the check fails.
 inline void require(bool ok, const char* message) {

@@ -45,7 +45,63 @@ size_t pool_round(size_t bytes) {
 }
 }  // namespace
 
+// ------------------------------------------------------------ fault hook --
+
+namespace {
+std::mutex g_fault_mutex;
+bool g_fault_loaded = false;
+FaultKind g_fault_kind = FaultKind::None;
+uint64_t g_fault_nth = 0, g_fault_seen = 0;
+
+// Parsed with C string functions: only a bad spec builds a message, so
+// arming or reading the hook allocates nothing.
+void parse_fault(const char* spec) {
+  const char* s = spec ? spec : "";
+  const char* colon = std::strchr(s, ':');
+  const size_t len = colon ? (size_t)(colon - s) : std::strlen(s);
+  FaultKind kind = FaultKind::None;
+  if (len == 5 && std::strncmp(s, "alloc", 5) == 0) kind = FaultKind::Alloc;
+  else if (len == 6 && std::strncmp(s, "launch", 6) == 0) kind = FaultKind::Launch;
+  else if (len == 7 && std::strncmp(s, "timeout", 7) == 0) kind = FaultKind::Timeout;
+  else if (len != 0) throw Error(std::string("fault spec must be alloc, launch or timeout[:N], not '") + s + "'");
+  uint64_t nth = 1;
+  if (colon) {
+    char* end = nullptr;
+    const unsigned long long v = std::strtoull(colon + 1, &end, 10);
+    if (end == colon + 1 || *end != '\0' || v == 0)
+      throw Error(std::string("fault spec: the occurrence must be a positive integer, not '") + s + "'");
+    nth = v;
+  }
+  g_fault_kind = kind;
+  g_fault_nth = nth;
+  g_fault_seen = 0;
+  g_fault_loaded = true;
+}
+}  // namespace
+
+void set_fault(const char* spec) {
+  std::lock_guard<std::mutex> g(g_fault_mutex);
+  parse_fault(spec);
+}
+
+bool fault_fires(FaultKind kind) {
+  std::lock_guard<std::mutex> g(g_fault_mutex);
+  if (!g_fault_loaded) {
+    try {
+      parse_fault(std::getenv("RTDL_MI355X_FAULT"));
+    } catch (const Error&) {
+      g_fault_loaded = true;  // an unusable environment value arms nothing
+      g_fault_kind = FaultKind::None;
+    }
+  }
+  if (g_fault_kind != kind) return false;
+  return ++g_fault_seen == g_fault_nth;
+}
+
 void* pool_alloc(size_t bytes) {
+  if (fault_fires(FaultKind::Alloc))
+    throw Error("injected fault (RTDL_MI355X_FAULT=alloc): device allocation of " + std::to_string(bytes) +
+                " bytes failed");
   const size_t size = pool_round(bytes);
   {
     std::lock_guard<std::mutex> g(g_pool_mutex);

@@ -34,12 +34,17 @@ namespace rtdl {
 namespace {
 
 constexpr uint32_t kStatusRejected = 4u;
+constexpr uint32_t kStatusFailed = 16u;
+constexpr uint32_t kStatusTimeout = 32u;
+constexpr uint32_t kStatusInvalid = 64u;
 
 struct Slot {
   hipStream_t stream = nullptr;
   std::unique_ptr<BatchEngine> engine;
   hipEvent_t done = nullptr;
   bool busy = false;
+  bool late = false;                       // the batch passed its deadline
+  double dispatched = 0.0;
   uint64_t range_at = 0, range_bytes = 0;  // the batch's rows in the arena
   std::vector<size_t> requests;           // request indices of the batch in flight
 };
@@ -89,21 +94,27 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
     views[i] = p->scene.view(0);
     types[i] = p->type;
   }
+  // a request that cannot be served is marked and skipped; the others run
+  std::vector<uint8_t> valid(n_requests, 1);
   for (size_t r = 0; r < n_requests; ++r) {
     const RtdlServeRequest& q = requests[r];
-    require(q.scene < n_scenes, "request scene index out of range");
-    require(q.kind <= kJobPipPositive, "unknown request kind");
-    const SceneType t = types[q.scene];
-    const bool point_kind = q.kind <= kJobBoundedKnn;
-    const bool ray_kind = q.kind >= kJobRayAnyHit && q.kind <= kJobRayClosest;
-    require(!point_kind || t == SceneType::Points3D, "neighbour requests need a 3-D point scene");
-    require(!ray_kind || t == SceneType::Triangles3D, "ray requests need a triangle scene");
-    require(q.kind != kJobPipPositive || t == SceneType::Polygons, "PIP requests need a polygon scene");
-    const float* arena = point_kind ? cfg.point_queries : ray_kind ? cfg.ray_queries : cfg.pip_queries;
-    require(q.query_count == 0 || arena != nullptr, "query arena for a request kind is null");
     if (r) require(q.arrival_s >= requests[r - 1].arrival_s, "arrival times must be non-decreasing");
     std::memset(&results[r], 0, sizeof(RtdlServeResult));
     results[r].arrival_s = q.arrival_s;
+    bool ok = q.scene < n_scenes && q.kind <= kJobPipPositive;
+    if (ok) {
+      const SceneType t = types[q.scene];
+      const bool point_kind = q.kind <= kJobBoundedKnn;
+      const bool ray_kind = q.kind >= kJobRayAnyHit && q.kind <= kJobRayClosest;
+      ok = (!point_kind || t == SceneType::Points3D) && (!ray_kind || t == SceneType::Triangles3D) &&
+           (q.kind != kJobPipPositive || t == SceneType::Polygons);
+      const float* arena = point_kind ? cfg.point_queries : ray_kind ? cfg.ray_queries : cfg.pip_queries;
+      ok = ok && (q.query_count == 0 || arena != nullptr);
+    }
+    if (!ok) {
+      valid[r] = 0;
+      results[r].status = kStatusInvalid;
+    }
   }
   require(cfg.k >= 1 && cfg.k <= 64 && cfg.k_max >= 1 && cfg.k_max <= 64,
           "served k and k_max must be in [1, 64]");
@@ -131,7 +142,7 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
   std::vector<uint64_t> capture_at(n_requests, 0);
   uint64_t capture_total = 0;
   for (size_t r = 0; r < n_requests; ++r) {
-    if (!requests[r].capture) continue;
+    if (!requests[r].capture || !valid[r]) continue;
     capture_at[r] = capture_total;
     capture_total += row_capacity(requests[r]) * job_row_bytes(requests[r].kind);
   }
@@ -169,32 +180,51 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
   auto now = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
 
   std::deque<size_t> pending;
+  std::deque<size_t> solo;  // requests of a failed batch, re-run one at a time
+  std::vector<uint8_t> retried(n_requests, 0);
   size_t next = 0, completed = 0, batch_id = 0;
   uint64_t inflight_bytes = 0, peak_bytes = 0, total_q = 0, total_rows = 0, batches = 0;
+  uint64_t n_retried = 0, n_failed = 0;
   std::vector<HostJob> jobs;
   std::vector<uint64_t> need_of(n_requests);
-  for (size_t r = 0; r < n_requests; ++r) need_of[r] = row_need(requests[r]);
+  for (size_t r = 0; r < n_requests; ++r) need_of[r] = valid[r] ? row_need(requests[r]) : 0;
 
   while (completed < n_requests) {
     bool progress = false;
     const double t = now();
-    while (next < n_requests && (!cfg.realtime || requests[next].arrival_s <= t)) pending.push_back(next++);
+    while (next < n_requests && (!cfg.realtime || requests[next].arrival_s <= t)) {
+      if (valid[next]) {
+        pending.push_back(next);
+      } else {
+        results[next].dispatch_s = results[next].done_s = t;
+        ++completed;
+      }
+      ++next;
+    }
 
     // completions
     for (Slot& s : slots) {
       if (!s.busy) continue;
       const hipError_t q = hipEventQuery(s.done);
-      if (q == hipErrorNotReady) continue;
+      if (q == hipErrorNotReady) {
+        if (cfg.timeout_ms && !s.late && (now() - s.dispatched) * 1e3 > cfg.timeout_ms) s.late = true;
+        continue;
+      }
       RTDL_HIP(q);
       const double td = now();
+      // a batch past its deadline (or a forced timeout) keeps its slot and
+      // rows until the device is done with them; its requests are marked
+      if (cfg.timeout_ms && (td - s.dispatched) * 1e3 > cfg.timeout_ms) s.late = true;
+      if (fault_fires(FaultKind::Timeout)) s.late = true;
       const JobCounters* c = s.engine->counters();
       for (size_t j = 0; j < s.requests.size(); ++j) {
         RtdlServeResult& res = results[s.requests[j]];
         res.done_s = td;
         res.rows = c[j].rows;
-        res.status |= c[j].status;
+        res.status |= c[j].status | (s.late ? kStatusTimeout : 0u);
         total_rows += c[j].rows;
       }
+      s.late = false;
       completed += s.requests.size();
       inflight_bytes -= s.range_bytes;
       s.range_bytes = 0;
@@ -203,22 +233,24 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
       progress = true;
     }
 
-    // dispatch
+    // dispatch: requests of a failed batch first, one per batch
     for (Slot& s : slots) {
-      if (pending.empty()) break;
+      if (pending.empty() && solo.empty()) break;
       if (s.busy) continue;
       uint64_t gap_at = 0, gap = 0;
       largest_gap(slots, arena_bytes, &gap_at, &gap);
       jobs.clear();
       uint64_t used = 0, queries = 0;
-      while (!pending.empty()) {
-        const size_t r = pending.front();
+      const bool alone = !solo.empty();
+      std::deque<size_t>& from = alone ? solo : pending;
+      while (!from.empty() && !(alone && !jobs.empty())) {
+        const size_t r = from.front();
         const RtdlServeRequest& q = requests[r];
         const uint64_t need = need_of[r];
         if (need > arena_bytes) {  // can never be admitted
           results[r].status |= kStatusRejected;
           results[r].dispatch_s = results[r].done_s = now();
-          pending.pop_front();
+          from.pop_front();
           ++completed;
           continue;
         }
@@ -240,19 +272,45 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
         s.requests.push_back(r);
         used += need;
         queries += q.query_count;
-        pending.pop_front();
+        from.pop_front();
       }
       if (jobs.empty()) break;  // the head of the line waits for memory
-      s.engine->enqueue(jobs, cfg.morton_sort != 0);
-      for (size_t j = 0; j < jobs.size(); ++j) {
-        const size_t r = s.requests[j];
-        if (!requests[r].capture) continue;
-        const uint64_t bytes = jobs[j].row_capacity * job_row_bytes(requests[r].kind);
-        if (bytes)
-          RTDL_HIP(hipMemcpyAsync(capture_pool + capture_at[r], jobs[j].rows, bytes, hipMemcpyDeviceToHost,
-                                  s.stream));
+      try {
+        s.engine->enqueue(jobs, cfg.morton_sort != 0);
+        for (size_t j = 0; j < jobs.size(); ++j) {
+          const size_t r = s.requests[j];
+          if (!requests[r].capture) continue;
+          const uint64_t bytes = jobs[j].row_capacity * job_row_bytes(requests[r].kind);
+          if (bytes)
+            RTDL_HIP(hipMemcpyAsync(capture_pool + capture_at[r], jobs[j].rows, bytes, hipMemcpyDeviceToHost,
+                                    s.stream));
+        }
+        RTDL_HIP(hipEventRecord(s.done, s.stream));
+      } catch (const Error&) {
+        // whatever was enqueued finishes before the slot and rows are reused;
+        // the batch's requests run again alone, so a failure stays with the
+        // request that causes it
+        RTDL_HIP(hipStreamSynchronize(s.stream));
+        const double tf = now();
+        for (size_t r : s.requests) {
+          if (alone || retried[r]) {
+            results[r].status |= kStatusFailed;
+            results[r].dispatch_s = results[r].done_s = tf;
+            ++completed;
+            ++n_failed;
+          } else {
+            retried[r] = 1;
+            ++n_retried;
+          }
+        }
+        if (!alone)
+          for (auto it = s.requests.rbegin(); it != s.requests.rend(); ++it) solo.push_front(*it);
+        s.requests.clear();
+        progress = true;
+        continue;
       }
-      RTDL_HIP(hipEventRecord(s.done, s.stream));
+      s.dispatched = now();
+      s.late = false;
       const double td = now();
       for (size_t r : s.requests) {
         results[r].dispatch_s = td;
@@ -272,7 +330,7 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
     if (!progress) {
       bool any_busy = false;
       for (const Slot& s : slots) any_busy = any_busy || s.busy;
-      if (!any_busy && pending.empty() && next < n_requests && cfg.realtime) {
+      if (!any_busy && pending.empty() && solo.empty() && next < n_requests && cfg.realtime) {
         const double wait = requests[next].arrival_s - now();
         if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1e6) - 100));
       } else {
@@ -283,7 +341,8 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
   const double wall = now();
   for (Slot& s : slots) RTDL_HIP(hipStreamSynchronize(s.stream));
   for (size_t r = 0; r < n_requests; ++r) {
-    if (!requests[r].capture || (results[r].status & kStatusRejected)) continue;
+    if (!requests[r].capture || (results[r].status & (kStatusRejected | kStatusFailed | kStatusInvalid)))
+      continue;
     // an overflowing request reserved more rows than its buffer holds; only
     // the buffer was copied
     const uint64_t rows = std::min<uint64_t>(results[r].rows, row_capacity(requests[r]));
@@ -300,6 +359,8 @@ void serve_replay(void* const* scenes, size_t n_scenes, const RtdlServeRequest* 
     stats->rows = total_rows;
     stats->peak_inflight_bytes = peak_bytes;
     stats->region_bytes = arena_bytes;
+    stats->retried = n_retried;
+    stats->failed = n_failed;
   }
 }
 

@@ -160,6 +160,7 @@ _SIGNATURES = {
     "rtdl_mi355x_run_prepared_point_in_polygon": ([_VP, _VP, _SZ, ctypes.c_uint32, *_ROWS, *_ERR], ctypes.c_int),
     "rtdl_mi355x_destroy_prepared": ([_VP], None),
     "rtdl_mi355x_set_walk_counting": ([ctypes.c_uint32, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_set_fault": ([ctypes.c_char_p, *_ERR], ctypes.c_int),
     "rtdl_mi355x_get_last_walk_stats": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64), *_ERR],
                                         ctypes.c_int),
     # reference-shaped prepared handles (the HIPRT parameter lists)
@@ -251,6 +252,15 @@ def _call_prepare(fn_name: str, *args) -> ctypes.c_void_p:
         detail = error.value.decode("utf-8", errors="replace")
         raise RuntimeError(f"{fn_name} failed with status {status}: {detail}")
     return handle
+
+
+def set_fault(spec: str) -> None:
+    """Arms a host-forced fault ("alloc", "launch" or "timeout", optionally
+    ":N" for the N-th occurrence from now); "" disarms.  For testing failure
+    handling: the library raises the failure on the host, no kernel faults."""
+    err = ctypes.create_string_buffer(512)
+    if _library().rtdl_mi355x_set_fault(spec.encode(), err, 512) != 0:
+        raise ValueError(err.value.decode())
 
 
 def set_walk_counting(on: bool) -> None:

@@ -592,3 +592,97 @@ def test_serve_admission_waits_for_memory_under_a_small_budget(gpu_ready):
         q = pq[o:o + n].astype(np.float64)
         qd = {"id": np.arange(n, dtype=np.uint32), "x": q[:, 0], "y": q[:, 1], "z": q[:, 2]}
         _check_nn(captured[i], oracle.neighbors("frn", qd, sdict, radius=0.03, k=64), False)
+
+
+def _frn_requests(rng, n_req, sizes_hi, scene_of=None, capture=True):
+    sizes = rng.integers(200, sizes_hi, n_req)
+    reqs = np.zeros(n_req, dtype=B.SERVE_REQUEST_DTYPE)
+    off = 0
+    for i in range(n_req):
+        reqs[i] = (B.JOB_FIXED_RADIUS, 0 if scene_of is None else scene_of(i), off, sizes[i], int(capture), 0.0)
+        off += sizes[i]
+    return reqs, int(sizes.sum())
+
+
+def _check_frn_request(captured, reqs, i, pq, sdict, radius, k):
+    o, n = int(reqs["query_offset"][i]), int(reqs["query_count"][i])
+    q = pq[o:o + n].astype(np.float64)
+    qd = {"id": np.arange(n, dtype=np.uint32), "x": q[:, 0], "y": q[:, 1], "z": q[:, 2]}
+    _check_nn(captured[i], oracle.neighbors("frn", qd, sdict, radius=radius, k=k), False)
+
+
+def test_serve_failures_stay_with_their_requests(gpu_ready):
+    """A launch failure in a batch (forced on the host: RTDL_MI355X_FAULT
+    "launch:3") sends the batch's requests through again one at a time, and
+    requests naming a missing scene or a scene of the wrong type are marked
+    invalid; every other request is served with exact rows."""
+    from rtdl_amd import mi355x_runtime as m
+
+    _say("case: serve failure isolation (forced launch failure, invalid requests)")
+    rng = np.random.default_rng(83)
+    sdict, s3 = _points3(rng, 20000)
+    pscene = _prepared_points(s3)
+    tris = cases.triangles(rng, 2000, sigma=0.01)
+    tscene = B.prepare_triangle_instances(_dev(cases.tri_matrix(tris)), [0, 2000])
+    bad = {5: 7, 9: 1}  # request -> scene: 7 does not exist, 1 holds triangles
+    reqs, total = _frn_requests(rng, 24, 1500, scene_of=lambda i: bad.get(i, 0))
+    pq = rng.random((total, 3)).astype(np.float32)
+    m.set_fault("launch:3")
+    try:
+        res, stats, captured = B.serve_replay([pscene, tscene], reqs, radius=0.03, k_max=16, k=1,
+                                              point_queries=_dev(pq), slots=2, realtime=False)
+    finally:
+        m.set_fault("")
+    for i in range(len(reqs)):
+        if i in bad:
+            assert res["status"][i] == B.STATUS_INVALID and i not in captured
+        else:
+            assert res["status"][i] == 0, (i, res["status"][i])
+            _check_frn_request(captured, reqs, i, pq, sdict, 0.03, 16)
+    assert stats["retried"] >= 1 and stats["failed"] == 0
+
+
+def test_serve_forced_timeout_marks_only_the_late_batch(gpu_ready):
+    """"timeout:1" treats the first batch to complete as past its deadline:
+    its requests carry RTDL_STATUS_TIMEOUT (their rows are complete, only
+    late); every other request is on time."""
+    from rtdl_amd import mi355x_runtime as m
+
+    _say("case: serve forced timeout")
+    rng = np.random.default_rng(84)
+    sdict, s3 = _points3(rng, 20000)
+    scene = _prepared_points(s3)
+    reqs, total = _frn_requests(rng, 12, 900)
+    pq = rng.random((total, 3)).astype(np.float32)
+    m.set_fault("timeout:1")
+    try:
+        res, stats, captured = B.serve_replay([scene], reqs, radius=0.03, k_max=16, k=1, point_queries=_dev(pq),
+                                              slots=1, realtime=False, max_batch_queries=2000)
+    finally:
+        m.set_fault("")
+    first = res["batch"] == res["batch"].min()
+    assert stats["batches"] >= 3 and first.any() and not first.all()
+    assert np.all(res["status"][first] == B.STATUS_TIMEOUT)
+    assert np.all(res["status"][~first] == 0)
+    for i in range(len(reqs)):
+        _check_frn_request(captured, reqs, i, pq, sdict, 0.03, 16)
+
+
+def test_forced_allocation_failure_leaves_the_library_usable(gpu_ready):
+    """An allocation failure (forced: "alloc") surfaces as an error of that
+    call; the next call gives exact rows."""
+    from rtdl_amd import mi355x_runtime as m
+
+    _say("case: forced allocation failure, then a normal call")
+    rng = np.random.default_rng(85)
+    q = cases.points(rng, 3000, 3)
+    s = cases.points(rng, 5000, 3)
+    m.set_fault("alloc")
+    try:
+        with pytest.raises(RuntimeError, match="injected fault"):
+            m.knn_rows_mi355x(q, s, k=4)
+    finally:
+        m.set_fault("")
+    got = m.knn_rows_mi355x(q, s, k=4)
+    want = oracle.neighbors("knn", q, s, k=4)
+    assert np.array_equal(got["neighbor_id"], want["nid"]) and np.array_equal(got["distance"], want["dist"])

@@ -96,3 +96,25 @@ def test_reference_shaped_prepared_calls_validate_before_the_device():
     for name in ("rtdl_mi355x_destroy_prepared_fixed_radius_neighbors_3d",
                  "rtdl_mi355x_destroy_prepared_ray_hitcount_3d", "rtdl_mi355x_destroy_prepared_ray_anyhit_2d"):
         getattr(lib, name)(None)  # a null handle is ignored, as delete nullptr is
+
+
+def test_fault_hook_parses_specs_and_fails_an_allocation_before_the_device():
+    """RTDL_MI355X_FAULT / rtdl_mi355x_set_fault: a bad spec is an error; an
+    armed "alloc" fault fails the next device allocation on the host, once,
+    with a message that names the hook; "" disarms."""
+    lib = m._library()
+    for bad in ("bogus", "alloc:0", "launch:x", "timeout:"):
+        with pytest.raises(ValueError, match="fault spec"):
+            m.set_fault(bad)
+    m.set_fault("alloc:2")
+    ptr = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(512)
+    try:
+        st = lib.rtdl_mi355x_device_alloc(64, ctypes.byref(ptr), err, 512)
+        first = err.value.decode()
+        assert "injected fault" not in first  # the first allocation is not the armed one
+        st = lib.rtdl_mi355x_device_alloc(64, ctypes.byref(ptr), err, 512)
+        assert st != 0 and ptr.value is None
+        assert "injected fault (RTDL_MI355X_FAULT=alloc)" in err.value.decode()
+    finally:
+        m.set_fault("")
