@@ -577,6 +577,8 @@ int rtdl_mi355x_serve_replay(void* const* scenes, size_t scene_count,
  * current device; copies are synchronous with respect to the host and to
  * every batch the library has completed. */
 int rtdl_mi355x_device_alloc(size_t bytes, void** ptr_out, char* error_out, size_t error_size);
+/* the calling thread's current HIP device (the device of its allocations) */
+int rtdl_mi355x_get_device(int* device_out, char* error_out, size_t error_size);
 int rtdl_mi355x_device_free(void* ptr, char* error_out, size_t error_size);
 int rtdl_mi355x_copy_to_device(void* dst, const void* src, size_t bytes, char* error_out,
                                size_t error_size);

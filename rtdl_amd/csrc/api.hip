@@ -849,6 +849,13 @@ int rtdl_mi355x_selftest_fp64(const double* a, const double* b, size_t n, double
   });
 }
 
+int rtdl_mi355x_get_device(int* device_out, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(device_out != nullptr, "device_out must not be null");
+    RTDL_HIP(hipGetDevice(device_out));
+  });
+}
+
 int rtdl_mi355x_device_alloc(size_t bytes, void** ptr_out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
     require(ptr_out != nullptr, "ptr_out must not be null");

@@ -39,6 +39,9 @@ class DeviceBuffer:
         _check(lib.rtdl_mi355x_device_alloc(self.nbytes, ctypes.byref(ptr), err, 1024), err,
                "rtdl_mi355x_device_alloc")
         self._ptr = ptr.value
+        from .rows import current_device
+
+        self.device = current_device()
 
     @classmethod
     def from_numpy(cls, array, dtype=None) -> "DeviceBuffer":
@@ -107,6 +110,17 @@ class DeviceBuffer:
 
             print(f"rtdl_amd.device: {exc}", file=sys.stderr)
 
+    # DLPack: zero-copy hand-off to other frameworks (torch.from_dlpack, ...)
+    def __dlpack__(self, stream=None):
+        from .rows import to_dlpack
+
+        return to_dlpack(self)
+
+    def __dlpack_device__(self):
+        from .rows import dlpack_device
+
+        return dlpack_device(self)
+
     def __getitem__(self, rows) -> "DeviceView":
         """A view of a contiguous range of rows (first axis): buf[a:b]."""
         if not isinstance(rows, slice) or rows.step not in (None, 1):
@@ -126,6 +140,17 @@ class DeviceView:
         self._ptr = ptr
         self.shape = shape
         self.dtype = base.dtype
+        self.device = getattr(base, "device", 0)
+
+    def __dlpack__(self, stream=None):
+        from .rows import to_dlpack
+
+        return to_dlpack(self)
+
+    def __dlpack_device__(self):
+        from .rows import dlpack_device
+
+        return dlpack_device(self)
 
     def data_ptr(self) -> int:
         return self._ptr
@@ -155,7 +180,9 @@ class DeviceView:
         start, stop, _ = rows.indices(self.shape[0])
         stop = max(start, stop)
         row_bytes = self.nbytes // max(1, self.shape[0])
-        return DeviceView(self.base, self._ptr + start * row_bytes, (stop - start, *self.shape[1:]))
+        view = DeviceView(self.base, self._ptr + start * row_bytes, (stop - start, *self.shape[1:]))
+        view.dtype = self.dtype
+        return view
 
 
 def to_device(array, dtype=None) -> DeviceBuffer:

@@ -161,6 +161,7 @@ _SIGNATURES = {
     "rtdl_mi355x_destroy_prepared": ([_VP], None),
     "rtdl_mi355x_set_walk_counting": ([ctypes.c_uint32, *_ERR], ctypes.c_int),
     "rtdl_mi355x_set_fault": ([ctypes.c_char_p, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_get_device": ([ctypes.POINTER(ctypes.c_int), *_ERR], ctypes.c_int),
     "rtdl_mi355x_get_last_walk_stats": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64), *_ERR],
                                         ctypes.c_int),
     # reference-shaped prepared handles (the HIPRT parameter lists)

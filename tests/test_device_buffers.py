@@ -40,6 +40,10 @@ class _FakeLib:
         ctypes.memset(dst, 0, nbytes)
         return 0
 
+    def rtdl_mi355x_get_device(self, device_ref, err, size):
+        device_ref._obj.value = 0
+        return 0
+
 
 @pytest.fixture
 def fake(monkeypatch):
@@ -79,3 +83,33 @@ def test_view_slicing_rejects_steps(fake):
     buf = dev.DeviceBuffer((10, 2), np.float32)
     with pytest.raises(TypeError):
         buf[0:10:2]
+
+
+def test_dlpack_capsule_describes_the_rows_in_place(fake):
+    """A DeviceBuffer / row view exports a "dltensor" capsule: ROCm device,
+    the buffer's own address (no copy), its shape and a compact layout; the
+    capsule keeps the buffer alive until it is released."""
+    from rtdl_amd import rows
+
+    buf = dev.DeviceBuffer((100, 24), np.uint8)
+    view = buf[10:30]
+    assert view.__dlpack_device__() == (rows.KDL_ROCM, 0)
+    cap = view.__dlpack__()
+    ptr = ctypes.pythonapi.PyCapsule_GetPointer
+    ptr.restype = ctypes.c_void_p
+    ptr.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    managed = ctypes.cast(ptr(cap, b"dltensor"), ctypes.POINTER(rows._DLManagedTensor)).contents
+    t = managed.dl_tensor
+    assert t.data == buf.ptr + 10 * 24
+    assert (t.device.device_type, t.device.device_id) == (rows.KDL_ROCM, 0)
+    assert t.ndim == 2 and [t.shape[0], t.shape[1]] == [20, 24]
+    assert (t.dtype.code, t.dtype.bits, t.dtype.lanes) == (1, 8, 1)
+    assert not t.strides and t.byte_offset == 0
+    n_live = len(rows._LIVE)
+    del view, buf
+    gc.collect()
+    assert fake.freed == []  # the capsule still holds the buffer
+    del cap, managed, t
+    gc.collect()
+    assert len(rows._LIVE) == n_live - 1
+    assert len(fake.freed) == 1
