@@ -54,6 +54,7 @@ CHECK_PER_REQUEST = 64      # sampled queries checked per captured request
 RADIUS = 0.01
 K_MAX = 64
 K_KNN = 8
+FRN_LEAF = 4   # points per leaf of the fixed-radius scenes (KNN scenes keep the default 2)
 TMAX = 0.5
 HBM_PEAK = 8.0e12
 KINDS = ("frn", "knn", "ray", "pip")
@@ -115,7 +116,7 @@ def make_scenes(B, rng, max_prims: float, n_scenes: int = 16):
         t0 = time.perf_counter()
         if kind in ("frn", "knn"):
             dev = DeviceBuffer.from_numpy(host)
-            handle = B.prepare_point_instances(dev, [0, n])
+            handle = B.prepare_point_instances(dev, [0, n], leaf_size=FRN_LEAF if kind == "frn" else 0)
         elif kind == "ray":
             dev = DeviceBuffer.from_numpy(host)
             handle = B.prepare_triangle_instances(dev, [0, n])
@@ -232,7 +233,7 @@ def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int, kinds=r
         fname = {0: "neighbors", 1: "neighbors", 2: "ray_any_hit", 3: "pip"}[k]
         t = fam[fname]
         rows = sum(r["rows"] for r in res)
-        nodes, prims = visits
+        nodes, prims, steps = visits
         used = {int(stream["scene"][i]) for i in idx[:len(jobs)]}
         if k == 3:
             # polygon refs (16 B) plus 8 B per vertex, at the mean vertex count
@@ -251,13 +252,16 @@ def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int, kinds=r
             "kernel": {0: "nbr_count/emit/sort_kernel", 1: f"batch_points_kernel<{K_KNN}>",
                        2: "batch_rays_kernel<0>", 3: "batch_pip_kernel"}[k],
             "queries": q_total, "jobs": len(jobs), "kernel_s": t, "node_visits": nodes,
-            "prim_tests": prims, "rows": rows, "algorithmic_bytes": w, "w_min_bytes": w_min,
+            "prim_tests": prims, "wave_steps": steps,
+            "lane_efficiency": nodes / (64.0 * steps) if steps else 0.0,
+            "rows": rows, "algorithmic_bytes": w, "w_min_bytes": w_min,
             "prim_record_bytes": pb,
             "achieved_GBps": w / t / 1e9 if t > 0 else 0.0,
             "queries_per_s": q_total / t if t > 0 else 0.0,
         }
         log(f"kernel {KINDS[k]}: {q_total} q in {t*1e3:.2f} ms ({q_total/t/1e6:.1f} Mq/s), "
-            f"{nodes/q_total:.1f} nodes/q, {prims/q_total:.1f} prims/q, W/t = {w/t/1e9:.0f} GB/s")
+            f"{nodes/q_total:.1f} nodes/q, {prims/q_total:.1f} prims/q, "
+            f"lane efficiency {nodes / (64.0 * max(steps, 1)):.2f}, W/t = {w/t/1e9:.0f} GB/s")
     return out
 
 
@@ -997,11 +1001,11 @@ def config3_leg(B, autotune: bool, seed: int = 20261203 + 3, jobs_checked: int =
         w_frn, w_pip = 0.0, 0.0
         for sub, sp in ((frn2, 8), (frn3, 12)):
             if sub:
-                _, _, (v, t) = B.run_batch_profiled(sub, count_visits=True)
+                _, _, (v, t, _) = B.run_batch_profiled(sub, count_visits=True)
                 q = sum(int(j.queries.shape[0]) for j in sub)
                 w_frn += q * sp + 32 * v + sp * t
         w_frn += 16 * rows_frn
-        _, _, (v_pip, t_pip) = B.run_batch_profiled(jobs[n_frn:], count_visits=True)
+        _, _, (v_pip, t_pip, _) = B.run_batch_profiled(jobs[n_frn:], count_visits=True)
         mean_v = float(ws["vcnt"].mean())
         w_pip = q_pip * 8 + 32 * v_pip + (16 + 8 * mean_v) * t_pip + 8 * rows_pip
         checked = config3_check(B, ws, jobs, res, rng, jobs_checked, per_job)
@@ -1059,6 +1063,8 @@ def main() -> int:
     ap.add_argument("--per-call-only", action="store_true",
                     help="run only the per-call 2M x 1M ray any-hit and 4M x 4M KNN k=16 calls")
     ap.add_argument("--config3-only", action="store_true", help="run only the 2,048-instance batch")
+    ap.add_argument("--fp64-calibration-mib", type=int, default=0,
+                    help="run only fp64_selftest_kernel over two inputs of this many MiB (counter calibration)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -1083,6 +1089,16 @@ def main() -> int:
         dst.copy_from(src)  # first touch
         dst.copy_from(src)  # the dispatch the counters are read from
         print(json.dumps({"copy_bytes": n}), flush=True)
+        return 0
+    if args.fp64_calibration_mib:
+        # a second counter calibration with a kernel of this library:
+        # fp64_selftest_kernel streams two FP64 inputs and writes two outputs
+        # of the same size (2 x n x 8 bytes each way)
+        n = (args.fp64_calibration_mib << 20) // 8
+        a = np.random.default_rng(1).random(n) + 1.0
+        for _ in range(2):  # the second dispatch is the one read
+            rtm.selftest_fp64(a, a)
+        print(json.dumps({"fp64_selftest_read_bytes": 16 * n, "fp64_selftest_write_bytes": 16 * n}), flush=True)
         return 0
 
     info = rtm.mi355x_context_probe()

@@ -136,8 +136,11 @@ int rtdl_mi355x_get_last_phase_timings(double* seconds_out, size_t capacity);
  * get_last_walk_stats gives the last call's traversal kernel time (HIP
  * events on the call's stream: the neighbour walk with its exact re-run for
  * k <= 64, the ray walk, the PIP counting walk) and, while counting is on,
- * visits[0] node records fetched and visits[1] primitive records tested
- * (counting kernels are slower: time and count in separate calls). */
+ * visits[0] node records fetched, visits[1] primitive records tested and
+ * visits[2] wavefront walk steps (per wavefront and work item, the node
+ * visits of its busiest lane, summed; visits[0] / (64 * visits[2]) is the
+ * walk's lane efficiency).  visits has 3 entries.  Counting kernels are
+ * slower: time and count in separate calls. */
 int rtdl_mi355x_set_walk_counting(uint32_t on, char* error_out, size_t error_size);
 
 /* Arms a host-forced fault for testing failure handling (no counterpart in
@@ -431,9 +434,10 @@ int rtdl_mi355x_run_batch_noalloc(const RtdlBatchJob* jobs, size_t job_count, ui
 
 /* rtdl_mi355x_run_batch with measurement: family_seconds (5 entries:
  * neighbour, ray any-hit, ray hit-count, ray closest-hit, PIP) receives each
- * family's kernel time from events on the launch stream; visits (2 entries,
- * optional, counting kernels) receives the node records fetched and the
- * primitive records tested by all traversals of the batch. */
+ * family's kernel time from events on the launch stream; visits (3 entries,
+ * optional, counting kernels) receives the node records fetched, the
+ * primitive records tested and the wavefront walk steps (as for
+ * rtdl_mi355x_get_last_walk_stats) of all traversals of the batch. */
 int rtdl_mi355x_run_batch_profiled(const RtdlBatchJob* jobs, size_t job_count,
                                    uint32_t morton_sort, RtdlBatchJobResult* results,
                                    double* family_seconds, uint64_t* visits, char* error_out,
@@ -499,6 +503,15 @@ int rtdl_mi355x_batch_graph_destroy(void* graph, char* error_out, size_t error_s
 int rtdl_mi355x_prepare_point_instances(const float* coords, size_t point_count,
                                         const uint64_t* instance_offsets, size_t instance_count,
                                         void** prepared_out, char* error_out, size_t error_size);
+/* The same with the points per leaf of this scene's trees (1-8; 0 takes the
+ * process tuning): a scene queried by fixed-radius jobs walks fewer nodes
+ * and fetches fewer bytes with larger leaves, while KNN walks prefer small
+ * ones, so each scene can be built for the jobs it serves (no counterpart in
+ * the reference). */
+int rtdl_mi355x_prepare_point_instances_leaf(const float* coords, size_t point_count,
+                                             const uint64_t* instance_offsets, size_t instance_count,
+                                             uint32_t leaf_size, void** prepared_out, char* error_out,
+                                             size_t error_size);
 /* coords: 9 FP32 per triangle (three vertices); ids are instance-local. */
 int rtdl_mi355x_prepare_triangle_instances(const float* coords, size_t triangle_count,
                                            const uint64_t* instance_offsets, size_t instance_count,

@@ -144,16 +144,19 @@ class Instances:
             pass
 
 
-def prepare_point_instances(coords, instance_offsets) -> Instances:
-    """coords: float32 device tensor (n, 3) (z = 0 for 2-D instances)."""
+def prepare_point_instances(coords, instance_offsets, *, leaf_size: int = 0) -> Instances:
+    """coords: float32 device tensor (n, 3) (z = 0 for 2-D instances).
+    `leaf_size`: points per leaf of this scene's trees (1-8), 0 for the
+    process tuning (autotune.Tuning.point_leaf)."""
     offs = np.ascontiguousarray(instance_offsets, dtype=np.uint64)
     assert coords.shape[-1] == 3
     _sync_foreign()
     handle = ctypes.c_void_p()
     err = ctypes.create_string_buffer(4096)
-    st = _lib().rtdl_mi355x_prepare_point_instances(
-        _ptr(coords), int(coords.shape[0]), offs.ctypes.data, len(offs) - 1, ctypes.byref(handle), err, 4096)
-    _check(st, err, "rtdl_mi355x_prepare_point_instances")
+    st = _lib().rtdl_mi355x_prepare_point_instances_leaf(
+        _ptr(coords), int(coords.shape[0]), offs.ctypes.data, len(offs) - 1, int(leaf_size),
+        ctypes.byref(handle), err, 4096)
+    _check(st, err, "rtdl_mi355x_prepare_point_instances_leaf")
     return Instances(handle, offs, "points")
 
 
@@ -332,10 +335,12 @@ FAMILIES = ("neighbors", "ray_any_hit", "ray_hit_count", "ray_closest_hit", "pip
 
 def run_batch_profiled(jobs: list[BatchJob], *, morton_sort: bool = True, count_visits: bool = False):
     """Like run_batch, returning per-family kernel seconds and, when
-    `count_visits`, the (node records, primitive records) visited."""
+    `count_visits`, the (node records, primitive records, wavefront walk
+    steps) visited; node records / (64 * walk steps) is the walks' lane
+    efficiency."""
     arr, res = _pack_jobs(jobs)
     fam = (ctypes.c_double * 5)()
-    visits = (ctypes.c_uint64 * 2)()
+    visits = (ctypes.c_uint64 * 3)()
     err = ctypes.create_string_buffer(4096)
     _sync_foreign()
     st = _lib().rtdl_mi355x_run_batch_profiled(ctypes.addressof(arr), len(jobs), int(bool(morton_sort)),
@@ -345,7 +350,7 @@ def run_batch_profiled(jobs: list[BatchJob], *, morton_sort: bool = True, count_
     out = [{"rows": int(res[i].rows), "status": int(res[i].status), "ambiguous": int(res[i].ambiguous)}
            for i in range(len(jobs))]
     seconds = {FAMILIES[f]: fam[f] for f in range(5)}
-    return out, seconds, ((int(visits[0]), int(visits[1])) if count_visits else None)
+    return out, seconds, (tuple(int(v) for v in visits) if count_visits else None)
 
 
 def job_rows(job: BatchJob, result: dict) -> np.ndarray:

@@ -120,6 +120,7 @@ int rtdl_mi355x_get_last_walk_stats(double* kernel_seconds, uint64_t* visits, ch
     if (visits) {
       visits[0] = w.nodes;
       visits[1] = w.prims;
+      visits[2] = w.steps;
     }
   });
 }
@@ -543,10 +544,19 @@ void rtdl_mi355x_destroy_prepared_ray_anyhit_2d(void* prepared) { rtdl_mi355x_de
 int rtdl_mi355x_prepare_point_instances(const float* coords, size_t point_count,
                                         const uint64_t* instance_offsets, size_t instance_count,
                                         void** prepared_out, char* error_out, size_t error_size) {
+  return rtdl_mi355x_prepare_point_instances_leaf(coords, point_count, instance_offsets, instance_count, 0u,
+                                                  prepared_out, error_out, error_size);
+}
+
+int rtdl_mi355x_prepare_point_instances_leaf(const float* coords, size_t point_count,
+                                             const uint64_t* instance_offsets, size_t instance_count,
+                                             uint32_t leaf_size, void** prepared_out, char* error_out,
+                                             size_t error_size) {
   return guarded(error_out, error_size, [&] {
     require(instance_offsets != nullptr && instance_count >= 1, "instance offsets are required");
     std::vector<uint64_t> offs(instance_offsets, instance_offsets + instance_count + 1);
-    make_prepared(prepared_out, [&](Prepared& p) { prepare_point_instances(p, coords, point_count, offs); });
+    make_prepared(prepared_out,
+                  [&](Prepared& p) { prepare_point_instances(p, coords, point_count, leaf_size, offs); });
   });
 }
 
@@ -808,7 +818,7 @@ static void run_batch_impl(const RtdlBatchJob* jobs, size_t job_count, uint32_t 
     if (seconds_out) *seconds_out = ms * 1e-3;
     if (family_seconds) engine->family_seconds(family_seconds);
     if (visits) {
-      RTDL_HIP(hipMemcpy(visits, engine->device_stats(), 2 * sizeof(unsigned long long),
+      RTDL_HIP(hipMemcpy(visits, engine->device_stats(), 3 * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost));
     }
     const JobCounters* c = engine->counters();

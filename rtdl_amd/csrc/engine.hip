@@ -211,6 +211,7 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
         }
       }
     }
+    if (STATS) close_item(st);
   }
   if (STATS) flush_lane_stats(stats, st);
 }
@@ -329,10 +330,24 @@ struct NbrCount {
   uint32_t count;    // neighbours counted in buckets 0..top
   uint32_t bucket;   // more than k: the bucket holding the k-th distance
   uint32_t n_cand;   // more than k: neighbours in buckets 0..bucket
+  uint32_t n_staged; // slots in the stage (a superset of the rows) when staged
   bool heavy;        // more than k neighbours: keep the k nearest
   bool refine;       // leave the query to the exact fix-up
-  bool staged;       // the last round's `count` slots are in the stage
+  bool staged;       // the last round's stage holds every row's slot
 };
+
+// A query's stage cursor once its stage could not keep every row: further
+// slots are dropped and the emit walk writes the rows instead.
+constexpr uint32_t kStageClosed = 0x80000000u;
+
+// Stage entries of scenes with at most 2^27 primitives carry the slot's
+// distance bucket in their top 5 bits, so compaction and the sort filter
+// without fetching the point again.
+constexpr uint32_t kStageSlotBits = 27;
+constexpr uint32_t kStageSlotMask = (1u << kStageSlotBits) - 1u;
+static_assert(kNbrBuckets <= (1u << (32 - kStageSlotBits)), "bucket must fit the stage entry");
+
+__device__ inline bool stage_packs(const SceneView& sc) { return sc.n_prims <= (1u << kStageSlotBits); }
 
 // The count walk with the FP64 visit taken out of the lanes' walks.  Each
 // lane walks its query's ball (one node step per lane per iteration) and only
@@ -352,6 +367,11 @@ struct NbrCount {
 // are exact as before.
 // Drains run after each point index of the leaves being prefiltered, so the
 // queue never holds more than 63 + 64 entries.
+// A drain adds at most 64 slots to one stage, so after each drain a stage
+// holding more than stage_slots - 64 is compacted in place by the wavefront:
+// only slots whose bucket is within the owner's current cutoff (top) stay,
+// as the others can no longer be rows (top only decreases within a round).
+// A stage that compaction cannot bring under that mark is closed.
 constexpr uint32_t kNbrQueueCap = 128;
 
 __device__ inline double shfl_f64(double x, uint32_t src) {
@@ -361,19 +381,32 @@ __device__ inline double shfl_f64(double x, uint32_t src) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// Lane q's value (q wave-uniform).
+__device__ inline uint64_t readlane64(uint64_t x, uint32_t q) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, q);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), q);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline double readlane_f64(double x, uint32_t q) {
+  return __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, x), q));
+}
+
 __device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 template <bool STATS>
 __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
     uint32_t* item_counter, NbrPlan* __restrict__ plans, unsigned long long* cand_count,
-    uint64_t cand_cap, uint32_t* __restrict__ stage, uint32_t stage_slots, uint4* amb, uint32_t* amb_count,
-    unsigned long long* stats) {
+    uint64_t cand_cap, uint32_t* __restrict__ stage, uint32_t stage_slots, uint32_t* __restrict__ emit_list,
+    uint32_t* emit_count, uint4* amb, uint32_t* amb_count, unsigned long long* stats) {
   // buckets 2w and 2w + 1 of thread t in the low and high half of hist[w * 256 + t]
   __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
   __shared__ uint32_t q_slot[4 * kNbrQueueCap];
   __shared__ uint8_t q_owner[4 * kNbrQueueCap];
-  __shared__ uint32_t l_count[256];  // neighbours counted this round, per thread
+  // per thread: neighbours counted this round (high word) and stage cursor
+  // (low word), updated together by one LDS atomic
+  __shared__ unsigned long long l_cs[256];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
   uint32_t* my_hist = hist + threadIdx.x;
@@ -400,6 +433,7 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
     }
     const float qf[3] = {(float)qx, (float)qy, (float)qz};
     const float m = coord_bound(scene_m(sc), qx, qy, qz);
+    const bool pack = stage_packs(sc);
     uint32_t* const stage_it = stage + (size_t)it * 64 * stage_slots;
     NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false, false};  // reference.py:357
     bool going = active;  // counts in the current round
@@ -410,7 +444,7 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
       bool shrunk = false;
       if (going) {
         for (uint32_t w2 = 0; w2 < kNbrBuckets / 2; ++w2) my_hist[256 * w2] = 0u;
-        l_count[threadIdx.x] = 0u;
+        l_cs[threadIdx.x] = 0ull;
       }
       uint32_t node = sc.root;
       bool walking = going && sc.root != kEndNode;
@@ -452,9 +486,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
               if (b <= otop) {  // not beyond the owner's k nearest
                 __hip_atomic_fetch_add(&hist[256 * (b >> 1) + wbase + o], 1u << (16 * (b & 1)), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                const uint32_t at = __hip_atomic_fetch_add(&l_count[wbase + o], 1u, __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (at < stage_slots) stage_it[(size_t)o * stage_slots + at] = slot;
+                const uint32_t at = (uint32_t)__hip_atomic_fetch_add(&l_cs[wbase + o], (1ull << 32) | 1ull,
+                                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (at < stage_slots) stage_it[(size_t)o * stage_slots + at] = pack ? slot | (b << kStageSlotBits) : slot;
               }
             }
           }
@@ -462,7 +496,7 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
           size -= n;
           wave_fence();
           if (going) {
-            const uint32_t count = l_count[threadIdx.x];
+            const uint32_t count = (uint32_t)(l_cs[threadIdx.x] >> 32);
             if (count >= next_check) {
               next_check = count + step;
               uint32_t cum = 0;
@@ -477,6 +511,43 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
               }
             }
           }
+          // compaction of the stages past the mark, one owner at a time
+          const uint32_t cur = (uint32_t)l_cs[threadIdx.x];
+          uint64_t full = __ballot(going && cur > stage_slots - 64 && cur < kStageClosed);
+          while (full) {
+            const uint32_t o = (uint32_t)__builtin_ctzll(full);
+            full &= full - 1;
+            const double ox = readlane_f64(qx, o), oy = readlane_f64(qy, o), oz = readlane_f64(qz, o);
+            const double oscale = readlane_f64(scale, o);
+            const uint32_t otop = __builtin_amdgcn_readlane(top, o);
+            const uint32_t n_st = min(__builtin_amdgcn_readlane(cur, o), stage_slots);
+            uint32_t* const base = stage_it + (size_t)o * stage_slots;
+            uint32_t kept = 0;
+            for (uint32_t e0 = 0; e0 < n_st; e0 += 64) {
+              const uint32_t e = e0 + lane;
+              bool keep = false;
+              uint32_t v = 0;
+              if (e < n_st) {
+                v = base[e];
+                if (pack) {
+                  keep = (v >> kStageSlotBits) <= otop;
+                } else {
+                  const float4 p = load_prim(sc.prims, v);
+                  keep = nbr_bucket(sqrt(point_d2(sc, v, p, ox, oy, oz)), oscale) <= otop;
+                }
+              }
+              // every lane has read its slot before any is rewritten (the
+              // ballot needs them all); later chunks lie past every write
+              const uint64_t km = __ballot(keep);
+              if (keep) base[kept + (uint32_t)__popcll(km & ((1ull << lane) - 1ull))] = v;
+              kept += (uint32_t)__popcll(km);
+            }
+            if (lane == o) {
+              const unsigned long long v = l_cs[threadIdx.x];
+              l_cs[threadIdx.x] = (v & ~0xFFFFFFFFull) | (kept > stage_slots - 64 ? kStageClosed : kept);
+            }
+          }
+          wave_fence();
         };
         // FP32 prefilter of the leaf points; survivors are queued in (point
         // index within the leaf, lane) order, with drains between indices
@@ -506,13 +577,15 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
       }
       wave_fence();
       if (going) {
-        const uint32_t count = l_count[threadIdx.x];
+        const unsigned long long cs = l_cs[threadIdx.x];
+        const uint32_t count = (uint32_t)(cs >> 32);
         going = false;
         r.count = count;
         r.heavy = shrunk || count > k;
         r.bucket = top;
         r.n_cand = 0;
-        r.staged = count <= stage_slots;
+        r.n_staged = (uint32_t)cs;
+        r.staged = r.n_staged <= stage_slots;
         if (r.heavy && count > kNbrCountMax) {
           r.refine = true;
         } else if (r.heavy) {
@@ -559,7 +632,15 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
     }
     if (active)
       plans[(size_t)it * 64 + lane] = NbrPlan{row_at,   cand_at, r.radius, r.radius_sq, refine ? 0u : n_rows,
-                                              r.heavy ? r.n_cand : 0u, r.bucket, qi, staged ? r.count : 0u, 0u};
+                                              r.heavy ? r.n_cand : 0u, r.bucket, qi, staged ? r.n_staged : 0u, 0u};
+    // queries whose rows the emit walk writes (not staged) join the emit list
+    const bool emit = n_rows && !refine && !staged;
+    const uint64_t em = __ballot(emit);
+    if (em) {
+      const uint32_t base = lane0_add(emit_count, (uint32_t)__popcll(em));
+      if (emit) emit_list[base + (uint32_t)__popcll(em & ((1ull << lane) - 1ull))] = it * 64 + lane;
+    }
+    if (STATS) close_item(st);
   }
   if (STATS) flush_lane_stats(stats, st);
 }
@@ -594,36 +675,27 @@ __device__ inline void nbr_emit_query(const DeviceJob& j, const NbrPlan& pl, Nbr
                      });
 }
 
+// Walks the queries of the emit list, 64 at a time (one per lane, any mix of
+// jobs), so a few unstaged queries no longer hold a wavefront per work item.
 __global__ __launch_bounds__(256) void nbr_emit_kernel(const DeviceJob* __restrict__ jobs,
                                                        const WorkItem* __restrict__ items,
-                                                       uint32_t n_items, uint32_t* item_counter,
                                                        const NbrPlan* __restrict__ plans,
-                                                       NbrCand* __restrict__ cands) {
+                                                       NbrCand* __restrict__ cands,
+                                                       const uint32_t* __restrict__ emit_list,
+                                                       const uint32_t* emit_count, uint32_t* chunk_counter) {
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t n = __builtin_amdgcn_readfirstlane(*emit_count);
   LaneStats st;
-  ItemQueue queue(item_counter, n_items);
   for (;;) {
-    const uint32_t it = queue.next();
-    if (it == kItemDone) break;
-    if (it == kItemRetry) continue;
-    const WorkItem w = items[it];
-    const DeviceJob& j = jobs[w.job];
-    const NbrPlan pl = lane < w.count ? plans[(size_t)it * 64 + lane] : NbrPlan{};
-    // every lane returns to the item fetch together; lanes without rows (or
-    // with staged rows) idle
-    if (pl.n_rows && !pl.staged) nbr_emit_query(j, pl, cands, st);
+    const uint32_t chunk = lane0_add(chunk_counter, 1u);
+    if ((uint64_t)chunk * 64 >= n) break;
+    const uint32_t e = chunk * 64 + lane;
+    if (e < n) {
+      const uint32_t p = emit_list[e];
+      const NbrPlan pl = plans[p];
+      nbr_emit_query(jobs[items[p >> 6].job], pl, cands, st);
+    }
   }
-}
-
-// Lane q's value (q wave-uniform).
-__device__ inline uint64_t readlane64(uint64_t x, uint32_t q) {
-  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, q);
-  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), q);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ inline double readlane_f64(double x, uint32_t q) {
-  return __builtin_bit_cast(double, readlane64(__builtin_bit_cast(uint64_t, x), q));
 }
 
 __device__ inline uint64_t shfl_xor64(uint64_t x, uint32_t m) {
@@ -683,8 +755,8 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
                                                              const NbrCand* __restrict__ cands,
                                                              const uint32_t* __restrict__ stage,
                                                              uint32_t stage_slots) {
-  __shared__ uint64_t pack_d[kNbrCandMax];
-  __shared__ uint32_t pack_i[kNbrCandMax];
+  // (distance bits low, high, id, 0) of the query being sorted
+  __shared__ uint4 pack[kNbrCandMax];
   const uint32_t lane = threadIdx.x;
   ItemQueue queue(item_counter, n_items);
   for (;;) {
@@ -720,6 +792,7 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
         const double qx = readlane_f64(mqx, q), qy = readlane_f64(mqy, q), qz = readlane_f64(mqz, q);
         const double scale = (double)kNbrBuckets / pl.radius;
         const uint32_t* sq = stage + ((size_t)it * 64 + q) * stage_slots;
+        const bool packed = stage_packs(j.scene);
         uint32_t kept = 0;
         for (uint32_t e0 = 0; e0 < pl.staged; e0 += 64) {
           const uint32_t e = e0 + lane;
@@ -727,33 +800,35 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
           uint64_t d = 0;
           uint32_t id = 0;
           if (e < pl.staged) {
-            const uint32_t slot = sq[e];
-            const float4 p = load_prim(j.scene.prims, slot);
-            const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
-            keep = pl.n_cand == 0 || nbr_bucket(dist, scale) <= pl.bucket;
-            d = __builtin_bit_cast(uint64_t, dist);
-            id = __float_as_uint(p.w);
+            const uint32_t v = sq[e];
+            const uint32_t slot = packed ? v & kStageSlotMask : v;
+            // a packed bucket past the k-th one is dropped without a fetch
+            if (!packed || pl.n_cand == 0 || (v >> kStageSlotBits) <= pl.bucket) {
+              const float4 p = load_prim(j.scene.prims, slot);
+              const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
+              keep = pl.n_cand == 0 || nbr_bucket(dist, scale) <= pl.bucket;
+              d = __builtin_bit_cast(uint64_t, dist);
+              id = __float_as_uint(p.w);
+            }
           }
           const unsigned long long mask = __ballot(keep);
           const uint32_t at = kept + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-          if (keep && at < kNbrCandMax) {
-            pack_d[at] = d;
-            pack_i[at] = id;
-          }
+          if (keep && at < kNbrCandMax) pack[at] = make_uint4((uint32_t)d, (uint32_t)(d >> 32), id, 0u);
           kept += (uint32_t)__popcll(mask);
         }
         __syncthreads();
         if (kept > kNbrCandMax && lane == 0) atomicOr(&j.counters->status, kStatusInternal);
         n = kept < kNbrCandMax ? kept : kNbrCandMax;
         if (lane < n) {
-          d0 = pack_d[lane];
-          i0 = pack_i[lane];
+          const uint4 e = pack[lane];
+          d0 = ((uint64_t)e.y << 32) | e.x;
+          i0 = e.z;
         }
         if (lane + 64 < n) {
-          d1 = pack_d[lane + 64];
-          i1 = pack_i[lane + 64];
+          const uint4 e = pack[lane + 64];
+          d1 = ((uint64_t)e.y << 32) | e.x;
+          i1 = e.z;
         }
-        __syncthreads();
       } else if (pl.n_cand == 0) {  // rows written in walk order, sorted in place
         n = pl.n_rows;
         if (lane < n) {
@@ -774,9 +849,27 @@ __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* _
           i1 = c.id;
         }
       }
-      wave_sort_dist_id(d0, i0, d1, i1, n);
-      if (lane < pl.n_rows)
-        write_neighbor_row(j, pl.row_at + lane, pl.qi, i0, __builtin_bit_cast(double, d0), lane + 1);
+      if (n <= 64) {
+        // rank sort: element `lane` goes to the number of elements before it
+        // in (distance, id) order (the pairs are distinct: ids are), found
+        // by reading every element from LDS as a broadcast
+        if (!pl.staged && lane < n) pack[lane] = make_uint4((uint32_t)d0, (uint32_t)(d0 >> 32), i0, 0u);
+        __syncthreads();
+        uint32_t rank = 0;
+#pragma unroll 4
+        for (uint32_t x = 0; x < n; ++x) {
+          const uint4 e = pack[x];
+          rank += dist_id_less(((uint64_t)e.y << 32) | e.x, e.z, d0, i0) ? 1u : 0u;
+        }
+        __syncthreads();  // every lane has read `pack` before the next query writes it
+        if (lane < n && rank < pl.n_rows)
+          write_neighbor_row(j, pl.row_at + rank, pl.qi, i0, __builtin_bit_cast(double, d0), rank + 1);
+      } else {
+        __syncthreads();
+        wave_sort_dist_id(d0, i0, d1, i1, n);
+        if (lane < pl.n_rows)
+          write_neighbor_row(j, pl.row_at + lane, pl.qi, i0, __builtin_bit_cast(double, d0), lane + 1);
+      }
     }
   }
 }
@@ -818,6 +911,7 @@ __global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __rest
           static_cast<ClosestRowRec*>(j.rows)[at] = ClosestRowRec{qi, res.best_id, res.best_t};
       }
     }
+    if (STATS) close_item(st);
   }
   if (STATS) flush_lane_stats(stats, st);
 }
@@ -883,6 +977,7 @@ __global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restr
         if (c > kPipHitCap) over[atomicAdd(over_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
       }
     }
+    if (STATS) close_item(st);
   }
   if (STATS) flush_lane_stats(stats, st);
 }
@@ -1046,10 +1141,10 @@ BatchEngine::BatchEngine(hipStream_t stream) : stream_(stream) {
   int dev = 0;
   RTDL_HIP(hipGetDevice(&dev));
   RTDL_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
-  stats_.resize(2);
+  stats_.resize(3);
   item_counters_.resize(kItemCounterSets * kXcds);
   amb_count_.resize(2);  // [0] neighbour queries for the exact fix-up, [1] PIP points past the hit cap
-  nbr_cand_count_.resize(1);
+  nbr_cand_count_.resize(2);  // [0] parking cursor, [1] (low word) emit list length
   for (int f = 0; f < kLaunchFamilies; ++f) {
     RTDL_HIP(hipEventCreate(&fam_start_[f]));
     RTDL_HIP(hipEventCreate(&fam_stop_[f]));
@@ -1061,10 +1156,10 @@ void BatchEngine::carve(Carver& dev, Carver& host, size_t max_jobs, size_t max_q
   max_jobs = std::max<size_t>(max_jobs, 1);
   max_queries = std::max<size_t>(max_queries, 1);
   const size_t max_items = max_queries / 64 + max_jobs;
-  dev.take(stats_, 2);
+  dev.take(stats_, 3);
   dev.take(item_counters_, kItemCounterSets * kXcds);
   dev.take(amb_count_, 2);
-  dev.take(nbr_cand_count_, 1);
+  dev.take(nbr_cand_count_, 2);
   dev.take(jobs_, max_jobs);
   dev.take(starts_, max_jobs + 1);
   dev.take(counters_, max_jobs);
@@ -1080,6 +1175,7 @@ void BatchEngine::carve(Carver& dev, Carver& host, size_t max_jobs, size_t max_q
   sort_ws_.carve(dev, std::max(max_queries, max_items));
   dev.take(nbr_cand_, std::max<size_t>(max_queries * kNbrCandPerQuery, 1u << 16));
   dev.take(nbr_stage_, max_items * 64 * kNbrStageMax);
+  dev.take(nbr_emit_, max_items * 64);
   host_jobs_ = host.take<DeviceJob>(max_jobs);
   host_starts_ = host.take<uint64_t>(max_jobs + 1);
   host_counters_ = host.take<JobCounters>(max_jobs);
@@ -1194,11 +1290,13 @@ void BatchEngine::reserve_nbr(const HostJob* jobs, size_t n) {
   const size_t want = (size_t)std::max<uint64_t>(nbr_queries * kNbrCandPerQuery, 1u << 16);
   const size_t stage = (size_t)nbr_items * 64 * kNbrStageMax;
   if (fixed_) {  // carved for max_queries / max_items, which reserve() checked
-    require(nbr_cand_.size() >= want && nbr_stage_.size() >= stage, "fixed engine: fixed-radius space");
+    require(nbr_cand_.size() >= want && nbr_stage_.size() >= stage && nbr_emit_.size() >= nbr_items * 64,
+            "fixed engine: fixed-radius space");
     return;
   }
   if (nbr_cand_.size() < want) nbr_cand_.resize(want);
   if (nbr_stage_.size() < stage) nbr_stage_.resize(stage);
+  if (nbr_emit_.size() < nbr_items * 64) nbr_emit_.resize(nbr_items * 64);
 }
 
 void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
@@ -1278,10 +1376,10 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
   const size_t zero_n = std::max<size_t>(n * 4, kItemCounterSets * kXcds + 1);
   zero_words_kernel<<<blocks_for(zero_n, 256), 256, 0, s>>>(
       reinterpret_cast<uint32_t*>(counters_.get()), n * 4, item_counters_.get(), kItemCounterSets * kXcds,
-      stats_.get(), 2);
+      stats_.get(), 3);
   RTDL_LAUNCH_CHECK();
   fill_u32(amb_count_.get(), 2, 0u, s);
-  fill_u32(reinterpret_cast<uint32_t*>(nbr_cand_count_.get()), 2, 0u, s);
+  fill_u32(reinterpret_cast<uint32_t*>(nbr_cand_count_.get()), 4, 0u, s);
   if (morton_sort && total > 1) {
     batch_keys_kernel<<<blocks_for(total, 256), 256, 0, s>>>(jobs_.get(), starts_.get(), (uint32_t)n,
                                                               total, keys_.get(), order_.get());
@@ -1354,23 +1452,25 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
         const uint32_t m = std::min<uint32_t>(kNbrChunkItems, n_items - c0);
         const WorkItem* ic = it + c0;
         const unsigned gm = grid_for(m);
-        require(nbr_stage_.size() >= (size_t)m * 64 * kNbrStageMax, "fixed-radius stage not reserved");
+        require(nbr_stage_.size() >= (size_t)m * 64 * kNbrStageMax && nbr_emit_.size() >= (size_t)m * 64,
+                "fixed-radius stage not reserved");
         if (c0) {
           fill_u32(ctr, kXcds, 0u, s);
           fill_u32(item_counters_.get() + kQueueNbrEmit * kXcds, 2 * kXcds, 0u, s);
-          fill_u32(reinterpret_cast<uint32_t*>(nbr_cand_count_.get()), 2, 0u, s);
+          fill_u32(reinterpret_cast<uint32_t*>(nbr_cand_count_.get()), 4, 0u, s);
         }
+        uint32_t* emit_count = reinterpret_cast<uint32_t*>(nbr_cand_count_.get() + 1);
         if (stats_on_)
           nbr_count_kernel<true><<<gm, 256, 0, s>>>(dj, ic, m, ctr, nbr_plan_.get(), nbr_cand_count_.get(),
                                                     nbr_cand_.size(), nbr_stage_.get(), kNbrStageMax,
-                                                    amb_.get(), amb_count_.get(), st);
+                                                    nbr_emit_.get(), emit_count, amb_.get(), amb_count_.get(), st);
         else
           nbr_count_kernel<false><<<gm, 256, 0, s>>>(dj, ic, m, ctr, nbr_plan_.get(), nbr_cand_count_.get(),
                                                      nbr_cand_.size(), nbr_stage_.get(), kNbrStageMax,
-                                                     amb_.get(), amb_count_.get(), st);
+                                                     nbr_emit_.get(), emit_count, amb_.get(), amb_count_.get(), st);
         RTDL_LAUNCH_CHECK();
-        nbr_emit_kernel<<<gm, 256, 0, s>>>(dj, ic, m, item_counters_.get() + kQueueNbrEmit * kXcds,
-                                           nbr_plan_.get(), nbr_cand_.get());
+        nbr_emit_kernel<<<gm, 256, 0, s>>>(dj, ic, nbr_plan_.get(), nbr_cand_.get(), nbr_emit_.get(), emit_count,
+                                           item_counters_.get() + kQueueNbrEmit * kXcds);
         RTDL_LAUNCH_CHECK();
         const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(m, (size_t)cus_ * 32));
         nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, ic, m, item_counters_.get() + kQueueNbrSort * kXcds,

@@ -222,6 +222,7 @@ class BatchEngine {
   DeviceArray<NbrPlan> nbr_plan_;        // cap_items_ * 64
   DeviceArray<NbrCand> nbr_cand_;        // parked candidates of a batch
   DeviceArray<uint32_t> nbr_stage_;      // kNbrStageMax slots per fixed-radius query of a sub-launch
+  DeviceArray<uint32_t> nbr_emit_;       // plan indices of the queries the emit walk writes
   DeviceArray<unsigned long long> nbr_cand_count_;
   DeviceArray<unsigned long long> stats_;
   SortWorkspace sort_ws_;

@@ -191,8 +191,8 @@ unsigned long long* walk_begin(hipStream_t s) {
   }
   unsigned long long* ctr = nullptr;
   if (w.stats.counting) {
-    if (!w.counters.size()) w.counters.resize(2);
-    fill_u32(reinterpret_cast<uint32_t*>(w.counters.get()), 4, 0u, s);
+    if (!w.counters.size()) w.counters.resize(3);
+    fill_u32(reinterpret_cast<uint32_t*>(w.counters.get()), 6, 0u, s);
     ctr = w.counters.get();
   }
   RTDL_HIP(hipEventRecord(w.start, s));
@@ -207,12 +207,13 @@ void walk_collect() {
   float ms = 0.0f;
   RTDL_HIP(hipEventElapsedTime(&ms, w.start, w.stop));
   w.stats.kernel_s = ms * 1e-3;
-  w.stats.nodes = w.stats.prims = 0;
+  w.stats.nodes = w.stats.prims = w.stats.steps = 0;
   if (w.stats.counting) {
-    unsigned long long v[2] = {0, 0};
+    unsigned long long v[3] = {0, 0, 0};
     RTDL_HIP(hipMemcpy(v, w.counters.get(), sizeof(v), hipMemcpyDeviceToHost));
     w.stats.nodes = v[0];
     w.stats.prims = v[1];
+    w.stats.steps = v[2];
   }
   w.stats.valid = true;
 }
@@ -588,9 +589,10 @@ void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* ver
   read_root_box(p);
 }
 
-void prepare_point_instances(Prepared& p, const float* coords, size_t n,
+void prepare_point_instances(Prepared& p, const float* coords, size_t n, uint32_t leaf_size,
                              const std::vector<uint64_t>& offsets) {
   require(n == 0 || coords != nullptr, "coords must not be null when count is nonzero");
+  require(leaf_size <= kMaxLeafSize, "leaf_size must be in [1, 8], or 0 for the tuned default");
   require(offsets.size() >= 2 && offsets.front() == 0 && offsets.back() == n,
           "instance offsets must start at 0 and end at the point count");
   hipStream_t s = p.stream;
@@ -600,7 +602,7 @@ void prepare_point_instances(Prepared& p, const float* coords, size_t n,
   DeviceArray<uint32_t> ids(n);
   widen_points(coords, n, d_offs.get(), (uint32_t)(offsets.size() - 1), c64.get(), ids.get(), s);
   build_scene(p.scene, PrimKind::Points, 3, n, c64.get(), ids.get(), nullptr, nullptr, 0, offsets,
-              current_tuning().point_leaf, s);
+              leaf_size ? leaf_size : current_tuning().point_leaf, s);
   read_root_box(p);
 }
 

@@ -39,7 +39,7 @@ struct WalkStats {
   bool counting = false;
   bool valid = false;
   double kernel_s = 0.0;
-  unsigned long long nodes = 0, prims = 0;
+  unsigned long long nodes = 0, prims = 0, steps = 0;
 };
 WalkStats& thread_walk_stats();
 
@@ -76,7 +76,8 @@ void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* ver
                       size_t n_doubles);
 
 // Multi-instance builders from device FP32 data (ids are instance-local).
-void prepare_point_instances(Prepared& p, const float* coords, size_t n,
+// leaf_size: points per leaf of the instance trees, 0 for the process tuning
+void prepare_point_instances(Prepared& p, const float* coords, size_t n, uint32_t leaf_size,
                              const std::vector<uint64_t>& offsets);
 void prepare_triangle_instances(Prepared& p, const float* coords, size_t n,
                                 const std::vector<uint64_t>& offsets);

@@ -16,9 +16,28 @@ namespace rtdl {
 struct LaneStats {
   unsigned long long nodes = 0;
   unsigned long long prims = 0;
+  unsigned long long mark = 0;   // nodes at the start of the current work item
+  unsigned long long steps = 0;  // lane 0: wavefront walk steps
 };
 
-__device__ inline void flush_lane_stats(unsigned long long* stats, const LaneStats& st) {
+// Ends a work item of the wavefront: its walk took at least as many lockstep
+// steps as its busiest lane visited nodes, so lane 0 adds that maximum.  The
+// ratio nodes / (64 * steps) is the walk's lane efficiency.  Every lane must
+// take part.
+__device__ inline void close_item(LaneStats& st) {
+  unsigned long long m = st.nodes - st.mark;
+  st.mark = st.nodes;
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(m, d, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) st.steps += m;
+}
+
+// Adds the wavefront's visits to stats[0..2] (nodes, primitives, walk steps).
+// Every lane must take part.
+__device__ inline void flush_lane_stats(unsigned long long* stats, LaneStats& st) {
+  close_item(st);
   unsigned long long nodes = st.nodes, prims = st.prims;
   for (int d = 32; d >= 1; d >>= 1) {
     nodes += __shfl_xor(nodes, d, 64);
@@ -27,6 +46,7 @@ __device__ inline void flush_lane_stats(unsigned long long* stats, const LaneSta
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(&stats[0], nodes);
     atomicAdd(&stats[1], prims);
+    atomicAdd(&stats[2], st.steps);
   }
 }
 

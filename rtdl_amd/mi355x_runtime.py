@@ -182,6 +182,8 @@ _SIGNATURES = {
     "rtdl_mi355x_run_batch_profiled": ([_VP, _SZ, ctypes.c_uint32, _VP, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_uint64), *_ERR], ctypes.c_int),
     "rtdl_mi355x_prepare_point_instances": ([_VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
+    "rtdl_mi355x_prepare_point_instances_leaf": ([_VP, _SZ, _VP, _SZ, ctypes.c_uint32, ctypes.POINTER(_VP), *_ERR],
+                                                 ctypes.c_int),
     "rtdl_mi355x_prepare_triangle_instances": ([_VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_prepare_polygon_instances": ([_VP, _VP, _SZ, _VP, _SZ, _VP, _SZ, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_serve_replay": ([_VP, _SZ, _VP, _SZ, _VP, _VP, _VP, *_ERR], ctypes.c_int),
@@ -273,13 +275,14 @@ def set_walk_counting(on: bool) -> None:
 
 def last_walk_stats() -> dict:
     """Traversal kernel seconds of this thread's last per-call traversal and,
-    when counting was on, its node and primitive visits."""
+    when counting was on, its node and primitive visits and wavefront walk
+    steps (the busiest lane's node visits per wavefront, summed)."""
     t = ctypes.c_double()
-    v = (ctypes.c_uint64 * 2)()
+    v = (ctypes.c_uint64 * 3)()
     err = ctypes.create_string_buffer(512)
     if _library().rtdl_mi355x_get_last_walk_stats(ctypes.byref(t), v, err, 512) != 0:
         raise RuntimeError(err.value.decode())
-    return {"kernel_s": t.value, "node_visits": int(v[0]), "prim_tests": int(v[1])}
+    return {"kernel_s": t.value, "node_visits": int(v[0]), "prim_tests": int(v[1]), "wave_steps": int(v[2])}
 
 
 def last_phase_timings() -> dict[str, float]:

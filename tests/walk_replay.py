@@ -228,10 +228,11 @@ def _nbr_bucket_edge(b: int, scale: float) -> float:
     return float(b + 1) / scale * 1.0000000000009094947
 
 
-def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -> tuple[int, int]:
-    """Nodes fetched and primitives tested by the batched fixed-radius count
-    walk of one work item: up to 64 queries (FP64 coordinates of FP32
-    values), one per lane of a wavefront, over all count rounds.
+def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -> tuple[int, int, int]:
+    """Nodes fetched, primitives tested and the busiest lane's node count
+    (the wavefront's walk steps) of the batched fixed-radius count walk of
+    one work item: up to 64 queries (FP64 coordinates of FP32 values), one per
+    lane of a wavefront, over all count rounds.
 
     The wavefront runs in lock step.  Each iteration every walking lane takes
     one node step of its rope preorder walk; the points of a leaf that pass
@@ -255,6 +256,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
     step = k // 2 if k // 2 > 4 else 4
     going = [True] * lanes
     n_nodes = n_prims = 0
+    lane_nodes = [0] * lanes
 
     def d2_of(lane: int, slot: int) -> float:
         if exact is not None:
@@ -283,6 +285,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
                 if not walking[i]:
                     continue
                 n_nodes += 1
+                lane_nodes[i] += 1
                 nd = nodes[node[i]]
                 link = int(nd["link"])
                 inside = box_dist2(qf[i], nd["lo"], nd["hi"]) <= thr2[i]
@@ -352,7 +355,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
                 rad[i] = min(rad[i], _nbr_bucket_edge(bucket, scale[i]))
                 rad_sq[i] = min(rad_sq[i], rad[i] * rad[i])
                 going[i] = True
-    return n_nodes, n_prims
+    return n_nodes, n_prims, max(lane_nodes, default=0)
 
 
 # ------------------------------------------------------------------- rays --
