@@ -253,7 +253,7 @@ def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int, kinds=r
                        2: "batch_rays_kernel<0>", 3: "batch_pip_kernel"}[k],
             "queries": q_total, "jobs": len(jobs), "kernel_s": t, "node_visits": nodes,
             "prim_tests": prims, "wave_steps": steps,
-            "lane_efficiency": nodes / (64.0 * steps) if steps else 0.0,
+            "nodes_per_lane_step": nodes / (64.0 * steps) if steps else 0.0,
             "rows": rows, "algorithmic_bytes": w, "w_min_bytes": w_min,
             "prim_record_bytes": pb,
             "achieved_GBps": w / t / 1e9 if t > 0 else 0.0,
@@ -261,7 +261,7 @@ def roofline_leg(B, scenes, stream, offsets, arenas, limit_queries: int, kinds=r
         }
         log(f"kernel {KINDS[k]}: {q_total} q in {t*1e3:.2f} ms ({q_total/t/1e6:.1f} Mq/s), "
             f"{nodes/q_total:.1f} nodes/q, {prims/q_total:.1f} prims/q, "
-            f"lane efficiency {nodes / (64.0 * max(steps, 1)):.2f}, W/t = {w/t/1e9:.0f} GB/s")
+            f"{nodes / (64.0 * max(steps, 1)):.2f} nodes per lane step, W/t = {w/t/1e9:.0f} GB/s")
     return out
 
 

@@ -138,9 +138,11 @@ int rtdl_mi355x_get_last_phase_timings(double* seconds_out, size_t capacity);
  * k <= 64, the ray walk, the PIP counting walk) and, while counting is on,
  * visits[0] node records fetched, visits[1] primitive records tested and
  * visits[2] wavefront walk steps (per wavefront and work item, the node
- * visits of its busiest lane, summed; visits[0] / (64 * visits[2]) is the
- * walk's lane efficiency).  visits has 3 entries.  Counting kernels are
- * slower: time and count in separate calls. */
+ * visits of its busiest lane, summed; for the batched fixed-radius count
+ * walk, which may test two nodes per step, its loop iterations), so that
+ * visits[0] / (64 * visits[2]) is the node visits per lane and step.  visits
+ * has 3 entries.  Counting kernels are slower: time and count in separate
+ * calls. */
 int rtdl_mi355x_set_walk_counting(uint32_t on, char* error_out, size_t error_size);
 
 /* Arms a host-forced fault for testing failure handling (no counterpart in

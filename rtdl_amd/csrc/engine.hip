@@ -350,8 +350,9 @@ static_assert(kNbrBuckets <= (1u << (32 - kStageSlotBits)), "bucket must fit the
 __device__ inline bool stage_packs(const SceneView& sc) { return sc.n_prims <= (1u << kStageSlotBits); }
 
 // The count walk with the FP64 visit taken out of the lanes' walks.  Each
-// lane walks its query's ball (one node step per lane per iteration) and only
-// prefilters leaf points in FP32; the points that pass go into a per-wavefront
+// lane walks its query's ball (one node step per lane per iteration, two when
+// it descends: the child that follows its parent in preorder is fetched with
+// it) and only prefilters leaf points in FP32; the points that pass go into a per-wavefront
 // queue in LDS as (owner lane, slot).  Whenever 64 are queued, or no lane is
 // still walking, the wavefront drains 64 of them at once with every lane
 // active: FP64 squared distance in the reference's order, the inclusive
@@ -451,9 +452,15 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
       uint32_t guard = sc.n_nodes + 1;  // a preorder walk visits each node at most once
       uint32_t head = 0, size = 0;      // the wavefront's queue (uniform)
       for (;;) {
+        if (STATS && lane == 0) ++st.steps;
         uint32_t first = 0, cnt = 0;
         if (walking) {
+          // the node and the one after it in preorder, fetched together: a
+          // descent goes there next, and its test then costs no second
+          // round trip to memory
+          const uint32_t after = node + 1 < sc.n_nodes ? node + 1 : node;
           const BvhNode nd = load_node(sc.nodes, node);
+          const BvhNode nd1 = load_node(sc.nodes, after);
           if (STATS) ++st.nodes;
           const bool in = box_dist2(qf, nd.lo, nd.hi) <= thr2;
           const bool leaf = node_is_leaf(nd.link);
@@ -461,8 +468,20 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
             first = leaf_first(nd.link);
             cnt = leaf_count(nd.link);
           }
-          node = (in && !leaf) ? nd.link : nd.escape;
-          walking = node != sc.end && --guard != 0;
+          uint32_t next = (in && !leaf) ? nd.link : nd.escape;
+          walking = next != sc.end && --guard != 0;
+          if (walking && in && !leaf && next == after) {  // descended: test the child now
+            if (STATS) ++st.nodes;
+            const bool in1 = box_dist2(qf, nd1.lo, nd1.hi) <= thr2;
+            const bool leaf1 = node_is_leaf(nd1.link);
+            if (in1 && leaf1) {
+              first = leaf_first(nd1.link);
+              cnt = leaf_count(nd1.link);
+            }
+            next = (in1 && !leaf1) ? nd1.link : nd1.escape;
+            walking = next != sc.end && --guard != 0;
+          }
+          node = next;
         }
         // One drain: the first min(64, size) queued points, one per lane, then
         // the owners' checkpoints (shrink to the k-th bucket's edge once k
@@ -640,9 +659,11 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
       const uint32_t base = lane0_add(emit_count, (uint32_t)__popcll(em));
       if (emit) emit_list[base + (uint32_t)__popcll(em & ((1ull << lane) - 1ull))] = it * 64 + lane;
     }
-    if (STATS) close_item(st);
   }
-  if (STATS) flush_lane_stats(stats, st);
+  if (STATS) {
+    st.mark = st.nodes;  // steps were counted per iteration
+    flush_lane_stats(stats, st);
+  }
 }
 
 // Writes one query's rows (or parks its candidates) on a second walk of its

@@ -22,7 +22,8 @@ struct LaneStats {
 
 // Ends a work item of the wavefront: its walk took at least as many lockstep
 // steps as its busiest lane visited nodes, so lane 0 adds that maximum.  The
-// ratio nodes / (64 * steps) is the walk's lane efficiency.  Every lane must
+// ratio nodes / (64 * steps) is the walk's lane efficiency (kernels whose
+// steps may visit more than one node count their steps themselves).  Every lane must
 // take part.
 __device__ inline void close_item(LaneStats& st) {
   unsigned long long m = st.nodes - st.mark;

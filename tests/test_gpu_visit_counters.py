@@ -86,10 +86,10 @@ def test_fixed_radius_visit_counters_equal_a_host_replay(gpu_ready):
         nodes = prims = steps = 0
         q64 = q3.astype(np.float64)
         for first in range(0, len(q64), 64):
-            n, p, busiest = W.frn_counts(sc, q64[first:first + 64], radius, k)
+            n, p, iters = W.frn_counts(sc, q64[first:first + 64], radius, k)
             nodes += n
             prims += p
-            steps += busiest
+            steps += iters
         assert (nodes, prims, steps) == visits, (radius, k, (nodes, prims, steps), visits)
 
 

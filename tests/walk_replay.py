@@ -229,13 +229,15 @@ def _nbr_bucket_edge(b: int, scale: float) -> float:
 
 
 def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -> tuple[int, int, int]:
-    """Nodes fetched, primitives tested and the busiest lane's node count
-    (the wavefront's walk steps) of the batched fixed-radius count walk of
-    one work item: up to 64 queries (FP64 coordinates of FP32 values), one per
-    lane of a wavefront, over all count rounds.
+    """Nodes tested, primitives tested and the wavefront's walk iterations of
+    the batched fixed-radius count walk of one work item: up to 64 queries
+    (FP64 coordinates of FP32 values), one per lane of a wavefront, over all
+    count rounds.
 
     The wavefront runs in lock step.  Each iteration every walking lane takes
-    one node step of its rope preorder walk; the points of a leaf that pass
+    one node step of its rope preorder walk, and a second one when the first
+    descends (to the child stored right after its parent, fetched with it),
+    with the same threshold; the points of the leaf it reaches that pass
     the FP32 prefilter are queued (point index within the leaf first, then
     lane), and after each point index, while 64 or more are queued (and, once
     no lane walks, while any are), a batch of up to 64 is taken in queue order: each point inside the
@@ -255,8 +257,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
     rad_sq = [radius * radius] * lanes
     step = k // 2 if k // 2 > 4 else 4
     going = [True] * lanes
-    n_nodes = n_prims = 0
-    lane_nodes = [0] * lanes
+    n_nodes = n_prims = n_steps = 0
 
     def d2_of(lane: int, slot: int) -> float:
         if exact is not None:
@@ -280,19 +281,24 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
         walking = [going[i] and root != end for i in range(lanes)]
         queue: list[tuple[int, int]] = []
         while True:
+            n_steps += 1
             leaf = [(0, 0)] * lanes
             for i in range(lanes):
                 if not walking[i]:
                     continue
-                n_nodes += 1
-                lane_nodes[i] += 1
-                nd = nodes[node[i]]
-                link = int(nd["link"])
-                inside = box_dist2(qf[i], nd["lo"], nd["hi"]) <= thr2[i]
-                if inside and link & LEAF:
-                    leaf[i] = (link & START_MASK, ((link >> 28) & 7) + 1)
-                node[i] = link if (inside and not link & LEAF) else int(nd["escape"])
-                walking[i] = node[i] != end
+                for half in range(2):
+                    cur = node[i]
+                    n_nodes += 1
+                    nd = nodes[cur]
+                    link = int(nd["link"])
+                    inside = box_dist2(qf[i], nd["lo"], nd["hi"]) <= thr2[i]
+                    if inside and link & LEAF:
+                        leaf[i] = (link & START_MASK, ((link >> 28) & 7) + 1)
+                    descend = inside and not link & LEAF
+                    node[i] = link if descend else int(nd["escape"])
+                    walking[i] = node[i] != end
+                    if not (walking[i] and descend and node[i] == cur + 1):
+                        break
             def drain() -> None:
                 nonlocal queue
                 batch, queue = queue[:64], queue[64:]
@@ -355,7 +361,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
                 rad[i] = min(rad[i], _nbr_bucket_edge(bucket, scale[i]))
                 rad_sq[i] = min(rad_sq[i], rad[i] * rad[i])
                 going[i] = True
-    return n_nodes, n_prims, max(lane_nodes, default=0)
+    return n_nodes, n_prims, n_steps
 
 
 # ------------------------------------------------------------------- rays --
