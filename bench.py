@@ -995,6 +995,12 @@ def config3_leg(B, autotune: bool, seed: int = 20261203 + 3, jobs_checked: int =
             replays = [graph.launch() for _ in range(4)]
         graph_s = float(np.median([s for _, s in replays[1:]]))
         graph_same = all([r["rows"] for r in g] == [r["rows"] for r in res] for g, _ in replays)
+        # the same batch with the point instances rebuilt from their device
+        # coordinates in front of it at every launch (build + traverse + emit)
+        with B.BatchGraph(jobs, rebuild=[(c3.pscene, c3.dev_build)]) as graph:
+            rebuilt = [graph.launch() for _ in range(4)]
+        graph_rebuild_s = float(np.median([s for _, s in rebuilt[1:]]))
+        graph_same = graph_same and all([r["rows"] for r in g] == [r["rows"] for r in res] for g, _ in rebuilt)
         # W from counting runs, per family and (fixed radius) per dimension
         frn2 = [j for j, i in zip(jobs[:n_frn], ws["frn_ids"]) if ws["dims"][i] == 2]
         frn3 = [j for j, i in zip(jobs[:n_frn], ws["frn_ids"]) if ws["dims"][i] == 3]
@@ -1013,7 +1019,8 @@ def config3_leg(B, autotune: bool, seed: int = 20261203 + 3, jobs_checked: int =
         AT.set_tuning(original)
     total_q = q_frn + q_pip
     t_frn, t_pip_s = fam["neighbors"], fam["pip"]
-    log(f"config [3]: eager {eager_s*1e3:.1f} ms, graph replay {graph_s*1e3:.1f} ms, "
+    log(f"config [3]: eager {eager_s*1e3:.1f} ms, graph replay {graph_s*1e3:.1f} ms "
+        f"({graph_rebuild_s*1e3:.1f} ms with the point instances rebuilt in the graph), "
         f"fixed radius {t_frn*1e3:.1f} ms ({w_frn/t_frn/1e9:.0f} GB/s), PIP {t_pip_s*1e3:.1f} ms "
         f"({w_pip/t_pip_s/1e9:.0f} GB/s), check {checked}")
     return {
@@ -1021,6 +1028,7 @@ def config3_leg(B, autotune: bool, seed: int = 20261203 + 3, jobs_checked: int =
                     "[0.005, 0.05], k_max 64; every 4th instance PIP vs 1,000 star polygons, positive hits",
         "instances": int(len(ws["sizes"])), "fixed_radius_queries": q_frn, "pip_queries": q_pip,
         "rows": int(rows_frn + rows_pip), "eager_s": eager_s, "graph_replay_s": graph_s,
+        "graph_rebuild_replay_s": graph_rebuild_s,
         "graph_rows_match_eager": bool(graph_same),
         "queries_per_s": total_q / graph_s, "eager_queries_per_s": total_q / eager_s,
         "families": {"fixed_radius": _roof(w_frn, t_frn), "pip": _roof(w_pip, t_pip_s)},

@@ -496,6 +496,27 @@ int rtdl_mi355x_batch_graph_launch(void* graph, RtdlBatchJobResult* results, dou
                                    char* error_out, size_t error_size);
 int rtdl_mi355x_batch_graph_destroy(void* graph, char* error_out, size_t error_size);
 
+/* Moving primitives (no counterpart in the reference's HIPRT API; the OptiX
+ * backend's rebuild-in-place of a geometry, src/native/optix/
+ * rtdl_optix_api.cpp:4323-4393 and 6249-6288, is the closest).
+ * rebuild_instances rebuilds the trees of a point or triangle instance scene
+ * (made by rtdl_mi355x_prepare_point_instances[_leaf] /
+ * _prepare_triangle_instances) from new FP32 device coordinates of the same
+ * primitives: same count, layout and instance offsets, so handles, ids and
+ * jobs stay valid.  batch_graph_create_rebuild records, in front of the batch,
+ * the rebuild of scenes[i] from the device buffer coords[i] (scene_count
+ * pairs; each scene at most once), so every launch rebuilds those scenes from
+ * the buffers' current contents and then runs the batch; launch reports an
+ * error a rebuild met (a coordinate that is not exact in FP32, such as NaN).
+ * The first rebuild of a scene keeps its build buffers for later ones (about
+ * 100 bytes per primitive); a scene in such a graph must not be queried,
+ * rebuilt or destroyed outside it while the graph is alive. */
+int rtdl_mi355x_rebuild_instances(void* prepared, const float* coords, char* error_out, size_t error_size);
+int rtdl_mi355x_batch_graph_create_rebuild(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                                           void* const* scenes, const float* const* coords,
+                                           size_t scene_count, void** graph_out, char* error_out,
+                                           size_t error_size);
+
 /* Multi-instance scenes for batched jobs: instance i owns the primitives
  * [instance_offsets[i], instance_offsets[i + 1]) of the device input and
  * gets its own LBVH subtree; neighbour / polygon ids are instance-local

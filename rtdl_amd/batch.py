@@ -160,6 +160,18 @@ def prepare_point_instances(coords, instance_offsets, *, leaf_size: int = 0) -> 
     return Instances(handle, offs, "points")
 
 
+def rebuild_instances(inst: Instances, coords) -> None:
+    """Rebuilds a point or triangle instance scene in place from new FP32
+    device coordinates of the same primitives (same shape and instance
+    offsets as at prepare time); ids, handles and jobs stay valid."""
+    per = 3 if inst.kind == "points" else 9
+    assert coords.shape[-1] == per and int(coords.shape[0]) == int(inst.offsets[-1]), "coords shape"
+    err = ctypes.create_string_buffer(4096)
+    _sync_foreign()
+    st = _lib().rtdl_mi355x_rebuild_instances(inst.handle, _ptr(coords), err, 4096)
+    _check(st, err, "rtdl_mi355x_rebuild_instances")
+
+
 def prepare_triangle_instances(coords, instance_offsets) -> Instances:
     """coords: float32 device tensor (n, 9): three vertices per triangle."""
     offs = np.ascontiguousarray(instance_offsets, dtype=np.uint64)
@@ -284,15 +296,32 @@ class BatchGraph:
     counts of every launch are those `run_batch` would give for the same
     jobs and contents."""
 
-    def __init__(self, jobs: list[BatchJob], *, morton_sort: bool = True):
+    def __init__(self, jobs: list[BatchJob], *, morton_sort: bool = True, rebuild=()):
+        """`rebuild`: (Instances, coords) pairs recorded in front of the
+        batch: every launch rebuilds each point / triangle instance scene
+        from the current contents of its FP32 device coordinate buffer (same
+        shape as at prepare time), then runs the batch."""
         self._jobs = list(jobs)
+        self._rebuild = [(inst, coords) for inst, coords in rebuild]
         self._arr, self._res = _pack_jobs(self._jobs)
         handle = ctypes.c_void_p()
         err = ctypes.create_string_buffer(4096)
         _sync_foreign()
-        st = _lib().rtdl_mi355x_batch_graph_create(ctypes.addressof(self._arr), len(self._jobs),
-                                                   int(bool(morton_sort)), ctypes.byref(handle), err, 4096)
-        _check(st, err, "rtdl_mi355x_batch_graph_create")
+        if self._rebuild:
+            n = len(self._rebuild)
+            scenes = (ctypes.c_void_p * n)(*[inst.handle.value for inst, _ in self._rebuild])
+            ptrs = (ctypes.c_void_p * n)(*[_ptr(c) for _, c in self._rebuild])
+            for inst, c in self._rebuild:
+                per = 3 if inst.kind == "points" else 9
+                assert c.shape[-1] == per and int(c.shape[0]) == int(inst.offsets[-1]), "coords shape"
+            st = _lib().rtdl_mi355x_batch_graph_create_rebuild(
+                ctypes.addressof(self._arr), len(self._jobs), int(bool(morton_sort)), scenes, ptrs, n,
+                ctypes.byref(handle), err, 4096)
+            _check(st, err, "rtdl_mi355x_batch_graph_create_rebuild")
+        else:
+            st = _lib().rtdl_mi355x_batch_graph_create(ctypes.addressof(self._arr), len(self._jobs),
+                                                       int(bool(morton_sort)), ctypes.byref(handle), err, 4096)
+            _check(st, err, "rtdl_mi355x_batch_graph_create")
         self._handle = handle
 
     def launch(self):

@@ -730,11 +730,27 @@ int rtdl_mi355x_set_tuning(const RtdlTuning* tuning, char* error_out, size_t err
 }
 
 namespace {
+// Raises the error a rebuild left in its status bits (read after `s` passed it).
+void check_rebuild(const Scene& scene, hipStream_t s) {
+  const uint32_t st = rebuild_status(scene, s);
+  require(!(st & kRebuildNotExact), "rebuild: a coordinate is not exact in FP32 (NaN?)");
+  require(!(st & kRebuildRefitIncomplete), "rebuild: the refit did not reach the root");
+  require(!(st & kRebuildRootsMoved), "rebuild: an instance root moved");
+}
+
+Prepared* rebuildable(void* handle) {
+  Prepared* p = prepared_from(handle);
+  require(p->instances && (p->scene.kind == PrimKind::Points || p->scene.kind == PrimKind::Triangles),
+          "only point and triangle instance scenes can be rebuilt");
+  return p;
+}
+
 struct BatchGraph {
   hipStream_t stream = nullptr;
   std::unique_ptr<BatchEngine> engine;
   hipGraphExec_t exec = nullptr;
   size_t job_count = 0;
+  std::vector<Prepared*> rebuilt;  // scenes the graph rebuilds before the batch
   ~BatchGraph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     engine.reset();
@@ -758,6 +774,46 @@ int rtdl_mi355x_batch_graph_create(const RtdlBatchJob* jobs, size_t job_count, u
   });
 }
 
+int rtdl_mi355x_rebuild_instances(void* prepared, const float* coords, char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    Prepared* p = rebuildable(prepared);
+    std::lock_guard<std::mutex> lock(p->lock);
+    require(p->scene.n_prims == 0 || coords != nullptr, "coords must not be null");
+    prepare_rebuild(p->scene, p->stream);
+    rebuild_scene(p->scene, coords, p->stream);
+    check_rebuild(p->scene, p->stream);
+    read_root_box(*p);
+  });
+}
+
+int rtdl_mi355x_batch_graph_create_rebuild(const RtdlBatchJob* jobs, size_t job_count, uint32_t morton_sort,
+                                           void* const* scenes, const float* const* coords,
+                                           size_t scene_count, void** graph_out, char* error_out,
+                                           size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(graph_out != nullptr, "graph_out must not be null");
+    *graph_out = nullptr;
+    require(scene_count == 0 || (scenes != nullptr && coords != nullptr), "scenes and coords must not be null");
+    auto g = std::make_unique<BatchGraph>();
+    for (size_t i = 0; i < scene_count; ++i) {
+      Prepared* p = rebuildable(scenes[i]);
+      require(p->scene.n_prims == 0 || coords[i] != nullptr, "coords must not be null");
+      for (Prepared* q : g->rebuilt) require(q != p, "a scene is rebuilt twice in one graph");
+      std::lock_guard<std::mutex> lock(p->lock);
+      prepare_rebuild(p->scene, p->stream);  // before the job views are taken: M in device memory
+      g->rebuilt.push_back(p);
+    }
+    const std::vector<HostJob> host = host_jobs_from(jobs, job_count);
+    RTDL_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    g->engine = std::make_unique<BatchEngine>(g->stream);
+    g->exec = g->engine->capture(host, morton_sort != 0, [&](hipStream_t s) {
+      for (size_t i = 0; i < scene_count; ++i) rebuild_scene(g->rebuilt[i]->scene, coords[i], s);
+    });
+    g->job_count = job_count;
+    *graph_out = g.release();
+  });
+}
+
 int rtdl_mi355x_batch_graph_launch(void* graph, RtdlBatchJobResult* results, double* seconds_out,
                                    char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
@@ -776,6 +832,7 @@ int rtdl_mi355x_batch_graph_launch(void* graph, RtdlBatchJobResult* results, dou
     (void)hipEventDestroy(start);
     (void)hipEventDestroy(stop);
     if (seconds_out) *seconds_out = ms * 1e-3;
+    for (Prepared* p : g->rebuilt) check_rebuild(p->scene, g->stream);
     const JobCounters* c = g->engine->counters();
     for (size_t i = 0; i < g->job_count; ++i) {
       results[i].rows = c[i].rows;

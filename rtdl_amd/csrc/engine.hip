@@ -1530,7 +1530,8 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
   if (!capturing_ && done_) RTDL_HIP(hipEventRecord(done_, s));
 }
 
-hipGraphExec_t BatchEngine::capture(const std::vector<HostJob>& jobs, bool morton_sort) {
+hipGraphExec_t BatchEngine::capture(const std::vector<HostJob>& jobs, bool morton_sort,
+                                    const std::function<void(hipStream_t)>& prologue) {
   // every allocation enqueue would make happens here, outside the capture
   uint64_t total = 0;
   for (const HostJob& h : jobs) total += h.nq;
@@ -1543,6 +1544,7 @@ hipGraphExec_t BatchEngine::capture(const std::vector<HostJob>& jobs, bool morto
   RTDL_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   capturing_ = true;
   try {
+    if (prologue) prologue(stream_);
     enqueue(jobs, morton_sort);
   } catch (...) {
     capturing_ = false;

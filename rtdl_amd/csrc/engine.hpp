@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -175,7 +176,10 @@ class BatchEngine {
   // descriptors, ordering sort, family launches, counter copy) with one
   // call.  Timing and visit counting are off in a captured batch.  The
   // engine then belongs to the graph: it must not enqueue other batches.
-  hipGraphExec_t capture(const std::vector<HostJob>& jobs, bool morton_sort);
+  // `prologue` (optional) enqueues work recorded before the batch, such as
+  // scene rebuilds; it must only enqueue capturable work on the stream given.
+  hipGraphExec_t capture(const std::vector<HostJob>& jobs, bool morton_sort,
+                         const std::function<void(hipStream_t)>& prologue = {});
   bool fixed() const { return fixed_; }
   // Valid once the batch has completed.
   const JobCounters* counters() const { return host_counters_; }

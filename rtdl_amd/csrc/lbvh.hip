@@ -24,6 +24,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "io.hpp"
 #include "primitives.hpp"
 #include "scene.hpp"
 
@@ -375,7 +376,9 @@ constexpr uint32_t kNotRefit = 0xFFFFFFFFu;
 __global__ void refit_round_kernel(uint32_t n_internal, uint32_t round, const uint32_t* left,
                                    const uint32_t* right, uint32_t* done, BvhNode* nodes) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_internal || done[k] != kNotRefit) return;
+  // once the root (internal node 0) is done, so is every node: later rounds
+  // of a fixed-length sequence return at once
+  if (k >= n_internal || done[0] != kNotRefit || done[k] != kNotRefit) return;
   const uint32_t a = left[k], b = right[k];
   const bool ready_a = a >= n_internal || done[a] < round;
   const bool ready_b = b >= n_internal || done[b] < round;
@@ -653,6 +656,242 @@ void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* c
   RTDL_HIP(hipMemcpyAsync(scene.host_ends.data(), scene.ends.get(), n_inst * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, stream));
   RTDL_HIP(hipStreamSynchronize(stream));
+}
+
+
+// ------------------------------------------------------------- rebuilds --
+
+struct RebuildWork {
+  PrimKind kind = PrimKind::Points;
+  size_t n = 0, m = 0, n_nodes = 0, n_internal = 0, n_tasks = 0;
+  uint32_t n_inst = 0;
+  int inst_bits = 0;
+  DeviceArray<uint64_t> inst_offsets, leaf_offsets;
+  DeviceArray<double> coords;
+  DeviceArray<uint32_t> ids;
+  DeviceArray<BoundsTask> tasks;
+  DeviceArray<float4> box_lo, box_hi, sorted_lo, sorted_hi;
+  DeviceArray<int> inst_bounds;
+  DeviceArray<uint32_t> flags;  // [0] M bits (read by traversals), [1] exact flag, [2] status
+  DeviceArray<float> fbounds;
+  DeviceArray<uint64_t> keys, leaf_keys;
+  DeviceArray<uint32_t> order, leaf_first, leaf_count, left, right, parent, range_lo, range_hi, done, pre,
+      roots, ends;
+  DeviceArray<unsigned long long> highest;
+  DeviceArray<BvhNode> work_nodes;
+  SortWorkspace sort_ws;
+};
+
+namespace {
+
+// The initial values build_scene uploads or memsets, written by a kernel.
+__global__ void rebuild_init_kernel(uint32_t n_inst, int* inst_bounds, uint32_t* flags, size_t m,
+                                    unsigned long long* highest, size_t n_internal, uint32_t* done,
+                                    size_t n_nodes, uint32_t* parent) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * (size_t)n_inst) inst_bounds[i] = (i % 6) < 3 ? 0x7FFFFFFF : (int)0x80000000;
+  if (i == 0) {
+    flags[0] = 0u;
+    flags[1] = 1u;
+    flags[2] = 0u;
+  }
+  if (i < m) highest[i] = 0ull;
+  if (i < n_internal) done[i] = kNotRefit;
+  if (i < n_nodes) parent[i] = kEndNode;
+}
+
+__device__ inline float float_from_ordered_dev(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+__global__ void decode_bounds_kernel(size_t n, const int* enc, float* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = float_from_ordered_dev(enc[i]);
+}
+
+// Error bits of a rebuild: the refit reached the root, the instance roots kept
+// their preorder positions (the jobs' scene views hold them) and every
+// coordinate stayed exact in FP32 (the scene keeps no FP64 copy).
+__global__ void rebuild_check_kernel(uint32_t n_inst, const uint32_t* roots, const uint32_t* ends,
+                                     const uint32_t* scene_roots, const uint32_t* scene_ends,
+                                     size_t n_internal, const uint32_t* done, uint32_t* flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    if (n_internal && done[0] == kNotRefit) atomicOr(&flags[2], kRebuildRefitIncomplete);
+    if (flags[1] == 0u) atomicOr(&flags[2], kRebuildNotExact);
+  }
+  if (i < n_inst && (roots[i] != scene_roots[i] || ends[i] != scene_ends[i]))
+    atomicOr(&flags[2], kRebuildRootsMoved);
+}
+
+// Refit rounds of a rebuild: a Karras tree over 64-bit keys with index
+// tie-breaks is at most about 100 levels deep, and rounds after the root's
+// return at once.
+constexpr uint32_t kRebuildRefitRounds = 128;
+
+}  // namespace
+
+void prepare_rebuild(Scene& scene, hipStream_t stream) {
+  require(scene.kind == PrimKind::Points || scene.kind == PrimKind::Triangles,
+          "only point and triangle instance scenes can be rebuilt");
+  require(scene.fp32_exact, "rebuilds need FP32-exact input (instance scenes from FP32 coordinates)");
+  if (scene.rebuild) return;
+  auto w = std::make_shared<RebuildWork>();
+  const size_t n = scene.n_prims;
+  const uint32_t n_inst = (uint32_t)scene.n_instances;
+  w->kind = scene.kind;
+  w->n = n;
+  w->n_inst = n_inst;
+  w->m = scene.n_leaves;
+  w->n_nodes = n ? 2 * w->m - 1 : 0;
+  w->n_internal = n ? w->m - 1 : 0;
+  while ((1u << w->inst_bits) < n_inst) ++w->inst_bits;
+  const std::vector<uint64_t>& offs = scene.host_offsets;
+  std::vector<uint64_t> leaf_offsets(n_inst + 1, 0);
+  for (uint32_t i = 0; i < n_inst; ++i)
+    leaf_offsets[i + 1] = leaf_offsets[i] + (offs[i + 1] - offs[i] + scene.leaf_size - 1) / scene.leaf_size;
+  std::vector<BoundsTask> tasks;
+  constexpr uint32_t kChunk = 16384;  // as build_scene
+  for (uint32_t i = 0; i < n_inst; ++i)
+    for (uint64_t b = offs[i]; b < offs[i + 1]; b += kChunk)
+      tasks.push_back({i, (uint32_t)b, (uint32_t)std::min<uint64_t>(b + kChunk, offs[i + 1])});
+  w->n_tasks = tasks.size();
+  const size_t per = scene.kind == PrimKind::Points ? 3 : 9;
+  auto at_least1 = [](size_t x) { return x ? x : (size_t)1; };
+  w->inst_offsets.resize(n_inst + 1);
+  w->leaf_offsets.resize(n_inst + 1);
+  w->coords.resize(at_least1(per * n));
+  w->ids.resize(at_least1(n));
+  w->tasks.resize(at_least1(tasks.size()));
+  w->box_lo.resize(at_least1(n));
+  w->box_hi.resize(at_least1(n));
+  w->sorted_lo.resize(at_least1(n));
+  w->sorted_hi.resize(at_least1(n));
+  w->inst_bounds.resize(at_least1(6 * (size_t)n_inst));
+  w->flags.resize(3);
+  w->fbounds.resize(at_least1(6 * (size_t)n_inst));
+  w->keys.resize(at_least1(n));
+  w->order.resize(at_least1(n));
+  w->leaf_keys.resize(at_least1(w->m));
+  w->leaf_first.resize(at_least1(w->m));
+  w->leaf_count.resize(at_least1(w->m));
+  w->left.resize(at_least1(w->n_internal));
+  w->right.resize(at_least1(w->n_internal));
+  w->range_lo.resize(at_least1(w->n_internal));
+  w->range_hi.resize(at_least1(w->n_internal));
+  w->done.resize(at_least1(w->n_internal));
+  w->parent.resize(at_least1(w->n_nodes));
+  w->pre.resize(at_least1(w->n_nodes));
+  w->highest.resize(at_least1(w->m));
+  w->work_nodes.resize(at_least1(w->n_nodes));
+  w->roots.resize(at_least1(n_inst));
+  w->ends.resize(at_least1(n_inst));
+  w->sort_ws.reserve(at_least1(n));
+  RTDL_HIP(hipMemcpyAsync(w->inst_offsets.get(), offs.data(), (n_inst + 1) * sizeof(uint64_t),
+                          hipMemcpyHostToDevice, stream));
+  RTDL_HIP(hipMemcpyAsync(w->leaf_offsets.get(), leaf_offsets.data(), (n_inst + 1) * sizeof(uint64_t),
+                          hipMemcpyHostToDevice, stream));
+  if (!tasks.empty())
+    RTDL_HIP(hipMemcpyAsync(w->tasks.get(), tasks.data(), tasks.size() * sizeof(BoundsTask),
+                            hipMemcpyHostToDevice, stream));
+  // M of the current tree, in device memory from now on
+  uint32_t m_bits;
+  std::memcpy(&m_bits, &scene.m, sizeof(m_bits));
+  const uint32_t init[3] = {m_bits, 1u, 0u};
+  RTDL_HIP(hipMemcpyAsync(w->flags.get(), init, sizeof(init), hipMemcpyHostToDevice, stream));
+  RTDL_HIP(hipStreamSynchronize(stream));
+  scene.m_device = reinterpret_cast<const float*>(w->flags.get());
+  scene.rebuild = std::move(w);
+}
+
+void rebuild_scene(Scene& scene, const float* coords, hipStream_t stream) {
+  require(scene.rebuild != nullptr, "scene was not prepared for rebuilds");
+  RebuildWork& w = *scene.rebuild;
+  const size_t n = w.n;
+  const uint32_t n_inst = w.n_inst;
+  if (!n) return;
+  require(coords != nullptr, "coords must not be null");
+  // 0. FP64 coordinates and instance-local ids, as the first build
+  if (w.kind == PrimKind::Points) {
+    widen_points(coords, n, w.inst_offsets.get(), n_inst, w.coords.get(), w.ids.get(), stream);
+  } else {
+    widen_f32(coords, 9 * n, w.coords.get(), stream);
+    instance_ids(n, w.inst_offsets.get(), n_inst, w.ids.get(), stream);
+  }
+  const size_t init_n = std::max({6 * (size_t)n_inst, w.m, w.n_nodes, (size_t)1});
+  rebuild_init_kernel<<<blocks_for(init_n, 256), 256, 0, stream>>>(
+      n_inst, w.inst_bounds.get(), w.flags.get(), w.m, w.highest.get(), w.n_internal, w.done.get(),
+      w.n_nodes, w.parent.get());
+  RTDL_LAUNCH_CHECK();
+  // 1. bounds, decoded on the device
+  bounds_kernel<<<(unsigned)w.n_tasks, 256, 0, stream>>>(scene.kind, w.tasks.get(), w.coords.get(), nullptr,
+                                                         nullptr, w.box_lo.get(), w.box_hi.get(),
+                                                         w.inst_bounds.get(), w.flags.get(), w.flags.get() + 1);
+  RTDL_LAUNCH_CHECK();
+  decode_bounds_kernel<<<blocks_for(6 * (size_t)n_inst, 256), 256, 0, stream>>>(
+      6 * (size_t)n_inst, w.inst_bounds.get(), w.fbounds.get());
+  RTDL_LAUNCH_CHECK();
+  // 2. Morton keys and sort
+  morton_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(n, scene.dim, w.box_lo.get(), w.box_hi.get(),
+                                                         w.inst_offsets.get(), n_inst, w.fbounds.get(),
+                                                         w.keys.get(), w.order.get());
+  RTDL_LAUNCH_CHECK();
+  radix_sort_pairs_async(w.keys.get(), w.order.get(), n, 32 + w.inst_bits, w.sort_ws, stream);
+  // 3. gather into Morton order, into the scene's own arrays
+  gather_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(
+      scene.kind, n, w.order.get(), w.coords.get(), w.ids.get(), nullptr, w.box_lo.get(), w.box_hi.get(),
+      scene.prims.get(), nullptr, scene.slot_to_input.get(), nullptr, nullptr, w.sorted_lo.get(),
+      w.sorted_hi.get());
+  RTDL_LAUNCH_CHECK();
+  // 4. leaves + Karras hierarchy, 5. leaf records and ropes (in the work nodes)
+  leaves_kernel<<<blocks_for(w.m, 256), 256, 0, stream>>>(w.m, w.leaf_offsets.get(), w.inst_offsets.get(),
+                                                           n_inst, scene.leaf_size, w.keys.get(),
+                                                           w.leaf_first.get(), w.leaf_count.get(),
+                                                           w.leaf_keys.get());
+  RTDL_LAUNCH_CHECK();
+  if (w.n_internal) {
+    karras_kernel<<<blocks_for(w.n_internal, 256), 256, 0, stream>>>(
+        (int)w.m, w.leaf_keys.get(), w.left.get(), w.right.get(), w.parent.get(), w.range_lo.get(),
+        w.range_hi.get(), w.highest.get());
+    RTDL_LAUNCH_CHECK();
+    links_kernel<<<blocks_for(w.n_internal, 256), 256, 0, stream>>>(
+        (uint32_t)w.m, w.left.get(), w.range_hi.get(), w.highest.get(), w.work_nodes.get());
+    RTDL_LAUNCH_CHECK();
+  }
+  leaf_nodes_kernel<<<blocks_for(w.m, 256), 256, 0, stream>>>(
+      (uint32_t)w.m, w.leaf_first.get(), w.leaf_count.get(), w.sorted_lo.get(), w.sorted_hi.get(),
+      w.highest.get(), w.work_nodes.get());
+  RTDL_LAUNCH_CHECK();
+  // 6. refit: a fixed number of rounds instead of a host poll of the root
+  if (w.n_internal)
+    for (uint32_t round = 1; round <= kRebuildRefitRounds; ++round) {
+      refit_round_kernel<<<blocks_for(w.n_internal, 256), 256, 0, stream>>>(
+          (uint32_t)w.n_internal, round, w.left.get(), w.right.get(), w.done.get(), w.work_nodes.get());
+      RTDL_LAUNCH_CHECK();
+    }
+  roots_kernel<<<blocks_for(n_inst, 64), 64, 0, stream>>>(
+      n_inst, (uint32_t)w.m, w.leaf_offsets.get(), w.parent.get(), w.range_lo.get(), w.range_hi.get(),
+      w.work_nodes.get(), w.roots.get(), w.ends.get());
+  RTDL_LAUNCH_CHECK();
+  // 7. preorder, written into the scene's node array
+  preorder_kernel<<<blocks_for(w.n_nodes, 256), 256, 0, stream>>>((uint32_t)w.m, w.parent.get(), w.left.get(),
+                                                                   w.range_lo.get(), w.pre.get());
+  RTDL_LAUNCH_CHECK();
+  permute_nodes_kernel<<<blocks_for(w.n_nodes, 256), 256, 0, stream>>>((uint32_t)w.n_nodes, w.work_nodes.get(),
+                                                                        w.pre.get(), scene.nodes.get());
+  RTDL_LAUNCH_CHECK();
+  remap_roots_kernel<<<blocks_for(n_inst, 64), 64, 0, stream>>>(n_inst, w.pre.get(), w.roots.get(), w.ends.get());
+  RTDL_LAUNCH_CHECK();
+  rebuild_check_kernel<<<blocks_for(std::max<size_t>(n_inst, 1), 64), 64, 0, stream>>>(
+      n_inst, w.roots.get(), w.ends.get(), scene.roots.get(), scene.ends.get(), w.n_internal, w.done.get(),
+      w.flags.get());
+  RTDL_LAUNCH_CHECK();
+}
+
+uint32_t rebuild_status(const Scene& scene, hipStream_t stream) {
+  require(scene.rebuild != nullptr, "scene was not prepared for rebuilds");
+  uint32_t st = 0;
+  RTDL_HIP(hipMemcpyAsync(&st, scene.rebuild->flags.get() + 2, sizeof(st), hipMemcpyDeviceToHost, stream));
+  RTDL_HIP(hipStreamSynchronize(stream));
+  return st;
 }
 
 }  // namespace rtdl

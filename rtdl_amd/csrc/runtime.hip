@@ -447,6 +447,8 @@ DeviceArray<uint8_t> upload_bytes(const void* host, size_t bytes, hipStream_t st
   return upload(static_cast<const uint8_t*>(host), bytes, stream);
 }
 
+}  // namespace
+
 void read_root_box(Prepared& p) {
   const SceneView v = p.scene.view(0);
   if (v.root == kEndNode) return;
@@ -459,6 +461,8 @@ void read_root_box(Prepared& p) {
     p.root_hi[a] = root.hi[a];
   }
 }
+
+namespace {
 
 void* host_rows(size_t count, size_t row_bytes) {
   void* rows = std::malloc(count ? count * row_bytes : 1);
@@ -604,6 +608,7 @@ void prepare_point_instances(Prepared& p, const float* coords, size_t n, uint32_
   build_scene(p.scene, PrimKind::Points, 3, n, c64.get(), ids.get(), nullptr, nullptr, 0, offsets,
               leaf_size ? leaf_size : current_tuning().point_leaf, s);
   read_root_box(p);
+  p.instances = true;
 }
 
 void prepare_triangle_instances(Prepared& p, const float* coords, size_t n,
@@ -621,6 +626,7 @@ void prepare_triangle_instances(Prepared& p, const float* coords, size_t n,
   build_scene(p.scene, PrimKind::Triangles, 3, n, c64.get(), ids.get(), nullptr, nullptr, 0, offsets,
               current_tuning().triangle_leaf, s);
   read_root_box(p);
+  p.instances = true;
 }
 
 void prepare_polygon_instances(Prepared& p, const uint32_t* voff, const uint32_t* vcnt, size_t n,

@@ -59,6 +59,7 @@ struct Prepared {
   size_t n_polygons = 0;
   uint32_t last_ambiguous = 0;  // queries of the last call recomputed exactly
   double radius = 0.0;          // fixed radius bound at prepare time (reference-shaped FRN handle)
+  bool instances = false;       // built by prepare_*_instances from FP32 device coordinates
   bool owns_stream = true;
   // own_stream = false borrows the calling thread's stream (per-call path).
   explicit Prepared(bool own_stream = true);
@@ -76,6 +77,8 @@ void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* ver
                       size_t n_doubles);
 
 // Multi-instance builders from device FP32 data (ids are instance-local).
+// Reads the root box of instance 0 into p.root_lo / root_hi (synchronises).
+void read_root_box(Prepared& p);
 // leaf_size: points per leaf of the instance trees, 0 for the process tuning
 void prepare_point_instances(Prepared& p, const float* coords, size_t n, uint32_t leaf_size,
                              const std::vector<uint64_t>& offsets);

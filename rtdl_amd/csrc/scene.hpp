@@ -5,11 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
 
 namespace rtdl {
+
+struct RebuildWork;  // the build buffers a scene keeps for in-place rebuilds
 
 enum class PrimKind : int { Points = 0, Triangles = 1, Polygons = 2 };
 
@@ -101,6 +104,10 @@ class Scene {
   std::vector<uint32_t> host_roots;
   std::vector<uint32_t> host_ends;
   std::vector<uint64_t> host_offsets;  // per instance, n_instances + 1
+  // set by prepare_rebuild: the buffers rebuild_scene reuses, and M in
+  // device memory (a rebuild changes it without the host seeing it)
+  std::shared_ptr<RebuildWork> rebuild;
+  const float* m_device = nullptr;
 
   SceneView view(size_t instance = 0) const {
     SceneView v{};
@@ -116,7 +123,7 @@ class Scene {
     v.n_prims = (uint32_t)n_prims;
     v.n_nodes = (uint32_t)nodes.size();
     v.m = m;
-    v.m_dev = nullptr;
+    v.m_dev = m_device;
     return v;
   }
   size_t device_bytes() const {
@@ -135,5 +142,21 @@ void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* c
                  const uint32_t* ids, const uint4* poly_refs, const double* poly_verts,
                  size_t n_verts, const std::vector<uint64_t>& instance_offsets,
                  uint32_t leaf_size, hipStream_t stream);
+
+// In-place rebuilds of a point or triangle instance scene built from FP32
+// input, for primitives that move while their count and instance layout stay
+// (the tree is rebuilt from scratch, not refit).  prepare_rebuild allocates
+// every buffer the rebuild needs for the scene's layout and switches the
+// scene's M to device memory; views taken before it keep the host M and must
+// not be used with rebuilt scenes.  rebuild_scene then enqueues only kernels
+// on `stream` (no allocation, host copy, memset or synchronisation), so a
+// stream capture can record it.  rebuild_status reads, after the stream has
+// passed the rebuild, the error bits of the last one (0 when it succeeded).
+constexpr uint32_t kRebuildRefitIncomplete = 1;  // the refit rounds did not reach the root
+constexpr uint32_t kRebuildRootsMoved = 2;       // an instance root left its preorder position
+constexpr uint32_t kRebuildNotExact = 4;         // a coordinate is not exact in FP32
+void prepare_rebuild(Scene& scene, hipStream_t stream);
+void rebuild_scene(Scene& scene, const float* coords, hipStream_t stream);
+uint32_t rebuild_status(const Scene& scene, hipStream_t stream);
 
 }  // namespace rtdl

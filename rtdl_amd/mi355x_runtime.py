@@ -190,6 +190,9 @@ _SIGNATURES = {
     "rtdl_mi355x_batch_graph_create": ([_VP, _SZ, ctypes.c_uint32, ctypes.POINTER(_VP), *_ERR], ctypes.c_int),
     "rtdl_mi355x_batch_graph_launch": ([_VP, _VP, ctypes.POINTER(ctypes.c_double), *_ERR], ctypes.c_int),
     "rtdl_mi355x_batch_graph_destroy": ([_VP, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_rebuild_instances": ([_VP, _VP, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_batch_graph_create_rebuild": ([_VP, _SZ, ctypes.c_uint32, _VP, _VP, _SZ, ctypes.POINTER(_VP),
+                                                *_ERR], ctypes.c_int),
     "rtdl_mi355x_scene_export": ([_VP, _VP, _VP, _VP, _VP, _VP, _VP, *_ERR], ctypes.c_int),
     "rtdl_mi355x_get_tuning": ([_VP, *_ERR], ctypes.c_int),
     "rtdl_mi355x_set_tuning": ([_VP, *_ERR], ctypes.c_int),

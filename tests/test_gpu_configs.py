@@ -10,8 +10,9 @@
 - Config [3]: the batched fixed-radius + PIP mix at full size (2,048
   instances of 1e3-2.6e5 points, 2-D and 3-D, radius log-uniform in
   [0.005, 0.05], every 4th instance PIP against 1,000 star polygons),
-  autotuned and replayed from a hipGraph; sampled queries of every job
-  checked against the C oracle over the whole instance.
+  autotuned and replayed from a hipGraph (also with the point instances
+  rebuilt in the graph); sampled queries of every job checked against the C
+  oracle over the whole instance.
 - Config [4]: the served stream at its scene scale (16 resident scenes of
   1e5-1e7 primitives) and request sizes (1-65,536 queries): half a second of
   Poisson arrivals at 20,000 requests/s; sampled queries of every 25th
@@ -132,6 +133,18 @@ def test_config3_batched_fixed_radius_and_pip_at_full_size(gpu_ready):
                 res, _ = graph.launch()
                 assert [r["rows"] for r in res] == [r["rows"] for r in eager]
                 assert all(r["status"] == 0 for r in res)
+                for i, want in sample_rows.items():
+                    key = "query_id" if i < len(ws["frn_ids"]) else "point_id"
+                    got = bench._rows_by_query(B.job_rows(jobs[i], res[i]), key, np.arange(0))[0]
+                    for name in want.dtype.names:
+                        assert np.array_equal(got[name], want[name]), (i, name)
+        # build + traverse + emit: the point instances rebuilt from their
+        # device coordinates in the graph (same coordinates, same rows)
+        with B.BatchGraph(jobs, rebuild=[(c3.pscene, c3.dev_build)]) as graph:
+            for rep in range(2):
+                _say(f"  graph replay with rebuild {rep}")
+                res, _ = graph.launch()
+                assert [r["rows"] for r in res] == [r["rows"] for r in eager]
                 for i, want in sample_rows.items():
                     key = "query_id" if i < len(ws["frn_ids"]) else "point_id"
                     got = bench._rows_by_query(B.job_rows(jobs[i], res[i]), key, np.arange(0))[0]
