@@ -474,6 +474,9 @@ typedef struct RtdlTuning {
   uint32_t triangle_leaf;  /* triangle scenes, 1..8 (default 1) */
   uint32_t polygon_leaf;   /* polygon scenes, 1..8 (default 1) */
   uint32_t blocks_per_cu;  /* persistent batch launches: 256-thread workgroups per CU, 1..64 (default 8) */
+  uint32_t knn_stack;      /* batched KNN walk: per-lane LDS node stack entries, 16, 32 or 48 (default 32);
+                              a walk that fills it walks the tree again */
+  uint32_t walk_block;     /* batched ray and PIP walks: threads per workgroup, 64, 128 or 256 (default 256) */
 } RtdlTuning;
 
 int rtdl_mi355x_get_tuning(RtdlTuning* out, char* error_out, size_t error_size);

@@ -1,8 +1,10 @@
 """Autotuner for the batched path (no counterpart in the reference).
 
 It picks the process-wide build and launch choices of the library (the LBVH
-leaf size per primitive type and the persistent launch size, see
-``RtdlTuning`` in include/rtdl_mi355x.h) for a caller's workload. It times
+leaf size per primitive type, the persistent launch size, the KNN walk's
+stack depth and the ray / PIP walks' workgroup size, the last two as kernel
+template variants; see ``RtdlTuning`` in include/rtdl_mi355x.h) for a
+caller's workload. It times
 candidates on that workload, one knob at a time (coordinate descent from the
 current values), and checks on the way that every candidate gives the same
 rows. The chosen values can be cached in a JSON file under a caller-chosen
@@ -29,11 +31,14 @@ class Tuning:
     triangle_leaf: int = 1
     polygon_leaf: int = 1
     blocks_per_cu: int = 8
+    knn_stack: int = 32
+    walk_block: int = 256
 
 
 class _CTuning(ctypes.Structure):
     _fields_ = [("point_leaf", ctypes.c_uint32), ("triangle_leaf", ctypes.c_uint32),
-                ("polygon_leaf", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32)]
+                ("polygon_leaf", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("knn_stack", ctypes.c_uint32), ("walk_block", ctypes.c_uint32)]
 
 
 # values tried per knob, in the order the descent visits the knobs
@@ -42,6 +47,8 @@ KNOBS: dict[str, tuple[int, ...]] = {
     "point_leaf": (2, 4, 6, 8),
     "triangle_leaf": (1, 2, 4),
     "polygon_leaf": (1, 2),
+    "knn_stack": (16, 32, 48),
+    "walk_block": (64, 128, 256),
 }
 
 
@@ -51,11 +58,12 @@ def get_tuning() -> Tuning:
     st = rt_native._library().rtdl_mi355x_get_tuning(ctypes.byref(out), err, 1024)
     if st != 0:
         raise RuntimeError(f"rtdl_mi355x_get_tuning failed: {err.value.decode(errors='replace')}")
-    return Tuning(out.point_leaf, out.triangle_leaf, out.polygon_leaf, out.blocks_per_cu)
+    return Tuning(out.point_leaf, out.triangle_leaf, out.polygon_leaf, out.blocks_per_cu, out.knn_stack,
+                  out.walk_block)
 
 
 def set_tuning(t: Tuning) -> None:
-    c = _CTuning(t.point_leaf, t.triangle_leaf, t.polygon_leaf, t.blocks_per_cu)
+    c = _CTuning(t.point_leaf, t.triangle_leaf, t.polygon_leaf, t.blocks_per_cu, t.knn_stack, t.walk_block)
     err = ctypes.create_string_buffer(1024)
     st = rt_native._library().rtdl_mi355x_set_tuning(ctypes.byref(c), err, 1024)
     if st != 0:

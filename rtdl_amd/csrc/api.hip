@@ -713,7 +713,7 @@ int rtdl_mi355x_get_tuning(RtdlTuning* out, char* error_out, size_t error_size) 
   return guarded(error_out, error_size, [&] {
     require(out != nullptr, "out must not be null");
     const Tuning t = current_tuning();
-    *out = RtdlTuning{t.point_leaf, t.triangle_leaf, t.polygon_leaf, t.blocks_per_cu};
+    *out = RtdlTuning{t.point_leaf, t.triangle_leaf, t.polygon_leaf, t.blocks_per_cu, t.knn_stack, t.walk_block};
   });
 }
 
@@ -725,6 +725,8 @@ int rtdl_mi355x_set_tuning(const RtdlTuning* tuning, char* error_out, size_t err
     t.triangle_leaf = tuning->triangle_leaf;
     t.polygon_leaf = tuning->polygon_leaf;
     t.blocks_per_cu = tuning->blocks_per_cu;
+    t.knn_stack = tuning->knn_stack;
+    t.walk_block = tuning->walk_block;
     set_tuning(t);
   });
 }

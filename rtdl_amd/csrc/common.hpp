@@ -153,6 +153,8 @@ struct Tuning {
   uint32_t triangle_leaf = 1;   // triangle scenes
   uint32_t polygon_leaf = 1;    // polygon scenes
   uint32_t blocks_per_cu = 8;   // persistent batch launches: 256-thread workgroups per CU
+  uint32_t knn_stack = 32;      // batched KNN walk: per-lane node stack entries (16, 32 or 48)
+  uint32_t walk_block = 256;    // batched ray and PIP walks: threads per workgroup (64, 128 or 256)
 };
 Tuning current_tuning();
 void set_tuning(const Tuning& t);  // validates; throws on out-of-range values

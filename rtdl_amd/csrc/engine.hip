@@ -152,13 +152,13 @@ __device__ inline void read_neighbor_row(const DeviceJob& j, uint64_t at, double
 // PackedTopK list in registers whose k-th key shrinks the walk; K is the list
 // size the job's k needs.  Queries whose FP32 keys tie at the boundary go to
 // batch_fix_kernel.
-template <int K, bool STATS>
+template <int K, bool STATS, uint32_t DEPTH>
 __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __restrict__ jobs,
                                                            const WorkItem* __restrict__ items,
                                                            uint32_t n_items, uint32_t* item_counter,
                                                            uint4* amb, uint32_t* amb_count,
                                                            unsigned long long* stats) {
-  __shared__ uint32_t stack[kStackDepth * 256];  // walk_nearest's per-lane stacks
+  __shared__ uint32_t stack[DEPTH * 256];  // walk_nearest's per-lane stacks
   const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void batch_points_kernel(const DeviceJob* __re
       qy = qp[1];
       qz = j.qstride >= 3 ? (double)qp[2] : 0.0;
       const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
-      ambiguous = collect_packed<K, STATS>(sc, sc.root, sc.end, qx, qy, qz, j.radius, k, list, st,
+      ambiguous = collect_packed<K, STATS, DEPTH>(sc, sc.root, sc.end, qx, qy, qz, j.radius, k, list, st,
                                            stack + threadIdx.x, 256);
 #pragma unroll
       for (int i = 0; i < K; ++i)
@@ -1134,24 +1134,39 @@ int family_of(uint32_t kind, uint32_t k) {
 // report group of a launch family: neighbours, any-hit, hit-count, closest, PIP
 int group_of(int family) { return family <= kFamNbr ? 0 : family - kFamNbr; }
 
+template <int K, uint32_t DEPTH>
+void launch_points_depth(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
+                         uint32_t n_items, uint32_t* counter, uint4* amb, uint32_t* amb_count,
+                         unsigned long long* st, hipStream_t s) {
+  if (stats)
+    batch_points_kernel<K, true, DEPTH><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+  else
+    batch_points_kernel<K, false, DEPTH><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
+  RTDL_LAUNCH_CHECK();
+}
+
+// The KNN walk's stack depth is a tuned template variant (Tuning::knn_stack).
 template <int K>
 void launch_points(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
                    uint32_t n_items, uint32_t* counter, uint4* amb, uint32_t* amb_count,
                    unsigned long long* st, hipStream_t s) {
-  if (stats)
-    batch_points_kernel<K, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
-  else
-    batch_points_kernel<K, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, amb, amb_count, st);
-  RTDL_LAUNCH_CHECK();
+  switch (current_tuning().knn_stack) {
+    case 16: launch_points_depth<K, 16>(stats, grid, jobs, items, n_items, counter, amb, amb_count, st, s); break;
+    case 48: launch_points_depth<K, 48>(stats, grid, jobs, items, n_items, counter, amb, amb_count, st, s); break;
+    default: launch_points_depth<K, 32>(stats, grid, jobs, items, n_items, counter, amb, amb_count, st, s); break;
+  }
 }
 
+// `threads` per workgroup: Tuning::walk_block (the kernels hold no LDS, so
+// only the scheduling granularity changes).
 template <int MODE>
-void launch_rays_batch(bool stats, unsigned grid, const DeviceJob* jobs, const WorkItem* items,
-                       uint32_t n_items, uint32_t* counter, unsigned long long* st, hipStream_t s) {
+void launch_rays_batch(bool stats, unsigned grid, unsigned threads, const DeviceJob* jobs,
+                       const WorkItem* items, uint32_t n_items, uint32_t* counter, unsigned long long* st,
+                       hipStream_t s) {
   if (stats)
-    batch_rays_kernel<MODE, true><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, st);
+    batch_rays_kernel<MODE, true><<<grid, threads, 0, s>>>(jobs, items, n_items, counter, st);
   else
-    batch_rays_kernel<MODE, false><<<grid, 256, 0, s>>>(jobs, items, n_items, counter, st);
+    batch_rays_kernel<MODE, false><<<grid, threads, 0, s>>>(jobs, items, n_items, counter, st);
   RTDL_LAUNCH_CHECK();
 }
 
@@ -1431,9 +1446,17 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
     RTDL_LAUNCH_CHECK();
     items_run = items_sorted_.get();
   }
-  const unsigned max_grid = (unsigned)cus_ * current_tuning().blocks_per_cu;
+  const Tuning tuning = current_tuning();
+  const unsigned max_grid = (unsigned)cus_ * tuning.blocks_per_cu;
   auto grid_for = [&](size_t items) {
     return (unsigned)std::max<size_t>(1, std::min<size_t>((items + 3) / 4, max_grid));
+  };
+  // ray and PIP walks: workgroups of walk_block threads, the same wavefront
+  // budget (max_grid 256-thread workgroups) and one wavefront per item
+  const unsigned walk_block = tuning.walk_block, walk_waves = walk_block / 64;
+  auto walk_grid_for = [&](size_t items) {
+    return (unsigned)std::max<size_t>(
+        1, std::min<size_t>((items + walk_waves - 1) / walk_waves, (size_t)max_grid * 4 / walk_waves));
   };
   unsigned long long* st = stats_on_ ? stats_.get() : nullptr;
   const DeviceJob* dj = jobs_.get();
@@ -1504,20 +1527,21 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
         RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamRayAny) {
-      launch_rays_batch<0>(stats_on_, grid, dj, it, n_items, ctr, st, s);
+      launch_rays_batch<0>(stats_on_, walk_grid_for(n_items), walk_block, dj, it, n_items, ctr, st, s);
     } else if (f == kFamRayCount) {
-      launch_rays_batch<1>(stats_on_, grid, dj, it, n_items, ctr, st, s);
+      launch_rays_batch<1>(stats_on_, walk_grid_for(n_items), walk_block, dj, it, n_items, ctr, st, s);
     } else if (f == kFamRayClosest) {
-      launch_rays_batch<2>(stats_on_, grid, dj, it, n_items, ctr, st, s);
+      launch_rays_batch<2>(stats_on_, walk_grid_for(n_items), walk_block, dj, it, n_items, ctr, st, s);
     } else {
       const double eps_sq = std::pow(1.0e-12, 2.0);  // reference.py:757 `eps ** 2`
       uint32_t* over_count = amb_count_.get() + 1;
+      const unsigned wg = walk_grid_for(n_items);
       if (stats_on_)
-        batch_pip_kernel<true><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
-                                                    over_count, st);
+        batch_pip_kernel<true><<<wg, walk_block, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
+                                                         over_count, st);
       else
-        batch_pip_kernel<false><<<grid, 256, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
-                                                     over_count, st);
+        batch_pip_kernel<false><<<wg, walk_block, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
+                                                          over_count, st);
       RTDL_LAUNCH_CHECK();
       batch_pip_over_kernel<<<64, 64, 0, s>>>(dj, pip_over_.get(), over_count, eps_sq);
       RTDL_LAUNCH_CHECK();

@@ -279,6 +279,8 @@ void set_tuning(const Tuning& t) {
   require(t.triangle_leaf >= 1 && t.triangle_leaf <= kMaxLeafSize, "triangle_leaf must be in [1, 8]");
   require(t.polygon_leaf >= 1 && t.polygon_leaf <= kMaxLeafSize, "polygon_leaf must be in [1, 8]");
   require(t.blocks_per_cu >= 1 && t.blocks_per_cu <= 64, "blocks_per_cu must be in [1, 64]");
+  require(t.knn_stack == 16 || t.knn_stack == 32 || t.knn_stack == 48, "knn_stack must be 16, 32 or 48");
+  require(t.walk_block == 64 || t.walk_block == 128 || t.walk_block == 256, "walk_block must be 64, 128 or 256");
   std::lock_guard<std::mutex> lock(g_tuning_mutex);
   g_tuning = t;
 }

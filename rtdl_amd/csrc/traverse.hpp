@@ -167,7 +167,8 @@ __device__ inline void walk_neighbors(const SceneView& s, uint32_t root, uint32_
     walk_near_first<STATS>(s, root, end, qf, thr2, st, visit);
 }
 
-// Depth of the per-lane node stack of walk_nearest.
+// Default depth of the per-lane node stack of walk_nearest (the batched KNN
+// walk takes the depth as a tuned template variant).
 constexpr uint32_t kStackDepth = 32;
 
 // Nearest-child-first walk for a k-nearest search: at an internal node both
@@ -175,10 +176,10 @@ constexpr uint32_t kStackDepth = 32;
 // pushes the farther one onto the lane's stack (LDS, entry d at
 // stack[d * stride]); a popped node is tested again against the threshold,
 // which has shrunk meanwhile.  Near primitives arrive first, so the list's
-// k-th key prunes most of the tree.  Returns false when the stack was full at
-// some push and a subtree was left out: the caller then walks the whole tree
-// again, meeting some primitives a second time.
-template <bool STATS, class Visit>
+// k-th key prunes most of the tree.  Returns false when the stack (DEPTH
+// entries) was full at some push and a subtree was left out: the caller then
+// walks the whole tree again, meeting some primitives a second time.
+template <bool STATS, uint32_t DEPTH = kStackDepth, class Visit>
 __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t end, const float qf[3],
                                     float& thr2, LaneStats& st, Visit&& visit, uint32_t* stack,
                                     uint32_t stride) {
@@ -202,7 +203,7 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
       const bool right_first = dr < dl;
       const float dn = right_first ? dr : dl, df = right_first ? dl : dr;
       if (df <= thr2) {
-        if (depth < kStackDepth)
+        if (depth < DEPTH)
           stack[stride * depth++] = right_first ? l : r;
         else
           complete = false;
@@ -338,10 +339,10 @@ __device__ inline uint64_t pack_key(double d2, uint32_t slot) {
 }
 __device__ inline uint32_t key_bits(uint64_t e) { return (uint32_t)(e >> 32); }
 
-// `stack` (LDS, kStackDepth entries at stride `stride`) selects the
+// `stack` (LDS, DEPTH entries at stride `stride`) selects the
 // nearest-child-first walk for a search without a radius; without one the
 // ring walk is used.
-template <int K, bool STATS>
+template <int K, bool STATS, uint32_t DEPTH = kStackDepth>
 __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_t end, double qx,
                                       double qy, double qz, double radius, uint32_t k,
                                       PackedTopK<K>& list, LaneStats& st, uint32_t* stack = nullptr,
@@ -371,7 +372,7 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
     }
   };
   if (stack && !(radius < __builtin_inf())) {
-    if (!walk_nearest<STATS>(s, root, end, qf, thr2, st, visit, stack, stride)) {
+    if (!walk_nearest<STATS, DEPTH>(s, root, end, qf, thr2, st, visit, stack, stride)) {
       // the stack overflowed: offer every primitive again, skipping the
       // entries already in the list
       walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float d2f) {

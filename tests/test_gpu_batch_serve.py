@@ -522,7 +522,8 @@ def test_batch_graph_replays_match_eager_runs(gpu_ready):
 def test_autotune_picks_the_fastest_and_every_candidate_gives_the_same_rows(gpu_ready):
     from rtdl_amd import autotune as T
 
-    _say("case: autotuner over leaf sizes and launch sizes, rows checked per candidate")
+    _say("case: autotuner over leaf sizes, launch sizes, KNN stack depth and walk workgroup size, "
+         "rows checked per candidate")
     rng = np.random.default_rng(95)
     centres = rng.random((64, 3))
     _, s3 = _points3(rng, 30000, "mixture", centres)
@@ -549,12 +550,15 @@ def test_autotune_picks_the_fastest_and_every_candidate_gives_the_same_rows(gpu_
     original = T.get_tuning()
     try:
         out = T.autotune_batch(build, jobs, repeats=2,
-                               knobs={"blocks_per_cu": (4, 16), "point_leaf": (4, 8), "triangle_leaf": (2, 4)})
+                               knobs={"blocks_per_cu": (4, 16), "point_leaf": (4, 8), "triangle_leaf": (2, 4),
+                                      "knn_stack": (16, 48), "walk_block": (64, 128)})
         table = out["table"]
-        assert len(table) == 7
+        assert len(table) == 11  # the start and two values of each knob, every one with the start's rows
+        assert {r["knn_stack"] for r in table} == {16, 32, 48}
+        assert {r["walk_block"] for r in table} == {64, 128, 256}
         fastest = min(table, key=lambda r: r["seconds"])
-        assert out["best"] == T.Tuning(**{k: fastest[k] for k in ("point_leaf", "triangle_leaf",
-                                                                  "polygon_leaf", "blocks_per_cu")})
+        assert out["best"] == T.Tuning(**{k: fastest[k] for k in ("point_leaf", "triangle_leaf", "polygon_leaf",
+                                                                  "blocks_per_cu", "knn_stack", "walk_block")})
         assert T.get_tuning() == out["best"]
     finally:
         T.set_tuning(original)
