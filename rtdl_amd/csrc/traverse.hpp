@@ -53,6 +53,28 @@ __device__ inline void flush_lane_stats(unsigned long long* stats, LaneStats& st
 
 __device__ inline float scene_m(const SceneView& s) { return s.m_dev ? *s.m_dev : s.m; }
 
+// One step of a rope (preorder + escape) walk that fetches the node together
+// with its successor in preorder: a descent into an internal node continues at
+// node + 1, whose record is then already at hand and is tested in the same
+// step, so a chain of descents costs half the dependent fetches.  On return
+// `node` / `nd` are the node the step ends on and `in` its test;
+// test(record, index) -> bool.
+template <bool STATS, class Test>
+__device__ inline void rope_pair_step(const SceneView& s, uint32_t& node, BvhNode& nd, bool& in, LaneStats& st,
+                                      Test&& test) {
+  const uint32_t after = node + 1 < s.n_nodes ? node + 1 : node;
+  nd = load_node(s.nodes, node);
+  const BvhNode nd1 = load_node(s.nodes, after);
+  if (STATS) ++st.nodes;
+  in = test(nd, node);
+  if (in && !node_is_leaf(nd.link) && nd.link == after) {
+    node = after;
+    nd = nd1;
+    if (STATS) ++st.nodes;
+    in = test(nd, node);
+  }
+}
+
 // -------------------------------------------------------- point candidates --
 
 // FP64 squared distance of slot `slot` from (qx, qy, qz), reference.py:882-890:
@@ -83,12 +105,14 @@ __device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t e
   // a preorder walk visits each node at most once; the guard only bounds the
   // loop should the tree ever be malformed
   for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
-    const BvhNode nd = load_node(s.nodes, node);
-    if (STATS) ++st.nodes;
-    const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
-    const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
-    // `skip`: a subtree the caller has already walked
-    const bool in = (node != skip) & (box_dist2(qf, lo, hi) <= thr2);
+    BvhNode nd;
+    bool in;
+    rope_pair_step<STATS>(s, node, nd, in, st, [&](const BvhNode& b, uint32_t i) {
+      const float lo[3] = {b.lo[0], b.lo[1], b.lo[2]};
+      const float hi[3] = {b.hi[0], b.hi[1], b.hi[2]};
+      // `skip`: a subtree the caller has already walked
+      return (i != skip) & (box_dist2(qf, lo, hi) <= thr2);
+    });
     const bool leaf = node_is_leaf(nd.link);
     if (in && leaf) {  // the only branch of the step: the leaf's primitives
       const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
@@ -563,11 +587,13 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
   uint32_t node = root;
   bool done = false;
   for (uint32_t guard = s.n_nodes + 1; node != end && !done && guard; --guard) {
-    const BvhNode nd = load_node(s.nodes, node);
-    if (STATS) ++st.nodes;
-    const float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
-    const float hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
-    const bool in = segment_box(o32, D32, inv, lo, hi, pad, slim);
+    BvhNode nd;
+    bool in;
+    rope_pair_step<STATS>(s, node, nd, in, st, [&](const BvhNode& b, uint32_t) {
+      const float lo[3] = {b.lo[0], b.lo[1], b.lo[2]};
+      const float hi[3] = {b.hi[0], b.hi[1], b.hi[2]};
+      return segment_box(o32, D32, inv, lo, hi, pad, slim);
+    });
     const bool leaf = node_is_leaf(nd.link);
     // into an overlapped internal node's subtree, else past the node
     const uint32_t next = (in && !leaf) ? nd.link : nd.escape;
@@ -679,10 +705,11 @@ __device__ inline void walk_pip(const SceneView& s, uint32_t root, uint32_t end,
   const float pad = m * 9.5367431640625e-07f + 1.0e-11f;
   uint32_t node = root;
   for (uint32_t guard = s.n_nodes + 1; node != end && guard; --guard) {
-    const BvhNode nd = load_node(s.nodes, node);
-    if (STATS) ++st.nodes;
-    const bool in = (pf0 >= nd.lo[0] - pad) & (pf0 <= nd.hi[0] + pad) & (pf1 >= nd.lo[1] - pad) &
-                    (pf1 <= nd.hi[1] + pad);
+    BvhNode nd;
+    bool in;
+    rope_pair_step<STATS>(s, node, nd, in, st, [&](const BvhNode& b, uint32_t) {
+      return (pf0 >= b.lo[0] - pad) & (pf0 <= b.hi[0] + pad) & (pf1 >= b.lo[1] - pad) & (pf1 <= b.hi[1] + pad);
+    });
     const bool leaf = node_is_leaf(nd.link);
     // into an overlapped internal node's subtree, else past the node
     const uint32_t next = (in && !leaf) ? nd.link : nd.escape;
