@@ -159,6 +159,11 @@ struct Tuning {
 Tuning current_tuning();
 void set_tuning(const Tuning& t);  // validates; throws on out-of-range values
 
+// A tree node as stored, in depth-first preorder: the left child of an
+// internal node is the next record, `link` holds its right child (so a walk
+// can fetch both children at once); a leaf's `link` holds the leaf flag, its
+// point count - 1 (3 bits) and its first slot.  `escape` is the node a walk
+// continues with after this node's subtree.
 struct alignas(16) BvhNode {
   float lo[3];
   uint32_t link;

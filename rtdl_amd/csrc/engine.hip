@@ -468,9 +468,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
             first = leaf_first(nd.link);
             cnt = leaf_count(nd.link);
           }
-          uint32_t next = (in && !leaf) ? nd.link : nd.escape;
+          uint32_t next = (in && !leaf) ? node + 1 : nd.escape;  // the left child is the next record
           walking = next != sc.end && --guard != 0;
-          if (walking && in && !leaf && next == after) {  // descended: test the child now
+          if (walking && in && !leaf) {  // descended: test the child (fetched with its parent) now
             if (STATS) ++st.nodes;
             const bool in1 = box_dist2(qf, nd1.lo, nd1.hi) <= thr2;
             const bool leaf1 = node_is_leaf(nd1.link);
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
               first = leaf_first(nd1.link);
               cnt = leaf_count(nd1.link);
             }
-            next = (in1 && !leaf1) ? nd1.link : nd1.escape;
+            next = (in1 && !leaf1) ? after + 1 : nd1.escape;
             walking = next != sc.end && --guard != 0;
           }
           node = next;
@@ -1453,6 +1453,9 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
   };
   // ray and PIP walks: workgroups of walk_block threads, the same wavefront
   // budget (max_grid 256-thread workgroups) and one wavefront per item
+  // the exact fix-up: its 48 KB LDS lists fit three workgroups per CU, and
+  // every one of them takes a share of the flagged queries
+  const unsigned fix_grid = (unsigned)cus_ * 3;
   const unsigned walk_block = tuning.walk_block, walk_waves = walk_block / 64;
   auto walk_grid_for = [&](size_t items) {
     return (unsigned)std::max<size_t>(
@@ -1485,7 +1488,7 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
         default: launch_points<64>(stats_on_, grid, dj, it, n_items, ctr, amb_.get(), amb_count_.get(), st, s); break;
       }
       if (f == last_nbr) {
-        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
+        batch_fix_kernel<<<fix_grid, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
         RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamNbr) {
@@ -1523,7 +1526,7 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
         RTDL_LAUNCH_CHECK();
       }
       if (f == last_nbr) {
-        batch_fix_kernel<<<64, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
+        batch_fix_kernel<<<fix_grid, kFixBlock, 0, s>>>(dj, amb_.get(), amb_count_.get());
         RTDL_LAUNCH_CHECK();
       }
     } else if (f == kFamRayAny) {

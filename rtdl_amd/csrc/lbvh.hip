@@ -328,11 +328,13 @@ __device__ inline uint32_t escape_after(uint32_t last_leaf, uint32_t m,
   return (m - 1) + last_leaf + 1;
 }
 
-__global__ void links_kernel(uint32_t m, const uint32_t* left, const uint32_t* range_hi,
+// Internal records hold the right child (the left one is the next record in
+// preorder) and the rope.
+__global__ void links_kernel(uint32_t m, const uint32_t* right, const uint32_t* range_hi,
                              const unsigned long long* highest, BvhNode* nodes) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m - 1) return;
-  nodes[k].link = left[k];
+  nodes[k].link = right[k];
   nodes[k].escape = escape_after(range_hi[k], m, highest);
 }
 
@@ -434,7 +436,8 @@ __global__ void preorder_kernel(uint32_t m, const uint32_t* parent, const uint32
 }
 
 // Stores the nodes in preorder, so a walk that descends reads consecutive
-// records (the left child of an internal node is the next record).
+// records (the left child of an internal node is the next record; `link`
+// keeps the right child's new position).
 __global__ void permute_nodes_kernel(uint32_t n_nodes, const BvhNode* in, const uint32_t* pre,
                                      BvhNode* out) {
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -606,7 +609,7 @@ void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* c
         range_hi.get(), highest.get());
     RTDL_LAUNCH_CHECK();
     links_kernel<<<blocks_for(n_internal, 256), 256, 0, stream>>>(
-        (uint32_t)m, left.get(), range_hi.get(), highest.get(), scene.nodes.get());
+        (uint32_t)m, right.get(), range_hi.get(), highest.get(), scene.nodes.get());
     RTDL_LAUNCH_CHECK();
   }
   // 5. leaf records and ropes
@@ -853,7 +856,7 @@ void rebuild_scene(Scene& scene, const float* coords, hipStream_t stream) {
         w.range_hi.get(), w.highest.get());
     RTDL_LAUNCH_CHECK();
     links_kernel<<<blocks_for(w.n_internal, 256), 256, 0, stream>>>(
-        (uint32_t)w.m, w.left.get(), w.range_hi.get(), w.highest.get(), w.work_nodes.get());
+        (uint32_t)w.m, w.right.get(), w.range_hi.get(), w.highest.get(), w.work_nodes.get());
     RTDL_LAUNCH_CHECK();
   }
   leaf_nodes_kernel<<<blocks_for(w.m, 256), 256, 0, stream>>>(

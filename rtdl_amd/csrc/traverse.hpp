@@ -67,7 +67,7 @@ __device__ inline void rope_pair_step(const SceneView& s, uint32_t& node, BvhNod
   const BvhNode nd1 = load_node(s.nodes, after);
   if (STATS) ++st.nodes;
   in = test(nd, node);
-  if (in && !node_is_leaf(nd.link) && nd.link == after) {
+  if (in && !node_is_leaf(nd.link)) {  // the left child is the next record
     node = after;
     nd = nd1;
     if (STATS) ++st.nodes;
@@ -125,14 +125,16 @@ __device__ inline void walk_points(const SceneView& s, uint32_t root, uint32_t e
         if (d2f <= thr2) visit(slot, p, d2f);
       }
     }
-    // into an overlapped internal node's subtree, else past the node
-    node = (in && !leaf) ? nd.link : nd.escape;
+    // into an overlapped internal node's subtree (its left child, the next
+    // record), else past the node
+    node = (in && !leaf) ? node + 1 : nd.escape;
   }
 }
 
 // Neighbour walk in expanding rings around the query.  A greedy descent
-// (nearer child box first; the right child of an internal node is its left
-// child's escape) finds the query's leaf and its ancestors up[1..7].  The leaf
+// (nearer child box first; both children are fetched together, the left one
+// being the next record and the right one the node's link) finds the query's
+// leaf and its ancestors up[1..7].  The leaf
 // is walked first, then each ancestor's subtree without the part already
 // walked, then the rest of the tree: candidates arrive roughly nearest first,
 // so a shrinking threshold (a full k list) prunes early, and every primitive
@@ -147,17 +149,18 @@ __device__ inline void walk_near_first(const SceneView& s, uint32_t root, uint32
 #pragma unroll
   for (int i = 0; i <= kUp; ++i) up[i] = root;
   BvhNode nd = load_node(s.nodes, root);
+  uint32_t cur = root;
   if (STATS) ++st.nodes;
   for (uint32_t guard = s.n_nodes + 1; !node_is_leaf(nd.link) && guard; --guard) {
-    const uint32_t l = nd.link;
+    const uint32_t l = cur + 1, r = nd.link;  // both children, fetched together
     const BvhNode ln = load_node(s.nodes, l);
-    const uint32_t r = ln.escape;
     const BvhNode rn = load_node(s.nodes, r);
     if (STATS) st.nodes += 2;
     const bool right = box_dist2(qf, rn.lo, rn.hi) < box_dist2(qf, ln.lo, ln.hi);
 #pragma unroll
     for (int i = kUp; i > 0; --i) up[i] = up[i - 1];
-    up[0] = right ? r : l;
+    cur = right ? r : l;
+    up[0] = cur;
     nd = right ? rn : ln;
   }
   uint32_t done = kEndNode;  // root of the subtree walked so far
@@ -211,6 +214,7 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
   bool complete = true;
   uint32_t depth = 0;
   BvhNode nd = load_node(s.nodes, root);
+  uint32_t cur = root;
   if (STATS) ++st.nodes;
   if (!(box_dist2(qf, nd.lo, nd.hi) <= thr2)) return true;
   // every node is pushed at most once; the guard only bounds the loop should
@@ -218,9 +222,10 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
   for (uint32_t guard = s.n_nodes + 1; guard; --guard) {
     bool leaf = true;
     while (!node_is_leaf(nd.link)) {
-      const uint32_t l = nd.link;
+      // both children, fetched together: the left one is the next record,
+      // the right one the node's link
+      const uint32_t l = cur + 1, r = nd.link;
       const BvhNode ln = load_node(s.nodes, l);
-      const uint32_t r = ln.escape;  // the right child follows the left subtree
       const BvhNode rn = load_node(s.nodes, r);
       if (STATS) st.nodes += 2;
       const float dl = box_dist2(qf, ln.lo, ln.hi), dr = box_dist2(qf, rn.lo, rn.hi);
@@ -237,6 +242,7 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
         break;
       }
       nd = right_first ? rn : ln;
+      cur = right_first ? r : l;
     }
     if (leaf) {
       const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
@@ -251,7 +257,8 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
     }
     bool found = false;
     while (depth) {
-      nd = load_node(s.nodes, stack[stride * --depth]);
+      cur = stack[stride * --depth];
+      nd = load_node(s.nodes, cur);
       if (STATS) ++st.nodes;
       if (box_dist2(qf, nd.lo, nd.hi) <= thr2) {
         found = true;
@@ -596,7 +603,7 @@ __device__ inline RayResult trace_ray(const SceneView& s, uint32_t root, uint32_
     });
     const bool leaf = node_is_leaf(nd.link);
     // into an overlapped internal node's subtree, else past the node
-    const uint32_t next = (in && !leaf) ? nd.link : nd.escape;
+    const uint32_t next = (in && !leaf) ? node + 1 : nd.escape;  // left child: the next record
     if (!(in && leaf)) {
       node = next;
       continue;
@@ -712,7 +719,7 @@ __device__ inline void walk_pip(const SceneView& s, uint32_t root, uint32_t end,
     });
     const bool leaf = node_is_leaf(nd.link);
     // into an overlapped internal node's subtree, else past the node
-    const uint32_t next = (in && !leaf) ? nd.link : nd.escape;
+    const uint32_t next = (in && !leaf) ? node + 1 : nd.escape;  // left child: the next record
     if (!(in && leaf)) {
       node = next;
       continue;

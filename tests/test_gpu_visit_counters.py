@@ -100,8 +100,10 @@ def _check_boxes(sc, leaf_points):
     link = nodes["link"].astype(np.int64)
     leaf = (link & W.LEAF) != 0
     internal = np.flatnonzero(~leaf)
-    left = link[internal]
-    right = nodes["escape"][left].astype(np.int64)  # the right child follows the left subtree
+    left = internal + 1  # the next record
+    right = link[internal]
+    # the right child also follows the left subtree in preorder
+    assert np.array_equal(nodes["escape"][left].astype(np.int64), right)
     assert np.array_equal(lo[internal], np.minimum(lo[left], lo[right]))
     assert np.array_equal(hi[internal], np.maximum(hi[left], hi[right]))
     first = link[leaf] & W.START_MASK

@@ -170,8 +170,8 @@ def knn_counts(sc: dict, q, k: int, *, instance: int = 0, stack_depth: int = 32)
         while True:
             leaf = True
             while not (int(nodes[cur]["link"]) & LEAF):
-                left = int(nodes[cur]["link"])
-                right = int(nodes[left]["escape"])
+                # the left child is the next record, the right one the link
+                left, right = cur + 1, int(nodes[cur]["link"])
                 state["nodes"] += 2
                 dl, dr = box(left), box(right)
                 right_first = dr < dl
@@ -203,7 +203,7 @@ def knn_counts(sc: dict, q, k: int, *, instance: int = 0, stack_depth: int = 32)
             if box(node) <= state["thr2"]:
                 link = int(nodes[node]["link"])
                 if not link & LEAF:
-                    node = link
+                    node += 1
                     continue
                 leaf_tests(link, True)
             node = int(nodes[node]["escape"])
@@ -295,9 +295,10 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
                     if inside and link & LEAF:
                         leaf[i] = (link & START_MASK, ((link >> 28) & 7) + 1)
                     descend = inside and not link & LEAF
-                    node[i] = link if descend else int(nd["escape"])
+                    # into the left child (the next record), else past the node
+                    node[i] = cur + 1 if descend else int(nd["escape"])
                     walking[i] = node[i] != end
-                    if not (walking[i] and descend and node[i] == cur + 1):
+                    if not (walking[i] and descend):
                         break
             def drain() -> None:
                 nonlocal queue
@@ -440,7 +441,7 @@ def ray_anyhit_counts(sc: dict, ray, *, instance: int = 0) -> tuple[int, int]:
             continue
         link = int(nd["link"])
         if not link & LEAF:
-            node = link
+            node += 1  # the left child is the next record
             continue
         for slot in leaf_range(link):
             n_prims += 1
