@@ -242,7 +242,22 @@ __global__ __launch_bounds__(kFixBlock) void batch_fix_kernel(const DeviceJob* _
     const float qf[3] = {(float)qx, (float)qy, (float)qz};
     const float m = coord_bound(scene_m(j.scene), qx, qy, qz);
     const double radius_sq = j.radius * j.radius;  // reference.py:357
-    float thr2 = frn_threshold2(j.radius, m);
+    // A KNN query arrives with the k rows of its packed walk already written:
+    // k real points within their largest distance bound the k-th nearest
+    // distance, so the exact walk starts pruned to that ball.  (Fixed-radius
+    // queries arrive with their rows unwritten.)
+    double bound = j.radius;
+    if (j.kind != kJobFixedRadius && a.w == k) {
+      double dmax = 0.0;
+      for (uint32_t r = 0; r < a.w; ++r) {
+        double d;
+        uint32_t id;
+        read_neighbor_row(j, (uint64_t)a.z + r, d, id);
+        dmax = fmax(dmax, d);
+      }
+      bound = fmin(bound, dmax);
+    }
+    float thr2 = frn_threshold2(bound, m);
     uint32_t count = 0;  // entries [0, count) sorted by (distance, id)
     LaneStats st;
     if (j.scene.root == kEndNode) continue;
