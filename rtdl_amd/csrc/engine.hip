@@ -1561,7 +1561,7 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
         batch_pip_kernel<false><<<wg, walk_block, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
                                                           over_count, st);
       RTDL_LAUNCH_CHECK();
-      batch_pip_over_kernel<<<64, 64, 0, s>>>(dj, pip_over_.get(), over_count, eps_sq);
+      batch_pip_over_kernel<<<(unsigned)cus_ * 4, 64, 0, s>>>(dj, pip_over_.get(), over_count, eps_sq);
       RTDL_LAUNCH_CHECK();
     }
     if (timing_on_) RTDL_HIP(hipEventRecord(fam_stop_[f], s));

@@ -677,26 +677,46 @@ __device__ inline bool point_on_segment(double px, double py, double ax, double 
 }
 
 // reference.py:717-742
+// Vertex (x, y) pairs fetched together by point_in_polygon, so their memory
+// latencies overlap instead of following one another edge by edge.
+constexpr uint32_t kPipVertexChunk = 4;
+
+__device__ inline void load_vertex(const double* vertices, uint32_t k, double& x, double& y) {
+  const GlobalU32x4 w = load_raw16(vertices + 2 * (size_t)k);  // 16-byte aligned (x, y)
+  x = __builtin_bit_cast(double, ((uint64_t)w[1] << 32) | w[0]);
+  y = __builtin_bit_cast(double, ((uint64_t)w[3] << 32) | w[2]);
+}
+
 __device__ inline bool point_in_polygon(double x, double y, const double* vertices, uint32_t n,
                                         double eps_sq) {
-  const auto* v = (const __attribute__((address_space(1))) double*)vertices;  // device memory
   // One pass over the edges a = v[k] -> b = v[k + 1]: any edge holding the
   // point answers true (the reference's first loop), otherwise the
   // even-odd parity of the crossings decides (its second loop, whose pair
   // (i, j = i - 1) is (b, a) here; the parity does not depend on the order).
   bool inside = false;
-  double ax = v[0], ay = v[1];
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t kb = (k + 1 == n) ? 0 : k + 1;
-    const double bx = v[2 * kb], by = v[2 * kb + 1];
-    if (point_on_segment(x, y, ax, ay, bx, by, eps_sq)) return true;
-    if ((by > y) != (ay > y)) {
-      double den = ay - by;
-      if (den == 0.0) den = 1.0e-20;
-      if (x <= (ax - bx) * (y - by) / den + bx) inside = !inside;
+  double ax, ay;
+  load_vertex(vertices, 0, ax, ay);
+  for (uint32_t k0 = 0; k0 < n; k0 += kPipVertexChunk) {
+    double bx[kPipVertexChunk], by[kPipVertexChunk];
+#pragma unroll
+    for (uint32_t t = 0; t < kPipVertexChunk; ++t) {
+      const uint32_t k = k0 + t;
+      bx[t] = ax;
+      by[t] = ay;
+      if (k < n) load_vertex(vertices, k + 1 == n ? 0u : k + 1, bx[t], by[t]);
     }
-    ax = bx;
-    ay = by;
+#pragma unroll
+    for (uint32_t t = 0; t < kPipVertexChunk; ++t) {
+      if (k0 + t >= n) break;
+      if (point_on_segment(x, y, ax, ay, bx[t], by[t], eps_sq)) return true;
+      if ((by[t] > y) != (ay > y)) {
+        double den = ay - by[t];
+        if (den == 0.0) den = 1.0e-20;
+        if (x <= (ax - bx[t]) * (y - by[t]) / den + bx[t]) inside = !inside;
+      }
+      ax = bx[t];
+      ay = by[t];
+    }
   }
   return inside;
 }
