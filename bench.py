@@ -1065,9 +1065,10 @@ def main() -> int:
     ap.add_argument("--skip-baselines", action="store_true")
     ap.add_argument("--skip-configs", action="store_true", help="omit the other configurations' GPU lines")
     ap.add_argument("--no-config3-autotune", action="store_true")
-    ap.add_argument("--roofline-only", choices=KINDS, default=None,
+    ap.add_argument("--roofline-only", choices=KINDS + ("all",), default=None,
                     help="only build the scenes and measure this kind's roofline batch (the batch the "
-                         "bench line's roofline is quoted on), for profiling that batch alone")
+                         "bench line's roofline is quoted on), or every kind's in turn ('all'), for "
+                         "profiling those batches alone")
     ap.add_argument("--per-call-only", action="store_true",
                     help="run only the per-call 2M x 1M ray any-hit and 4M x 4M KNN k=16 calls")
     ap.add_argument("--config3-only", action="store_true", help="run only the 2,048-instance batch")
@@ -1130,11 +1131,13 @@ def main() -> int:
     roof_stream = make_stream(rng_roof, 0.25, args.rate, scenes)
     roof_offsets = assign_windows(rng_roof, roof_stream, scenes)
     if args.roofline_only:
+        only = list(KINDS) if args.roofline_only == "all" else [args.roofline_only]
         roof = roofline_leg(B, scenes, roof_stream, roof_offsets, pools_dev, args.roofline_queries,
-                            kinds=[KINDS.index(args.roofline_only)])
-        print(json.dumps({"roofline_only": args.roofline_only, "roofline_seed": ROOF_SEED,
-                          "max_prims": args.max_prims, "roofline_queries": args.roofline_queries,
-                          **roof[args.roofline_only]}), flush=True)
+                            kinds=[KINDS.index(k) for k in only])
+        for k in only:
+            print(json.dumps({"roofline_only": k, "roofline_seed": ROOF_SEED,
+                              "max_prims": args.max_prims, "roofline_queries": args.roofline_queries,
+                              **roof[k]}), flush=True)
         return 0
 
     step_s = STREAM_SECONDS / args.steps
