@@ -381,14 +381,23 @@ __device__ inline bool stage_packs(const SceneView& sc) { return sc.n_prims <= (
 // distance of each point, and the set of points counted in buckets up to the
 // final cutoff does not depend on the order they are drained in, so the rows
 // are exact as before.
-// Drains run after each point index of the leaves being prefiltered, so the
-// queue never holds more than 63 + 64 entries.
+// Leaves reach the prefilter through a second per-wavefront queue: a lane
+// that meets a leaf only queues (owner lane, leaf), and whenever 64 leaves are
+// queued, or no lane is still walking, the wavefront takes 64 of them, one per
+// lane, and prefilters their points against the owners' current thresholds.
+// Lanes meet leaves in different steps, so prefiltering in the walk itself
+// would run the leaf loop nearly every step for a few lanes.  The owners'
+// thresholds only shrink, so filtering later keeps every point that can still
+// be counted.
+// Point drains run after each point index of the leaves being prefiltered, so
+// the point queue never holds more than 63 + 64 entries.
 // A drain adds at most 64 slots to one stage, so after each drain a stage
 // holding more than stage_slots - 64 is compacted in place by the wavefront:
 // only slots whose bucket is within the owner's current cutoff (top) stay,
 // as the others can no longer be rows (top only decreases within a round).
 // A stage that compaction cannot bring under that mark is closed.
 constexpr uint32_t kNbrQueueCap = 128;
+constexpr uint32_t kNbrLeafQueueCap = 128;
 
 __device__ inline double shfl_f64(double x, uint32_t src) {
   const uint64_t b = __builtin_bit_cast(uint64_t, x);
@@ -420,6 +429,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
   __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
   __shared__ uint32_t q_slot[4 * kNbrQueueCap];
   __shared__ uint8_t q_owner[4 * kNbrQueueCap];
+  // queued leaves: the leaf's link word (count and first slot) and its owner
+  __shared__ uint32_t lq_link[4 * kNbrLeafQueueCap];
+  __shared__ uint8_t lq_owner[4 * kNbrLeafQueueCap];
   // per thread: neighbours counted this round (high word) and stage cursor
   // (low word), updated together by one LDS atomic
   __shared__ unsigned long long l_cs[256];
@@ -428,6 +440,8 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
   uint32_t* my_hist = hist + threadIdx.x;
   uint32_t* qs = q_slot + (threadIdx.x >> 6) * kNbrQueueCap;
   uint8_t* qo = q_owner + (threadIdx.x >> 6) * kNbrQueueCap;
+  uint32_t* lql = lq_link + (threadIdx.x >> 6) * kNbrLeafQueueCap;
+  uint8_t* lqo = lq_owner + (threadIdx.x >> 6) * kNbrLeafQueueCap;
   auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
@@ -447,6 +461,8 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
       qi = job_query(j, w.first + lane);
       load_query3(j, qi, qx, qy, qz);
     }
+    // queries are FP32, so qf holds them exactly: the FP64 coordinates are
+    // (double)qf and are not kept in registers
     const float qf[3] = {(float)qx, (float)qy, (float)qz};
     const float m = coord_bound(scene_m(sc), qx, qy, qz);
     const bool pack = stage_packs(sc);
@@ -465,10 +481,11 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
       uint32_t node = sc.root;
       bool walking = going && sc.root != kEndNode;
       uint32_t guard = sc.n_nodes + 1;  // a preorder walk visits each node at most once
-      uint32_t head = 0, size = 0;      // the wavefront's queue (uniform)
+      uint32_t head = 0, size = 0;      // the wavefront's point queue (uniform)
+      uint32_t lhead = 0, lsize = 0;    // the wavefront's leaf queue (uniform)
       for (;;) {
         if (STATS && lane == 0) ++st.steps;
-        uint32_t first = 0, cnt = 0;
+        uint32_t leaf_link = 0;  // the leaf this lane met in this step, if any
         if (walking) {
           // the node and the one after it in preorder, fetched together: a
           // descent goes there next, and its test then costs no second
@@ -479,20 +496,14 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
           if (STATS) ++st.nodes;
           const bool in = box_dist2(qf, nd.lo, nd.hi) <= thr2;
           const bool leaf = node_is_leaf(nd.link);
-          if (in && leaf) {
-            first = leaf_first(nd.link);
-            cnt = leaf_count(nd.link);
-          }
+          if (in && leaf) leaf_link = nd.link;
           uint32_t next = (in && !leaf) ? node + 1 : nd.escape;  // the left child is the next record
           walking = next != sc.end && --guard != 0;
           if (walking && in && !leaf) {  // descended: test the child (fetched with its parent) now
             if (STATS) ++st.nodes;
             const bool in1 = box_dist2(qf, nd1.lo, nd1.hi) <= thr2;
             const bool leaf1 = node_is_leaf(nd1.link);
-            if (in1 && leaf1) {
-              first = leaf_first(nd1.link);
-              cnt = leaf_count(nd1.link);
-            }
+            if (in1 && leaf1) leaf_link = nd1.link;
             next = (in1 && !leaf1) ? after + 1 : nd1.escape;
             walking = next != sc.end && --guard != 0;
           }
@@ -509,7 +520,8 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
           const uint32_t slot = qs[e];
           const uint32_t o = has ? qo[e] : lane;
           // the owner's query and ball (every lane takes part in the shuffles)
-          const double ox = shfl_f64(qx, o), oy = shfl_f64(qy, o), oz = shfl_f64(qz, o);
+          const double ox = __shfl(qf[0], (int)o, 64), oy = __shfl(qf[1], (int)o, 64);
+          const double oz = __shfl(qf[2], (int)o, 64);
           const double orsq = shfl_f64(r.radius_sq, o), oscale = shfl_f64(scale, o);
           const uint32_t otop = __shfl(top, (int)o, 64);
           if (has) {
@@ -551,7 +563,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
           while (full) {
             const uint32_t o = (uint32_t)__builtin_ctzll(full);
             full &= full - 1;
-            const double ox = readlane_f64(qx, o), oy = readlane_f64(qy, o), oz = readlane_f64(qz, o);
+            const double ox = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(qf[0]), o));
+            const double oy = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(qf[1]), o));
+            const double oz = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(qf[2]), o));
             const double oscale = readlane_f64(scale, o);
             const uint32_t otop = __builtin_amdgcn_readlane(top, o);
             const uint32_t n_st = min(__builtin_amdgcn_readlane(cur, o), stage_slots);
@@ -583,31 +597,59 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
           }
           wave_fence();
         };
-        // FP32 prefilter of the leaf points; survivors are queued in (point
-        // index within the leaf, lane) order, with drains between indices
-        for (uint32_t e = 0;; ++e) {
-          const bool act = e < cnt;
-          if (!__ballot(act)) break;
-          bool push = false;
-          const uint32_t slot = first + e;
-          if (act) {
-            const float4 p = load_prim(sc.prims, slot);
-            if (STATS) ++st.prims;
-            const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
-            push = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)) <= thr2;
+        // One leaf drain: the first min(64, lsize) queued leaves, one per
+        // lane; their points are prefiltered in FP32 against the owner's
+        // current threshold and the survivors queued in (point index within
+        // the leaf, queue position) order, with point drains between indices.
+        auto leaf_drain = [&]() {
+          const uint32_t n = lsize < 64 ? lsize : 64;
+          wave_fence();
+          const bool has = lane < n;
+          const uint32_t e = (lhead + (has ? lane : 0u)) & (kNbrLeafQueueCap - 1);
+          const uint32_t link = lql[e];
+          const uint32_t o = has ? lqo[e] : lane;
+          // the owner's query and threshold (every lane takes part in the shuffles)
+          const float ox = __shfl(qf[0], (int)o, 64), oy = __shfl(qf[1], (int)o, 64);
+          const float oz = __shfl(qf[2], (int)o, 64), othr2 = __shfl(thr2, (int)o, 64);
+          const uint32_t first = leaf_first(link), cnt = has ? leaf_count(link) : 0u;
+          lhead = (lhead + n) & (kNbrLeafQueueCap - 1);
+          lsize -= n;
+          for (uint32_t i = 0;; ++i) {
+            const bool act = i < cnt;
+            if (!__ballot(act)) break;
+            bool push = false;
+            const uint32_t slot = first + i;
+            if (act) {
+              const float4 p = load_prim(sc.prims, slot);
+              if (STATS) ++st.prims;
+              const float fx = p.x - ox, fy = p.y - oy, fz = p.z - oz;
+              push = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)) <= othr2;
+            }
+            const uint64_t pm = __ballot(push);
+            if (push) {
+              const uint32_t pos = (head + size + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))) & (kNbrQueueCap - 1);
+              qs[pos] = slot;
+              qo[pos] = (uint8_t)o;
+            }
+            size += (uint32_t)__popcll(pm);
+            while (size >= 64) drain();
           }
-          const uint64_t pm = __ballot(push);
-          if (push) {
-            const uint32_t pos = (head + size + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))) & (kNbrQueueCap - 1);
-            qs[pos] = slot;
-            qo[pos] = (uint8_t)lane;
-          }
-          size += (uint32_t)__popcll(pm);
-          while (size >= 64) drain();
+          wave_fence();
+        };
+        // the leaves met in this step join the leaf queue in lane order
+        const uint64_t lm = __ballot(leaf_link != 0u);
+        if (leaf_link != 0u) {
+          const uint32_t pos = (lhead + lsize + (uint32_t)__popcll(lm & ((1ull << lane) - 1ull))) & (kNbrLeafQueueCap - 1);
+          lql[pos] = leaf_link;
+          lqo[pos] = (uint8_t)lane;
         }
-        const bool any_walk = __ballot(walking) != 0;
-        while (!any_walk && size != 0) drain();
-        if (!any_walk && size == 0) break;
+        lsize += (uint32_t)__popcll(lm);
+        while (lsize >= 64) leaf_drain();
+        if (__ballot(walking) == 0) {  // the round's walks are over: empty both queues
+          while (lsize != 0) leaf_drain();
+          while (size != 0) drain();
+          break;
+        }
       }
       wave_fence();
       if (going) {

@@ -237,16 +237,21 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
     The wavefront runs in lock step.  Each iteration every walking lane takes
     one node step of its rope preorder walk, and a second one when the first
     descends (to the child stored right after its parent, fetched with it),
-    with the same threshold; the points of the leaf it reaches that pass
-    the FP32 prefilter are queued (point index within the leaf first, then
-    lane), and after each point index, while 64 or more are queued (and, once
-    no lane walks, while any are), a batch of up to 64 is taken in queue order: each point inside the
-    owner's FP64 radius whose distance bucket does not lie beyond the owner's
-    last counted bucket is counted; then every lane whose count reached its
-    checkpoint (every max(k/2, 4)) shrinks its threshold to the edge of the
-    bucket holding its k-th neighbour.  Rounds are wavefront-synchronous: a
-    lane whose buckets up to the k-th still hold more than 128 neighbours
-    counts again over the finer buckets of that ball (at most four rounds)."""
+    with the same threshold; the leaf it reaches joins a leaf queue (lane
+    order).  While 64 or more leaves are queued (and, once no lane walks,
+    while any are), a batch of up to 64 is taken in queue order and the
+    points of its leaves are prefiltered in FP32 against their owners'
+    current thresholds; the survivors join a point queue (point index within
+    the leaf first, then position in the batch), and after each point index,
+    while 64 or more points are queued (and, once no lane walks and the leaf
+    queue is empty, while any are), a batch of up to 64 is taken in queue
+    order: each point inside the owner's FP64 radius whose distance bucket
+    does not lie beyond the owner's last counted bucket is counted; then
+    every lane whose count reached its checkpoint (every max(k/2, 4)) shrinks
+    its threshold to the edge of the bucket holding its k-th neighbour.
+    Rounds are wavefront-synchronous: a lane whose buckets up to the k-th
+    still hold more than 128 neighbours counts again over the finer buckets
+    of that ball (at most four rounds)."""
     nodes, prims, exact = sc["nodes"], sc["prims"], sc["exact"]
     root, end = int(sc["roots"][instance]), int(sc["ends"][instance])
     lanes = len(queries)
@@ -280,6 +285,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
         node = [root] * lanes
         walking = [going[i] and root != end for i in range(lanes)]
         queue: list[tuple[int, int]] = []
+        leaves: list[tuple[int, int, int]] = []  # (owner, first, count)
         while True:
             n_steps += 1
             leaf = [(0, 0)] * lanes
@@ -325,25 +331,35 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
                             thr2[i] = frn_threshold2(min(rad[i], _nbr_bucket_edge(c, scale[i])), m[i])
                             break
 
-            for e in range(8):
-                if not any(e < leaf[i][1] for i in range(lanes)):
-                    break
-                for i in range(lanes):
-                    first, cnt = leaf[i]
-                    if e >= cnt:
-                        continue
-                    n_prims += 1
-                    slot = first + e
-                    p = prims[slot]
-                    fx, fy, fz = F32(p[0] - qf[i][0]), F32(p[1] - qf[i][1]), F32(p[2] - qf[i][2])
-                    if fma32(fz, fz, fma32(fy, fy, F32(fx * fx))) <= thr2[i]:
-                        queue.append((i, slot))
-                while len(queue) >= 64:
+            def leaf_drain() -> None:
+                nonlocal leaves, n_prims
+                batch, leaves = leaves[:64], leaves[64:]
+                limits = [thr2[o] for o, _, _ in batch]  # the owners' thresholds when taken
+                for e in range(8):
+                    if not any(e < cnt for _, _, cnt in batch):
+                        break
+                    for (o, first, cnt), lim in zip(batch, limits):
+                        if e >= cnt:
+                            continue
+                        n_prims += 1
+                        slot = first + e
+                        p = prims[slot]
+                        fx, fy, fz = F32(p[0] - qf[o][0]), F32(p[1] - qf[o][1]), F32(p[2] - qf[o][2])
+                        if fma32(fz, fz, fma32(fy, fy, F32(fx * fx))) <= lim:
+                            queue.append((o, slot))
+                    while len(queue) >= 64:
+                        drain()
+
+            for i in range(lanes):
+                if leaf[i][1]:
+                    leaves.append((i, leaf[i][0], leaf[i][1]))
+            while len(leaves) >= 64:
+                leaf_drain()
+            if not any(walking):
+                while leaves:
+                    leaf_drain()
+                while queue:
                     drain()
-            any_walk = any(walking)
-            while not any_walk and queue:
-                drain()
-            if not any_walk and not queue:
                 break
         for i in range(lanes):
             if not going[i]:
