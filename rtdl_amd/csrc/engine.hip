@@ -419,6 +419,145 @@ __device__ inline double readlane_f64(double x, uint32_t q) {
 
 __device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+__device__ inline uint64_t shfl_xor64(uint64_t x, uint32_t m) {
+  const uint32_t lo = __shfl_xor((uint32_t)x, (int)m, 64);
+  const uint32_t hi = __shfl_xor((uint32_t)(x >> 32), (int)m, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// (distance bits, id) order; distances are non-negative, so their bit
+// patterns order like the values.
+__device__ inline bool dist_id_less(uint64_t da, uint32_t ia, uint64_t db, uint32_t ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+// One bitonic compare-exchange of element `idx` (held by this lane) with
+// element idx ^ stride (held by lane ^ stride).
+__device__ inline void bitonic_lanes(uint64_t& d, uint32_t& id, uint32_t idx, uint32_t size,
+                                     uint32_t stride) {
+  const uint64_t od = shfl_xor64(d, stride);
+  const uint32_t oi = __shfl_xor(id, (int)stride, 64);
+  const bool keep_min = ((idx & stride) == 0) == ((idx & size) == 0);
+  const bool take = keep_min ? dist_id_less(od, oi, d, id) : dist_id_less(d, id, od, oi);
+  d = take ? od : d;
+  id = take ? oi : id;
+}
+
+// Sorts n <= 128 (distance bits, id) pairs held two per lane (element lane in
+// (d0, i0), element lane + 64 in (d1, i1); pads are ~0) with a bitonic
+// network over the next power of two >= n.  Every lane must take part.
+__device__ inline void wave_sort_dist_id(uint64_t& d0, uint32_t& i0, uint64_t& d1, uint32_t& i1,
+                                         uint32_t n) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t size_max = 2;
+  while (size_max < n) size_max <<= 1;
+  for (uint32_t size = 2; size <= size_max; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride == 64) {  // element lane vs lane + 64, one ascending block
+        const bool sw = dist_id_less(d1, i1, d0, i0);
+        const uint64_t td = d0;
+        const uint32_t ti = i0;
+        d0 = sw ? d1 : d0;
+        i0 = sw ? i1 : i0;
+        d1 = sw ? td : d1;
+        i1 = sw ? ti : i1;
+      } else {
+        bitonic_lanes(d0, i0, lane, size, stride);
+        if (size_max > 64) bitonic_lanes(d1, i1, lane + 64, size, stride);
+      }
+    }
+  }
+}
+
+// Writes rows 1..n_rows of a query from n <= kNbrCandMax (distance bits, id)
+// pairs in (distance, id) order.  Element `lane` is in (d0, i0), element
+// lane + 64 in (d1, i1) (pads ~0).  p_lo, p_hi and p_id are three LDS planes
+// of kNbrCandMax words of this wavefront; `in_planes` says the n pairs are
+// already there (element x at index x).  Every lane must take part.
+__device__ inline void nbr_write_sorted(const DeviceJob& j, uint64_t row_at, uint32_t qi, uint32_t n_rows,
+                                        uint32_t n, uint64_t d0, uint32_t i0, uint64_t d1, uint32_t i1,
+                                        uint32_t* p_lo, uint32_t* p_hi, uint32_t* p_id, bool in_planes) {
+  const uint32_t lane = threadIdx.x & 63;
+  if (n <= 64) {
+    // rank sort: element `lane` goes to the number of elements before it in
+    // (distance, id) order (the pairs are distinct: ids are), found by reading
+    // every element from LDS as a broadcast
+    if (!in_planes && lane < n) {
+      p_lo[lane] = (uint32_t)d0;
+      p_hi[lane] = (uint32_t)(d0 >> 32);
+      p_id[lane] = i0;
+    }
+    wave_fence();
+    uint32_t rank = 0;
+#pragma unroll 4
+    for (uint32_t x = 0; x < n; ++x) {
+      const uint64_t dx = ((uint64_t)p_hi[x] << 32) | p_lo[x];
+      rank += dist_id_less(dx, p_id[x], d0, i0) ? 1u : 0u;
+    }
+    wave_fence();  // every lane has read the planes before they are written again
+    if (lane < n && rank < n_rows)
+      write_neighbor_row(j, row_at + rank, qi, i0, __builtin_bit_cast(double, d0), rank + 1);
+  } else {
+    wave_sort_dist_id(d0, i0, d1, i1, n);
+    if (lane < n_rows)
+      write_neighbor_row(j, row_at + lane, qi, i0, __builtin_bit_cast(double, d0), lane + 1);
+  }
+}
+
+// Sorts one staged query's rows: recomputes the FP64 distances of the slots
+// the count walk staged, keeps buckets 0..bucket (all of them for a query
+// with at most k neighbours, n_cand == 0) and writes the first n_rows in
+// (distance, id) order.  Every lane must take part.
+__device__ inline void nbr_sort_staged(const DeviceJob& j, const uint32_t* sq, uint32_t n_staged, uint32_t n_rows,
+                                       uint32_t n_cand, uint32_t bucket, double scale, double qx, double qy,
+                                       double qz, uint64_t row_at, uint32_t qi, uint32_t* p_lo, uint32_t* p_hi,
+                                       uint32_t* p_id) {
+  const uint32_t lane = threadIdx.x & 63;
+  const bool packed = stage_packs(j.scene);
+  uint32_t kept = 0;
+  for (uint32_t e0 = 0; e0 < n_staged; e0 += 64) {
+    const uint32_t e = e0 + lane;
+    bool keep = false;
+    uint64_t d = 0;
+    uint32_t id = 0;
+    if (e < n_staged) {
+      const uint32_t v = sq[e];
+      const uint32_t slot = packed ? v & kStageSlotMask : v;
+      // a packed bucket past the k-th one is dropped without a fetch
+      if (!packed || n_cand == 0 || (v >> kStageSlotBits) <= bucket) {
+        const float4 p = load_prim(j.scene.prims, slot);
+        const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
+        keep = n_cand == 0 || nbr_bucket(dist, scale) <= bucket;
+        d = __builtin_bit_cast(uint64_t, dist);
+        id = __float_as_uint(p.w);
+      }
+    }
+    const unsigned long long mask = __ballot(keep);
+    const uint32_t at = kept + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (keep && at < kNbrCandMax) {
+      p_lo[at] = (uint32_t)d;
+      p_hi[at] = (uint32_t)(d >> 32);
+      p_id[at] = id;
+    }
+    kept += (uint32_t)__popcll(mask);
+  }
+  wave_fence();
+  if (kept > kNbrCandMax && lane == 0) atomicOr(&j.counters->status, kStatusInternal);
+  const uint32_t n = kept < kNbrCandMax ? kept : kNbrCandMax;
+  uint64_t d0 = ~0ull, d1 = ~0ull;
+  uint32_t i0 = ~0u, i1 = ~0u;
+  if (lane < n) {
+    d0 = ((uint64_t)p_hi[lane] << 32) | p_lo[lane];
+    i0 = p_id[lane];
+  }
+  if (lane + 64 < n) {
+    d1 = ((uint64_t)p_hi[lane + 64] << 32) | p_lo[lane + 64];
+    i1 = p_id[lane + 64];
+  }
+  nbr_write_sorted(j, row_at, qi, n_rows, n, d0, i0, d1, i1, p_lo, p_hi, p_id, true);
+  wave_fence();  // the planes are free again once every lane has read them
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
@@ -469,8 +608,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
     uint32_t* const stage_it = stage + (size_t)it * 64 * stage_slots;
     NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false, false};  // reference.py:357
     bool going = active;  // counts in the current round
+    double scale = 0.0;  // bucket scale of the lane's last counting round
     for (uint32_t round = 0; round < 4 && __ballot(going); ++round) {
-      const double scale = (double)kNbrBuckets / r.radius;
+      scale = (double)kNbrBuckets / r.radius;
       float thr2 = frn_threshold2(r.radius, m);
       uint32_t top = kNbrBuckets - 1, next_check = k;
       bool shrunk = false;
@@ -716,6 +856,29 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
       const uint32_t base = lane0_add(emit_count, (uint32_t)__popcll(em));
       if (emit) emit_list[base + (uint32_t)__popcll(em & ((1ull << lane) - 1ull))] = it * 64 + lane;
     }
+    // Staged queries are sorted here, one at a time by the whole wavefront,
+    // while their stages and points are still in this XCD's L2 (the sort
+    // kernel only takes the emit list).  Between items the wavefront's point
+    // and leaf queues and its stage cursors are free: they hold the sort's
+    // three LDS planes of kNbrCandMax words.
+    uint64_t todo = __ballot(staged);
+    if (todo) {
+      wave_fence();
+      uint32_t* const p_id = reinterpret_cast<uint32_t*>(l_cs + wbase);
+      const uint32_t n_cand = r.heavy ? r.n_cand : 0u;
+      while (todo) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(todo);
+        todo &= todo - 1;
+        const double ox = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(qf[0]), q));
+        const double oy = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(qf[1]), q));
+        const double oz = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(qf[2]), q));
+        nbr_sort_staged(j, stage_it + (size_t)q * stage_slots, __builtin_amdgcn_readlane(r.n_staged, q),
+                        __builtin_amdgcn_readlane(n_rows, q), __builtin_amdgcn_readlane(n_cand, q),
+                        __builtin_amdgcn_readlane(r.bucket, q), readlane_f64(scale, q), ox, oy, oz,
+                        readlane64(row_at, q), __builtin_amdgcn_readlane(qi, q), qs, lql, p_id);
+      }
+      wave_fence();
+    }
   }
   if (STATS) {
     st.mark = st.nodes;  // steps were counted per iteration
@@ -776,179 +939,53 @@ __global__ __launch_bounds__(256) void nbr_emit_kernel(const DeviceJob* __restri
   }
 }
 
-__device__ inline uint64_t shfl_xor64(uint64_t x, uint32_t m) {
-  const uint32_t lo = __shfl_xor((uint32_t)x, (int)m, 64);
-  const uint32_t hi = __shfl_xor((uint32_t)(x >> 32), (int)m, 64);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// (distance bits, id) order; distances are non-negative, so their bit
-// patterns order like the values.
-__device__ inline bool dist_id_less(uint64_t da, uint32_t ia, uint64_t db, uint32_t ib) {
-  return da < db || (da == db && ia < ib);
-}
-
-// One bitonic compare-exchange of element `idx` (held by this lane) with
-// element idx ^ stride (held by lane ^ stride).
-__device__ inline void bitonic_lanes(uint64_t& d, uint32_t& id, uint32_t idx, uint32_t size,
-                                     uint32_t stride) {
-  const uint64_t od = shfl_xor64(d, stride);
-  const uint32_t oi = __shfl_xor(id, (int)stride, 64);
-  const bool keep_min = ((idx & stride) == 0) == ((idx & size) == 0);
-  const bool take = keep_min ? dist_id_less(od, oi, d, id) : dist_id_less(d, id, od, oi);
-  d = take ? od : d;
-  id = take ? oi : id;
-}
-
-// Sorts n <= 128 (distance bits, id) pairs held two per lane (element lane in
-// (d0, i0), element lane + 64 in (d1, i1); pads are ~0) with a bitonic
-// network over the next power of two >= n.  Every lane must take part.
-__device__ inline void wave_sort_dist_id(uint64_t& d0, uint32_t& i0, uint64_t& d1, uint32_t& i1,
-                                         uint32_t n) {
-  const uint32_t lane = threadIdx.x & 63;
-  uint32_t size_max = 2;
-  while (size_max < n) size_max <<= 1;
-  for (uint32_t size = 2; size <= size_max; size <<= 1) {
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      if (stride == 64) {  // element lane vs lane + 64, one ascending block
-        const bool sw = dist_id_less(d1, i1, d0, i0);
-        const uint64_t td = d0;
-        const uint32_t ti = i0;
-        d0 = sw ? d1 : d0;
-        i0 = sw ? i1 : i0;
-        d1 = sw ? td : d1;
-        i1 = sw ? ti : i1;
-      } else {
-        bitonic_lanes(d0, i0, lane, size, stride);
-        if (size_max > 64) bitonic_lanes(d1, i1, lane + 64, size, stride);
-      }
-    }
-  }
-}
-
+// Sorts the rows of the queries of the emit list (the count walk sorts the
+// staged ones itself), one query at a time per wavefront: parked candidates
+// (the first n_rows of them in (distance, id) order are the rows) or rows
+// the emit walk wrote in walk order.
 __global__ __launch_bounds__(kSortBlock) void nbr_sort_kernel(const DeviceJob* __restrict__ jobs,
                                                              const WorkItem* __restrict__ items,
-                                                             uint32_t n_items, uint32_t* item_counter,
                                                              const NbrPlan* __restrict__ plans,
                                                              const NbrCand* __restrict__ cands,
-                                                             const uint32_t* __restrict__ stage,
-                                                             uint32_t stage_slots) {
-  // (distance bits low, high, id, 0) of the query being sorted
-  __shared__ uint4 pack[kNbrCandMax];
+                                                             const uint32_t* __restrict__ emit_list,
+                                                             const uint32_t* emit_count, uint32_t* counter) {
+  __shared__ uint32_t planes[3 * kNbrCandMax];
   const uint32_t lane = threadIdx.x;
-  ItemQueue queue(item_counter, n_items);
+  const uint32_t n_list = __builtin_amdgcn_readfirstlane(*emit_count);
   for (;;) {
-    const uint32_t it = queue.next();
-    if (it == kItemDone) break;
-    if (it == kItemRetry) continue;
-    const WorkItem w = items[it];
-    const DeviceJob& j = jobs[w.job];
-    // each lane loads one query's plan and coordinates; the loop below takes
-    // query q's from lane q (readlane) instead of 64 dependent loads
-    const NbrPlan mine = lane < w.count ? plans[(size_t)it * 64 + lane] : NbrPlan{};
-    double mqx = 0.0, mqy = 0.0, mqz = 0.0;
-    if (mine.staged) load_query3(j, mine.qi, mqx, mqy, mqz);
-    for (uint32_t q = 0; q < w.count; ++q) {  // one query at a time, all lanes
-      NbrPlan pl;
-      pl.row_at = readlane64(mine.row_at, q);
-      pl.cand_at = readlane64(mine.cand_at, q);
-      pl.radius = readlane_f64(mine.radius, q);
-      pl.radius_sq = readlane_f64(mine.radius_sq, q);
-      pl.n_rows = __builtin_amdgcn_readlane(mine.n_rows, q);
-      pl.n_cand = __builtin_amdgcn_readlane(mine.n_cand, q);
-      pl.bucket = __builtin_amdgcn_readlane(mine.bucket, q);
-      pl.qi = __builtin_amdgcn_readlane(mine.qi, q);
-      pl.staged = __builtin_amdgcn_readlane(mine.staged, q);
-      if (pl.n_rows == 0 || (pl.n_cand == 0 && pl.n_rows == 1 && !pl.staged)) continue;
-      uint64_t d0 = ~0ull, d1 = ~0ull;
-      uint32_t i0 = ~0u, i1 = ~0u;
-      uint32_t n;
-      if (pl.staged) {
-        // the count walk's slots: recompute the FP64 distances, keep buckets
-        // 0..b (all of them for a query with at most k neighbours) and pack
-        // the kept ones, at most kNbrCandMax, into LDS
-        const double qx = readlane_f64(mqx, q), qy = readlane_f64(mqy, q), qz = readlane_f64(mqz, q);
-        const double scale = (double)kNbrBuckets / pl.radius;
-        const uint32_t* sq = stage + ((size_t)it * 64 + q) * stage_slots;
-        const bool packed = stage_packs(j.scene);
-        uint32_t kept = 0;
-        for (uint32_t e0 = 0; e0 < pl.staged; e0 += 64) {
-          const uint32_t e = e0 + lane;
-          bool keep = false;
-          uint64_t d = 0;
-          uint32_t id = 0;
-          if (e < pl.staged) {
-            const uint32_t v = sq[e];
-            const uint32_t slot = packed ? v & kStageSlotMask : v;
-            // a packed bucket past the k-th one is dropped without a fetch
-            if (!packed || pl.n_cand == 0 || (v >> kStageSlotBits) <= pl.bucket) {
-              const float4 p = load_prim(j.scene.prims, slot);
-              const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
-              keep = pl.n_cand == 0 || nbr_bucket(dist, scale) <= pl.bucket;
-              d = __builtin_bit_cast(uint64_t, dist);
-              id = __float_as_uint(p.w);
-            }
-          }
-          const unsigned long long mask = __ballot(keep);
-          const uint32_t at = kept + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-          if (keep && at < kNbrCandMax) pack[at] = make_uint4((uint32_t)d, (uint32_t)(d >> 32), id, 0u);
-          kept += (uint32_t)__popcll(mask);
-        }
-        __syncthreads();
-        if (kept > kNbrCandMax && lane == 0) atomicOr(&j.counters->status, kStatusInternal);
-        n = kept < kNbrCandMax ? kept : kNbrCandMax;
-        if (lane < n) {
-          const uint4 e = pack[lane];
-          d0 = ((uint64_t)e.y << 32) | e.x;
-          i0 = e.z;
-        }
-        if (lane + 64 < n) {
-          const uint4 e = pack[lane + 64];
-          d1 = ((uint64_t)e.y << 32) | e.x;
-          i1 = e.z;
-        }
-      } else if (pl.n_cand == 0) {  // rows written in walk order, sorted in place
-        n = pl.n_rows;
-        if (lane < n) {
-          double dist;
-          read_neighbor_row(j, pl.row_at + lane, dist, i0);
-          d0 = __builtin_bit_cast(uint64_t, dist);
-        }
-      } else {  // parked candidates; the first n_rows of them are the rows
-        n = pl.n_cand;
-        if (lane < n) {
-          const NbrCand c = cands[pl.cand_at + lane];
-          d0 = __builtin_bit_cast(uint64_t, c.distance);
-          i0 = c.id;
-        }
-        if (lane + 64 < n) {
-          const NbrCand c = cands[pl.cand_at + lane + 64];
-          d1 = __builtin_bit_cast(uint64_t, c.distance);
-          i1 = c.id;
-        }
+    const uint32_t i = lane0_add(counter, 1u);
+    if (i >= n_list) break;
+    const uint32_t p = __builtin_amdgcn_readfirstlane(emit_list[i]);
+    const NbrPlan pl = plans[p];
+    const uint32_t n_rows = __builtin_amdgcn_readfirstlane(pl.n_rows);
+    const uint32_t n_cand = __builtin_amdgcn_readfirstlane(pl.n_cand);
+    if (n_rows == 0 || (n_cand == 0 && n_rows == 1)) continue;
+    const DeviceJob& j = jobs[items[p >> 6].job];
+    uint64_t d0 = ~0ull, d1 = ~0ull;
+    uint32_t i0 = ~0u, i1 = ~0u;
+    uint32_t n;
+    if (n_cand == 0) {  // rows written in walk order, sorted in place
+      n = n_rows;
+      if (lane < n) {
+        double dist;
+        read_neighbor_row(j, pl.row_at + lane, dist, i0);
+        d0 = __builtin_bit_cast(uint64_t, dist);
       }
-      if (n <= 64) {
-        // rank sort: element `lane` goes to the number of elements before it
-        // in (distance, id) order (the pairs are distinct: ids are), found
-        // by reading every element from LDS as a broadcast
-        if (!pl.staged && lane < n) pack[lane] = make_uint4((uint32_t)d0, (uint32_t)(d0 >> 32), i0, 0u);
-        __syncthreads();
-        uint32_t rank = 0;
-#pragma unroll 4
-        for (uint32_t x = 0; x < n; ++x) {
-          const uint4 e = pack[x];
-          rank += dist_id_less(((uint64_t)e.y << 32) | e.x, e.z, d0, i0) ? 1u : 0u;
-        }
-        __syncthreads();  // every lane has read `pack` before the next query writes it
-        if (lane < n && rank < pl.n_rows)
-          write_neighbor_row(j, pl.row_at + rank, pl.qi, i0, __builtin_bit_cast(double, d0), rank + 1);
-      } else {
-        __syncthreads();
-        wave_sort_dist_id(d0, i0, d1, i1, n);
-        if (lane < pl.n_rows)
-          write_neighbor_row(j, pl.row_at + lane, pl.qi, i0, __builtin_bit_cast(double, d0), lane + 1);
+    } else {  // parked candidates; the first n_rows of them are the rows
+      n = n_cand;
+      if (lane < n) {
+        const NbrCand c = cands[pl.cand_at + lane];
+        d0 = __builtin_bit_cast(uint64_t, c.distance);
+        i0 = c.id;
+      }
+      if (lane + 64 < n) {
+        const NbrCand c = cands[pl.cand_at + lane + 64];
+        d1 = __builtin_bit_cast(uint64_t, c.distance);
+        i1 = c.id;
       }
     }
+    nbr_write_sorted(j, pl.row_at, pl.qi, n_rows, n, d0, i0, d1, i1, planes, planes + kNbrCandMax,
+                     planes + 2 * kNbrCandMax, false);
   }
 }
 
@@ -1577,9 +1614,8 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
                                            item_counters_.get() + kQueueNbrEmit * kXcds);
         RTDL_LAUNCH_CHECK();
         const unsigned g64 = (unsigned)std::max<size_t>(1, std::min<size_t>(m, (size_t)cus_ * 32));
-        nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, ic, m, item_counters_.get() + kQueueNbrSort * kXcds,
-                                                   nbr_plan_.get(), nbr_cand_.get(), nbr_stage_.get(),
-                                                   kNbrStageMax);
+        nbr_sort_kernel<<<g64, kSortBlock, 0, s>>>(dj, ic, nbr_plan_.get(), nbr_cand_.get(), nbr_emit_.get(),
+                                                   emit_count, item_counters_.get() + kQueueNbrSort * kXcds);
         RTDL_LAUNCH_CHECK();
       }
       if (f == last_nbr) {
