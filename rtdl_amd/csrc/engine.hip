@@ -1031,15 +1031,41 @@ __global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __rest
   if (STATS) flush_lane_stats(stats, st);
 }
 
-// A point inside more than kPipHitCap polygons reserves all its rows, writes
-// the first kPipHitCap from registers and is queued for batch_pip_over_kernel,
-// which walks it again and writes every hit, so the row set stays exact.
+// PIP jobs with the polygon test taken out of the lanes' walks.  Each lane
+// walks its point's rope preorder walk (two node records per step); the
+// polygons of the leaves it reaches go into a per-wavefront queue in LDS as
+// (owner lane, slot), and whenever 64 are queued, or no lane is still walking,
+// the wavefront tests 64 of them at once with every lane active: the exact
+// box prefilter of positive hits (reference.py:122-125), then the
+// point-in-polygon test (reference.py:717-742).  Lanes reach leaves in
+// different steps, so testing in the walk itself ran the long edge loop for a
+// few lanes at a time.  A containing polygon goes to its owner's hit list in
+// LDS (input index and id of the first kPipLdsHits, every hit counted).  At
+// the end of the work item each lane writes its rows ordered by polygon input
+// index.  A point inside more than kPipLdsHits polygons reserves all its rows
+// and is queued for batch_pip_over_kernel, which walks it again and writes
+// every hit, so the row set stays exact.
+constexpr uint32_t kPipQueueCap = 128;
+constexpr uint32_t kPipLdsHits = 8;
+
+// One wavefront's share of the PIP kernel's LDS (dynamic shared memory, one
+// per wavefront of the workgroup).
+struct PipWaveLds {
+  uint32_t q_slot[kPipQueueCap];
+  uint32_t hit_count[64];
+  uint32_t hit_input[kPipLdsHits * 64];  // entry i of lane l at [i * 64 + l]
+  uint32_t hit_id[kPipLdsHits * 64];
+  uint8_t q_owner[kPipQueueCap];
+};
+
 template <bool STATS>
-__global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 6) void batch_pip_kernel(const DeviceJob* __restrict__ jobs,
                                                         const WorkItem* __restrict__ items,
                                                         uint32_t n_items, uint32_t* item_counter,
                                                         double eps_sq, uint4* over, uint32_t* over_count,
                                                         unsigned long long* stats) {
+  extern __shared__ uint4 pip_lds[];
+  PipWaveLds& L = reinterpret_cast<PipWaveLds*>(pip_lds)[threadIdx.x >> 6];
   const uint32_t lane = threadIdx.x & 63;
   LaneStats st;
   ItemQueue queue(item_counter, n_items);
@@ -1049,49 +1075,107 @@ __global__ __launch_bounds__(256) void batch_pip_kernel(const DeviceJob* __restr
     if (it == kItemRetry) continue;
     const WorkItem w = items[it];
     const DeviceJob& j = jobs[w.job];
+    const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
     const bool active = lane < w.count;
-    // (input index << 32 | slot), ascending; ~0 marks an empty entry
-    uint64_t hit[kPipHitCap];
-#pragma unroll
-    for (int i = 0; i < (int)kPipHitCap; ++i) hit[i] = ~0ull;
-    uint32_t found = 0;
     uint32_t qi = 0;
+    float pf0 = 0.0f, pf1 = 0.0f;  // queries are FP32: (double)pf is the point exactly
     if (active) {
       qi = job_query(j, w.first + lane);
       const float* p = j.queries + (size_t)qi * j.qstride;
-      const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
-      walk_pip<STATS>(sc, sc.root, sc.end, p[0], p[1], true, eps_sq, st,
-                      [&](uint32_t slot) {
-                        if (found < kPipHitCap) {
-                          hit[kPipHitCap - 1] = ((uint64_t)sc.slot_to_input[slot] << 32) | slot;
-#pragma unroll
-                          for (int i = (int)kPipHitCap - 1; i > 0; --i) {
-                            const uint64_t a = hit[i - 1], b = hit[i];
-                            hit[i - 1] = a < b ? a : b;
-                            hit[i] = a < b ? b : a;
-                          }
-                        }
-                        ++found;
-                      });
+      pf0 = p[0];
+      pf1 = p[1];
     }
-    const uint32_t c = active ? found : 0u;
+    const float m = coord_bound(scene_m(sc), pf0, pf1, 0.0);
+    const float pad = m * 9.5367431640625e-07f + 1.0e-11f;
+    L.hit_count[lane] = 0u;
+    uint32_t node = sc.root;
+    bool walking = active && sc.root != sc.end;
+    uint32_t guard = sc.n_nodes + 1;  // a preorder walk visits each node at most once
+    uint32_t head = 0, size = 0;      // the wavefront's queue (uniform)
+    // One drain: the first min(64, size) queued (owner, polygon) pairs, one
+    // per lane.
+    auto drain = [&]() {
+      const uint32_t n = size < 64 ? size : 64;
+      wave_fence();
+      const bool has = lane < n;
+      const uint32_t e = (head + (has ? lane : 0u)) & (kPipQueueCap - 1);
+      const uint32_t slot = L.q_slot[e];
+      const uint32_t o = has ? L.q_owner[e] : lane;
+      const double x = __shfl(pf0, (int)o, 64), y = __shfl(pf1, (int)o, 64);
+      if (has) {
+        const size_t bx = 4 * (size_t)slot;  // reference.py:122-125 exact bbox prefilter
+        if (!(x < load_f64(sc.poly_box, bx) || x > load_f64(sc.poly_box, bx + 2) ||
+              y < load_f64(sc.poly_box, bx + 1) || y > load_f64(sc.poly_box, bx + 3))) {
+          const uint4 ref = load_u4(sc.poly, slot);
+          if (point_in_polygon(x, y, sc.poly_verts + 2 * (size_t)ref.x, ref.y, eps_sq)) {
+            const uint32_t h = __hip_atomic_fetch_add(&L.hit_count[o], 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (h < kPipLdsHits) {
+              L.hit_input[h * 64 + o] = sc.slot_to_input[slot];
+              L.hit_id[h * 64 + o] = ref.z;
+            }
+          }
+        }
+      }
+      head = (head + n) & (kPipQueueCap - 1);
+      size -= n;
+      wave_fence();
+    };
+    for (;;) {
+      uint32_t leaf_link = 0;
+      if (walking) {
+        BvhNode nd;
+        bool in;
+        rope_pair_step<STATS>(sc, node, nd, in, st, [&](const BvhNode& bb, uint32_t) {
+          return (pf0 >= bb.lo[0] - pad) & (pf0 <= bb.hi[0] + pad) & (pf1 >= bb.lo[1] - pad) &
+                 (pf1 <= bb.hi[1] + pad);
+        });
+        const bool leaf = node_is_leaf(nd.link);
+        if (in && leaf) leaf_link = nd.link;
+        // into an overlapped internal node's subtree, else past the node
+        node = (in && !leaf) ? node + 1 : nd.escape;  // left child: the next record
+        walking = node != sc.end && --guard != 0;
+      }
+      // the leaf's polygons join the queue in (index within the leaf, lane) order
+      const uint32_t first = leaf_first(leaf_link), cnt = leaf_link ? leaf_count(leaf_link) : 0u;
+      for (uint32_t e = 0;; ++e) {
+        const bool act = e < cnt;
+        const uint64_t pm = __ballot(act);
+        if (!pm) break;
+        if (act) {
+          if (STATS) ++st.prims;
+          const uint32_t pos = (head + size + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))) & (kPipQueueCap - 1);
+          L.q_slot[pos] = first + e;
+          L.q_owner[pos] = (uint8_t)lane;
+        }
+        size += (uint32_t)__popcll(pm);
+        while (size >= 64) drain();
+      }
+      if (__ballot(walking) == 0) {
+        while (size != 0) drain();
+        break;
+      }
+    }
+    wave_fence();
+    const uint32_t c = active ? L.hit_count[lane] : 0u;
     const uint64_t at0 = wave_reserve(j.counters, c);
     if (c) {
       if (at0 + c > j.row_capacity) {
         atomicOr(&j.counters->status, kStatusRowOverflow);
+      } else if (c > kPipLdsHits) {
+        over[atomicAdd(over_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
       } else {
-        uint32_t r = 0;
-#pragma unroll
-        for (int i = 0; i < (int)kPipHitCap; ++i) {
-          if (hit[i] != ~0ull) {
-            const uint32_t slot = (uint32_t)hit[i];
-            static_cast<PipRowRec*>(j.rows)[at0 + r] = PipRowRec{qi, j.scene.poly[slot].z, 1u};
-            ++r;
-          }
+        // rows in polygon input order: each hit goes to its rank among the
+        // lane's hits (input indices are distinct)
+        for (uint32_t h = 0; h < c; ++h) {
+          const uint32_t in_h = L.hit_input[h * 64 + lane];
+          uint32_t rank = 0;
+          for (uint32_t g = 0; g < c; ++g) rank += L.hit_input[g * 64 + lane] < in_h ? 1u : 0u;
+          static_cast<PipRowRec*>(j.rows)[at0 + rank] = PipRowRec{qi, L.hit_id[h * 64 + lane], 1u};
         }
-        if (c > kPipHitCap) over[atomicAdd(over_count, 1u)] = make_uint4(w.job, qi, (uint32_t)at0, c);
       }
     }
+    wave_fence();  // every lane has read its hits before the next item clears the counts
     if (STATS) close_item(st);
   }
   if (STATS) flush_lane_stats(stats, st);
@@ -1633,10 +1717,10 @@ void BatchEngine::enqueue(const HostJob* jobs, size_t n, bool morton_sort) {
       uint32_t* over_count = amb_count_.get() + 1;
       const unsigned wg = walk_grid_for(n_items);
       if (stats_on_)
-        batch_pip_kernel<true><<<wg, walk_block, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
+        batch_pip_kernel<true><<<wg, walk_block, walk_waves * sizeof(PipWaveLds), s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
                                                          over_count, st);
       else
-        batch_pip_kernel<false><<<wg, walk_block, 0, s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
+        batch_pip_kernel<false><<<wg, walk_block, walk_waves * sizeof(PipWaveLds), s>>>(dj, it, n_items, ctr, eps_sq, pip_over_.get(),
                                                           over_count, st);
       RTDL_LAUNCH_CHECK();
       batch_pip_over_kernel<<<(unsigned)cus_ * 4, 64, 0, s>>>(dj, pip_over_.get(), over_count, eps_sq);

@@ -58,7 +58,7 @@ size_t job_row_bytes(uint32_t kind);
 // Upper bound of rows a job can emit (PIP: `pip_cap` per point).
 uint64_t job_row_capacity(uint32_t kind, uint32_t nq, uint32_t k, uint32_t pip_cap);
 
-constexpr uint32_t kPipHitCap = 16;   // containing polygons held in registers per point
+constexpr uint32_t kPipHitCap = 16;   // default PIP row budget per point of a served request
 constexpr int kPointBuckets = 7;    // KNN list sizes 1, 2, 4, ..., 64
 constexpr int kLaunchFamilies = kPointBuckets + 5;  // + fixed radius, ray any / count / closest, PIP
 constexpr uint32_t kStatusRowOverflow = 1u;
