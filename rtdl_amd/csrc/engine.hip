@@ -507,54 +507,88 @@ __device__ inline void nbr_write_sorted(const DeviceJob& j, uint64_t row_at, uin
 // Sorts one staged query's rows: recomputes the FP64 distances of the slots
 // the count walk staged, keeps buckets 0..bucket (all of them for a query
 // with at most k neighbours, n_cand == 0) and writes the first n_rows in
-// (distance, id) order.  Every lane must take part.
+// (distance, id) order.  The buckets order the rows already (a bucket is a
+// monotone function of the distance), and the owner's histogram of its last
+// counting round holds how many kept rows each bucket has: each kept row is
+// placed in its bucket's region of the LDS planes and then ranked against
+// the rows of that region only.  hist_q is the owner's histogram column;
+// p_lo, p_hi and p_id are three planes of kNbrCandMax words, p_b one of
+// kNbrCandMax bytes and b_cnt kNbrBuckets words, all of this wavefront.
+// Every lane must take part.
 __device__ inline void nbr_sort_staged(const DeviceJob& j, const uint32_t* sq, uint32_t n_staged, uint32_t n_rows,
                                        uint32_t n_cand, uint32_t bucket, double scale, double qx, double qy,
-                                       double qz, uint64_t row_at, uint32_t qi, uint32_t* p_lo, uint32_t* p_hi,
-                                       uint32_t* p_id) {
+                                       double qz, uint64_t row_at, uint32_t qi, const uint32_t* hist_q,
+                                       uint32_t* p_lo, uint32_t* p_hi, uint32_t* p_id, uint8_t* p_b,
+                                       uint32_t* b_cnt) {
   const uint32_t lane = threadIdx.x & 63;
   const bool packed = stage_packs(j.scene);
+  const uint32_t last = n_cand ? bucket : kNbrBuckets - 1;  // the last bucket kept
+  // lane c < kNbrBuckets: rows in bucket c and the region's first position
+  uint32_t cnt_c = 0;
+  if (lane <= last) cnt_c = (hist_q[256 * (lane >> 1)] >> (16 * (lane & 1))) & 0xFFFFu;  // last < kNbrBuckets
+  uint32_t total;
+  const uint32_t off_c = wave_exclusive_sum(cnt_c, &total);
+  if (lane < kNbrBuckets) b_cnt[lane] = 0u;
+  wave_fence();
   uint32_t kept = 0;
   for (uint32_t e0 = 0; e0 < n_staged; e0 += 64) {
     const uint32_t e = e0 + lane;
     bool keep = false;
     uint64_t d = 0;
-    uint32_t id = 0;
+    uint32_t id = 0, b = 0;
     if (e < n_staged) {
       const uint32_t v = sq[e];
       const uint32_t slot = packed ? v & kStageSlotMask : v;
-      // a packed bucket past the k-th one is dropped without a fetch
-      if (!packed || n_cand == 0 || (v >> kStageSlotBits) <= bucket) {
+      // a packed bucket past the last one kept is dropped without a fetch
+      if (!packed || (v >> kStageSlotBits) <= last) {
         const float4 p = load_prim(j.scene.prims, slot);
         const double dist = sqrt(point_d2(j.scene, slot, p, qx, qy, qz));
-        keep = n_cand == 0 || nbr_bucket(dist, scale) <= bucket;
+        b = nbr_bucket(dist, scale);
+        keep = b <= last;
         d = __builtin_bit_cast(uint64_t, dist);
         id = __float_as_uint(p.w);
       }
     }
-    const unsigned long long mask = __ballot(keep);
-    const uint32_t at = kept + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    if (keep && at < kNbrCandMax) {
-      p_lo[at] = (uint32_t)d;
-      p_hi[at] = (uint32_t)(d >> 32);
-      p_id[at] = id;
+    const uint32_t base = __shfl(off_c, (int)(keep ? b : 0u), 64);
+    if (keep) {
+      const uint32_t pos = base + __hip_atomic_fetch_add(&b_cnt[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (pos < kNbrCandMax) {
+        p_lo[pos] = (uint32_t)d;
+        p_hi[pos] = (uint32_t)(d >> 32);
+        p_id[pos] = id;
+        p_b[pos] = (uint8_t)b;
+      }
     }
-    kept += (uint32_t)__popcll(mask);
+    kept += (uint32_t)__popcll(__ballot(keep));
   }
   wave_fence();
-  if (kept > kNbrCandMax && lane == 0) atomicOr(&j.counters->status, kStatusInternal);
+  // the kept rows are exactly the points counted in buckets 0..last
+  if ((kept > kNbrCandMax || kept != total) && lane == 0) atomicOr(&j.counters->status, kStatusInternal);
   const uint32_t n = kept < kNbrCandMax ? kept : kNbrCandMax;
-  uint64_t d0 = ~0ull, d1 = ~0ull;
-  uint32_t i0 = ~0u, i1 = ~0u;
-  if (lane < n) {
-    d0 = ((uint64_t)p_hi[lane] << 32) | p_lo[lane];
-    i0 = p_id[lane];
+  for (uint32_t h = 0; h < n; h += 64) {
+    const uint32_t x = h + lane;
+    const bool has = x < n;
+    uint64_t d = 0;
+    uint32_t id = 0, b = 0;
+    if (has) {
+      d = ((uint64_t)p_hi[x] << 32) | p_lo[x];
+      id = p_id[x];
+      b = p_b[x];
+    }
+    // every lane takes part in both shuffles (a shuffle reads only active lanes)
+    const uint32_t start = __shfl(off_c, (int)b, 64), len_b = __shfl(cnt_c, (int)b, 64);
+    const uint32_t len = has ? len_b : 0u;
+    // rank within the bucket's region (the pairs are distinct: ids are)
+    uint32_t rank = 0;
+    for (uint32_t t = 0; __ballot(t < len); ++t) {
+      if (t < len) {
+        const uint32_t y = start + t;
+        rank += dist_id_less(((uint64_t)p_hi[y] << 32) | p_lo[y], p_id[y], d, id) ? 1u : 0u;
+      }
+    }
+    const uint32_t pos = start + rank;
+    if (has && pos < n_rows) write_neighbor_row(j, row_at + pos, qi, id, __builtin_bit_cast(double, d), pos + 1);
   }
-  if (lane + 64 < n) {
-    d1 = ((uint64_t)p_hi[lane + 64] << 32) | p_lo[lane + 64];
-    i1 = p_id[lane + 64];
-  }
-  nbr_write_sorted(j, row_at, qi, n_rows, n, d0, i0, d1, i1, p_lo, p_hi, p_id, true);
   wave_fence();  // the planes are free again once every lane has read them
 }
 
@@ -567,10 +601,10 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
   // buckets 2w and 2w + 1 of thread t in the low and high half of hist[w * 256 + t]
   __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
   __shared__ uint32_t q_slot[4 * kNbrQueueCap];
-  __shared__ uint8_t q_owner[4 * kNbrQueueCap];
+  __shared__ __align__(16) uint8_t q_owner[4 * kNbrQueueCap];
   // queued leaves: the leaf's link word (count and first slot) and its owner
   __shared__ uint32_t lq_link[4 * kNbrLeafQueueCap];
-  __shared__ uint8_t lq_owner[4 * kNbrLeafQueueCap];
+  __shared__ __align__(16) uint8_t lq_owner[4 * kNbrLeafQueueCap];
   // per thread: neighbours counted this round (high word) and stage cursor
   // (low word), updated together by one LDS atomic
   __shared__ unsigned long long l_cs[256];
@@ -875,7 +909,8 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
         nbr_sort_staged(j, stage_it + (size_t)q * stage_slots, __builtin_amdgcn_readlane(r.n_staged, q),
                         __builtin_amdgcn_readlane(n_rows, q), __builtin_amdgcn_readlane(n_cand, q),
                         __builtin_amdgcn_readlane(r.bucket, q), readlane_f64(scale, q), ox, oy, oz,
-                        readlane64(row_at, q), __builtin_amdgcn_readlane(qi, q), qs, lql, p_id);
+                        readlane64(row_at, q), __builtin_amdgcn_readlane(qi, q), hist + wbase + q, qs, lql, p_id,
+                        lqo, reinterpret_cast<uint32_t*>(qo));
       }
       wave_fence();
     }
