@@ -101,7 +101,7 @@ int rtdl_mi355x_context_probe(char* device_name, size_t device_name_size, int* d
   });
 }
 
-void rtdl_mi355x_free_rows(void* rows) { std::free(rows); }
+void rtdl_mi355x_free_rows(void* rows) { free_host_rows(rows); }
 
 int rtdl_mi355x_set_fault(const char* spec, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] { set_fault(spec); });

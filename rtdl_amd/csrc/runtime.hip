@@ -451,6 +451,54 @@ DeviceArray<uint8_t> upload_bytes(const void* host, size_t bytes, hipStream_t st
 
 }  // namespace
 
+namespace {
+
+// Pinned row buffers for large per-call row sets.  The rows are copied
+// straight from device memory into the buffer the caller receives: one DMA at
+// the link's rate, with no host copy and no first-touch page faults of a
+// fresh allocation.  A buffer the caller releases (rtdl_mi355x_free_rows) is
+// kept for a later call of about its size, up to kPinnedCacheBytes of
+// released buffers; beyond that it is freed.
+constexpr size_t kPinnedRowsMin = 1u << 20;
+constexpr size_t kPinnedCacheBytes = size_t(4) << 30;
+constexpr size_t kPinnedGranule = 2u << 20;
+
+struct PinnedRows {
+  std::mutex mu;
+  std::map<void*, size_t> live;         // buffers held by callers: capacity
+  std::multimap<size_t, void*> cached;  // released buffers by capacity
+  size_t cached_bytes = 0;
+};
+
+PinnedRows& pinned_rows() {
+  static PinnedRows* rows = new PinnedRows();  // lives until the process ends
+  return *rows;
+}
+
+}  // namespace
+
+void free_host_rows(void* rows) {
+  if (!rows) return;
+  PinnedRows& pr = pinned_rows();
+  size_t cap = 0;
+  {
+    std::lock_guard<std::mutex> g(pr.mu);
+    auto it = pr.live.find(rows);
+    if (it == pr.live.end()) {
+      std::free(rows);  // pageable rows
+      return;
+    }
+    cap = it->second;
+    pr.live.erase(it);
+    if (pr.cached_bytes + cap <= kPinnedCacheBytes) {
+      pr.cached.emplace(cap, rows);
+      pr.cached_bytes += cap;
+      return;
+    }
+  }
+  (void)hipHostFree(rows);
+}
+
 void read_root_box(Prepared& p) {
   const SceneView v = p.scene.view(0);
   if (v.root == kEndNode) return;
@@ -466,6 +514,34 @@ void read_root_box(Prepared& p) {
 
 namespace {
 
+// A pinned buffer of at least `bytes`, or null (the caller then uses pageable
+// memory): a released one that fits without wasting more than half, else a
+// new allocation.
+void* take_pinned_rows(size_t bytes) {
+  PinnedRows& pr = pinned_rows();
+  {
+    std::lock_guard<std::mutex> g(pr.mu);
+    auto it = pr.cached.lower_bound(bytes);
+    if (it != pr.cached.end() && it->first / 2 <= bytes) {
+      void* p = it->second;
+      const size_t cap = it->first;
+      pr.cached.erase(it);
+      pr.cached_bytes -= cap;
+      pr.live[p] = cap;
+      return p;
+    }
+  }
+  const size_t cap = (bytes + kPinnedGranule - 1) / kPinnedGranule * kPinnedGranule;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, cap, hipHostMallocPortable) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(pr.mu);
+  pr.live[p] = cap;
+  return p;
+}
+
 void* host_rows(size_t count, size_t row_bytes) {
   void* rows = std::malloc(count ? count * row_bytes : 1);
   require(rows != nullptr, "host row allocation failed");
@@ -474,17 +550,19 @@ void* host_rows(size_t count, size_t row_bytes) {
 
 void copy_rows_back(const void* device_rows, size_t count, size_t row_bytes, void** rows_out,
                     size_t* n_out, hipStream_t stream) {
-  void* rows = host_rows(count, row_bytes);
+  const size_t bytes = count * row_bytes;
+  void* rows = bytes >= kPinnedRowsMin ? take_pinned_rows(bytes) : nullptr;
+  const bool pinned = rows != nullptr;
+  if (!pinned) rows = host_rows(count, row_bytes);
   if (count) {
-    const size_t bytes = count * row_bytes;
     try {
-      if (bytes < kStageMin ||
+      if (pinned || bytes < kStageMin ||
           !staged_copy(static_cast<uint8_t*>(rows), static_cast<const uint8_t*>(device_rows), bytes, stream)) {
         RTDL_HIP(hipMemcpyAsync(rows, device_rows, bytes, hipMemcpyDeviceToHost, stream));
         RTDL_HIP(hipStreamSynchronize(stream));
       }
     } catch (const Error& e) {
-      std::free(rows);
+      free_host_rows(rows);
       throw Error(std::string("row copy-back failed: ") + e.what());
     }
   }

@@ -79,6 +79,9 @@ void prepare_polygons(Prepared& p, const void* refs, size_t n, const double* ver
 // Multi-instance builders from device FP32 data (ids are instance-local).
 // Reads the root box of instance 0 into p.root_lo / root_hi (synchronises).
 void read_root_box(Prepared& p);
+// Releases a row buffer returned by a per-call or prepared entry point
+// (pageable memory, or a pinned buffer kept for later calls).
+void free_host_rows(void* rows);
 // leaf_size: points per leaf of the instance trees, 0 for the process tuning
 void prepare_point_instances(Prepared& p, const float* coords, size_t n, uint32_t leaf_size,
                              const std::vector<uint64_t>& offsets);
