@@ -123,7 +123,10 @@ int rtdl_mi355x_get_version(int* major, int* minor, int* patch);
 int rtdl_mi355x_context_probe(char* device_name, size_t device_name_size, int* device_type,
                               int* api_version, char* error_out, size_t error_size);
 
-/* replaces rtdl_hiprt_free_rows (src/native/hiprt/rtdl_hiprt_api.cpp:11-13) */
+/* replaces rtdl_hiprt_free_rows (src/native/hiprt/rtdl_hiprt_api.cpp:11-13).
+ * Row sets of 1 MiB or more live in pinned host memory; a released one is
+ * kept (up to 4 GiB of released buffers) for a later call of about its size.
+ * Only pointers returned as rows by this library may be passed. */
 void rtdl_mi355x_free_rows(void* rows);
 
 /* Per-phase wall times of the last call on this thread, in seconds:
