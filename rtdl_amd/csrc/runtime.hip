@@ -481,6 +481,7 @@ void free_host_rows(void* rows) {
   if (!rows) return;
   PinnedRows& pr = pinned_rows();
   size_t cap = 0;
+  std::vector<void*> evict;
   {
     std::lock_guard<std::mutex> g(pr.mu);
     auto it = pr.live.find(rows);
@@ -490,13 +491,24 @@ void free_host_rows(void* rows) {
     }
     cap = it->second;
     pr.live.erase(it);
-    if (pr.cached_bytes + cap <= kPinnedCacheBytes) {
-      pr.cached.emplace(cap, rows);
-      pr.cached_bytes += cap;
-      return;
+    if (cap <= kPinnedCacheBytes) {
+      // smaller released buffers make room first: a large one costs the
+      // most to pin again
+      while (pr.cached_bytes + cap > kPinnedCacheBytes && !pr.cached.empty() && pr.cached.begin()->first <= cap) {
+        auto small = pr.cached.begin();
+        evict.push_back(small->second);
+        pr.cached_bytes -= small->first;
+        pr.cached.erase(small);
+      }
+      if (pr.cached_bytes + cap <= kPinnedCacheBytes) {
+        pr.cached.emplace(cap, rows);
+        pr.cached_bytes += cap;
+        rows = nullptr;
+      }
     }
   }
-  (void)hipHostFree(rows);
+  for (void* e : evict) (void)hipHostFree(e);
+  if (rows) (void)hipHostFree(rows);
 }
 
 void read_root_box(Prepared& p) {
