@@ -97,3 +97,53 @@ def test_dlpack_rows_reach_torch_in_place(gpu_ready):
         pytest.skip(proc.stdout.strip())
     assert proc.returncode == 0, proc.stdout[-2000:] + proc.stderr[-3000:]
     assert proc.stdout.strip().startswith("ok")
+
+
+def test_pinned_per_call_rows_survive_later_calls_and_are_reused(gpu_ready):
+    """Per-call row sets of 1 MiB or more come back in pinned buffers that
+    free_rows keeps for later calls: rows the caller still holds must not
+    change when later calls run (and reuse released buffers), and every
+    call's rows equal the oracle's."""
+    import gc
+
+    from oracle import native as oracle
+    from rtdl_amd import mi355x_runtime as m
+
+    _say("case: pinned per-call rows held across later calls, released buffers reused")
+    rng = np.random.default_rng(123)
+    s = rng.random((30000, 3)).astype(np.float32).astype(np.float64)
+    search = {"id": np.arange(len(s), dtype=np.uint32), "x": s[:, 0], "y": s[:, 1], "z": s[:, 2]}
+
+    def queries(n, seed):
+        q = np.random.default_rng(seed).random((n, 3)).astype(np.float32).astype(np.float64)
+        return {"id": np.arange(n, dtype=np.uint32), "x": q[:, 0], "y": q[:, 1], "z": q[:, 2]}
+
+    qa, qb = queries(6000, 1), queries(5000, 2)  # 6000 x 16 x 24 B = 2.3 MB of rows
+    want_a = oracle.neighbors("knn", qa, search, k=16)
+    want_b = oracle.neighbors("knn", qb, search, k=16)
+
+    def check(got, want):
+        assert len(got) == len(want)
+        assert np.array_equal(got["query_id"], want["qid"])
+        assert np.array_equal(got["neighbor_id"], want["nid"])
+        assert np.array_equal(got["distance"], want["dist"])
+
+    a1 = m.knn_rows_mi355x(qa, search, k=16)
+    assert a1.nbytes >= 1 << 20
+    keep = a1.copy()
+    b1 = m.knn_rows_mi355x(qb, search, k=16)  # a1 is still held: b1 gets another buffer
+    check(a1, want_a)
+    check(b1, want_b)
+    assert np.array_equal(a1, keep)
+    del b1
+    gc.collect()
+    for _ in range(3):  # released buffers are handed out again
+        b2 = m.knn_rows_mi355x(qb, search, k=16)
+        check(b2, want_b)
+        assert np.array_equal(a1, keep)
+        del b2
+        gc.collect()
+    del a1
+    gc.collect()
+    a2 = m.knn_rows_mi355x(qa, search, k=16)
+    check(a2, want_a)
