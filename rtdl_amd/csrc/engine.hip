@@ -577,7 +577,9 @@ __device__ inline void nbr_sort_staged(const DeviceJob& j, const uint32_t* sq, u
     }
     // every lane takes part in both shuffles (a shuffle reads only active lanes)
     const uint32_t start = __shfl(off_c, (int)b, 64), len_b = __shfl(cnt_c, (int)b, 64);
-    const uint32_t len = has ? len_b : 0u;
+    // (a region never reaches past the planes: start + len_b <= total, and
+    // total <= kNbrCandMax unless the status flag above is set)
+    const uint32_t len = has ? min(len_b, kNbrCandMax - min(start, kNbrCandMax)) : 0u;
     // rank within the bucket's region (the pairs are distinct: ids are)
     uint32_t rank = 0;
     for (uint32_t t = 0; __ballot(t < len); ++t) {
