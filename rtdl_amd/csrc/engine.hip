@@ -1083,7 +1083,7 @@ __global__ __launch_bounds__(256) void batch_rays_kernel(const DeviceJob* __rest
 // and is queued for batch_pip_over_kernel, which walks it again and writes
 // every hit, so the row set stays exact.
 constexpr uint32_t kPipQueueCap = 128;
-constexpr uint32_t kPipLdsHits = 8;
+constexpr uint32_t kPipLdsHits = 16;
 
 // One wavefront's share of the PIP kernel's LDS (dynamic shared memory, one
 // per wavefront of the workgroup).
