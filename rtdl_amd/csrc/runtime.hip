@@ -547,7 +547,21 @@ void* take_pinned_rows(size_t bytes) {
   void* p = nullptr;
   if (hipHostMalloc(&p, cap, hipHostMallocPortable) != hipSuccess) {
     (void)hipGetLastError();
-    return nullptr;
+    // pinned memory may be exhausted by released buffers: give them back and
+    // try once more
+    std::vector<void*> evict;
+    {
+      std::lock_guard<std::mutex> g(pr.mu);
+      for (auto& kv : pr.cached) evict.push_back(kv.second);
+      pr.cached.clear();
+      pr.cached_bytes = 0;
+    }
+    for (void* e : evict) (void)hipHostFree(e);
+    p = nullptr;
+    if (evict.empty() || hipHostMalloc(&p, cap, hipHostMallocPortable) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
   }
   std::lock_guard<std::mutex> g(pr.mu);
   pr.live[p] = cap;
