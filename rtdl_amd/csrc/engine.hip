@@ -721,15 +721,27 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
             const uint32_t count = (uint32_t)(l_cs[threadIdx.x] >> 32);
             if (count >= next_check) {
               next_check = count + step;
-              uint32_t cum = 0;
-              for (uint32_t c = 0; c < top; ++c) {
-                cum += bucket_count(c);
-                if (cum >= k) {  // the k nearest lie in buckets 0..c: walk no farther
-                  top = c;
-                  shrunk = true;
-                  thr2 = frn_threshold2(fmin(r.radius, nbr_bucket_edge(c, scale)), m);
-                  break;
+              // the first bucket c < top whose cumulative count reaches k;
+              // the histogram words are read four at a time (eight buckets),
+              // so their LDS latencies overlap instead of adding up
+              uint32_t cum = 0, kth = top;
+              for (uint32_t w0 = 0; w0 < kNbrBuckets / 2 && 2 * w0 < top && kth == top; w0 += 4) {
+                uint32_t wv[4];
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i) wv[i] = my_hist[256 * (w0 + i)];
+#pragma unroll
+                for (uint32_t i = 0; i < 8; ++i) {
+                  const uint32_t c = 2 * w0 + i;
+                  if (kth == top && c < top) {
+                    cum += (wv[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+                    if (cum >= k) kth = c;
+                  }
                 }
+              }
+              if (kth < top) {  // the k nearest lie in buckets 0..kth: walk no farther
+                top = kth;
+                shrunk = true;
+                thr2 = frn_threshold2(fmin(r.radius, nbr_bucket_edge(kth, scale)), m);
               }
             }
           }
