@@ -853,13 +853,25 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
         if (r.heavy && count > kNbrCountMax) {
           r.refine = true;
         } else if (r.heavy) {
+          // the first bucket c <= top whose cumulative count reaches k (the
+          // histogram read four words at a time, as at the checkpoints)
           uint32_t cum = 0;
-          for (uint32_t c = 0; c <= top; ++c) {
-            cum += bucket_count(c);
-            if (cum >= k) {
-              r.bucket = c;
-              r.n_cand = cum;
-              break;
+          bool found = false;
+          for (uint32_t w0 = 0; w0 < kNbrBuckets / 2 && 2 * w0 <= top && !found; w0 += 4) {
+            uint32_t wv[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4; ++i) wv[i] = my_hist[256 * (w0 + i)];
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i) {
+              const uint32_t c = 2 * w0 + i;
+              if (!found && c <= top) {
+                cum += (wv[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+                if (cum >= k) {
+                  r.bucket = c;
+                  r.n_cand = cum;
+                  found = true;
+                }
+              }
             }
           }
           if (r.n_cand > kNbrCandMax) {
