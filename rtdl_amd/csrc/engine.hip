@@ -368,7 +368,7 @@ __device__ inline bool stage_packs(const SceneView& sc) { return sc.n_prims <= (
 // lane walks its query's ball (one node step per lane per iteration, two when
 // it descends: the child that follows its parent in preorder is fetched with
 // it) and only prefilters leaf points in FP32; the points that pass go into a per-wavefront
-// queue in LDS as (owner lane, slot).  Whenever 64 are queued, or no lane is
+// queue in LDS as (owner lane, slot, FP32 coordinates).  Whenever 64 are queued, or no lane is
 // still walking, the wavefront drains 64 of them at once with every lane
 // active: FP64 squared distance in the reference's order, the inclusive
 // radius test, the distance bucket, the owner's histogram (LDS atomics) and
@@ -595,7 +595,7 @@ __device__ inline void nbr_sort_staged(const DeviceJob& j, const uint32_t* sq, u
 }
 
 template <bool STATS>
-__global__ __launch_bounds__(256, 6) void nbr_count_kernel(
+__global__ __launch_bounds__(256, 5) void nbr_count_kernel(
     const DeviceJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t n_items,
     uint32_t* item_counter, NbrPlan* __restrict__ plans, unsigned long long* cand_count,
     uint64_t cand_cap, uint32_t* __restrict__ stage, uint32_t stage_slots, uint32_t* __restrict__ emit_list,
@@ -604,6 +604,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
   __shared__ uint32_t hist[kNbrBuckets / 2 * 256];
   __shared__ uint32_t q_slot[4 * kNbrQueueCap];
   __shared__ __align__(16) uint8_t q_owner[4 * kNbrQueueCap];
+  // the queued points' FP32 coordinates, so that the drain does not fetch the
+  // point again (an FP64 scene's drain still reads its exact coordinates)
+  __shared__ float q_px[4 * kNbrQueueCap], q_py[4 * kNbrQueueCap], q_pz[4 * kNbrQueueCap];
   // queued leaves: the leaf's link word (count and first slot) and its owner
   __shared__ uint32_t lq_link[4 * kNbrLeafQueueCap];
   __shared__ __align__(16) uint8_t lq_owner[4 * kNbrLeafQueueCap];
@@ -615,6 +618,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
   uint32_t* my_hist = hist + threadIdx.x;
   uint32_t* qs = q_slot + (threadIdx.x >> 6) * kNbrQueueCap;
   uint8_t* qo = q_owner + (threadIdx.x >> 6) * kNbrQueueCap;
+  float* const qpx = q_px + (threadIdx.x >> 6) * kNbrQueueCap;
+  float* const qpy = q_py + (threadIdx.x >> 6) * kNbrQueueCap;
+  float* const qpz = q_pz + (threadIdx.x >> 6) * kNbrQueueCap;
   uint32_t* lql = lq_link + (threadIdx.x >> 6) * kNbrLeafQueueCap;
   uint8_t* lqo = lq_owner + (threadIdx.x >> 6) * kNbrLeafQueueCap;
   auto bucket_count = [&](uint32_t b) { return (my_hist[256 * (b >> 1)] >> (16 * (b & 1))) & 0xFFFFu; };
@@ -701,7 +707,7 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
           const double orsq = shfl_f64(r.radius_sq, o), oscale = shfl_f64(scale, o);
           const uint32_t otop = __shfl(top, (int)o, 64);
           if (has) {
-            const float4 p = load_prim(sc.prims, slot);
+            const float4 p = make_float4(qpx[e], qpy[e], qpz[e], 0.f);
             const double d2 = point_d2(sc, slot, p, ox, oy, oz);
             if (d2 <= orsq) {  // reference.py:361 inclusive radius
               const uint32_t b = nbr_bucket(sqrt(d2), oscale);
@@ -807,8 +813,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
             if (!__ballot(act)) break;
             bool push = false;
             const uint32_t slot = first + i;
+            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
             if (act) {
-              const float4 p = load_prim(sc.prims, slot);
+              p = load_prim(sc.prims, slot);
               if (STATS) ++st.prims;
               const float fx = p.x - ox, fy = p.y - oy, fz = p.z - oz;
               push = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)) <= othr2;
@@ -818,6 +825,9 @@ __global__ __launch_bounds__(256, 6) void nbr_count_kernel(
               const uint32_t pos = (head + size + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))) & (kNbrQueueCap - 1);
               qs[pos] = slot;
               qo[pos] = (uint8_t)o;
+              qpx[pos] = p.x;
+              qpy[pos] = p.y;
+              qpz[pos] = p.z;
             }
             size += (uint32_t)__popcll(pm);
             while (size >= 64) drain();
