@@ -206,66 +206,115 @@ constexpr uint32_t kStackDepth = 32;
 // k-th key prunes most of the tree.  Returns false when the stack (DEPTH
 // entries) was full at some push and a subtree was left out: the caller then
 // walks the whole tree again, meeting some primitives a second time.
+// The walk is one flat loop in which every lane takes one action per
+// iteration: expand an internal node (fetch both children: the left one is
+// the next record, the right one the node's link), visit two points of a
+// leaf, or test a node popped from the stack.  Each action fetches at most two
+// 32-byte records, so the lanes of a wavefront issue their fetches in the
+// same load instructions whatever each one is doing; nested descend and pop
+// loops would instead leave most lanes idle while a few descend or pop.  The
+// sequence of nodes and points of each lane is that of the nested form.
 template <bool STATS, uint32_t DEPTH = kStackDepth, class Visit>
 __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t end, const float qf[3],
                                     float& thr2, LaneStats& st, Visit&& visit, uint32_t* stack,
                                     uint32_t stride) {
   if (root == kEndNode || root == end) return true;
+  constexpr uint32_t kExpand = 0, kLeaf = 1, kTest = 2, kDone = 3;
   bool complete = true;
   uint32_t depth = 0;
-  BvhNode nd = load_node(s.nodes, root);
-  uint32_t cur = root;
-  if (STATS) ++st.nodes;
-  if (!(box_dist2(qf, nd.lo, nd.hi) <= thr2)) return true;
-  // every node is pushed at most once; the guard only bounds the loop should
-  // the tree ever be malformed
-  for (uint32_t guard = s.n_nodes + 1; guard; --guard) {
-    bool leaf = true;
-    while (!node_is_leaf(nd.link)) {
-      // both children, fetched together: the left one is the next record,
-      // the right one the node's link
-      const uint32_t l = cur + 1, r = nd.link;
-      const BvhNode ln = load_node(s.nodes, l);
-      const BvhNode rn = load_node(s.nodes, r);
-      if (STATS) st.nodes += 2;
-      const float dl = box_dist2(qf, ln.lo, ln.hi), dr = box_dist2(qf, rn.lo, rn.hi);
-      const bool right_first = dr < dl;
-      const float dn = right_first ? dr : dl, df = right_first ? dl : dr;
-      if (df <= thr2) {
-        if (depth < DEPTH)
-          stack[stride * depth++] = right_first ? l : r;
-        else
-          complete = false;
-      }
-      if (!(dn <= thr2)) {
-        leaf = false;
-        break;
-      }
-      nd = right_first ? rn : ln;
-      cur = right_first ? r : l;
+  uint32_t state = kTest, cur = root;  // the root is tested first
+  uint32_t first = 0, cnt = 0, j = 0;  // kLeaf: the leaf's points, next index j
+  uint32_t right = 0;                  // kExpand: the right child
+  // every node is pushed at most once and a leaf takes at most four visits,
+  // so the guard only bounds the loop should the tree ever be malformed
+  for (uint32_t guard = 4 * s.n_nodes + 4; guard && state != kDone; --guard) {
+    // the iteration's fetch: record A (a node, or points j and j + 1) and,
+    // when expanding, record B (the right child)
+    const uint8_t* pa;
+    if (state == kLeaf)
+      pa = reinterpret_cast<const uint8_t*>(s.prims + first + j);
+    else
+      pa = reinterpret_cast<const uint8_t*>(s.nodes + (state == kExpand ? cur + 1 : cur));
+    // a leaf's odd last point: the second half of A would lie past the leaf
+    const bool a_half = state != kLeaf || j + 1 < cnt;
+    const GlobalU32x4 a0 = load_raw16(pa);
+    const GlobalU32x4 a1 = load_raw16(a_half ? pa + 16 : pa);
+    GlobalU32x4 b0 = a0, b1 = a1;
+    if (state == kExpand) {
+      const uint8_t* pb = reinterpret_cast<const uint8_t*>(s.nodes + right);
+      b0 = load_raw16(pb);
+      b1 = load_raw16(pb + 16);
     }
-    if (leaf) {
-      const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
-      for (uint32_t j = 0; j < cnt; ++j) {
-        const uint32_t slot = first + j;
-        const float4 p = load_prim(s.prims, slot);
-        if (STATS) ++st.prims;
-        const float fx = p.x - qf[0], fy = p.y - qf[1], fz = p.z - qf[2];
+    bool pop = false;
+    if (state == kLeaf) {
+      const float4 p0 = make_float4(__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2]),
+                                    __uint_as_float(a0[3]));
+      if (STATS) ++st.prims;
+      {
+        const float fx = p0.x - qf[0], fy = p0.y - qf[1], fz = p0.z - qf[2];
         const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
-        if (d2f <= thr2) visit(slot, p, d2f);
+        if (d2f <= thr2) visit(first + j, p0, d2f);
+      }
+      if (j + 1 < cnt) {
+        const float4 p1 = make_float4(__uint_as_float(a1[0]), __uint_as_float(a1[1]), __uint_as_float(a1[2]),
+                                      __uint_as_float(a1[3]));
+        if (STATS) ++st.prims;
+        const float fx = p1.x - qf[0], fy = p1.y - qf[1], fz = p1.z - qf[2];
+        const float d2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+        if (d2f <= thr2) visit(first + j + 1, p1, d2f);
+      }
+      j += 2;
+      pop = j >= cnt;
+    } else {
+      // A is a node: the left child when expanding, else the node to test
+      const float alo[3] = {__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2])};
+      const float ahi[3] = {__uint_as_float(a1[0]), __uint_as_float(a1[1]), __uint_as_float(a1[2])};
+      const float da = box_dist2(qf, alo, ahi);
+      uint32_t next = cur, link = a0[3];
+      bool go = false;
+      if (state == kExpand) {
+        if (STATS) st.nodes += 2;
+        const float blo[3] = {__uint_as_float(b0[0]), __uint_as_float(b0[1]), __uint_as_float(b0[2])};
+        const float bhi[3] = {__uint_as_float(b1[0]), __uint_as_float(b1[1]), __uint_as_float(b1[2])};
+        const float db = box_dist2(qf, blo, bhi);
+        const bool right_first = db < da;
+        const float dn = right_first ? db : da, df = right_first ? da : db;
+        if (df <= thr2) {
+          if (depth < DEPTH)
+            stack[stride * depth++] = right_first ? cur + 1 : right;
+          else
+            complete = false;
+        }
+        go = dn <= thr2;
+        next = right_first ? right : cur + 1;
+        link = right_first ? b0[3] : a0[3];
+      } else {  // kTest
+        if (STATS) ++st.nodes;
+        go = da <= thr2;
+      }
+      if (go) {
+        cur = next;
+        if (node_is_leaf(link)) {
+          state = kLeaf;
+          first = leaf_first(link);
+          cnt = leaf_count(link);
+          j = 0;
+        } else {
+          state = kExpand;
+          right = link;
+        }
+      } else {
+        pop = true;
       }
     }
-    bool found = false;
-    while (depth) {
-      cur = stack[stride * --depth];
-      nd = load_node(s.nodes, cur);
-      if (STATS) ++st.nodes;
-      if (box_dist2(qf, nd.lo, nd.hi) <= thr2) {
-        found = true;
-        break;
+    if (pop) {
+      if (depth) {
+        cur = stack[stride * --depth];
+        state = kTest;
+      } else {
+        state = kDone;
       }
     }
-    if (!found) break;
   }
   return complete;
 }
