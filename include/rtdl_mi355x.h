@@ -468,6 +468,16 @@ int rtdl_mi355x_scene_export(void* prepared, RtdlSceneExport* info, void* nodes,
                              double* exact, uint32_t* roots, uint32_t* ends, char* error_out,
                              size_t error_size);
 
+/* Host copy of a prepared point scene's compact nodes (no counterpart in the
+ * reference; for checking them and the fixed-radius count walk that reads
+ * them against a replay on the host): node_count 16-byte records (three
+ * words of 16-bit box steps (lo x, lo y), (lo z, hi x), (hi y, hi z), then a
+ * leaf's link or an internal node's escape, node_count meaning the end) and
+ * the 8-float frame (origin[3], step[3], unusable flag, 0).  has_compact
+ * receives 0 for scenes without them (non-point or empty scenes). */
+int rtdl_mi355x_scene_export_compact(void* prepared, uint32_t* has_compact, void* qnodes, float* frame,
+                                     char* error_out, size_t error_size);
+
 /* Process-wide build and launch choices (no counterpart in the reference;
  * for the autotuner in rtdl_amd/autotune.py and callers that pin a tuned
  * configuration).  Every setting gives the same rows; scenes built and

@@ -709,6 +709,20 @@ int rtdl_mi355x_scene_export(void* prepared, RtdlSceneExport* info, void* nodes,
   });
 }
 
+int rtdl_mi355x_scene_export_compact(void* prepared, uint32_t* has_compact, void* qnodes, float* frame,
+                                     char* error_out, size_t error_size) {
+  return guarded(error_out, error_size, [&] {
+    require(has_compact != nullptr, "has_compact must not be null");
+    Prepared* p = prepared_from(prepared);
+    std::lock_guard<std::mutex> lock(p->lock);
+    const Scene& sc = p->scene;
+    *has_compact = sc.qnodes.size() ? 1u : 0u;
+    if (!sc.qnodes.size()) return;
+    if (qnodes) RTDL_HIP(hipMemcpy(qnodes, sc.qnodes.get(), sc.qnodes.bytes(), hipMemcpyDeviceToHost));
+    if (frame) RTDL_HIP(hipMemcpy(frame, sc.qframe.get(), sc.qframe.bytes(), hipMemcpyDeviceToHost));
+  });
+}
+
 int rtdl_mi355x_get_tuning(RtdlTuning* out, char* error_out, size_t error_size) {
   return guarded(error_out, error_size, [&] {
     require(out != nullptr, "out must not be null");

@@ -176,6 +176,20 @@ __host__ __device__ inline bool node_is_leaf(uint32_t link) { return (link & kLe
 __host__ __device__ inline uint32_t leaf_first(uint32_t link) { return link & kMaxLeafStart; }
 __host__ __device__ inline uint32_t leaf_count(uint32_t link) { return ((link >> 28) & 7u) + 1u; }
 
+// Compact copy of a point scene's nodes for the fixed-radius count walk, the
+// hottest rope walk: 16 bytes per node (one load instead of two).  The box
+// is quantised to 16-bit steps of a per-scene frame (origin o and power-of-two
+// step s per axis, chosen so that o + q * s is exact in FP32 for every q of
+// the box), rounded outward, so the dequantised box contains the node's FP32
+// box.  The word is a leaf's link, or an internal node's escape (the node
+// count stands for the end of the walk).  A leaf needs no escape: in preorder
+// the node after a leaf is the next record.  The frame holds o[3], s[3] and,
+// in frame[6], 1 when a scene box is not finite (no compact walk then).
+constexpr uint32_t kQFrameFloats = 8;
+__host__ __device__ inline uint32_t qnode_word_of(const BvhNode& nd, uint32_t n_nodes) {
+  return node_is_leaf(nd.link) ? nd.link : (nd.escape == kEndNode ? n_nodes : nd.escape);
+}
+
 // ---------------------------------------------- conservative FP32 bounds --
 //
 // Traversal and primitive prefilters run in FP32; the final predicate is

@@ -648,6 +648,16 @@ __global__ __launch_bounds__(256, 5) void nbr_count_kernel(
     const float m = coord_bound(scene_m(sc), qx, qy, qz);
     const bool pack = stage_packs(sc);
     uint32_t* const stage_it = stage + (size_t)it * 64 * stage_slots;
+    // the walk reads the scene's compact nodes when its frame is usable
+    const bool compact = sc.qnodes != nullptr && sc.qframe[6] == 0.0f;
+    float q_o[3] = {0.0f, 0.0f, 0.0f}, q_s[3] = {0.0f, 0.0f, 0.0f};
+    if (compact) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        q_o[a] = sc.qframe[a];
+        q_s[a] = sc.qframe[3 + a];
+      }
+    }
     NbrCount r{j.radius, j.radius * j.radius, 0u, kNbrBuckets - 1, 0u, false, false, false};  // reference.py:357
     bool going = active;  // counts in the current round
     double scale = 0.0;  // bucket scale of the lane's last counting round
@@ -673,21 +683,29 @@ __global__ __launch_bounds__(256, 5) void nbr_count_kernel(
           // descent goes there next, and its test then costs no second
           // round trip to memory
           const uint32_t after = node + 1 < sc.n_nodes ? node + 1 : node;
-          const BvhNode nd = load_node(sc.nodes, node);
-          const BvhNode nd1 = load_node(sc.nodes, after);
+          QNode nd, nd1;
+          if (compact) {
+            nd = load_qnode(sc.qnodes, node, q_o, q_s);
+            nd1 = load_qnode(sc.qnodes, after, q_o, q_s);
+          } else {
+            nd = qnode_of(load_node(sc.nodes, node), sc.n_nodes);
+            nd1 = qnode_of(load_node(sc.nodes, after), sc.n_nodes);
+          }
           if (STATS) ++st.nodes;
           const bool in = box_dist2(qf, nd.lo, nd.hi) <= thr2;
-          const bool leaf = node_is_leaf(nd.link);
-          if (in && leaf) leaf_link = nd.link;
-          uint32_t next = (in && !leaf) ? node + 1 : nd.escape;  // the left child is the next record
-          walking = next != sc.end && --guard != 0;
+          const bool leaf = node_is_leaf(nd.word);
+          if (in && leaf) leaf_link = nd.word;
+          // into the left child (the next record) or past a leaf (also the
+          // next record), else to the escape
+          uint32_t next = (in || leaf) ? node + 1 : nd.word;
+          walking = next != sc.end && next < sc.n_nodes && --guard != 0;
           if (walking && in && !leaf) {  // descended: test the child (fetched with its parent) now
             if (STATS) ++st.nodes;
             const bool in1 = box_dist2(qf, nd1.lo, nd1.hi) <= thr2;
-            const bool leaf1 = node_is_leaf(nd1.link);
-            if (in1 && leaf1) leaf_link = nd1.link;
-            next = (in1 && !leaf1) ? after + 1 : nd1.escape;
-            walking = next != sc.end && --guard != 0;
+            const bool leaf1 = node_is_leaf(nd1.word);
+            if (in1 && leaf1) leaf_link = nd1.word;
+            next = (in1 || leaf1) ? after + 1 : nd1.word;
+            walking = next != sc.end && next < sc.n_nodes && --guard != 0;
           }
           node = next;
         }

@@ -456,6 +456,99 @@ __global__ void remap_roots_kernel(uint32_t n_inst, const uint32_t* pre, uint32_
   if (ends[i] != kEndNode) ends[i] = pre[ends[i]];
 }
 
+// The quantisation frame of a point scene's compact nodes, from the union of
+// its instance roots' boxes (one workgroup of 256 threads).  Per axis the
+// step s is the smallest power of two with s >= extent / 2^15 and
+// s >= max|coordinate| / 2^22, and the origin o = floor(lo / s) * s, so every
+// o + q * s (0 <= q < 2^16) is an integer multiple of s below 2^24 * s in
+// magnitude: exact in FP32.
+__global__ void qframe_kernel(uint32_t n_inst, const uint32_t* roots, const BvhNode* nodes, float* frame) {
+  __shared__ float r_lo[3][256], r_hi[3][256];
+  __shared__ uint32_t r_bad[256];
+  const uint32_t t = threadIdx.x;
+  float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+  float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+  uint32_t bad = 0;
+  for (uint32_t i = t; i < n_inst; i += 256) {
+    const uint32_t r = roots[i];
+    if (r == kEndNode) continue;
+    const BvhNode nd = nodes[r];
+    for (int a = 0; a < 3; ++a) {
+      if (!(fabsf(nd.lo[a]) < __builtin_inff()) || !(fabsf(nd.hi[a]) < __builtin_inff())) bad = 1;
+      lo[a] = fminf(lo[a], nd.lo[a]);
+      hi[a] = fmaxf(hi[a], nd.hi[a]);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    r_lo[a][t] = lo[a];
+    r_hi[a][t] = hi[a];
+  }
+  r_bad[t] = bad;
+  __syncthreads();
+  if (t != 0) return;
+  for (uint32_t u = 1; u < 256; ++u) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], r_lo[a][u]);
+      hi[a] = fmaxf(hi[a], r_hi[a][u]);
+    }
+    bad |= r_bad[u];
+  }
+  for (int a = 0; a < 3; ++a) {
+    double o = 0.0, s = 1.0;
+    if (!bad && lo[a] <= hi[a]) {
+      const double ext = (double)hi[a] - (double)lo[a];
+      const double mx = fmax(fabs((double)lo[a]), fabs((double)hi[a]));
+      const double need = fmax(fmax(ext * 0x1p-15, mx * 0x1p-22), 0x1p-100);
+      int e = 0;
+      const double f = frexp(need, &e);  // need = f * 2^e, f in [0.5, 1)
+      s = ldexp(1.0, f == 0.5 ? e - 1 : e);
+      o = floor((double)lo[a] / s) * s;
+    }
+    frame[a] = (float)o;
+    frame[3 + a] = (float)s;
+  }
+  frame[6] = bad ? 1.0f : 0.0f;
+  frame[7] = 0.0f;
+}
+
+// Compact node i: its box quantised outward in the frame (the largest q_lo
+// with o + q_lo * s <= lo and the smallest q_hi with o + q_hi * s >= hi, both
+// checked in the FP32 arithmetic of the walk) and its word.
+__global__ void qnodes_kernel(uint32_t n_nodes, const BvhNode* nodes, const float* frame, uint4* qnodes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const BvhNode nd = nodes[i];
+  uint32_t q[6] = {0, 0, 0, 0xFFFFu, 0xFFFFu, 0xFFFFu};
+  if (frame[6] == 0.0f) {
+    for (int a = 0; a < 3; ++a) {
+      const float o = frame[a], s = frame[3 + a];
+      double ql = floor(((double)nd.lo[a] - (double)o) / (double)s);
+      double qh = ceil(((double)nd.hi[a] - (double)o) / (double)s);
+      ql = fmin(fmax(ql, 0.0), 65535.0);
+      qh = fmin(fmax(qh, 0.0), 65535.0);
+      uint32_t l = (uint32_t)ql, h = (uint32_t)qh;
+      while (l > 0 && o + (float)l * s > nd.lo[a]) --l;
+      while (h < 0xFFFFu && o + (float)h * s < nd.hi[a]) ++h;
+      q[a] = l;
+      q[3 + a] = h;
+    }
+  }
+  qnodes[i] = make_uint4(q[0] | (q[1] << 16), q[2] | (q[3] << 16), q[4] | (q[5] << 16), qnode_word_of(nd, n_nodes));
+}
+
+// Frame and compact nodes of a point scene whose nodes are in their final
+// preorder places (kernels only: a captured rebuild records them too).
+void build_qnodes(Scene& scene, uint32_t n_inst, hipStream_t stream) {
+  const size_t n_nodes = scene.nodes.size();
+  require(scene.qnodes.size() == n_nodes && scene.qframe.size() == kQFrameFloats, "compact nodes not sized");
+  if (!n_nodes) return;
+  qframe_kernel<<<1, 256, 0, stream>>>(n_inst, scene.roots.get(), scene.nodes.get(), scene.qframe.get());
+  RTDL_LAUNCH_CHECK();
+  qnodes_kernel<<<blocks_for(n_nodes, 256), 256, 0, stream>>>((uint32_t)n_nodes, scene.nodes.get(),
+                                                               scene.qframe.get(), scene.qnodes.get());
+  RTDL_LAUNCH_CHECK();
+}
+
 }  // namespace
 
 void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* coords,
@@ -653,6 +746,12 @@ void build_scene(Scene& scene, PrimKind kind, int dim, size_t n, const double* c
     RTDL_LAUNCH_CHECK();
     RTDL_HIP(hipStreamSynchronize(stream));  // before the old nodes return to the pool
     scene.nodes = std::move(ordered);
+  }
+  // 8. point scenes: the compact nodes of the fixed-radius count walk
+  if (kind == PrimKind::Points) {
+    scene.qnodes.resize(n_nodes);
+    scene.qframe.resize(kQFrameFloats);
+    build_qnodes(scene, n_inst, stream);
   }
   RTDL_HIP(hipMemcpyAsync(scene.host_roots.data(), scene.roots.get(), n_inst * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, stream));
@@ -887,6 +986,8 @@ void rebuild_scene(Scene& scene, const float* coords, hipStream_t stream) {
       n_inst, w.roots.get(), w.ends.get(), scene.roots.get(), scene.ends.get(), w.n_internal, w.done.get(),
       w.flags.get());
   RTDL_LAUNCH_CHECK();
+  // 8. the compact nodes (the instance roots keep their places, as checked)
+  if (scene.qnodes.size()) build_qnodes(scene, n_inst, stream);
 }
 
 uint32_t rebuild_status(const Scene& scene, hipStream_t stream) {

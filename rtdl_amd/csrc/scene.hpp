@@ -38,6 +38,14 @@ struct SceneView {
   uint32_t n_nodes;
   float m;             // >= every |coordinate| in the scene, rounded up
   const float* m_dev;  // when set, M is read from device memory instead
+  const uint4* qnodes;  // point scenes: compact nodes (16 bytes each) for the count walk
+  const float* qframe;  // their quantisation frame (kQFrameFloats)
+};
+
+// A compact node (see qnode_word_of): its dequantised box and word.
+struct QNode {
+  float lo[3], hi[3];
+  uint32_t word;
 };
 
 // Loads from scene arrays through the global address space.  The arrays
@@ -62,6 +70,33 @@ __device__ inline BvhNode load_node(const BvhNode* nodes, uint32_t i) {
   n.hi[1] = __uint_as_float(b[1]);
   n.hi[2] = __uint_as_float(b[2]);
   n.escape = b[3];
+  return n;
+}
+
+// A full node in the compact form (box as stored, word as qnode_word_of).
+__device__ inline QNode qnode_of(const BvhNode& nd, uint32_t n_nodes) {
+  QNode n;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    n.lo[ax] = nd.lo[ax];
+    n.hi[ax] = nd.hi[ax];
+  }
+  n.word = qnode_word_of(nd, n_nodes);
+  return n;
+}
+
+// Compact node i: halves (lo x, lo y), (lo z, hi x), (hi y, hi z) of three
+// words, then the word; box = o + q * s per axis (exact in FP32).
+__device__ inline QNode load_qnode(const uint4* qnodes, uint32_t i, const float o[3], const float s[3]) {
+  const GlobalU32x4 a = load_raw16(qnodes + i);
+  const uint32_t q[6] = {a[0] & 0xFFFFu, a[0] >> 16, a[1] & 0xFFFFu, a[1] >> 16, a[2] & 0xFFFFu, a[2] >> 16};
+  QNode n;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    n.lo[ax] = o[ax] + (float)q[ax] * s[ax];
+    n.hi[ax] = o[ax] + (float)q[3 + ax] * s[ax];
+  }
+  n.word = a[3];
   return n;
 }
 
@@ -108,6 +143,10 @@ class Scene {
   // device memory (a rebuild changes it without the host seeing it)
   std::shared_ptr<RebuildWork> rebuild;
   const float* m_device = nullptr;
+  // point scenes: the compact nodes of the fixed-radius count walk and their
+  // quantisation frame, refreshed by every build and rebuild
+  DeviceArray<uint4> qnodes;
+  DeviceArray<float> qframe;
 
   SceneView view(size_t instance = 0) const {
     SceneView v{};
@@ -124,11 +163,13 @@ class Scene {
     v.n_nodes = (uint32_t)nodes.size();
     v.m = m;
     v.m_dev = m_device;
+    v.qnodes = qnodes.size() ? qnodes.get() : nullptr;
+    v.qframe = qnodes.size() ? qframe.get() : nullptr;
     return v;
   }
   size_t device_bytes() const {
     return nodes.bytes() + prims.bytes() + exact.bytes() + slot_to_input.bytes() + poly.bytes() +
-           poly_box.bytes() + poly_verts.bytes() + roots.bytes() + ends.bytes();
+           poly_box.bytes() + poly_verts.bytes() + roots.bytes() + ends.bytes() + qnodes.bytes() + qframe.bytes();
   }
 };
 

@@ -194,6 +194,7 @@ _SIGNATURES = {
     "rtdl_mi355x_batch_graph_create_rebuild": ([_VP, _SZ, ctypes.c_uint32, _VP, _VP, _SZ, ctypes.POINTER(_VP),
                                                 *_ERR], ctypes.c_int),
     "rtdl_mi355x_scene_export": ([_VP, _VP, _VP, _VP, _VP, _VP, _VP, *_ERR], ctypes.c_int),
+    "rtdl_mi355x_scene_export_compact": ([_VP, _VP, _VP, _VP, *_ERR], ctypes.c_int),
     "rtdl_mi355x_get_tuning": ([_VP, *_ERR], ctypes.c_int),
     "rtdl_mi355x_set_tuning": ([_VP, *_ERR], ctypes.c_int),
     # device buffers (see rtdl_amd.device)

@@ -114,7 +114,43 @@ def _check_boxes(sc, leaf_points):
         ok = j < count
         p = leaf_points[first[ok] + j]
         assert np.all(llo[ok] <= p) and np.all(lhi[ok] >= p)
+    # in preorder the node after a leaf is the next record (the last leaf ends the walk)
+    esc = nodes["escape"][leaf].astype(np.int64)
+    idx = np.flatnonzero(leaf)
+    assert np.all((esc == idx + 1) | (esc == 0xFFFFFFFF))
+    if sc["qnodes"] is not None:
+        _check_compact(sc, leaf)
     return int(len(internal))
+
+
+def _check_compact(sc, leaf):
+    """The compact nodes of the count walk: every dequantised box contains the
+    node's FP32 box and is the tightest one of the frame's grid, the frame's
+    steps are powers of two (so the dequantisation is exact), and the words
+    are the leaves' links and the internal nodes' escapes."""
+    nodes, fr = sc["nodes"], sc["qframe"]
+    n = len(nodes)
+    assert fr[6] == 0 and fr[7] == 0
+    o, st = fr[:3].astype(np.float64), fr[3:6].astype(np.float64)
+    m, e = np.frexp(st)
+    assert np.all(m == 0.5), st  # powers of two
+    assert np.all(np.floor(o / st) * st == o)
+    lo = nodes["lo"].astype(np.float32)
+    hi = nodes["hi"].astype(np.float32)
+    qlo, qhi, word = W.compact_boxes(sc)
+    assert np.all(qlo <= lo) and np.all(qhi >= hi)
+    # one step tighter would no longer contain the box (exact in FP64: o + q * s
+    # has at most 24 significant bits)
+    q = sc["qnodes"]
+    ql = np.stack([q[:, 0] & 0xFFFF, q[:, 0] >> 16, q[:, 1] & 0xFFFF], 1).astype(np.float64)
+    qh = np.stack([q[:, 1] >> 16, q[:, 2] & 0xFFFF, q[:, 2] >> 16], 1).astype(np.float64)
+    assert np.array_equal(o + ql * st, qlo.astype(np.float64))
+    assert np.all((ql == 65535) | (o + (ql + 1) * st > lo))
+    assert np.all((qh == 0) | (o + (qh - 1) * st < hi))
+    link = nodes["link"].astype(np.int64)
+    esc = nodes["escape"].astype(np.int64)
+    want = np.where(leaf, link, np.where(esc == 0xFFFFFFFF, n, esc))
+    assert np.array_equal(word.astype(np.int64), want)
 
 
 def test_lbvh_node_boxes_are_the_unions_of_their_children(gpu_ready):
@@ -128,10 +164,18 @@ def test_lbvh_node_boxes_are_the_unions_of_their_children(gpu_ready):
         scene = B.prepare_point_instances(DeviceBuffer.from_numpy(s3), [0, n])
         sc = W.export_scene(scene.handle)
         _check_boxes(sc, sc["prims"][:, :3])
+    # several instances far from the origin: one frame over all of them
+    pts = cases.points(rng, 30000, 3, kind="uniform", centres=centres)
+    s3 = (np.stack([pts["x"], pts["y"], pts["z"]], 1) * 3.0 + 1000.0).astype(np.float32)
+    scene = B.prepare_point_instances(DeviceBuffer.from_numpy(s3), [0, 7000, 7001, 30000])
+    sc = W.export_scene(scene.handle)
+    assert sc["qnodes"] is not None
+    _check_boxes(sc, sc["prims"][:, :3])
     t = cases.triangles(rng, 50000, sigma=0.01)
     scene = B.prepare_triangle_instances(DeviceBuffer.from_numpy(cases.tri_matrix(t).astype(np.float32)),
                                          [0, len(t["id"])])
     sc = W.export_scene(scene.handle)
+    assert sc["qnodes"] is None  # compact nodes are for point scenes
     v = sc["prims"].reshape(-1, 3, 4)[:, :, :3]
     _check_boxes(sc, v.min(axis=1))
     _check_boxes(sc, v.max(axis=1))

@@ -46,8 +46,33 @@ def export_scene(handle) -> dict:
                                     exact.ctypes.data if info.exact_doubles else None, roots.ctypes.data,
                                     ends.ctypes.data, err, 1024):
         raise RuntimeError(err.value.decode())
-    return {"nodes": nodes, "prims": prims, "exact": exact if info.exact_doubles else None,
-            "roots": roots, "ends": ends, "m": F32(info.m), "prim_float4s": info.prim_float4s}
+    out = {"nodes": nodes, "prims": prims, "exact": exact if info.exact_doubles else None,
+           "roots": roots, "ends": ends, "m": F32(info.m), "prim_float4s": info.prim_float4s,
+           "qnodes": None, "qframe": None}
+    has = ctypes.c_uint32(0)
+    if lib.rtdl_mi355x_scene_export_compact(handle, ctypes.byref(has), None, None, err, 1024):
+        raise RuntimeError(err.value.decode())
+    if has.value:
+        qnodes = np.zeros((info.node_count, 4), dtype=np.uint32)
+        frame = np.zeros(8, dtype=np.float32)
+        if lib.rtdl_mi355x_scene_export_compact(handle, ctypes.byref(has), qnodes.ctypes.data, frame.ctypes.data,
+                                                err, 1024):
+            raise RuntimeError(err.value.decode())
+        out["qnodes"], out["qframe"] = qnodes, frame
+    return out
+
+
+def compact_boxes(sc: dict):
+    """Dequantised boxes (lo, hi: float32 [n, 3]) and words of a scene's
+    compact nodes: box = origin + q * step per axis in FP32, as the count
+    walk computes it (exact for the frame's steps)."""
+    qn, fr = sc["qnodes"], sc["qframe"]
+    q = np.stack([qn[:, 0] & 0xFFFF, qn[:, 0] >> 16, qn[:, 1] & 0xFFFF,
+                  qn[:, 1] >> 16, qn[:, 2] & 0xFFFF, qn[:, 2] >> 16], 1).astype(F32)
+    o, st = fr[:3].astype(F32), fr[3:6].astype(F32)
+    lo = (o + q[:, :3] * st).astype(F32)
+    hi = (o + q[:, 3:] * st).astype(F32)
+    return lo, hi, qn[:, 3].copy()
 
 
 # ------------------------------------------------------------ arithmetic --
@@ -254,6 +279,11 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
     of that ball (at most four rounds)."""
     nodes, prims, exact = sc["nodes"], sc["prims"], sc["exact"]
     root, end = int(sc["roots"][instance]), int(sc["ends"][instance])
+    # the walk tests the compact nodes' boxes when the scene's frame is usable
+    if sc.get("qnodes") is not None and sc["qframe"][6] == 0:
+        box_lo, box_hi, _ = compact_boxes(sc)
+    else:
+        box_lo, box_hi = nodes["lo"], nodes["hi"]
     lanes = len(queries)
     q = [tuple(float(v) for v in row) for row in queries]
     qf = [(F32(x), F32(y), F32(z)) for x, y, z in q]
@@ -297,7 +327,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
                     n_nodes += 1
                     nd = nodes[cur]
                     link = int(nd["link"])
-                    inside = box_dist2(qf[i], nd["lo"], nd["hi"]) <= thr2[i]
+                    inside = box_dist2(qf[i], box_lo[cur], box_hi[cur]) <= thr2[i]
                     if inside and link & LEAF:
                         leaf[i] = (link & START_MASK, ((link >> 28) & 7) + 1)
                     descend = inside and not link & LEAF
