@@ -214,22 +214,30 @@ constexpr uint32_t kStackDepth = 32;
 // same load instructions whatever each one is doing; nested descend and pop
 // loops would instead leave most lanes idle while a few descend or pop.  The
 // sequence of nodes and points of each lane is that of the nested form.
-template <bool STATS, uint32_t DEPTH = kStackDepth, class Visit>
-__device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t end, const float qf[3],
-                                    float& thr2, LaneStats& st, Visit&& visit, uint32_t* stack,
-                                    uint32_t stride) {
-  if (root == kEndNode || root == end) return true;
-  constexpr uint32_t kExpand = 0, kLeaf = 1, kTest = 2, kDone = 3;
-  bool complete = true;
-  uint32_t depth = 0;
-  uint32_t state = kTest, cur = root;  // the root is tested first
-  uint32_t first = 0, cnt = 0, j = 0;  // kLeaf: the leaf's points, next index j
-  uint32_t right = 0;                  // kExpand: the right child
-  // every node is pushed at most once and a leaf takes at most four visits,
-  // so the guard only bounds the loop should the tree ever be malformed
-  for (uint32_t guard = 4 * s.n_nodes + 4; guard && state != kDone; --guard) {
-    // the iteration's fetch: record A (a node, or points j and j + 1) and,
-    // when expanding, record B (the right child)
+template <uint32_t DEPTH = kStackDepth>
+struct NearestWalk {
+  static constexpr uint32_t kExpand = 0, kLeaf = 1, kTest = 2, kDone = 3;
+  uint32_t state, cur;
+  uint32_t first, cnt, j;  // kLeaf: the leaf's points, next index j
+  uint32_t right;          // kExpand: the right child
+  uint32_t depth;          // entries on the lane's stack
+  bool complete;           // no push was lost to a full stack
+
+  __device__ inline void init(uint32_t root, uint32_t end) {
+    state = (root == kEndNode || root == end) ? kDone : kTest;  // the root is tested first
+    cur = root;
+    first = cnt = j = right = depth = 0;
+    complete = true;
+  }
+  __device__ inline bool done() const { return state == kDone; }
+
+  // One action of the lane: expand, visit up to two points, or test a popped
+  // node.  Returns false once the walk is over.
+  template <bool STATS, class Visit>
+  __device__ inline bool step(const SceneView& s, const float qf[3], float& thr2, LaneStats& st, Visit&& visit,
+                              uint32_t* stack, uint32_t stride) {
+    // the action's fetch: record A (a node, or points j and j + 1) and, when
+    // expanding, record B (the right child)
     const uint8_t* pa;
     if (state == kLeaf)
       pa = reinterpret_cast<const uint8_t*>(s.prims + first + j);
@@ -237,14 +245,19 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
       pa = reinterpret_cast<const uint8_t*>(s.nodes + (state == kExpand ? cur + 1 : cur));
     // a leaf's odd last point: the second half of A would lie past the leaf
     const bool a_half = state != kLeaf || j + 1 < cnt;
+    // the four loads are issued together, before any of them is waited for
+    // (record B repeats A for a lane that does not expand), so an expansion
+    // costs one round trip to memory, not two
+    const uint8_t* pb = state == kExpand ? reinterpret_cast<const uint8_t*>(s.nodes + right) : pa;
     const GlobalU32x4 a0 = load_raw16(pa);
     const GlobalU32x4 a1 = load_raw16(a_half ? pa + 16 : pa);
-    GlobalU32x4 b0 = a0, b1 = a1;
-    if (state == kExpand) {
-      const uint8_t* pb = reinterpret_cast<const uint8_t*>(s.nodes + right);
-      b0 = load_raw16(pb);
-      b1 = load_raw16(pb + 16);
-    }
+    const GlobalU32x4 b0 = load_raw16(pb);
+    const GlobalU32x4 b1 = load_raw16(a_half ? pb + 16 : pb);
+    // an empty use of all four in the common path keeps the compiler from
+    // sinking the B loads into the expanding branch behind a wait for A:
+    // the loads are issued back to back and waited for once
+    __builtin_amdgcn_sched_barrier(0);
+    __asm__ volatile("" : : "v"(a0), "v"(a1), "v"(b0), "v"(b1));
     bool pop = false;
     if (state == kLeaf) {
       const float4 p0 = make_float4(__uint_as_float(a0[0]), __uint_as_float(a0[1]), __uint_as_float(a0[2]),
@@ -315,8 +328,21 @@ __device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t 
         state = kDone;
       }
     }
+    return state != kDone;
   }
-  return complete;
+};
+
+template <bool STATS, uint32_t DEPTH = kStackDepth, class Visit>
+__device__ inline bool walk_nearest(const SceneView& s, uint32_t root, uint32_t end, const float qf[3],
+                                    float& thr2, LaneStats& st, Visit&& visit, uint32_t* stack,
+                                    uint32_t stride) {
+  NearestWalk<DEPTH> w;
+  w.init(root, end);
+  // every node is pushed at most once and a leaf takes at most four visits,
+  // so the guard only bounds the loop should the tree ever be malformed
+  for (uint32_t guard = 4 * s.n_nodes + 4; guard && !w.done(); --guard)
+    w.template step<STATS>(s, qf, thr2, st, visit, stack, stride);
+  return w.complete;
 }
 
 // Exact list: K (distance, id) entries in registers, k of them live in slots
@@ -419,6 +445,60 @@ __device__ inline uint64_t pack_key(double d2, uint32_t slot) {
 }
 __device__ inline uint32_t key_bits(uint64_t e) { return (uint32_t)(e >> 32); }
 
+// One candidate of a k-nearest search offered to its packed list: FP64
+// distance in the reference's order, the radius test, the (fl32(d2), slot)
+// key; once the list is full its k-th key shrinks the walk's threshold.
+// drop_min keeps the smallest key bits of any candidate dropped.
+template <int K>
+__device__ inline void knn_offer(const SceneView& s, uint32_t slot, const float4& p, double qx, double qy,
+                                 double qz, double radius, double radius_sq, float m, PackedTopK<K>& list,
+                                 float& thr2, uint32_t& drop_min) {
+  const double d2 = point_d2(s, slot, p, qx, qy, qz);
+  if (d2 > radius_sq) return;
+  const uint64_t v = pack_key(d2, slot);
+  if (v < list.e[K - 1]) {
+    const uint64_t out = list.insert(v);
+    if (out != ~0ull) drop_min = min(drop_min, key_bits(out));
+    if (list.full()) {
+      // every candidate whose key lies within one ulp of the k-th key
+      // stays inside (the factor covers 2^-23 of d2 and its rounding)
+      const double kth = (double)__uint_as_float(key_bits(list.e[K - 1]));
+      thr2 = frn_threshold2(fmin(radius, sqrt(kth) * 1.0000002384185791015625), m);
+    }
+  } else {
+    drop_min = min(drop_min, key_bits(v));
+  }
+}
+
+// Whether a finished k-nearest list may differ from the exact one: dropped
+// keys are never below the final k-th key (it only shrinks), and kept keys
+// ascend, so each test below is a difference of at most one ulp.
+template <int K>
+__device__ inline bool knn_ambiguous(const PackedTopK<K>& list, uint32_t k, uint32_t drop_min) {
+  bool ambiguous = list.full() && drop_min - key_bits(list.e[K - 1]) <= 1u;
+#pragma unroll
+  for (int i = 1; i < K; ++i) {
+    if (i > K - (int)k && list.e[i] != ~0ull && key_bits(list.e[i]) - key_bits(list.e[i - 1]) <= 1u)
+      ambiguous = true;
+  }
+  return ambiguous;
+}
+
+// After a nearest-first walk whose stack overflowed: every primitive is
+// offered again, skipping the entries already in the list.
+template <int K, bool STATS>
+__device__ inline void knn_rewalk(const SceneView& s, uint32_t root, uint32_t end, const float qf[3], double qx,
+                                  double qy, double qz, double radius, double radius_sq, float m,
+                                  PackedTopK<K>& list, float& thr2, uint32_t& drop_min, LaneStats& st) {
+  walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float) {
+    const uint64_t v = pack_key(point_d2(s, slot, p, qx, qy, qz), slot);
+    bool dup = false;
+#pragma unroll
+    for (int i = 0; i < K; ++i) dup = dup || list.e[i] == v;
+    if (!dup) knn_offer<K>(s, slot, p, qx, qy, qz, radius, radius_sq, m, list, thr2, drop_min);
+  });
+}
+
 // `stack` (LDS, DEPTH entries at stride `stride`) selects the
 // nearest-child-first walk for a search without a radius; without one the
 // ring walk is used.
@@ -435,46 +515,15 @@ __device__ inline bool collect_packed(const SceneView& s, uint32_t root, uint32_
   uint32_t drop_min = 0xFFFFFFFFu;  // smallest key bits of any dropped candidate
   if (root == kEndNode) return false;
   auto visit = [&](uint32_t slot, const float4& p, float) {
-    const double d2 = point_d2(s, slot, p, qx, qy, qz);
-    if (d2 > radius_sq) return;
-    const uint64_t v = pack_key(d2, slot);
-    if (v < list.e[K - 1]) {
-      const uint64_t out = list.insert(v);
-      if (out != ~0ull) drop_min = min(drop_min, key_bits(out));
-      if (list.full()) {
-        // every candidate whose key lies within one ulp of the k-th key
-        // stays inside (the factor covers 2^-23 of d2 and its rounding)
-        const double kth = (double)__uint_as_float(key_bits(list.e[K - 1]));
-        thr2 = frn_threshold2(fmin(radius, sqrt(kth) * 1.0000002384185791015625), m);
-      }
-    } else {
-      drop_min = min(drop_min, key_bits(v));
-    }
+    knn_offer<K>(s, slot, p, qx, qy, qz, radius, radius_sq, m, list, thr2, drop_min);
   };
   if (stack && !(radius < __builtin_inf())) {
-    if (!walk_nearest<STATS, DEPTH>(s, root, end, qf, thr2, st, visit, stack, stride)) {
-      // the stack overflowed: offer every primitive again, skipping the
-      // entries already in the list
-      walk_points<STATS>(s, root, end, qf, thr2, st, [&](uint32_t slot, const float4& p, float d2f) {
-        const uint64_t v = pack_key(point_d2(s, slot, p, qx, qy, qz), slot);
-        bool dup = false;
-#pragma unroll
-        for (int i = 0; i < K; ++i) dup = dup || list.e[i] == v;
-        if (!dup) visit(slot, p, d2f);
-      });
-    }
+    if (!walk_nearest<STATS, DEPTH>(s, root, end, qf, thr2, st, visit, stack, stride))
+      knn_rewalk<K, STATS>(s, root, end, qf, qx, qy, qz, radius, radius_sq, m, list, thr2, drop_min, st);
   } else {
     walk_neighbors<STATS>(s, root, end, qf, radius, thr2, st, visit);
   }
-  // dropped keys are never below the final k-th key (it only shrinks), and
-  // kept keys ascend, so each test below is a difference of at most one ulp
-  bool ambiguous = list.full() && drop_min - key_bits(list.e[K - 1]) <= 1u;
-#pragma unroll
-  for (int i = 1; i < K; ++i) {
-    if (i > K - (int)k && list.e[i] != ~0ull && key_bits(list.e[i]) - key_bits(list.e[i - 1]) <= 1u)
-      ambiguous = true;
-  }
-  return ambiguous;
+  return knn_ambiguous<K>(list, k, drop_min);
 }
 
 // ------------------------------------------------------------------- rays --
