@@ -1274,19 +1274,51 @@ __global__ __launch_bounds__(256, 6) void batch_pip_kernel(const DeviceJob* __re
 __global__ __launch_bounds__(64) void batch_pip_over_kernel(const DeviceJob* __restrict__ jobs,
                                                             const uint4* __restrict__ over,
                                                             const uint32_t* over_count, double eps_sq) {
+  // one wavefront per point: the walk is the same in every lane, and each
+  // polygon met is tested with its edges spread over the lanes (points inside
+  // many polygons lie where polygons are dense: long walks, many tests)
   const uint32_t n = *over_count;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
     const uint4 o = over[i];
     const DeviceJob& j = jobs[o.x];
+    const SceneView sc = j.scene;
     const float* p = j.queries + (size_t)o.y * j.qstride;
+    const double x = p[0], y = p[1];
     PipRowRec* rows = static_cast<PipRowRec*>(j.rows) + o.z;
     uint32_t r = 0;
+    const float pf0 = p[0], pf1 = p[1];
+    const float m = coord_bound(scene_m(sc), x, y, 0.0);
+    const float pad = m * 9.5367431640625e-07f + 1.0e-11f;  // as walk_pip
     LaneStats st;
-    walk_pip<false>(j.scene, j.scene.root, j.scene.end, p[0], p[1], true, eps_sq, st,
-                    [&](uint32_t slot) {
-                      if (r < o.w) rows[r] = PipRowRec{o.y, j.scene.poly[slot].z, 1u};
-                      ++r;
-                    });
+    uint32_t node = sc.root;
+    for (uint32_t guard = sc.n_nodes + 1; node != sc.end && guard; --guard) {
+      BvhNode nd;
+      bool in;
+      rope_pair_step<false>(sc, node, nd, in, st, [&](const BvhNode& bb, uint32_t) {
+        return (pf0 >= bb.lo[0] - pad) & (pf0 <= bb.hi[0] + pad) & (pf1 >= bb.lo[1] - pad) &
+               (pf1 <= bb.hi[1] + pad);
+      });
+      const bool leaf = node_is_leaf(nd.link);
+      if (!(in && leaf)) {
+        node = (in && !leaf) ? node + 1 : nd.escape;  // left child: the next record
+        continue;
+      }
+      const uint32_t first = leaf_first(nd.link), cnt = leaf_count(nd.link);
+      for (uint32_t e = 0; e < cnt; ++e) {
+        const uint32_t slot = first + e;
+        const size_t bx = 4 * (size_t)slot;  // reference.py:122-125 exact bbox prefilter
+        if (x < load_f64(sc.poly_box, bx) || x > load_f64(sc.poly_box, bx + 2) || y < load_f64(sc.poly_box, bx + 1) ||
+            y > load_f64(sc.poly_box, bx + 3))
+          continue;
+        const uint4 ref = load_u4(sc.poly, slot);
+        if (point_in_polygon_wave(x, y, sc.poly_verts + 2 * (size_t)ref.x, ref.y, eps_sq)) {
+          if (lane == 0 && r < o.w) rows[r] = PipRowRec{o.y, ref.z, 1u};
+          ++r;
+        }
+      }
+      node = nd.escape;
+    }
   }
 }
 

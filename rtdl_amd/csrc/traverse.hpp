@@ -819,6 +819,36 @@ __device__ inline bool point_in_polygon(double x, double y, const double* vertic
   return inside;
 }
 
+// point_in_polygon with the edges spread over the 64 lanes of a wavefront
+// (every lane takes part; x, y and the polygon are the same in all lanes):
+// the edges' on-edge and crossing tests are those of point_in_polygon, and
+// "any edge holds the point, else the parity of the crossings" does not
+// depend on the order the edges are taken in.
+__device__ inline bool point_in_polygon_wave(double x, double y, const double* vertices, uint32_t n,
+                                             double eps_sq) {
+  const uint32_t lane = threadIdx.x & 63;
+  bool on_any = false;
+  uint32_t parity = 0;
+  for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+    const uint32_t k = k0 + lane;
+    bool on = false, cross = false;
+    if (k < n) {
+      double ax, ay, bx, by;
+      load_vertex(vertices, k, ax, ay);
+      load_vertex(vertices, k + 1 == n ? 0u : k + 1, bx, by);
+      on = point_on_segment(x, y, ax, ay, bx, by, eps_sq);
+      if ((by > y) != (ay > y)) {
+        double den = ay - by;
+        if (den == 0.0) den = 1.0e-20;
+        cross = x <= (ax - bx) * (y - by) / den + bx;
+      }
+    }
+    on_any = on_any || __ballot(on) != 0;
+    parity ^= (uint32_t)__popcll(__ballot(cross)) & 1u;
+  }
+  return on_any || parity != 0;
+}
+
 // Calls hit(slot) for every polygon that contains (x, y) under the
 // reference's rules.  Boxes are grown by more than the on-edge tolerance
 // (1e-12 from the boundary) plus FP32 rounding of the point.
