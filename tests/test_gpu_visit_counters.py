@@ -164,6 +164,13 @@ def test_lbvh_node_boxes_are_the_unions_of_their_children(gpu_ready):
         scene = B.prepare_point_instances(DeviceBuffer.from_numpy(s3), [0, n])
         sc = W.export_scene(scene.handle)
         _check_boxes(sc, sc["prims"][:, :3])
+    # coordinates of both signs
+    pts = cases.points(rng, 20000, 3, kind="uniform", centres=centres)
+    s3 = (np.stack([pts["x"], pts["y"], pts["z"]], 1) * 3.0 - 2.0).astype(np.float32)
+    scene = B.prepare_point_instances(DeviceBuffer.from_numpy(s3), [0, len(s3)])
+    sc = W.export_scene(scene.handle)
+    assert sc["qframe"][0] < 0 < sc["qframe"][0] + 65535 * sc["qframe"][3]
+    _check_boxes(sc, sc["prims"][:, :3])
     # several instances far from the origin: one frame over all of them
     pts = cases.points(rng, 30000, 3, kind="uniform", centres=centres)
     s3 = (np.stack([pts["x"], pts["y"], pts["z"]], 1) * 3.0 + 1000.0).astype(np.float32)
@@ -179,6 +186,37 @@ def test_lbvh_node_boxes_are_the_unions_of_their_children(gpu_ready):
     v = sc["prims"].reshape(-1, 3, 4)[:, :, :3]
     _check_boxes(sc, v.min(axis=1))
     _check_boxes(sc, v.max(axis=1))
+
+
+def test_compact_nodes_fall_back_when_a_scene_box_is_not_finite(gpu_ready):
+    """A point at infinity makes the quantisation frame unusable: the frame is
+    flagged and the count walk reads the full nodes; the rows are those of the
+    finite points (a point at infinity is never within a finite radius)."""
+    from oracle import native as oracle
+
+    _say("case: scene with a point at infinity: flagged frame, count walk on full nodes")
+    rng = np.random.default_rng(101)
+    s3 = rng.random((5000, 3)).astype(np.float32)
+    s3[17] = (np.inf, 0.5, 0.5)
+    scene = B.prepare_point_instances(DeviceBuffer.from_numpy(s3), [0, len(s3)])
+    sc = W.export_scene(scene.handle)
+    assert sc["qnodes"] is not None and sc["qframe"][6] == 1
+    q3 = rng.random((400, 3)).astype(np.float32)
+    job = B.BatchJob(scene, B.JOB_FIXED_RADIUS, DeviceBuffer.from_numpy(q3), radius=0.08, k=16)
+    (res,), _ = B.run_batch([job])
+    assert res["status"] == 0
+    got = B.job_rows(job, res)
+    got = got[np.lexsort((got["neighbor_id"], got["query_id"]))]
+    keep = np.flatnonzero(np.isfinite(s3).all(axis=1))
+    s = s3[keep].astype(np.float64)
+    q = q3.astype(np.float64)
+    want = oracle.neighbors("frn", {"id": np.arange(len(q)), "x": q[:, 0], "y": q[:, 1], "z": q[:, 2]},
+                            {"id": keep, "x": s[:, 0], "y": s[:, 1], "z": s[:, 2]}, radius=0.08, k=16)
+    want = want[np.lexsort((want["nid"], want["qid"]))]
+    assert len(got) == len(want) > 0
+    assert np.array_equal(got["query_id"], want["qid"])
+    assert np.array_equal(got["neighbor_id"], want["nid"])
+    assert np.array_equal(got["distance"], want["dist"])
 
 
 def _sorted_rows(rows, key):
