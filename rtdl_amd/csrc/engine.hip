@@ -635,7 +635,10 @@ __global__ __launch_bounds__(256, 5) void nbr_count_kernel(
     const SceneView sc = j.scene;  // wave-uniform: kept in scalar registers
     const bool active = lane < w.count;
     const uint32_t k = j.k;
-    const uint32_t step = k / 2 > 4u ? k / 2 : 4u;
+    // a lane checks for a k-th bucket each time it has counted another
+    // max(k/8, 4) neighbours: frequent checks shrink the ball early, and a
+    // check reads the histogram eight buckets per LDS round trip
+    const uint32_t step = k / 8 > 4u ? k / 8 : 4u;
     uint32_t qi = 0;
     double qx = 0.0, qy = 0.0, qz = 0.0;
     if (active) {

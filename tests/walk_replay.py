@@ -272,7 +272,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
     queue is empty, while any are), a batch of up to 64 is taken in queue
     order: each point inside the owner's FP64 radius whose distance bucket
     does not lie beyond the owner's last counted bucket is counted; then
-    every lane whose count reached its checkpoint (every max(k/2, 4)) shrinks
+    every lane whose count reached its checkpoint (every max(k/8, 4)) shrinks
     its threshold to the edge of the bucket holding its k-th neighbour.
     Rounds are wavefront-synchronous: a lane whose buckets up to the k-th
     still hold more than 128 neighbours counts again over the finer buckets
@@ -290,7 +290,7 @@ def frn_counts(sc: dict, queries, radius: float, k: int, *, instance: int = 0) -
     m = [coord_bound(sc["m"], *row) for row in q]
     rad = [radius] * lanes
     rad_sq = [radius * radius] * lanes
-    step = k // 2 if k // 2 > 4 else 4
+    step = k // 8 if k // 8 > 4 else 4
     going = [True] * lanes
     n_nodes = n_prims = n_steps = 0
 
